@@ -1,0 +1,203 @@
+/*
+ * visualtur_amd.h -- C ABI of the MI355X isosurface ray-casting library.
+ *
+ * The reference (carlosduelo/visualTur) exposes this path as C++ classes
+ * whose methods launch CUDA kernels directly.  Here the same path sits
+ * behind plain C entry points (opaque handles, plain pointers and sizes,
+ * int status codes, no exceptions); the C++ classes in visualTur.hpp,
+ * Camera.hpp, Octree.hpp, lruCache.hpp, rayCaster.hpp and FileManager.hpp
+ * are written on top of these.  Each declaration names the reference
+ * interface it replaces.
+ *
+ * Conventions
+ *   - every function returns VT_OK (0) or a negative VT_E_* status;
+ *     vt_last_error() returns the message of the last failure on the
+ *     calling thread;
+ *   - pointers named d_* are device (HBM) pointers, e.g. from vt_malloc;
+ *   - one vt_context per host thread and device; calls on one context are
+ *     not re-entrant.  Its kernels run on the context's own HIP stream.
+ */
+#ifndef VISUALTUR_AMD_H
+#define VISUALTUR_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    VT_OK = 0,
+    VT_E_ARG = -1,      /* invalid argument or size */
+    VT_E_HIP = -2,      /* HIP runtime error */
+    VT_E_IO = -3,       /* file could not be opened / read / written */
+    VT_E_FORMAT = -4,   /* bad octree magic or malformed run list */
+    VT_E_STATE = -5,    /* call out of order (e.g. no octree uploaded) */
+    VT_E_NOMEM = -6,    /* allocation failed */
+    VT_E_RANGE = -7     /* a sample fell outside its brick slot */
+};
+
+typedef struct vt_context vt_context;
+typedef struct vt_file vt_file;
+
+/* Camera pose and screen, world units = voxels.  Same fields as the
+ * reference Camera+Screen state (Camera.hpp:31-41, Screen.hpp:11-18);
+ * h and w are the screen plane size 2*d*tan(fov) (Screen.cpp:18-19). */
+typedef struct {
+    float pos[3];
+    float look[3];
+    float up[3];
+    float right[3];
+    float h, w, distance;
+    int32_t W, H;
+} vt_camera;
+
+/* Per-frame constants the reference spreads over visualTur/rayCaster
+ * (visualTur.cu:30, rayCaster.cu:245, visualTur.hpp:24-27). */
+typedef struct {
+    int32_t levelO;        /* octreeLevel: level the traversal emits */
+    int32_t levelC;        /* levelCubes: brick level of the cache */
+    float iso;             /* isovalue (the reference takes it from the octree file) */
+    float light[3];        /* light position, (0,512,0) in the reference */
+    float color_scale;     /* 256 in the reference */
+} vt_render_params;
+
+/* Hit record, 32 bytes per ray (the reference keeps only RGBA; these are
+ * the node/brick/voxel of visibleCube_t at the moment of the hit,
+ * config.hpp:13-19). */
+typedef struct {
+    uint64_t node;         /* levelO node Morton code, 0 on miss */
+    uint64_t brick;        /* levelC brick code, 0 on miss */
+    int32_t voxel[3];      /* floor of the secant point */
+    float depth;           /* |Xnew - origin| */
+} vt_hit;
+
+typedef struct {
+    int32_t passes;             /* kernel passes (the reference's while(notEnd) iterations) */
+    int32_t err;                /* nonzero: a sample fell outside its slot */
+    int64_t bricks_requested;   /* distinct bricks missing per pass, summed */
+    int64_t bricks_uploaded;    /* bricks copied into the device pool */
+    int64_t host_tier_misses;   /* bricks read from the file during the frame */
+    int64_t rays_parked;        /* ray-waits summed over passes */
+    uint64_t samples;           /* trilinear samples (when stats enabled) */
+    uint64_t nodes;             /* nodes marched (when stats enabled) */
+    double ms_total;            /* frame wall time */
+    double ms_kernel;           /* device time of the render passes */
+    double ms_service;          /* host time spent servicing misses */
+} vt_frame_stats;
+
+typedef struct {
+    int64_t gpu_slots, gpu_resident;
+    int64_t cpu_slots, cpu_resident;
+    int64_t lookups_missed;     /* device misses (bricks requested) */
+    int64_t uploads;            /* device pool fills */
+    int64_t file_reads;         /* host tier fills from the file */
+    int64_t evictions;
+} vt_cache_stats;
+
+/* ------------------------------------------------------------ context */
+
+/* Replaces the implicit CUDA context + cudaSetDevice (testVisualTur.cu:18-23). */
+int vt_context_create(int device, vt_context **out);
+int vt_context_destroy(vt_context *ctx);
+/* Message of the last failure on this thread (ctx may be NULL). */
+int vt_last_error(vt_context *ctx, char *buf, size_t len);
+/* Replaces cudaDeviceSynchronize after each launch (rayCaster.cu:382). */
+int vt_synchronize(vt_context *ctx);
+
+/* Device memory helpers (the reference calls cudaMalloc/cudaMemcpy inline,
+ * e.g. visualTur.cu:17, testVisualTur.cu:70,90). */
+int vt_malloc(vt_context *ctx, size_t bytes, void **out);
+int vt_free(vt_context *ctx, void *p);
+int vt_memcpy_h2d(vt_context *ctx, void *d_dst, const void *src, size_t bytes);
+int vt_memcpy_d2h(vt_context *ctx, void *dst, const void *d_src, size_t bytes);
+int vt_memset(vt_context *ctx, void *d_dst, int value, size_t bytes);
+
+/* -------------------------------------------------------------- octree */
+
+/* Replaces the upload half of Octree::Octree (Octree.cu:820-848): run lists
+ * of levels 0..maxLevel, concatenated, sizes[l] u64 entries each (sorted
+ * inclusive [start,end] pairs).  Malformed lists are rejected. */
+int vt_octree_upload(vt_context *ctx, const uint64_t *runs, const int32_t *sizes,
+                     int32_t nLevels, int32_t maxLevel);
+/* Replaces Octree::Octree(file, camera, maxLevel) (Octree.cu:769-863):
+ * reads the reference binary file, uploads levels 0..maxLevel, returns the
+ * file's isovalue (Octree::getIsosurface, Octree.cu:872) and nLevels
+ * (Octree::getnLevels, Octree.cu:877). */
+int vt_octree_load_file(vt_context *ctx, const char *path, int32_t maxLevel,
+                        float *iso, int32_t *nLevels, int32_t realDim[3]);
+
+/* -------------------------------------------------------------- camera */
+
+/* Replaces Camera::UpdateRays / cuda_updateRays (Camera.cu:39-83): W*H
+ * normalized float3 directions into d_dirs (pixel id = i*W + j). */
+int vt_camera_rays(vt_context *ctx, const vt_camera *cam, float *d_dirs);
+/* Screen::Screen plane size (Screen.cpp:18-19): h = 2 d tan(fovH deg),
+ * w = 2 d tan(fovW deg) -- the fov arguments are half-angles. */
+int vt_screen_size(float distance, float fovH, float fovW, float *h, float *w);
+
+/* --------------------------------------------------------- brick cache */
+
+/* Replaces lruCache::lruCache (lruCache.cu:101-129): a device pool of
+ * gpuSlots bricks of (cubeDim+2*cubeInc)^3 floats at level levelC, backed
+ * by a pinned host tier of cpuSlots bricks (0 = no host tier). */
+int vt_cache_create(vt_context *ctx, int32_t levelC, int32_t cubeDim, int32_t cubeInc,
+                    int64_t gpuSlots, int64_t cpuSlots);
+/* Brick source = HDF5 dataset (lruCache.cu:118 opens a FileManager). */
+int vt_cache_attach_hdf5(vt_context *ctx, const char *file, const char *dataset);
+/* Brick source = a volume already in HBM, C order [X][Y][Z] (misses are
+ * assembled on the device; no host tier involved). */
+int vt_cache_attach_device_volume(vt_context *ctx, const float *d_vol, int32_t X, int32_t Y,
+                                  int32_t Z);
+/* Start filling the host tier from the file on the reader thread
+ * (wait=1 blocks until done). */
+int vt_cache_prefetch_host(vt_context *ctx, int wait);
+/* Load the given bricks (level-levelC codes) into the device pool ahead of
+ * use, as far as capacity allows (lruCache::updateCube, lruCache.cu:168-296). */
+int vt_cache_load_bricks(vt_context *ctx, const uint64_t *codes, int64_t count);
+/* Drop every device-resident brick. */
+int vt_cache_clear(vt_context *ctx);
+/* lruCache::printStatistics (lruCache.cu:318-322) as numbers. */
+int vt_cache_get_stats(vt_context *ctx, vt_cache_stats *out);
+/* Replaces lruCache::changeDimensionCube (lruCache.cu:141-166). */
+int vt_cache_destroy(vt_context *ctx);
+
+/* -------------------------------------------------------------- render */
+
+/* Replaces visualTur::updateVisibleCubes (visualTur.cu:167-228): renders
+ * one full frame into d_rgba (W*H float4, pixel id = i*W + j, caller-owned
+ * device memory as in testVisualTur.cu:70) and, if non-NULL, d_hits (W*H
+ * vt_hit).  Runs traversal+march+shade passes until every ray is painted,
+ * loading missing bricks between passes.  stats may be NULL. */
+int vt_render_frame(vt_context *ctx, const vt_camera *cam, const vt_render_params *p,
+                    float *d_rgba, vt_hit *d_hits, vt_frame_stats *stats);
+/* Enable per-frame sample/node counters and the visited-brick trace. */
+int vt_set_frame_options(vt_context *ctx, int collect_stats, int record_trace);
+/* Visited-brick trace of the last frame: out[i] = 1 if brick
+ * (i + 8^levelC) was marched by at least one ray.  n = 8^levelC. */
+int vt_get_brick_trace(vt_context *ctx, uint8_t *out, int64_t n);
+
+/* Octree::getBoxIntersected repeated to exhaustion (Octree.cu:883-901):
+ * for every pixel, the first maxNodes level-finalLevel nodes in emission
+ * order into d_nodes[id*maxNodes + k], and the emission count (capped at
+ * capTotal) into d_counts[id]. */
+int vt_trace_nodes(vt_context *ctx, const vt_camera *cam, int32_t finalLevel,
+                   int32_t maxNodes, int32_t capTotal, uint64_t *d_nodes, int32_t *d_counts);
+
+/* ---------------------------------------------------------------- files */
+
+/* FileManager::FileManager (FileManager.cu:11-37), read-only. */
+int vt_file_open(const char *file, const char *dataset, vt_file **out, int64_t dims[3]);
+int vt_file_close(vt_file *f);
+/* FileManager::readHDF5_Voxel_Array (FileManager.cu:86-167): box [s,e),
+ * zero outside the dataset, C order, host memory. */
+int vt_file_read_box(vt_file *f, const int32_t s[3], const int32_t e[3], float *out);
+/* FileManager::CreateFile (FileManager.cu:169-243); chunk 0 = contiguous. */
+int vt_file_create(const char *file, const char *dataset, const float *data,
+                   int64_t X, int64_t Y, int64_t Z, int32_t chunk);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,201 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle on the
+same seeded inputs."""
+import math
+
+import numpy as np
+import pytest
+
+from gpu_scenes import Scene, assert_same, bitexact_fraction, blobs_volume, flat_runs, orbit_camera
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def test_camera_rays_bit_exact(vt, vto, ctx):
+    for k, (W, H) in enumerate([(1, 1), (7, 5), (64, 48), (257, 131)]):
+        cam = orbit_camera(vt, W, H, (32, 32, 32), 90.0, 37.0 * k, 11.0 * k)
+        g = ctx.camera_rays(cam)
+        c = vto.ray_dirs(cam)
+        assert np.array_equal(g.view(np.uint32), c.view(np.uint32)), (W, H)
+
+
+@pytest.mark.parametrize("n,level_o,seed", [(5, 2, 1), (5, 3, 2), (6, 4, 3), (6, 3, 4)])
+def test_trace_nodes_matches_reference_dfs(vt, vto, ctx, n, level_o, seed):
+    D = 1 << n
+    vol = blobs_volume(D, 8, seed=seed)
+    sc = Scene(vto, vol, 0.45, level_o, min(level_o, 2))
+    flat, sizes = flat_runs(sc.runs, level_o)
+    ctx.octree_upload(flat, sizes, sc.n, level_o)
+    K = 24
+    for t, theta in enumerate([0.0, 33.0, 141.0, 250.0]):
+        cam = orbit_camera(vt, 40, 30, (D / 2, D / 2, D / 2), 2.2 * D, theta, 15.0 + 10 * t)
+        log(f"trace n={n} levelO={level_o} theta={theta}")
+        nodes, counts = ctx.trace_nodes(cam, level_o, K, 1 << 20)
+        dirs = vto.ray_dirs(cam)
+        for r in range(cam.W * cam.H):
+            exp, cnt = sc.O.traverse(cam.pos, dirs[r], level_o, max_out=K)
+            assert counts[r] == cnt, (r, counts[r], cnt)
+            assert nodes[r, : min(cnt, K)].tolist() == exp, r
+
+
+def test_degenerate_axis_aligned_rays(vt, vto, ctx):
+    """Rays with zero direction components from grid-aligned origins give
+    NaN slab distances; the kernel must follow the reference's literal
+    bubble sort there."""
+    vol = blobs_volume(32, 10, seed=9)
+    sc = Scene(vto, vol, 0.3, 3, 2)
+    sc.setup_gpu(vt, ctx, slots=64)
+    for pos, look, up in [((16, 16, 60), (0, 0, -1), (0, 1, 0)), ((8, 8, 8), (1, 0, 0), (0, 1, 0)),
+                          ((16, 40, 16), (0, -1, 0), (0, 0, 1)), ((4, 12, -10), (0, 0, 1), (0, 1, 0))]:
+        right = np.cross(np.asarray(look, np.float64), np.asarray(up, np.float64))
+        for W, H in [(1, 1), (3, 3), (5, 5)]:
+            cam = vt.make_camera(W, H, pos, look, up, right, fov_v_deg=20.0)
+            rg, hg, _ = sc.render_gpu(vt, ctx, cam)
+            rc, hc = sc.render_cpu(vto, cam)
+            assert_same(hg, rg, hc, rc)
+
+
+@pytest.mark.parametrize("level_o,level_c,inc", [(3, 3, 3), (3, 2, 3), (4, 2, 3), (2, 1, 3)])
+def test_render_all_resident(vt, vto, ctx, level_o, level_c, inc):
+    vol = blobs_volume(64, 12, seed=level_o * 7 + level_c)
+    sc = Scene(vto, vol, 0.25, level_o, level_c)
+    sc.setup_gpu(vt, ctx, slots=1 << (3 * level_c), inc=inc)
+    for theta in (0.0, 75.0, 200.0):
+        cam = orbit_camera(vt, 96, 80, (32, 32, 32), 100.0, theta)
+        log(f"render levelO={level_o} levelC={level_c} theta={theta}")
+        rg, hg, st = sc.render_gpu(vt, ctx, cam)
+        rc, hc = sc.render_cpu(vto, cam)
+        assert st.err == 0
+        assert_same(hg, rg, hc, rc)
+        assert bitexact_fraction(hg, rg, hc, rc) == 1.0
+        assert (hc["node"] != 0).mean() > 0.1   # the view actually hits the surface
+
+
+@pytest.mark.parametrize("slots", [1, 2, 5, 17])
+def test_capacity_invariance(vt, vto, ctx, slots):
+    """Hits do not depend on the device cache capacity (the reference's GPU
+    tier could overwrite a brick another ray still used, lruCache.cu:254-266)."""
+    vol = blobs_volume(64, 12, seed=11)
+    sc = Scene(vto, vol, 0.25, 3, 2)
+    sc.setup_gpu(vt, ctx, slots=slots)
+    cam = orbit_camera(vt, 64, 64, (32, 32, 32), 100.0, 30.0)
+    rg, hg, st = sc.render_gpu(vt, ctx, cam)
+    rc, hc = sc.render_cpu(vto, cam)
+    assert_same(hg, rg, hc, rc)
+    assert st.passes >= 2
+    cs = ctx.cache_stats()
+    assert cs["gpu_resident"] <= slots
+
+
+def test_literal_octree_skips_last_octant(vt, vto, ctx):
+    """Q1 (CreateOctree.cu:550): with the reference's builder the (+,+,+)
+    octant is absent and the GPU follows the file it is given."""
+    vol = blobs_volume(32, 12, seed=21)
+    sc = Scene(vto, vol, 0.4, 3, 2, literal=True)
+    sc.setup_gpu(vt, ctx, slots=64)
+    cam = orbit_camera(vt, 64, 64, (16, 16, 16), 70.0, 45.0, 35.0)
+    rg, hg, _ = sc.render_gpu(vt, ctx, cam)
+    rc, hc = sc.render_cpu(vto, cam)
+    assert_same(hg, rg, hc, rc)
+    # no hit inside octant 7 = [16,32)^3
+    hit = hc["node"] != 0
+    v = hc["voxel"][hit]
+    assert not np.any((v[:, 0] >= 16) & (v[:, 1] >= 16) & (v[:, 2] >= 16))
+
+
+def test_hdf5_host_tier_paths(vt, vto, ctx, tmp_path):
+    vol = blobs_volume(64, 10, seed=31, dims=(64, 60, 50))
+    sc = Scene(vto, vol, 0.5, 3, 2)
+    h5 = tmp_path / "vol.h5"
+    vt.file_create(h5, "value", vol, chunk=16)
+    octf = tmp_path / "vol.oct"
+    vto.octree_write(octf, sc.iso, 64, vol.shape, sc.n, sc.runs)
+    cam = orbit_camera(vt, 80, 64, (32, 30, 25), 140.0, 120.0)
+    rc, hc = sc.render_cpu(vto, cam)
+    for cpu_slots, gpu_slots in [(64, 64), (64, 6), (9, 5)]:
+        log(f"hdf5 cpu_slots={cpu_slots} gpu_slots={gpu_slots}")
+        iso, n, rd = ctx.octree_load_file(octf, 3)
+        assert (n, rd) == (6, (64, 60, 50)) and iso == np.float32(0.5)
+        ctx.cache_create(2, 16, 3, gpu_slots, cpu_slots)
+        ctx.cache_attach_hdf5(h5, "value")
+        if cpu_slots >= 64:
+            ctx.cache_prefetch_host(wait=True)
+        prm = vt.render_params(3, 2, iso)
+        R = cam.W * cam.H
+        d_rgba, d_hits = ctx.alloc(R * 16), ctx.alloc(R * 32)
+        ctx.render_frame(cam, prm, d_rgba, d_hits)
+        rg, hg = d_rgba.download(np.float32, (R, 4)), d_hits.download(vt.HIT_DTYPE, (R,))
+        d_rgba.free(); d_hits.free()
+        assert_same(hg, rg, hc, rc)
+
+
+def test_file_read_box_matches_oracle(vt, vto, tmp_path):
+    rng = np.random.default_rng(4)
+    vol = rng.standard_normal((20, 18, 16)).astype(np.float32)
+    p = tmp_path / "r.h5"
+    vt.file_create(p, "value", vol, chunk=8)
+    V = vto.Volume(vol)
+    with vt.File(p, "value") as f:
+        assert f.dims == (20, 18, 16)
+        for s, e in [((0, 0, 0), (10, 10, 10)), ((-3, -2, -1), (7, 8, 9)), ((15, 15, 10), (25, 25, 20)),
+                     ((30, 0, 0), (34, 4, 4)), ((-8, -8, -8), (-1, 3, 3))]:
+            assert np.array_equal(f.read_box(s, e), V.read_box(s, e))
+
+
+def test_error_paths(vt, vto, ctx, tmp_path):
+    c = vt.Context(0)
+    try:
+        cam = vt.make_camera(8, 8, (0, 0, 50))
+        prm = vt.render_params(2, 1, 0.5)
+        buf = c.alloc(64 * 16)
+        with pytest.raises(vt.VTError) as e:
+            c.render_frame(cam, prm, buf, None)
+        assert e.value.code == -5   # VT_E_STATE: no octree
+        bad = np.array([9, 8], np.uint64)   # start after end
+        with pytest.raises(vt.VTError) as e:
+            c.octree_upload(np.concatenate([np.array([1, 1], np.uint64), bad]), [2, 2], 3, 1)
+        assert e.value.code == -4   # VT_E_FORMAT
+        f = tmp_path / "bad.oct"
+        f.write_bytes(b"\x01\x02\x03\x04" + b"\x00" * 40)
+        with pytest.raises(vt.VTError) as e:
+            c.octree_load_file(f, 2)
+        assert e.value.code == -4
+        with pytest.raises(vt.VTError) as e:
+            vt.File(tmp_path / "missing.h5", "value")
+        assert e.value.code == -3   # VT_E_IO
+        buf.free()
+    finally:
+        c.close()
+
+
+def test_halo_too_small_is_reported(vt, vto, ctx):
+    """With cubeInc=0 the march reads past the brick's upper faces (the
+    reference reads out of its slot, Q4).  Every read is bounds-checked:
+    the frame returns VT_E_RANGE instead of reading outside the slot."""
+    vol = blobs_volume(64, 12, seed=11)
+    sc = Scene(vto, vol, 0.25, 3, 2)
+    sc.setup_gpu(vt, ctx, slots=64, inc=0)
+    cam = orbit_camera(vt, 64, 64, (32, 32, 32), 100.0, 30.0)
+    with pytest.raises(vt.VTError) as e:
+        sc.render_gpu(vt, ctx, cam)
+    assert e.value.code == -7
+
+
+def test_c1_full_frame(vt, vto, ctx):
+    """Config 1: 256^3, 16 blobs, iso 0.5, 32^3 bricks (levels 0..3, all 512
+    resident), one 512x512 frame, 45 deg fov, camera 2.5 extents from the
+    centre looking at it."""
+    vol = blobs_volume(256, 16, seed=0xC0FFEE)
+    sc = Scene(vto, vol, 0.5, 3, 3)
+    sc.setup_gpu(vt, ctx, slots=512)
+    cam = vt.make_camera(512, 512, (128.0, 128.0, 128.0 + 2.5 * 256.0), fov_v_deg=45.0)
+    log("c1 gpu")
+    rg, hg, st = sc.render_gpu(vt, ctx, cam)
+    log(f"c1 gpu done passes={st.passes}; oracle")
+    rc, hc = sc.render_cpu(vto, cam)
+    assert_same(hg, rg, hc, rc)
+    assert bitexact_fraction(hg, rg, hc, rc) == 1.0
+    assert (hc["node"] != 0).mean() > 0.1

@@ -1,0 +1,190 @@
+"""Oracle slab test, ray generation, trilinear sampling, march and shading,
+pinned by hand KATs and closed forms (affine fields are reproduced exactly
+by trilinear interpolation and by the secant step)."""
+import math
+
+import numpy as np
+import pytest
+
+
+# ------------------------------------------------------------------- slab
+def test_slab_kats(vto):
+    # axis-parallel ray through the unit box; zero components give +-inf slabs
+    hit, tn, tf = vto.slab((-5, 0.5, 0.5), (1, 0, 0), (0, 0, 0), (1, 1, 1))
+    assert hit and tn == 5.0 and tf == 6.0
+    # reversed direction: the swapped branch (divx < 0)
+    hit, tn, tf = vto.slab((7, 0.5, 0.5), (-1, 0, 0), (0, 0, 0), (2, 2, 2))
+    assert hit and tn == 5.0 and tf == 7.0
+    # origin inside: tnear clamps to 0 (Octree.cu:322-325)
+    hit, tn, tf = vto.slab((0.5, 0.5, 0.5), (0, 0, 1), (0, 0, 0), (1, 1, 1))
+    assert hit and tn == 0.0 and tf == 0.5
+    # box behind the origin still "hits" with tfar < 0 (root has no tfar test)
+    hit, tn, tf = vto.slab((0.5, 0.5, 5), (0, 0, 1), (0, 0, 0), (1, 1, 1))
+    assert hit and tf < 0
+    # clean miss
+    hit, tn, tf = vto.slab((5, 5, -5), (0, 0, 1), (0, 0, 0), (1, 1, 1))
+    assert not hit
+    # -0.0 direction: 1/-0 = -inf takes the swapped branch, still a hit
+    hit, tn, tf = vto.slab((0.5, -3, 0.5), (-0.0, 1, 0), (0, 0, 0), (1, 1, 1))
+    assert hit and tn == 3.0 and tf == 4.0
+
+
+def test_slab_matches_float64(vto):
+    rng = np.random.default_rng(1)
+    for _ in range(2000):
+        o = rng.uniform(-20, 40, 3).astype(np.float32)
+        d = rng.normal(size=3)
+        d = (d / np.linalg.norm(d)).astype(np.float32)
+        mn = rng.integers(0, 16, 3)
+        side = int(rng.integers(1, 9))
+        mx = mn + side
+        hit, tn, tf = vto.slab(o, d, mn, mx)
+        t0 = (mn - o.astype(np.float64)) / d
+        t1 = (mx - o.astype(np.float64)) / d
+        lo, hi = np.minimum(t0, t1).max(), np.maximum(t0, t1).min()
+        if abs(lo - hi) < 1e-3:
+            continue   # grazing: float rounding decides
+        assert hit == (lo <= hi)
+        if hit:
+            assert tf == pytest.approx(hi, abs=1e-3)
+            assert tn == pytest.approx(max(lo, 0.0), abs=1e-3)
+
+
+# ---------------------------------------------------------------- camera
+def test_screen_size_half_angle(vto):
+    h, w = vto.screen_size(50.0, 22.5, 30.0)
+    assert h == pytest.approx(2 * 50 * math.tan(math.radians(22.5)), rel=1e-6)
+    assert w == pytest.approx(2 * 50 * math.tan(math.radians(30.0)), rel=1e-6)
+
+
+def test_ray_dirs_closed_form(vto):
+    W, H, d = 7, 5, 50.0
+    h, w = vto.screen_size(d, 20.0, 25.0)
+    cam = dict(pos=(1, 2, 3), look=(0, 0, -1), up=(0, 1, 0), right=(1, 0, 0), h=h, w=w, distance=d, W=W, H=H)
+    dirs = vto.ray_dirs(cam)
+    for i in range(H):
+        for j in range(W):
+            A = np.array([-(w / 2) + (w / W) * (j + 0.5), -(h / 2) + (h / H) * (i + 0.5), -d])
+            A /= np.linalg.norm(A)
+            np.testing.assert_allclose(dirs[i * W + j], A, atol=2e-7)
+    assert np.allclose(np.linalg.norm(dirs.astype(np.float64), axis=1), 1.0, atol=2e-7)
+
+
+# ---------------------------------------------------------------- sampling
+def test_sample_affine_exact_and_padding(vto):
+    n = 12
+    g = np.array([0.25, -0.5, 0.125])
+    idx = np.indices((n, n, n)).astype(np.float64)
+    vol = (g[0] * idx[0] + g[1] * idx[1] + g[2] * idx[2] + 3.0).astype(np.float32)
+    V = vto.Volume(vol)
+    rng = np.random.default_rng(2)
+    for _ in range(500):
+        p = rng.uniform(0, n - 1.001, 3).astype(np.float32)
+        assert V.sample(p) == pytest.approx(float(g @ p.astype(np.float64) + 3.0), abs=2e-5)
+    assert V.voxel(-1, 0, 0) == 0.0 and V.voxel(0, n, 0) == 0.0
+    assert V.sample((3.0, 4.0, 5.0)) == vol[3, 4, 5]
+    # (n-1, n-1, n-1) + 0.5 blends the last voxel with the zero padding
+    assert V.sample((n - 0.5, 0.0, 0.0)) == pytest.approx(0.5 * vol[n - 1, 0, 0], rel=1e-6)
+
+
+def test_sample_negative_coordinate_extrapolates(vto):
+    """modff truncates toward zero (rayCaster.cu:89): at x=-0.25 the weights
+    are (1.25, -0.25) on voxels 0 and 1."""
+    vol = np.zeros((4, 4, 4), np.float32)
+    vol[0, 0, 0], vol[1, 0, 0] = 2.0, 6.0
+    V = vto.Volume(vol)
+    assert V.sample((-0.25, 0.0, 0.0)) == pytest.approx(2.0 * 1.25 + 6.0 * -0.25)
+    assert math.isnan(V.sample((float("nan"), 0.0, 0.0)))
+
+
+def test_read_box_filemanager_pattern(vto):
+    """testFileManager.cu:8-104: a box with negative offsets is zero outside
+    the dataset and equals the shifted read inside."""
+    rng = np.random.default_rng(3)
+    vol = rng.standard_normal((20, 18, 16)).astype(np.float32)
+    V = vto.Volume(vol)
+    base = V.read_box((0, 0, 0), (10, 10, 10))
+    assert np.array_equal(base, vol[:10, :10, :10])
+    for off in [(-3, 0, 0), (-2, -5, -1), (0, -4, 0)]:
+        s = off
+        e = (off[0] + 10, off[1] + 10, off[2] + 10)
+        box = V.read_box(s, e)
+        for x in range(10):
+            for y in range(10):
+                for z in range(10):
+                    gx, gy, gz = s[0] + x, s[1] + y, s[2] + z
+                    inside = gx >= 0 and gy >= 0 and gz >= 0
+                    assert box[x, y, z] == (vol[gx, gy, gz] if inside else 0.0)
+    # beyond the upper faces and fully outside
+    box = V.read_box((15, 15, 10), (25, 25, 20))
+    assert np.array_equal(box[:5, :3, :6], vol[15:20, 15:18, 10:16]) and box[5:].sum() == 0
+    assert V.read_box((30, 0, 0), (34, 4, 4)).sum() == 0
+
+
+# ------------------------------------------------------------------ frame
+def one_leaf_octree(vto, n, level_o, vol, iso):
+    nl, runs = vto.build_octree(vol, iso, literal=False)
+    assert nl == n
+    return vto.Octree(runs, n, level_o)
+
+
+def test_affine_march_depth_and_shading(vto):
+    """Affine field f = g.x + c: trilinear and secant are exact along the ray,
+    so the hit depth, voxel and shading are closed forms."""
+    n = 5
+    idx = np.indices((32, 32, 32)).astype(np.float64)
+    # f = 1 - z/32: below iso next to the zero padding at the top face, so the
+    # first crossing along a downward ray is the plane z = 13
+    vol = (1.0 - idx[2] / 32.0).astype(np.float32)
+    iso = 19.0 / 32.0
+    oct = one_leaf_octree(vto, n, 2, vol, iso)
+    h, w = vto.screen_size(50.0, 10.0, 10.0)
+    cam = dict(pos=(16.2, 15.7, 40.0), look=(0, 0, -1), up=(0, 1, 0), right=(1, 0, 0), h=h, w=w,
+               distance=50.0, W=9, H=9)
+    prm = vto.params(2, 1, iso)
+    rgba, hits, _ = vto.render(vto.Volume(vol), oct, cam, prm)
+    dirs = vto.ray_dirs(cam)
+    o = np.array(cam["pos"], np.float64)
+    for r in range(81):
+        d = dirs[r].astype(np.float64)
+        t = (13.0 - o[2]) / d[2]
+        X = o + t * d
+        assert hits[r]["node"] != 0
+        assert hits[r]["depth"] == pytest.approx(t, abs=1e-4)
+        assert tuple(hits[r]["voxel"]) == tuple(np.floor(X).astype(int))
+        # n = normalize(f(X-z)-f(X+z)) = (0,0,1); l = normalize(X - light)
+        l = X - np.array([0.0, 512.0, 0.0])
+        l /= np.linalg.norm(l)
+        dif = abs(l[2])
+        a = X[1] / 256.0
+        np.testing.assert_allclose(rgba[r], [(1 - a) * dif, a * dif, 0.0, 1.0], atol=2e-5)
+
+
+def test_constant_iso_field_gives_nan_hit(vto):
+    """Q12 KAT: a node entered where f == iso for two consecutive samples
+    makes a = 0/0 (rayCaster.cu:227), a NaN secant point and NaN shading.
+    f = 0.5 everywhere; only cells touching the zero padding of the upper
+    faces are valid, so the ray along +x from (8,9.5,9.5) first marches the
+    node [12,16]x[8,12]x[8,12] starting at x=12 with f=0.5 twice."""
+    vol = np.full((16, 16, 16), 0.5, np.float32)
+    oct = one_leaf_octree(vto, 4, 2, vol, 0.5)
+    h, w = vto.screen_size(50.0, 1.0, 1.0)
+    cam = dict(pos=(8.0, 9.5, 9.5), look=(1, 0, 0), up=(0, 1, 0), right=(0, 0, 1), h=h, w=w,
+               distance=50.0, W=1, H=1)
+    assert vto.ray_dirs(cam)[0].tolist() == [1.0, 0.0, 0.0]
+    rgba, hits, _ = vto.render(vto.Volume(vol), oct, cam, vto.params(2, 1, 0.5))
+    assert hits[0]["node"] == vto.encode(12, 8, 8, 2, 4)
+    assert math.isnan(hits[0]["depth"])
+    assert math.isnan(rgba[0][0]) and math.isnan(rgba[0][1]) and math.isnan(rgba[0][2])
+    assert rgba[0][3] == 1.0
+
+
+def test_render_background_is_white(vto):
+    vol = np.zeros((16, 16, 16), np.float32)
+    vol[8, 8, 8] = 1.0
+    oct = one_leaf_octree(vto, 4, 2, vol, 0.5)
+    h, w = vto.screen_size(50.0, 20.0, 20.0)
+    cam = dict(pos=(100.0, 100.0, 100.0), look=(0, 0, 1), up=(0, 1, 0), right=(-1, 0, 0), h=h, w=w,
+               distance=50.0, W=4, H=4)   # looking away from the volume
+    rgba, hits, _ = vto.render(vto.Volume(vol), oct, cam, vto.params(2, 1, 0.5))
+    assert (rgba == 1.0).all() and (hits["node"] == 0).all()

@@ -1,0 +1,153 @@
+"""Oracle traversal order and octree builder, pinned by independent
+formulations: brute-force path-key sorting for the DFS (Octree.cu:620-739)
+and the min/max closed form for node validity (CreateOctree.cu:425-573)."""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+from conftest import blobs_volume
+
+
+def random_octree(vto, rng, n, density):
+    """Random leaf-level node set ORed up to the root (well-formed runs)."""
+    sets = [None] * (n + 1)
+    D = 1 << n
+    leaf = set()
+    for x, y, z in itertools.product(range(D), repeat=3):
+        if rng.random() < density:
+            leaf.add(vto.encode(x, y, z, n, n))
+    sets[n] = leaf
+    for l in range(n - 1, -1, -1):
+        sets[l] = {c >> 3 for c in sets[l + 1]}
+    runs = []
+    for l in range(n + 1):
+        codes = sorted(sets[l])
+        r = []
+        for c in codes:
+            if r and r[-1] + 1 == c:
+                r[-1] = c
+            else:
+                r += [c, c]
+        runs.append(np.array(r, np.uint64))
+    return sets, runs
+
+
+def brute_force_order(vto, sets, n, final, o, d):
+    """Emitted nodes = final-level nodes whose ancestors (root excepted) are all
+    members, slab-hit and tfar >= 0; ordered by the path of sibling keys
+    (tnear, tfar, k) from level 1 down."""
+    if 1 not in sets[0]:
+        return []
+    hit, _, _ = vto.slab(o, d, (0, 0, 0), (1 << n,) * 3)
+    if not hit:
+        return []
+    out = []
+    for c in sets[final]:
+        key = []
+        ok = True
+        for l in range(1, final + 1):
+            a = c >> (3 * (final - l))
+            if a not in sets[l]:
+                ok = False
+                break
+            mn = vto.decode(a, l, n)
+            s = 1 << (n - l)
+            h, tn, tf = vto.slab(o, d, mn, (mn[0] + s, mn[1] + s, mn[2] + s))
+            if not (h and tf >= 0.0):
+                ok = False
+                break
+            key.append((tn, tf, a & 7))
+        if ok:
+            out.append((key, c))
+    out.sort(key=lambda kc: kc[0])
+    return [c for _, c in out]
+
+
+@pytest.mark.parametrize("n,final,density", [(3, 2, 0.3), (3, 3, 0.05), (4, 3, 0.02), (4, 2, 0.05)])
+def test_dfs_order_equals_path_key_sort(vto, n, final, density):
+    rng = np.random.default_rng(n * 10 + final)
+    sets, runs = random_octree(vto, rng, n, density)
+    oct = vto.Octree(runs, n, final)
+    D = 1 << n
+    for _ in range(150):
+        o = rng.uniform(-D, 2 * D, 3).astype(np.float32)
+        tgt = rng.uniform(0, D, 3)
+        d = tgt - o
+        d = (d / np.linalg.norm(d)).astype(np.float32)
+        got, cnt = oct.traverse(o, d, final, max_out=4096)
+        exp = brute_force_order(vto, sets, n, final, o, d)
+        assert got == exp
+
+
+def closed_form_levels(vol, iso, n):
+    """valid(level l node) <=> min < iso <= max over its (s+1)^3 zero-padded
+    samples; computed from per-cell min/max with a max/min pyramid."""
+    D = 1 << n
+    pad = np.zeros((D + 1, D + 1, D + 1), np.float32)
+    X, Y, Z = vol.shape
+    pad[:X, :Y, :Z] = vol
+    lo = pad[:-1, :-1, :-1].copy()
+    hi = lo.copy()
+    for dx, dy, dz in itertools.product((0, 1), repeat=3):
+        c = pad[dx:dx + D, dy:dy + D, dz:dz + D]
+        lo = np.minimum(lo, c)
+        hi = np.maximum(hi, c)
+    levels = {}
+    for l in range(n, -1, -1):
+        valid = (lo < iso) & (iso <= hi)
+        levels[l] = valid
+        if l > 0:
+            s = lo.shape[0] // 2
+            lo = lo.reshape(s, 2, s, 2, s, 2).min(axis=(1, 3, 5))
+            hi = hi.reshape(s, 2, s, 2, s, 2).max(axis=(1, 3, 5))
+    return levels
+
+
+def runs_to_set(runs):
+    out = set()
+    for j in range(0, runs.size, 2):
+        out.update(range(int(runs[j]), int(runs[j + 1]) + 1))
+    return out
+
+
+@pytest.mark.parametrize("shape,iso", [((16, 16, 16), 0.5), ((13, 16, 9), 0.3), ((32, 32, 32), 0.6)])
+def test_builder_closed_form(vto, shape, iso):
+    vol = blobs_volume(max(shape), 6, seed=sum(shape), dims=shape)
+    n = int(math.ceil(math.log2(max(shape))))
+    levels = closed_form_levels(vol, iso, n)
+    for literal in (False, True):
+        nl, runs = vto.build_octree(vol, iso, literal=literal)
+        assert nl == n
+        for l in range(n + 1):
+            r = runs[l]
+            assert r.size % 2 == 0
+            # sorted, disjoint, and merged (a gap between consecutive runs)
+            assert all(r[j] <= r[j + 1] for j in range(0, r.size, 2))
+            assert all(r[j] > r[j - 1] + 1 for j in range(2, r.size, 2))
+            got = runs_to_set(r)
+            idx = np.argwhere(levels[l])
+            exp = {vto.encode(int(x) << (n - l), int(y) << (n - l), int(z) << (n - l), l, n) for x, y, z in idx}
+            if literal and 1 <= l < n:
+                # CreateOctree.cu:550 never visits the last node of an inner level
+                exp.discard((2 << (3 * l)) - 1)
+            if literal and l == 0:
+                exp = {1} if runs_to_set(runs[1]) else set()
+            assert got == exp, (literal, l)
+
+
+def test_octree_file_roundtrip(vto, tmp_path):
+    vol = blobs_volume(16, 4, seed=5)
+    n, runs = vto.build_octree(vol, 0.4, literal=True)
+    p = tmp_path / "t.oct"
+    vto.octree_write(p, 0.4, 16, (16, 16, 16), n, runs)
+    iso, dim, rd, nl, back = vto.octree_read(p)
+    assert (iso, dim, rd, nl) == (np.float32(0.4), 16, (16, 16, 16), n)
+    assert all(np.array_equal(a, b) for a, b in zip(runs, back))
+    raw = p.read_bytes()
+    assert int.from_bytes(raw[:4], "little") == 919278872
+    bad = tmp_path / "bad.oct"
+    bad.write_bytes(b"\x00" * 4 + raw[4:])
+    with pytest.raises(ValueError):
+        vto.octree_read(bad)

@@ -1,0 +1,353 @@
+"""visualtur_amd -- MI355X isosurface ray casting (the visualTur hot path).
+
+Python access to the C ABI of include/visualtur_amd.h through ctypes.  The
+library is the HIP build in visualtur_amd/_build/libvt_hip.so; there is no
+fallback: if it is missing or no GPU is present, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from pathlib import Path
+
+import numpy as np
+
+__all__ = [
+    "VTError", "Context", "DeviceBuffer", "Camera", "RenderParams", "FrameStats", "CacheStats",
+    "HIT_DTYPE", "lib", "make_camera", "look_at", "screen_size", "file_create", "File",
+]
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "_build" / "libvt_hip.so"
+_lib = None
+
+
+class VTError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("pos", ctypes.c_float * 3), ("look", ctypes.c_float * 3), ("up", ctypes.c_float * 3),
+                ("right", ctypes.c_float * 3), ("h", ctypes.c_float), ("w", ctypes.c_float),
+                ("distance", ctypes.c_float), ("W", ctypes.c_int32), ("H", ctypes.c_int32)]
+
+
+class RenderParams(ctypes.Structure):
+    _fields_ = [("levelO", ctypes.c_int32), ("levelC", ctypes.c_int32), ("iso", ctypes.c_float),
+                ("light", ctypes.c_float * 3), ("color_scale", ctypes.c_float)]
+
+
+class FrameStats(ctypes.Structure):
+    _fields_ = [("passes", ctypes.c_int32), ("err", ctypes.c_int32), ("bricks_requested", ctypes.c_int64),
+                ("bricks_uploaded", ctypes.c_int64), ("host_tier_misses", ctypes.c_int64),
+                ("rays_parked", ctypes.c_int64), ("samples", ctypes.c_uint64), ("nodes", ctypes.c_uint64),
+                ("ms_total", ctypes.c_double), ("ms_kernel", ctypes.c_double), ("ms_service", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class CacheStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int64) for k in ("gpu_slots", "gpu_resident", "cpu_slots", "cpu_resident",
+                                             "lookups_missed", "uploads", "file_reads", "evictions")]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+HIT_DTYPE = np.dtype([("node", "<u8"), ("brick", "<u8"), ("voxel", "<i4", (3,)), ("depth", "<f4")])
+assert HIT_DTYPE.itemsize == 32
+
+_P = ctypes.c_void_p
+_SIGS = {
+    "vt_last_error": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_size_t]),
+    "vt_context_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_P)]),
+    "vt_context_destroy": (ctypes.c_int, [_P]),
+    "vt_synchronize": (ctypes.c_int, [_P]),
+    "vt_malloc": (ctypes.c_int, [_P, ctypes.c_size_t, ctypes.POINTER(_P)]),
+    "vt_free": (ctypes.c_int, [_P, _P]),
+    "vt_memcpy_h2d": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
+    "vt_memcpy_d2h": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
+    "vt_memset": (ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.c_size_t]),
+    "vt_octree_upload": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32, ctypes.c_int32]),
+    "vt_octree_load_file": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_float),
+                                           ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
+    "vt_camera_rays": (ctypes.c_int, [_P, ctypes.POINTER(Camera), _P]),
+    "vt_screen_size": (ctypes.c_int, [ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                      ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+    "vt_cache_create": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64,
+                                       ctypes.c_int64]),
+    "vt_cache_attach_hdf5": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_char_p]),
+    "vt_cache_attach_device_volume": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "vt_cache_prefetch_host": (ctypes.c_int, [_P, ctypes.c_int]),
+    "vt_cache_load_bricks": (ctypes.c_int, [_P, _P, ctypes.c_int64]),
+    "vt_cache_clear": (ctypes.c_int, [_P]),
+    "vt_cache_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(CacheStats)]),
+    "vt_cache_destroy": (ctypes.c_int, [_P]),
+    "vt_render_frame": (ctypes.c_int, [_P, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams), _P, _P,
+                                       ctypes.POINTER(FrameStats)]),
+    "vt_set_frame_options": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
+    "vt_get_brick_trace": (ctypes.c_int, [_P, _P, ctypes.c_int64]),
+    "vt_trace_nodes": (ctypes.c_int, [_P, ctypes.POINTER(Camera), ctypes.c_int32, ctypes.c_int32,
+                                      ctypes.c_int32, _P, _P]),
+    "vt_file_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_P),
+                                    ctypes.POINTER(ctypes.c_int64)]),
+    "vt_file_close": (ctypes.c_int, [_P]),
+    "vt_file_read_box": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), _P]),
+    "vt_file_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _P, ctypes.c_int64, ctypes.c_int64,
+                                      ctypes.c_int64, ctypes.c_int32]),
+}
+
+
+def lib() -> ctypes.CDLL:
+    """Load libvt_hip.so (built by __graft_entry__.build()); raises if absent."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise FileNotFoundError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, ctx=None) -> None:
+    if rc != 0:
+        buf = ctypes.create_string_buffer(1024)
+        lib().vt_last_error(ctx, buf, len(buf))
+        raise VTError(rc, buf.value.decode(errors="replace"))
+
+
+def screen_size(distance: float, fov_h: float, fov_w: float) -> tuple[float, float]:
+    """Screen::Screen plane size (Screen.cpp:18-19); fov arguments are half-angles."""
+    h, w = ctypes.c_float(), ctypes.c_float()
+    _check(lib().vt_screen_size(distance, fov_h, fov_w, ctypes.byref(h), ctypes.byref(w)))
+    return h.value, w.value
+
+
+def look_at(eye, target, up_hint=(0.0, 1.0, 0.0)):
+    """Orthonormal (look, up, right) with right = look x up as in Camera.cu:20."""
+    e = np.asarray(eye, np.float64)
+    t = np.asarray(target, np.float64)
+    look = t - e
+    look /= np.linalg.norm(look)
+    right = np.cross(look, np.asarray(up_hint, np.float64))
+    right /= np.linalg.norm(right)
+    up = np.cross(right, look)
+    return look.astype(np.float32), up.astype(np.float32), right.astype(np.float32)
+
+
+def make_camera(W: int, H: int, pos, look=(0, 0, -1), up=(0, 1, 0), right=None, fov_v_deg: float = 45.0,
+                distance: float = 50.0, fov_w_half_deg: float | None = None) -> Camera:
+    """Camera with a *full* vertical field of view fov_v_deg: the reference's
+    Screen takes a half-angle (Screen.cpp:18), so fov_v_deg/2 is passed, and
+    the horizontal half-angle follows the aspect ratio unless given."""
+    look = np.asarray(look, np.float32)
+    up = np.asarray(up, np.float32)
+    if right is None:
+        right = np.cross(look.astype(np.float64), up.astype(np.float64)).astype(np.float32)
+    half_v = fov_v_deg / 2.0
+    if fov_w_half_deg is None:
+        fov_w_half_deg = math.degrees(math.atan(math.tan(math.radians(half_v)) * W / H))
+    h, w = screen_size(distance, half_v, fov_w_half_deg)
+    c = Camera()
+    c.pos[:] = [float(v) for v in np.asarray(pos, np.float32)]
+    c.look[:] = [float(v) for v in look]
+    c.up[:] = [float(v) for v in up]
+    c.right[:] = [float(v) for v in np.asarray(right, np.float32)]
+    c.h, c.w, c.distance = h, w, distance
+    c.W, c.H = W, H
+    return c
+
+
+class DeviceBuffer:
+    """Device allocation owned by a Context."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _check(lib().vt_malloc(ctx.handle, self.nbytes, ctypes.byref(p)), ctx.handle)
+        self.ptr = p.value
+
+    def upload(self, arr: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        _check(lib().vt_memcpy_h2d(self.ctx.handle, self.ptr, a.ctypes.data, a.nbytes), self.ctx.handle)
+        return self
+
+    def download(self, dtype, shape) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        _check(lib().vt_memcpy_d2h(self.ctx.handle, out.ctypes.data, self.ptr, out.nbytes), self.ctx.handle)
+        return out
+
+    def zero(self) -> None:
+        _check(lib().vt_memset(self.ctx.handle, self.ptr, 0, self.nbytes), self.ctx.handle)
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().vt_free(self.ctx.handle, self.ptr)
+            self.ptr = None
+
+
+class Context:
+    """One vt_context (device, stream, octree, brick cache)."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib().vt_context_create(device, ctypes.byref(h)))
+        self.handle = h.value
+        self._keep = []
+
+    def close(self) -> None:
+        if self.handle:
+            lib().vt_context_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _c(self, rc):
+        _check(rc, self.handle)
+
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    def to_device(self, arr: np.ndarray) -> DeviceBuffer:
+        a = np.ascontiguousarray(arr)
+        return DeviceBuffer(self, a.nbytes).upload(a)
+
+    def synchronize(self) -> None:
+        self._c(lib().vt_synchronize(self.handle))
+
+    # ---- octree
+    def octree_upload(self, runs: np.ndarray, sizes, n_levels: int, max_level: int) -> None:
+        r = np.ascontiguousarray(runs, np.uint64)
+        s = np.ascontiguousarray(np.asarray(sizes)[: max_level + 1], np.int32)
+        self._c(lib().vt_octree_upload(self.handle, r.ctypes.data, s.ctypes.data, n_levels, max_level))
+
+    def octree_load_file(self, path: str, max_level: int):
+        iso, n = ctypes.c_float(), ctypes.c_int32()
+        rd = (ctypes.c_int32 * 3)()
+        self._c(lib().vt_octree_load_file(self.handle, str(path).encode(), max_level, ctypes.byref(iso),
+                                          ctypes.byref(n), rd))
+        return iso.value, n.value, tuple(rd)
+
+    # ---- cache
+    def cache_create(self, level_c: int, cube_dim: int, cube_inc: int, gpu_slots: int, cpu_slots: int = 0):
+        self._c(lib().vt_cache_create(self.handle, level_c, cube_dim, cube_inc, gpu_slots, cpu_slots))
+
+    def cache_attach_device_volume(self, vol: DeviceBuffer, dims) -> None:
+        self._keep.append(vol)
+        self._c(lib().vt_cache_attach_device_volume(self.handle, vol.ptr, *[int(d) for d in dims]))
+
+    def cache_attach_hdf5(self, file: str, dataset: str) -> None:
+        self._c(lib().vt_cache_attach_hdf5(self.handle, str(file).encode(), dataset.encode()))
+
+    def cache_prefetch_host(self, wait: bool = True) -> None:
+        self._c(lib().vt_cache_prefetch_host(self.handle, 1 if wait else 0))
+
+    def cache_load_bricks(self, codes) -> None:
+        c = np.ascontiguousarray(codes, np.uint64)
+        self._c(lib().vt_cache_load_bricks(self.handle, c.ctypes.data, c.size))
+
+    def cache_clear(self) -> None:
+        self._c(lib().vt_cache_clear(self.handle))
+
+    def cache_stats(self) -> dict:
+        s = CacheStats()
+        self._c(lib().vt_cache_get_stats(self.handle, ctypes.byref(s)))
+        return s.as_dict()
+
+    # ---- render
+    def set_frame_options(self, collect_stats: bool = False, record_trace: bool = False) -> None:
+        self._c(lib().vt_set_frame_options(self.handle, int(collect_stats), int(record_trace)))
+
+    def render_frame(self, cam: Camera, params: RenderParams, d_rgba: DeviceBuffer, d_hits: DeviceBuffer | None):
+        st = FrameStats()
+        self._c(lib().vt_render_frame(self.handle, ctypes.byref(cam), ctypes.byref(params), d_rgba.ptr,
+                                      d_hits.ptr if d_hits is not None else None, ctypes.byref(st)))
+        return st
+
+    def brick_trace(self, level_c: int) -> np.ndarray:
+        n = 1 << (3 * level_c)
+        out = np.empty(n, np.uint8)
+        self._c(lib().vt_get_brick_trace(self.handle, out.ctypes.data, n))
+        return out
+
+    def trace_nodes(self, cam: Camera, final_level: int, max_nodes: int, cap_total: int):
+        R = cam.W * cam.H
+        nodes = self.alloc(max(R * max_nodes * 8, 8))
+        counts = self.alloc(R * 4)
+        try:
+            self._c(lib().vt_trace_nodes(self.handle, ctypes.byref(cam), final_level, max_nodes, cap_total,
+                                         nodes.ptr, counts.ptr))
+            n = nodes.download(np.uint64, (R, max_nodes))
+            c = counts.download(np.int32, (R,))
+        finally:
+            nodes.free()
+            counts.free()
+        return n, c
+
+    def camera_rays(self, cam: Camera) -> np.ndarray:
+        R = cam.W * cam.H
+        d = self.alloc(R * 12)
+        try:
+            self._c(lib().vt_camera_rays(self.handle, ctypes.byref(cam), d.ptr))
+            return d.download(np.float32, (R, 3))
+        finally:
+            d.free()
+
+
+def render_params(level_o: int, level_c: int, iso: float, light=(0.0, 512.0, 0.0),
+                  color_scale: float = 256.0) -> RenderParams:
+    p = RenderParams()
+    p.levelO, p.levelC, p.iso = level_o, level_c, iso
+    p.light[:] = [float(v) for v in light]
+    p.color_scale = color_scale
+    return p
+
+
+def file_create(path: str, dataset: str, vol: np.ndarray, chunk: int = 0) -> None:
+    """FileManager::CreateFile (FileManager.cu:169-243), optional chunking."""
+    v = np.ascontiguousarray(vol, np.float32)
+    X, Y, Z = v.shape
+    _check(lib().vt_file_create(str(path).encode(), dataset.encode(), v.ctypes.data, X, Y, Z, chunk))
+
+
+class File:
+    """FileManager (FileManager.cu:11-167): read-only HDF5 dataset access."""
+
+    def __init__(self, path: str, dataset: str):
+        h = ctypes.c_void_p()
+        dims = (ctypes.c_int64 * 3)()
+        _check(lib().vt_file_open(str(path).encode(), dataset.encode(), ctypes.byref(h), dims))
+        self.handle = h.value
+        self.dims = tuple(int(d) for d in dims)
+
+    def read_box(self, s, e) -> np.ndarray:
+        s3 = (ctypes.c_int32 * 3)(*[int(v) for v in s])
+        e3 = (ctypes.c_int32 * 3)(*[int(v) for v in e])
+        out = np.empty(tuple(abs(int(e[k]) - int(s[k])) for k in range(3)), np.float32)
+        _check(lib().vt_file_read_box(self.handle, s3, e3, out.ctypes.data))
+        return out
+
+    def close(self) -> None:
+        if self.handle:
+            lib().vt_file_close(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -1,0 +1,373 @@
+// vt_device.hpp -- device-side building blocks of the isosurface hot path.
+//
+// Everything here is plain FP32 with no contraction (the file sets
+// `fp contract(off)` and the build passes -ffp-contract=off); division and
+// sqrtf are the correctly rounded HIP defaults.  The arithmetic order of each
+// routine follows the reference line for line so that node codes, brick ids
+// and hit voxels come out bit-identical to the reference algorithm.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace vt {
+
+// ------------------------------------------------------------------ Morton
+// A node code at level l is a 1 sentinel bit followed by l (x,y,z) bit
+// triples, x highest (mortonCodeUtil.hpp:125-148).
+
+__host__ __device__ __forceinline__ uint32_t compact3(uint64_t x)
+{
+    // keep every third bit (bit 3b -> bit b), inverse of the bit dilation
+    x &= 0x1249249249249249ull;
+    x = (x ^ (x >> 2)) & 0x10c30c30c30c30c3ull;
+    x = (x ^ (x >> 4)) & 0x100f00f00f00f00full;
+    x = (x ^ (x >> 8)) & 0x001f0000ff0000ffull;
+    x = (x ^ (x >> 16)) & 0x001f00000000ffffull;
+    x = (x ^ (x >> 32)) & 0x00000000001fffffull;
+    return (uint32_t)x;
+}
+
+__host__ __device__ __forceinline__ uint64_t dilate3(uint32_t v)
+{
+    uint64_t x = v & 0x1fffffu;
+    x = (x | (x << 32)) & 0x001f00000000ffffull;
+    x = (x | (x << 16)) & 0x001f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+// level of a node code (code >= 1), mortonCodeUtil.hpp:21-65
+__device__ __forceinline__ int code_level(uint64_t c)
+{
+    return (63 - __clzll((long long)c)) / 3;
+}
+
+// min corner (voxels) of node `c` at level `l` in a 2^n cube,
+// mortonCodeUtil.hpp:99-123
+__host__ __device__ __forceinline__ void node_min(uint64_t c, int l, int n, int& x, int& y, int& z)
+{
+    uint64_t m = c ^ (1ull << (3 * l));
+    int s = n - l;
+    x = (int)(compact3(m >> 2) << s);
+    y = (int)(compact3(m >> 1) << s);
+    z = (int)(compact3(m) << s);
+}
+
+// ------------------------------------------------------------------- rays
+
+struct Camera {
+    float pos[3], look[3], up[3], right[3];
+    float h, w, distance;
+    int W, H;
+};
+
+struct Ray {
+    float ox, oy, oz;
+    float dx, dy, dz;
+    float ix, iy, iz;   // 1/d, computed once (Octree.cu:268 recomputes the same value)
+};
+
+// Camera.cu:58-73; normalize as v * (1/sqrt(dot(v,v)))
+__device__ __forceinline__ void make_ray(const Camera& c, int i, int j, Ray& r)
+{
+    float ih = c.h / (float)c.H;
+    float iw = c.w / (float)c.W;
+    float su = -(c.h / 2.0f) + (ih * ((float)i + 0.5f));
+    float sr = -(c.w / 2.0f) + (iw * ((float)j + 0.5f));
+    float ax = c.look[0] * c.distance, ay = c.look[1] * c.distance, az = c.look[2] * c.distance;
+    ax = ax + c.up[0] * su; ay = ay + c.up[1] * su; az = az + c.up[2] * su;
+    ax = ax + c.right[0] * sr; ay = ay + c.right[1] * sr; az = az + c.right[2] * sr;
+    float inv = 1.0f / sqrtf(ax * ax + ay * ay + az * az);
+    r.dx = ax * inv; r.dy = ay * inv; r.dz = az * inv;
+    r.ox = c.pos[0]; r.oy = c.pos[1]; r.oz = c.pos[2];
+    r.ix = 1 / r.dx; r.iy = 1 / r.dy; r.iz = 1 / r.dz;
+}
+
+// ------------------------------------------------------------------- slab
+// Octree.cu:258-330 (_cuda_RayAABB2) == rayCaster.cu:12-79
+__device__ __forceinline__ bool slab(const Ray& r, int mnx, int mny, int mnz, int side,
+                                     float& tnear, float& tfar)
+{
+    bool hit = true;
+    float tmin, tmax, tymin, tymax, tzmin, tzmax;
+    float fx0 = (float)mnx, fx1 = (float)(mnx + side);
+    float fy0 = (float)mny, fy1 = (float)(mny + side);
+    float fz0 = (float)mnz, fz1 = (float)(mnz + side);
+    if (r.ix >= 0) { tmin = (fx0 - r.ox) * r.ix; tmax = (fx1 - r.ox) * r.ix; }
+    else           { tmin = (fx1 - r.ox) * r.ix; tmax = (fx0 - r.ox) * r.ix; }
+    if (r.iy >= 0) { tymin = (fy0 - r.oy) * r.iy; tymax = (fy1 - r.oy) * r.iy; }
+    else           { tymin = (fy1 - r.oy) * r.iy; tymax = (fy0 - r.oy) * r.iy; }
+    if ((tmin > tymax) || (tymin > tmax)) hit = false;
+    if (tymin > tmin) tmin = tymin;
+    if (tymax < tmax) tmax = tymax;
+    if (r.iz >= 0) { tzmin = (fz0 - r.oz) * r.iz; tzmax = (fz1 - r.oz) * r.iz; }
+    else           { tzmin = (fz1 - r.oz) * r.iz; tzmax = (fz0 - r.oz) * r.iz; }
+    if ((tmin > tzmax) || (tzmin > tmax)) hit = false;
+    if (tzmin > tmin) tmin = tzmin;
+    if (tzmax < tmax) tmax = tzmax;
+    tnear = (tmin < 0.0f) ? 0.0f : tmin;
+    tfar = tmax;
+    return hit;
+}
+
+// --------------------------------------------------------------- octree
+// Membership is held as one byte per parent: byte (parent - 8^(l-1)) of the
+// level-l table has bit k set when child (parent<<3)+k is in the level-l run
+// list.  Built on the host from the reference run lists.
+
+struct OctreeView {
+    const uint8_t* child_bytes;   // levels 1..maxLevel concatenated
+    uint64_t level_off[24];       // byte offset of level l's table (index by l)
+    int n;                        // cube side 2^n
+    int final_level;              // levelO: the level the traversal emits
+    int root_valid;               // level-0 run list contains node 1
+};
+
+__device__ __forceinline__ uint32_t child_mask(const OctreeView& o, uint64_t parent, int plevel)
+{
+    uint64_t idx = parent - (1ull << (3 * plevel));
+    return o.child_bytes[o.level_off[plevel + 1] + idx];
+}
+
+// Literal restatement of the bubble sort of Octree.cu:665-706 on <= 8 keys.
+// Only used when a key is NaN (degenerate rays); otherwise the rank form
+// below gives the identical stable order.
+__device__ __noinline__ uint32_t bubble_order(const float* tn_in, const float* tf_in, uint32_t valid)
+{
+    float tn[8], tf[8];
+    int ks[8];
+    int nv = 0;
+    for (int k = 0; k < 8; ++k)
+        if ((valid >> k) & 1) { tn[nv] = tn_in[k]; tf[nv] = tf_in[k]; ks[nv] = k; nv++; }
+    int n = nv;
+    while (n != 0) {
+        int newn = 0;
+        for (int id = 1; id < nv; ++id) {
+            bool sw = (tn[id - 1] > tn[id]) || (tn[id - 1] == tn[id] && tf[id - 1] > tf[id]);
+            if (sw) {
+                int a = ks[id]; ks[id] = ks[id - 1]; ks[id - 1] = a;
+                float f = tn[id]; tn[id] = tn[id - 1]; tn[id - 1] = f;
+                f = tf[id]; tf[id] = tf[id - 1]; tf[id - 1] = f;
+                newn = id;
+            }
+        }
+        n = newn;
+    }
+    uint32_t order = (uint32_t)nv << 24;
+    for (int p = 0; p < nv; ++p) order |= (uint32_t)ks[p] << (3 * p);
+    return order;
+}
+
+// Children of `parent` that are members, slab-hit and have tfar >= 0,
+// in the order the reference pushes/pops them (Octree.cu:375-532, 663-712).
+// Packed result: bits [3p,3p+3) = child k at position p, bits [24,28) = count.
+__device__ __forceinline__ uint32_t sorted_children(const Ray& r, const OctreeView& o,
+                                                    uint64_t parent, int plevel)
+{
+    uint32_t members = child_mask(o, parent, plevel);
+    if (members == 0) return 0;
+    int px, py, pz;
+    node_min(parent, plevel, o.n, px, py, pz);
+    int side = 1 << (o.n - plevel - 1);
+    float tn[8], tf[8];
+    uint32_t valid = 0;
+    bool has_nan = false;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        tn[k] = 0.0f; tf[k] = 0.0f;
+        if ((members >> k) & 1) {
+            float a, b;
+            bool hit = slab(r, px + ((k >> 2) & 1) * side, py + ((k >> 1) & 1) * side,
+                            pz + (k & 1) * side, side, a, b);
+            if (hit && b >= 0.0f) {
+                valid |= 1u << k;
+                tn[k] = a; tf[k] = b;
+                has_nan |= (a != a);   // b is not NaN here (b >= 0 held)
+            }
+        }
+    }
+    if (valid == 0) return 0;
+    if (has_nan) return bubble_order(tn, tf, valid);
+    // Without NaN the comparison (tnear, then tfar) is a strict weak order and
+    // bubble sort is stable, so position = number of valid children that sort
+    // strictly before, plus equal keys with a smaller k.
+    uint32_t order = (uint32_t)__popc(valid) << 24;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        if (!((valid >> k) & 1)) continue;
+        int rank = 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            if (q == k || !((valid >> q) & 1)) continue;
+            bool before = (tn[q] < tn[k]) || (tn[q] == tn[k] && tf[q] < tf[k]) ||
+                          (tn[q] == tn[k] && tf[q] == tf[k] && q < k);
+            rank += before ? 1 : 0;
+        }
+        order |= (uint32_t)k << (3 * rank);
+    }
+    return order;
+}
+
+__device__ __forceinline__ int order_count(uint32_t order) { return (int)(order >> 24); }
+__device__ __forceinline__ int order_at(uint32_t order, int p) { return (int)((order >> (3 * p)) & 7u); }
+
+// Next node of the reference's DFS emission sequence at level final_level.
+//   descend = true : `node` itself has not been explored yet
+//   descend = false: `node` (and its subtree) is finished
+// Returns 0 when the traversal is exhausted.  This is the stack DFS of
+// Octree.cu:620-720 without the stack: the parent of a node is code>>3 and
+// its position among its siblings is found by recomputing the parent's
+// sorted child list, which is a pure function of (parent, ray).
+__device__ __forceinline__ uint64_t next_node(const Ray& r, const OctreeView& o,
+                                              uint64_t node, bool descend)
+{
+    int level = code_level(node);
+    for (;;) {
+        if (descend) {
+            if (level == o.final_level) return node;
+            uint32_t ord = sorted_children(r, o, node, level);
+            if (order_count(ord) > 0) {
+                node = (node << 3) | (uint64_t)order_at(ord, 0);
+                level += 1;
+                continue;
+            }
+            descend = false;
+        }
+        if (level == 0) return 0;
+        uint64_t parent = node >> 3;
+        int k = (int)(node & 7u);
+        uint32_t ord = sorted_children(r, o, parent, level - 1);
+        int cnt = order_count(ord);
+        int p = 0;
+        while (p < cnt && order_at(ord, p) != k) ++p;
+        if (p + 1 < cnt) {
+            node = (parent << 3) | (uint64_t)order_at(ord, p + 1);
+            descend = true;
+        } else {
+            node = parent;
+            level -= 1;
+        }
+    }
+}
+
+// Root acceptance of Octree.cu:729: level 0 holds node 1 and the ray meets
+// the cube (no tfar test at the root).
+__device__ __forceinline__ bool root_ok(const Ray& r, const OctreeView& o)
+{
+    if (!o.root_valid) return false;
+    float a, b;
+    return slab(r, 0, 0, 0, 1 << o.n, a, b);
+}
+
+// ------------------------------------------------------------ sampling
+
+__device__ __forceinline__ bool coord_ok(float p)
+{
+    return fabsf(p) < 16777216.0f;   // finite and exactly integral-convertible
+}
+
+// rayCaster.cu:86-106 (getElementInterpolate) on a brick slot of side S
+// whose voxel (0,0,0) is global voxel (bx,by,bz).  A read outside the slot
+// sets `err` and yields 0; a non-finite coordinate yields NaN.
+__device__ __forceinline__ float sample(const float* __restrict__ slot, int S, int bx, int by, int bz,
+                                        float px, float py, float pz, uint32_t& err)
+{
+    if (!coord_ok(px) || !coord_ok(py) || !coord_ok(pz)) return __builtin_nanf("");
+    float ix, iy, iz;
+    float fx = modff(px, &ix);
+    float fy = modff(py, &iy);
+    float fz = modff(pz, &iz);
+    int x0 = (int)ix - bx, y0 = (int)iy - by, z0 = (int)iz - bz;
+    if ((unsigned)x0 >= (unsigned)(S - 1) || (unsigned)y0 >= (unsigned)(S - 1) ||
+        (unsigned)z0 >= (unsigned)(S - 1)) {
+        err |= 1u;
+        return 0.0f;
+    }
+    const float* p = slot + ((size_t)x0 * S + y0) * S + z0;
+    size_t sx = (size_t)S * S, sy = (size_t)S;
+    float v000 = p[0], v001 = p[1], v010 = p[sy], v011 = p[sy + 1];
+    float v100 = p[sx], v101 = p[sx + 1], v110 = p[sx + sy], v111 = p[sx + sy + 1];
+    float c00 = v000 * (1.0f - fx) + v100 * fx;
+    float c01 = v001 * (1.0f - fx) + v101 * fx;
+    float c10 = v010 * (1.0f - fx) + v110 * fx;
+    float c11 = v011 * (1.0f - fx) + v111 * fx;
+    float c0 = c00 * (1.0f - fy) + c10 * fy;
+    float c1 = c01 * (1.0f - fy) + c11 * fy;
+    return c0 * (1.0f - fz) + c1 * fz;
+}
+
+__device__ __forceinline__ int cvt_rzi(float v)
+{
+    // truncate with saturation, NaN -> 0 (the hardware conversion semantics)
+    if (v != v) return 0;
+    if (v >= 2147483647.0f) return 2147483647;
+    if (v <= -2147483648.0f) return (int)0x80000000;
+    return (int)v;
+}
+
+// rayCaster.cu:156-237 for one node whose brick slot is `slot`.
+// Returns true on a crossing with the secant point in X*.
+__device__ __forceinline__ bool march_node(const Ray& r, int nx, int ny, int nz, int side,
+                                           const float* __restrict__ slot, int S,
+                                           int bx, int by, int bz, float iso,
+                                           float& Xx, float& Xy, float& Xz,
+                                           uint32_t& err, uint32_t& nsamples)
+{
+    float tnear, tfar;
+    if (!slab(r, nx, ny, nz, side, tnear, tfar)) return false;
+    float X0 = r.ox + tnear * r.dx, X1 = r.oy + tnear * r.dy, X2 = r.oz + tnear * r.dz;
+    float F0 = X0, F1 = X1, F2 = X2;
+    float s0 = 0.5f * r.dx, s1 = 0.5f * r.dy, s2 = 0.5f * r.dz;
+    int maxStep = cvt_rzi(ceilf((tfar - tnear) * 2.0f));
+    if (maxStep > 4 * side + 8) maxStep = 4 * side + 8;
+    if (maxStep < 0) return false;
+    float ant = sample(slot, S, bx, by, bz, X0, X1, X2, err);
+    nsamples += 1;
+    for (int steps = 1; steps <= maxStep; ++steps) {
+        X0 = X0 + s0; X1 = X1 + s1; X2 = X2 + s2;
+        float sig = sample(slot, S, bx, by, bz, X0, X1, X2, err);
+        nsamples += 1;
+        if (((iso - ant) < 0.0f && (iso - sig) < 0.0f) || ((iso - ant) > 0.0f && (iso - sig) > 0.0f)) {
+            ant = sig;
+            F0 = X0; F1 = X1; F2 = X2;
+        } else {
+            float a = (iso - ant) / (sig - ant);
+            Xx = F0 * (1.0f - a) + X0 * a;
+            Xy = F1 * (1.0f - a) + X1 * a;
+            Xz = F2 * (1.0f - a) + X2 * a;
+            return true;
+        }
+    }
+    return false;
+}
+
+// rayCaster.cu:131-137 and 238-251
+__device__ __forceinline__ void shade(const float* __restrict__ slot, int S, int bx, int by, int bz,
+                                      float Xx, float Xy, float Xz,
+                                      float lx, float ly, float lz, float color_scale,
+                                      float4& out, uint32_t& err)
+{
+    float n0 = (sample(slot, S, bx, by, bz, Xx - 1.0f, Xy, Xz, err) -
+                sample(slot, S, bx, by, bz, Xx + 1.0f, Xy, Xz, err)) / 2.0f;
+    float n1 = (sample(slot, S, bx, by, bz, Xx, Xy - 1.0f, Xz, err) -
+                sample(slot, S, bx, by, bz, Xx, Xy + 1.0f, Xz, err)) / 2.0f;
+    float n2 = (sample(slot, S, bx, by, bz, Xx, Xy, Xz - 1.0f, err) -
+                sample(slot, S, bx, by, bz, Xx, Xy, Xz + 1.0f, err)) / 2.0f;
+    float ninv = 1.0f / sqrtf(n0 * n0 + n1 * n1 + n2 * n2);
+    n0 = n0 * ninv; n1 = n1 * ninv; n2 = n2 * ninv;
+    float l0 = Xx - lx, l1 = Xy - ly, l2 = Xz - lz;
+    float linv = 1.0f / sqrtf(l0 * l0 + l1 * l1 + l2 * l2);
+    l0 = l0 * linv; l1 = l1 * linv; l2 = l2 * linv;
+    float dif = fabsf(n0 * l0 + n1 * l1 + n2 * l2);
+    float a = Xy / color_scale;
+    out.x = (1 - a) * dif;
+    out.y = (a) * dif;
+    out.z = 0.0f * dif;
+    out.w = 1.0f;
+}
+
+}  // namespace vt

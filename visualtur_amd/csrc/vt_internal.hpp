@@ -1,0 +1,88 @@
+// vt_internal.hpp -- shared host/device declarations of the HIP library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include "vt_device.hpp"
+
+namespace vt {
+
+struct HitRecord {        // 32 bytes, one per ray
+    uint64_t node;        // levelO node code of the hit, 0 on miss
+    uint64_t brick;       // levelC brick code, 0 on miss
+    int32_t voxel[3];     // floor of the secant point
+    float depth;          // |Xnew - origin|
+};
+static_assert(sizeof(HitRecord) == 32, "hit record layout");
+
+enum RayPhase : uint8_t { PHASE_START = 0, PHASE_NEED = 1, PHASE_DONE = 2 };
+
+// per-frame device counters (one 64-byte block)
+struct FrameCounters {
+    uint32_t req_count;       // distinct bricks requested in this pass
+    uint32_t parked_count;    // rays parked waiting for a brick
+    uint32_t err;             // bit 0: a sample fell outside its brick slot
+    uint32_t pad0;
+    unsigned long long samples;   // trilinear samples taken (march + normal)
+    unsigned long long nodes;     // nodes marched
+    uint32_t pad[8];
+};
+
+struct RenderArgs {
+    Camera cam;
+    OctreeView oct;
+    int level_c;
+    int shift;                   // 3*(levelO - levelC)
+    uint64_t brick_base;         // 8^levelC
+    const int32_t* brick_slot;   // brick index -> slot, -1 when absent
+    const float* pool;           // slot-major brick pool
+    int S;                       // slot side = cubeDim + 2*inc
+    int inc;
+    size_t slot_elems;           // S^3
+    float iso, lx, ly, lz, color_scale;
+    // ray state
+    uint64_t* ray_node;
+    uint8_t* ray_phase;
+    const uint32_t* in_list;     // null: first pass over the whole frame
+    uint32_t in_count;
+    uint32_t* out_list;
+    uint32_t* req_bits;
+    uint32_t* req_list;
+    uint8_t* slot_ref;
+    uint8_t* brick_trace;        // optional, 1 byte per brick
+    FrameCounters* ctr;
+    float4* rgba;
+    HitRecord* hits;
+    int tiles_x;                 // 16x16 pixel tiles per row
+    int tiles_total;
+    int collect_stats;
+};
+
+struct TraceArgs {
+    Camera cam;
+    OctreeView oct;
+    int max_nodes;
+    int cap_total;
+    uint64_t* nodes;             // [R * max_nodes]
+    int32_t* counts;             // [R]
+};
+
+// launchers (vt_kernels.hip)
+hipError_t launch_render_pass(const RenderArgs& a, hipStream_t s);
+hipError_t launch_trace_nodes(const TraceArgs& a, hipStream_t s);
+// Assemble bricks with halo from a device volume [X][Y][Z] (zero outside).
+hipError_t launch_bricks_from_volume(const float* vol, int X, int Y, int Z,
+                                     const int64_t* brick_codes, const int32_t* slots, int count,
+                                     int level_c, int n, int cube_dim, int inc,
+                                     float* pool, hipStream_t s);
+// Copy whole brick slots from (device-visible) host memory into the pool.
+hipError_t launch_copy_bricks(const float* const* src, const int32_t* dst_slots, int count,
+                              size_t slot_elems, float* pool, hipStream_t s);
+// table[idx[i]] = val[i]
+hipError_t launch_scatter_i32(int32_t* table, const int64_t* idx, const int32_t* val, int count,
+                              hipStream_t s);
+// camera rays (Camera.cu:39-74)
+hipError_t launch_camera_rays(const Camera& cam, float* dirs, hipStream_t s);
+
+}  // namespace vt

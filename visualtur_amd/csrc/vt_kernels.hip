@@ -1,0 +1,301 @@
+// vt_kernels.hip -- gfx950 kernels of the isosurface hot path.
+//
+// k_render_pass fuses what the reference runs as three stages per pass
+// (Octree::getBoxIntersected, lruCache::updateCache, rayCaster::render,
+// visualTur.cu:175-220): one thread per ray walks the octree in the
+// reference's front-to-back order, marches every emitted node whose brick is
+// resident, shades the first crossing and stops.  A ray whose next brick is
+// not resident records the brick in the request set and parks; the host
+// loads the requested bricks and relaunches over the parked rays only.
+#include "vt_internal.hpp"
+
+#pragma clang fp contract(off)
+
+namespace vt {
+
+namespace {
+
+constexpr int kBlock = 256;   // 4 wave64s = one 16x16 pixel tile
+
+// Wave-aggregated atomic add on a 32-bit counter: one atomic per wave.
+__device__ __forceinline__ uint32_t wave_atomic_add(uint32_t* ctr, bool pred)
+{
+    unsigned long long mask = __ballot(pred);
+    if (!pred) return 0;
+    int lane = (int)__lane_id();
+    unsigned long long lower = mask & ((1ull << lane) - 1ull);
+    int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(ctr, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    return base + (uint32_t)__popcll(lower);
+}
+
+__device__ __forceinline__ void write_miss(const RenderArgs& a, uint32_t id)
+{
+    a.rgba[id] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);   // rayCaster.cu:146-153
+    HitRecord h;
+    h.node = 0; h.brick = 0; h.voxel[0] = h.voxel[1] = h.voxel[2] = 0; h.depth = 0.0f;
+    a.hits[id] = h;
+    a.ray_phase[id] = PHASE_DONE;
+}
+
+__device__ __forceinline__ int floor_to_int(float x) { return cvt_rzi(floorf(x)); }
+
+}  // namespace
+
+__global__ __launch_bounds__(kBlock) void k_render_pass(RenderArgs a)
+{
+    const int W = a.cam.W, H = a.cam.H;
+    uint32_t id;
+    bool active = true;
+    if (a.in_list) {
+        uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
+        active = idx < a.in_count;
+        id = active ? a.in_list[idx] : 0;
+    } else {
+        // XCD-aware tile order: consecutive workgroups are dealt round-robin
+        // to the 8 XCDs, so give each XCD a contiguous run of tiles (shared
+        // bricks stay in that XCD's L2).
+        int b = blockIdx.x;
+        int per_xcd = (a.tiles_total + 7) / 8;
+        int t = (b % 8) * per_xcd + (b / 8);
+        active = t < a.tiles_total;
+        int ty = active ? t / a.tiles_x : 0, tx = active ? t % a.tiles_x : 0;
+        int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        int j = tx * 16 + (w & 1) * 8 + (lane & 7);
+        int i = ty * 16 + (w >> 1) * 8 + (lane >> 3);
+        active = active && i < H && j < W;
+        id = active ? (uint32_t)(i * W + j) : 0;
+    }
+
+    uint32_t err = 0, nsamples = 0, nnodes = 0;
+    bool park = false;
+    uint32_t req_idx = 0;
+    bool new_req = false;
+
+    if (active) {
+        int i = (int)(id / (uint32_t)W), j = (int)(id % (uint32_t)W);
+        Ray r;
+        make_ray(a.cam, i, j, r);
+        uint64_t node;
+        if (a.in_list == nullptr) {
+            node = root_ok(r, a.oct) ? next_node(r, a.oct, 1ull, true) : 0ull;
+        } else {
+            node = a.ray_node[id];   // parked on this node, not yet marched
+        }
+        const int n = a.oct.n, levelO = a.oct.final_level;
+        const int side = 1 << (n - levelO);
+        bool done = false;
+        while (!done) {
+            if (node == 0) {
+                write_miss(a, id);
+                done = true;
+                break;
+            }
+            uint64_t brick = node >> a.shift;
+            uint64_t bidx = brick - a.brick_base;
+            int slot = a.brick_slot[bidx];
+            if (slot < 0) {
+                // request the brick once per pass and park the ray on `node`
+                uint32_t bit = 1u << (bidx & 31);
+                uint32_t old = atomicOr(&a.req_bits[bidx >> 5], bit);
+                new_req = !(old & bit);
+                req_idx = (uint32_t)bidx;
+                a.ray_node[id] = node;
+                a.ray_phase[id] = PHASE_NEED;
+                park = true;
+                break;
+            }
+            a.slot_ref[slot] = 1;
+            if (a.brick_trace) a.brick_trace[bidx] = 1;
+            int nx, ny, nz, bx, by, bz;
+            node_min(node, levelO, n, nx, ny, nz);
+            node_min(brick, a.level_c, n, bx, by, bz);
+            bx -= a.inc; by -= a.inc; bz -= a.inc;
+            const float* sp = a.pool + (size_t)slot * a.slot_elems;
+            float Xx, Xy, Xz;
+            nnodes += 1;
+            if (march_node(r, nx, ny, nz, side, sp, a.S, bx, by, bz, a.iso, Xx, Xy, Xz, err, nsamples)) {
+                float4 c;
+                shade(sp, a.S, bx, by, bz, Xx, Xy, Xz, a.lx, a.ly, a.lz, a.color_scale, c, err);
+                nsamples += 6;
+                a.rgba[id] = c;
+                HitRecord h;
+                h.node = node;
+                h.brick = brick;
+                h.voxel[0] = floor_to_int(Xx);
+                h.voxel[1] = floor_to_int(Xy);
+                h.voxel[2] = floor_to_int(Xz);
+                float dx = Xx - r.ox, dy = Xy - r.oy, dz = Xz - r.oz;
+                h.depth = sqrtf(dx * dx + dy * dy + dz * dz);
+                a.hits[id] = h;
+                a.ray_phase[id] = PHASE_DONE;
+                done = true;
+                break;
+            }
+            node = next_node(r, a.oct, node, false);
+        }
+    }
+
+    // parked rays and first-time requests: one atomic per wave each
+    uint32_t pslot = wave_atomic_add(&a.ctr->parked_count, park);
+    if (park) a.out_list[pslot] = id;
+    uint32_t rslot = wave_atomic_add(&a.ctr->req_count, new_req);
+    if (new_req) a.req_list[rslot] = req_idx;
+    if (err) atomicOr(&a.ctr->err, err);
+    if (a.collect_stats) {
+        unsigned long long s = nsamples, nn = nnodes;
+        for (int off = 32; off > 0; off >>= 1) {
+            s += __shfl_xor(s, off);
+            nn += __shfl_xor(nn, off);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&a.ctr->samples, s);
+            atomicAdd(&a.ctr->nodes, nn);
+        }
+    }
+}
+
+hipError_t launch_render_pass(const RenderArgs& a, hipStream_t s)
+{
+    unsigned grid;
+    if (a.in_list) {
+        if (a.in_count == 0) return hipSuccess;
+        grid = (a.in_count + kBlock - 1) / kBlock;
+    } else {
+        int per_xcd = (a.tiles_total + 7) / 8;
+        grid = (unsigned)(per_xcd * 8);
+    }
+    hipLaunchKernelGGL(k_render_pass, dim3(grid), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+// --------------------------------------------------------- traversal trace
+// Emission sequence of the octree walk alone, as repeated calls of
+// Octree::getBoxIntersected would produce it (Octree.cu:883-901).
+__global__ __launch_bounds__(kBlock) void k_trace_nodes(TraceArgs a)
+{
+    uint32_t id = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t R = (uint32_t)a.cam.W * (uint32_t)a.cam.H;
+    if (id >= R) return;
+    int i = (int)(id / (uint32_t)a.cam.W), j = (int)(id % (uint32_t)a.cam.W);
+    Ray r;
+    make_ray(a.cam, i, j, r);
+    int count = 0;
+    if (root_ok(r, a.oct)) {
+        uint64_t node = next_node(r, a.oct, 1ull, true);
+        while (node != 0 && count < a.cap_total) {
+            if (count < a.max_nodes) a.nodes[(size_t)id * a.max_nodes + count] = node;
+            count++;
+            node = next_node(r, a.oct, node, false);
+        }
+    }
+    a.counts[id] = count;
+}
+
+hipError_t launch_trace_nodes(const TraceArgs& a, hipStream_t s)
+{
+    uint32_t R = (uint32_t)a.cam.W * (uint32_t)a.cam.H;
+    hipLaunchKernelGGL(k_trace_nodes, dim3((R + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ camera rays
+__global__ void k_camera_rays(Camera cam, float* dirs)
+{
+    uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t R = (uint32_t)cam.W * (uint32_t)cam.H;
+    if (id >= R) return;
+    Ray r;
+    make_ray(cam, (int)(id / (uint32_t)cam.W), (int)(id % (uint32_t)cam.W), r);
+    dirs[3 * (size_t)id + 0] = r.dx;
+    dirs[3 * (size_t)id + 1] = r.dy;
+    dirs[3 * (size_t)id + 2] = r.dz;
+}
+
+hipError_t launch_camera_rays(const Camera& cam, float* dirs, hipStream_t s)
+{
+    uint32_t R = (uint32_t)cam.W * (uint32_t)cam.H;
+    hipLaunchKernelGGL(k_camera_rays, dim3((R + 255) / 256), dim3(256), 0, s, cam, dirs);
+    return hipGetLastError();
+}
+
+// ----------------------------------------------------------- brick staging
+// One workgroup per brick: slot voxel (x,y,z) = global voxel
+// (bmin - inc + (x,y,z)), zero outside the dataset (FileManager.cu:88-112).
+__global__ __launch_bounds__(256) void k_bricks_from_volume(const float* __restrict__ vol,
+                                                            int X, int Y, int Z,
+                                                            const int64_t* __restrict__ codes,
+                                                            const int32_t* __restrict__ slots,
+                                                            int level_c, int n, int S, int inc,
+                                                            float* __restrict__ pool)
+{
+    int b = blockIdx.x;
+    int bx, by, bz;
+    node_min((uint64_t)codes[b], level_c, n, bx, by, bz);
+    bx -= inc; by -= inc; bz -= inc;
+    float* dst = pool + (size_t)slots[b] * ((size_t)S * S * S);
+    int total = S * S * S;
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+        int z = e % S, y = (e / S) % S, x = e / (S * S);
+        int gx = bx + x, gy = by + y, gz = bz + z;
+        float v = 0.0f;
+        if (gx >= 0 && gy >= 0 && gz >= 0 && gx < X && gy < Y && gz < Z)
+            v = vol[((size_t)gx * Y + gy) * Z + gz];
+        dst[e] = v;
+    }
+}
+
+hipError_t launch_bricks_from_volume(const float* vol, int X, int Y, int Z,
+                                     const int64_t* brick_codes, const int32_t* slots, int count,
+                                     int level_c, int n, int cube_dim, int inc,
+                                     float* pool, hipStream_t s)
+{
+    if (count <= 0) return hipSuccess;
+    int S = cube_dim + 2 * inc;
+    hipLaunchKernelGGL(k_bricks_from_volume, dim3(count), dim3(256), 0, s, vol, X, Y, Z,
+                       brick_codes, slots, level_c, n, S, inc, pool);
+    return hipGetLastError();
+}
+
+// Whole-slot copies from device-visible pinned host memory (one workgroup
+// per brick, 16-byte accesses; the slot size is a multiple of 4 floats when
+// S is even, otherwise the tail loop finishes it).
+__global__ __launch_bounds__(256) void k_copy_bricks(const float* const* __restrict__ src,
+                                                     const int32_t* __restrict__ dst_slots,
+                                                     size_t slot_elems, float* __restrict__ pool)
+{
+    int b = blockIdx.x;
+    const float* s = src[b];
+    float* d = pool + (size_t)dst_slots[b] * slot_elems;
+    size_t n4 = slot_elems / 4;
+    const float4* s4 = reinterpret_cast<const float4*>(s);
+    float4* d4 = reinterpret_cast<float4*>(d);
+    for (size_t e = threadIdx.x; e < n4; e += blockDim.x) d4[e] = s4[e];
+    for (size_t e = n4 * 4 + threadIdx.x; e < slot_elems; e += blockDim.x) d[e] = s[e];
+}
+
+hipError_t launch_copy_bricks(const float* const* src, const int32_t* dst_slots, int count,
+                              size_t slot_elems, float* pool, hipStream_t s)
+{
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copy_bricks, dim3(count), dim3(256), 0, s, src, dst_slots, slot_elems, pool);
+    return hipGetLastError();
+}
+
+__global__ void k_scatter_i32(int32_t* table, const int64_t* idx, const int32_t* val, int count)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) table[idx[i]] = val[i];
+}
+
+hipError_t launch_scatter_i32(int32_t* table, const int64_t* idx, const int32_t* val, int count,
+                              hipStream_t s)
+{
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter_i32, dim3((count + 255) / 256), dim3(256), 0, s, table, idx, val, count);
+    return hipGetLastError();
+}
+
+}  // namespace vt

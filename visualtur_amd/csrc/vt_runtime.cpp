@@ -1,0 +1,1031 @@
+// vt_runtime.cpp -- host runtime behind the C ABI (include/visualtur_amd.h).
+//
+// Owns the device state of one rendering context: the octree membership
+// tables, the brick pool with its page table and clock eviction, the pinned
+// host tier filled from HDF5 by a reader thread, the per-ray parking state,
+// and the pass loop of a frame.
+#include "visualtur_amd.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "vt_hdf5.hpp"
+#include "vt_internal.hpp"
+
+using namespace vt;
+
+static_assert(sizeof(vt_camera) == sizeof(vt::Camera), "camera layout");
+static_assert(sizeof(vt_hit) == sizeof(vt::HitRecord), "hit layout");
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return fail(VT_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+double now_ms()
+{
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t count)
+    {
+        if (count <= n && p) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T));
+        if (e == hipSuccess) n = count;
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+template <class T>
+struct PinBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t count)
+    {
+        if (count <= n && p) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipHostMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T), hipHostMallocDefault);
+        if (e == hipSuccess) n = count;
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+// ------------------------------------------------------------ host tier
+// Pinned, device-visible brick slots.  When it can hold every brick of the
+// cube, a reader thread fills it slab by slab (x-slabs of bricks, one HDF5
+// hyperslab read per slab); otherwise bricks are read on demand into
+// clock-evicted slots.
+struct HostTier {
+    int level_c = 0, n = 0, cube = 0, inc = 0, S = 0;
+    int64_t nbricks = 0, capacity = 0;
+    size_t slot_elems = 0;
+    float* pool = nullptr;        // pinned host memory
+    float* pool_dev = nullptr;    // the same memory as seen by kernels
+    std::string file, dataset;
+    H5Volume sync_reader;         // on-demand reads (service thread)
+    std::vector<int32_t> slot_of; // brick index -> host slot
+    std::vector<int64_t> brick_of;
+    std::vector<uint8_t> ref;
+    std::vector<uint64_t> pin;
+    int64_t hand = 0, used = 0;
+    // prefetch state
+    bool full_mode = false;
+    std::thread reader;
+    std::mutex m;
+    std::condition_variable cv;
+    int slabs_done = 0;           // x-slabs of bricks fully loaded
+    bool reader_failed = false;
+    std::string reader_err;
+    int64_t file_reads = 0;
+
+    ~HostTier()
+    {
+        if (reader.joinable()) reader.join();
+        if (pool) (void)hipHostFree(pool);
+    }
+
+    int64_t side_bricks() const { return 1ll << level_c; }
+
+    // linear x-major brick index of (bx,by,bz) brick coordinates
+    static uint64_t brick_code(int bx, int by, int bz, int level_c)
+    {
+        return (1ull << (3 * level_c)) | (dilate3((uint32_t)bx) << 2) | (dilate3((uint32_t)by) << 1) |
+               dilate3((uint32_t)bz);
+    }
+
+    void run_prefetch()
+    {
+        H5Volume vol;
+        std::string err;
+        if (!vol.open(file, dataset, 0, err)) {
+            std::lock_guard<std::mutex> g(m);
+            reader_failed = true;
+            reader_err = err;
+            cv.notify_all();
+            return;
+        }
+        const int64_t B = side_bricks();
+        const int64_t ext = B * cube;   // cube extent in voxels
+        const int64_t SY = ext + 2 * inc, SZ = ext + 2 * inc;
+        std::vector<float> slab((size_t)S * SY * SZ);
+        for (int64_t bx = 0; bx < B; ++bx) {
+            int s[3] = {(int)(bx * cube - inc), -inc, -inc};
+            int e[3] = {(int)(bx * cube - inc + S), (int)(ext + inc), (int)(ext + inc)};
+            if (!vol.read_box(s, e, slab.data(), err)) {
+                std::lock_guard<std::mutex> g(m);
+                reader_failed = true;
+                reader_err = err;
+                cv.notify_all();
+                return;
+            }
+            for (int64_t by = 0; by < B; ++by)
+                for (int64_t bz = 0; bz < B; ++bz) {
+                    uint64_t code = brick_code((int)bx, (int)by, (int)bz, level_c);
+                    int64_t bidx = (int64_t)(code - (1ull << (3 * level_c)));
+                    float* dst = pool + (size_t)slot_of[bidx] * slot_elems;
+                    for (int x = 0; x < S; ++x)
+                        for (int y = 0; y < S; ++y) {
+                            const float* src = slab.data() + ((size_t)x * SY + (size_t)(by * cube + y)) * SZ +
+                                               (size_t)(bz * cube);
+                            memcpy(dst + ((size_t)x * S + y) * S, src, (size_t)S * sizeof(float));
+                        }
+                }
+            {
+                std::lock_guard<std::mutex> g(m);
+                slabs_done = (int)bx + 1;
+                file_reads += B * B;
+            }
+            cv.notify_all();
+        }
+    }
+
+    int start(int n_levels)
+    {
+        n = n_levels;
+        if (capacity >= nbricks) {
+            full_mode = true;
+            for (int64_t b = 0; b < nbricks; ++b) { slot_of[b] = (int32_t)b; brick_of[b] = b; }
+            used = nbricks;
+            reader = std::thread([this] { run_prefetch(); });
+        }
+        return VT_OK;
+    }
+
+    // device-visible pointer of brick `bidx`, reading it if needed.
+    int get(int64_t bidx, uint64_t epoch, const float** out, int64_t* misses)
+    {
+        if (full_mode) {
+            int32_t code_bx;
+            {
+                int x, y, z;
+                node_min(bidx + (1ull << (3 * level_c)), level_c, level_c, x, y, z);
+                code_bx = x;
+            }
+            std::unique_lock<std::mutex> lk(m);
+            cv.wait(lk, [&] { return reader_failed || slabs_done > code_bx; });
+            if (reader_failed) return fail(VT_E_IO, "host tier reader: " + reader_err);
+            *out = pool_dev + (size_t)slot_of[bidx] * slot_elems;
+            return VT_OK;
+        }
+        int32_t s = slot_of[bidx];
+        if (s >= 0) {
+            ref[s] = 1;
+            pin[s] = epoch;
+            *out = pool_dev + (size_t)s * slot_elems;
+            return VT_OK;
+        }
+        // choose a host slot: unused first, then clock skipping pinned
+        int64_t slot = -1;
+        if (used < capacity) {
+            slot = used++;
+        } else {
+            for (int64_t it = 0; it < 3 * capacity; ++it) {
+                int64_t c = hand;
+                hand = (hand + 1) % capacity;
+                if (pin[c] == epoch) continue;
+                if (ref[c]) { ref[c] = 0; continue; }
+                slot = c;
+                break;
+            }
+            if (slot < 0) return fail(VT_E_STATE, "host tier: every slot is pinned");
+            slot_of[brick_of[slot]] = -1;
+        }
+        if (!sync_reader.is_open()) {
+            std::string err;
+            if (!sync_reader.open(file, dataset, 64ull << 20, err)) return fail(VT_E_IO, err);
+        }
+        int x, y, z;
+        node_min(bidx + (1ull << (3 * level_c)), level_c, n, x, y, z);
+        int s3[3] = {x - inc, y - inc, z - inc};
+        int e3[3] = {x - inc + S, y - inc + S, z - inc + S};
+        std::string err;
+        if (!sync_reader.read_box(s3, e3, pool + (size_t)slot * slot_elems, err)) return fail(VT_E_IO, err);
+        brick_of[slot] = bidx;
+        slot_of[bidx] = (int32_t)slot;
+        ref[slot] = 1;
+        pin[slot] = epoch;
+        file_reads++;
+        (*misses)++;
+        *out = pool_dev + (size_t)slot * slot_elems;
+        return VT_OK;
+    }
+};
+
+// ----------------------------------------------------------- brick cache
+struct BrickCache {
+    int level_c = 0, cube = 0, inc = 0, S = 0;
+    size_t slot_elems = 0;
+    int64_t nbricks = 0, gpu_slots = 0, cpu_slots = 0;
+    float* d_pool = nullptr;
+    int32_t* d_table = nullptr;
+    uint8_t* d_ref = nullptr;
+    std::vector<int32_t> table;
+    std::vector<int64_t> slot_brick;
+    std::vector<uint8_t> ref, ref_tmp;
+    std::vector<uint64_t> pin;
+    uint64_t epoch = 0;
+    int64_t hand = 0, used = 0;
+    // sources
+    const float* d_vol = nullptr;
+    int vol_dims[3] = {0, 0, 0};
+    std::unique_ptr<HostTier> host;
+    // staging
+    PinBuf<const float*> h_src;
+    PinBuf<int32_t> h_slots;
+    PinBuf<int64_t> h_codes;
+    PinBuf<int64_t> h_tidx;
+    PinBuf<int32_t> h_tval;
+    DevBuf<const float*> d_src;
+    DevBuf<int32_t> d_slots;
+    DevBuf<int64_t> d_codes;
+    DevBuf<int64_t> d_tidx;
+    DevBuf<int32_t> d_tval;
+    // stats
+    int64_t stat_missed = 0, stat_uploads = 0, stat_evictions = 0;
+
+    ~BrickCache()
+    {
+        host.reset();
+        if (d_pool) (void)hipFree(d_pool);
+        if (d_table) (void)hipFree(d_table);
+        if (d_ref) (void)hipFree(d_ref);
+        h_src.release(); h_slots.release(); h_codes.release(); h_tidx.release(); h_tval.release();
+        d_src.release(); d_slots.release(); d_codes.release(); d_tidx.release(); d_tval.release();
+    }
+
+    int64_t pick_victim()
+    {
+        if (used < gpu_slots) return used++;
+        for (int64_t it = 0; it < 3 * gpu_slots; ++it) {
+            int64_t c = hand;
+            hand = (hand + 1) % gpu_slots;
+            if (pin[c] == epoch) continue;
+            if (ref[c]) { ref[c] = 0; continue; }
+            return c;
+        }
+        return -1;
+    }
+};
+
+}  // namespace
+
+struct vt_file {
+    H5Volume vol;
+};
+
+struct vt_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // octree
+    bool octree_ready = false;
+    int n = 0, max_level = 0;
+    uint8_t* d_child = nullptr;
+    uint64_t level_off[24] = {0};
+    int root_valid = 0;
+    // cache
+    std::unique_ptr<BrickCache> cache;
+    // ray state
+    DevBuf<uint64_t> ray_node;
+    DevBuf<uint8_t> ray_phase;
+    DevBuf<uint32_t> list[2];
+    DevBuf<HitRecord> hits_scratch;
+    DevBuf<uint32_t> req_bits, req_list;
+    DevBuf<uint8_t> trace;
+    FrameCounters* d_ctr = nullptr;
+    FrameCounters* h_ctr = nullptr;
+    PinBuf<uint32_t> h_req;
+    int collect_stats = 0, record_trace = 0;
+    int64_t trace_n = 0;
+
+    ~vt_context()
+    {
+        cache.reset();
+        if (d_child) (void)hipFree(d_child);
+        ray_node.release(); ray_phase.release(); list[0].release(); list[1].release();
+        hits_scratch.release(); req_bits.release(); req_list.release(); trace.release();
+        if (d_ctr) (void)hipFree(d_ctr);
+        if (h_ctr) (void)hipHostFree(h_ctr);
+        h_req.release();
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int set_device(vt_context* ctx)
+{
+    HIP_TRY(hipSetDevice(ctx->device));
+    return VT_OK;
+}
+
+OctreeView make_view(const vt_context* ctx, int final_level)
+{
+    OctreeView v;
+    v.child_bytes = ctx->d_child;
+    for (int l = 0; l < 24; ++l) v.level_off[l] = ctx->level_off[l];
+    v.n = ctx->n;
+    v.final_level = final_level;
+    v.root_valid = ctx->root_valid;
+    return v;
+}
+
+Camera to_cam(const vt_camera* c)
+{
+    Camera k;
+    memcpy(&k, c, sizeof k);
+    return k;
+}
+
+int check_camera(const vt_camera* cam)
+{
+    if (!cam) return fail(VT_E_ARG, "camera is NULL");
+    if (cam->W <= 0 || cam->H <= 0) return fail(VT_E_ARG, "screen size must be positive");
+    if ((int64_t)cam->W * cam->H > (1ll << 30)) return fail(VT_E_ARG, "screen too large");
+    return VT_OK;
+}
+
+// Load `count` requested bricks (indices relative to 8^levelC) into the
+// device pool as far as capacity allows.  Bricks loaded here are pinned
+// against eviction until the next call.
+int service_requests(vt_context* ctx, const uint32_t* reqs, int64_t count, vt_frame_stats* st)
+{
+    BrickCache& c = *ctx->cache;
+    // fold the device reference bits into the clock
+    if (c.used > 0) {
+        HIP_TRY(hipMemcpyAsync(c.ref_tmp.data(), c.d_ref, (size_t)c.gpu_slots, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        for (int64_t s = 0; s < c.gpu_slots; ++s) c.ref[s] |= c.ref_tmp[s];
+        HIP_TRY(hipMemsetAsync(c.d_ref, 0, (size_t)c.gpu_slots, ctx->stream));
+    }
+    std::vector<uint32_t> sorted(reqs, reqs + count);
+    std::sort(sorted.begin(), sorted.end());
+    int64_t n_load = std::min<int64_t>((int64_t)sorted.size(), c.gpu_slots);
+    if (c.host && !c.host->full_mode) n_load = std::min<int64_t>(n_load, c.host->capacity);
+    c.epoch++;
+    uint64_t host_epoch = c.epoch;
+    if (c.h_src.ensure((size_t)n_load) != hipSuccess || c.h_slots.ensure((size_t)n_load) != hipSuccess ||
+        c.h_codes.ensure((size_t)n_load) != hipSuccess || c.h_tidx.ensure((size_t)(2 * n_load)) != hipSuccess ||
+        c.h_tval.ensure((size_t)(2 * n_load)) != hipSuccess)
+        return fail(VT_E_NOMEM, "pinned staging allocation failed");
+    int64_t nt = 0;
+    int64_t misses = 0;
+    for (int64_t q = 0; q < n_load; ++q) {
+        int64_t b = sorted[(size_t)q];
+        int64_t slot = c.pick_victim();
+        if (slot < 0) return fail(VT_E_STATE, "brick cache: no evictable slot");
+        int64_t old = c.slot_brick[(size_t)slot];
+        if (old >= 0) {
+            c.table[(size_t)old] = -1;
+            c.h_tidx.p[nt] = old; c.h_tval.p[nt] = -1; nt++;
+            c.stat_evictions++;
+        }
+        c.slot_brick[(size_t)slot] = b;
+        c.table[(size_t)b] = (int32_t)slot;
+        c.h_tidx.p[nt] = b; c.h_tval.p[nt] = (int32_t)slot; nt++;
+        c.pin[(size_t)slot] = c.epoch;
+        c.ref[(size_t)slot] = 1;
+        c.h_slots.p[q] = (int32_t)slot;
+        c.h_codes.p[q] = b + (int64_t)(1ull << (3 * c.level_c));
+        if (c.host) {
+            const float* src = nullptr;
+            int rc = c.host->get(b, host_epoch, &src, &misses);
+            if (rc != VT_OK) return rc;
+            c.h_src.p[q] = src;
+        }
+    }
+    if (n_load > 0) {
+        if (c.d_slots.ensure((size_t)n_load) != hipSuccess || c.d_tidx.ensure((size_t)nt) != hipSuccess ||
+            c.d_tval.ensure((size_t)nt) != hipSuccess)
+            return fail(VT_E_NOMEM, "device staging allocation failed");
+        HIP_TRY(hipMemcpyAsync(c.d_slots.p, c.h_slots.p, (size_t)n_load * 4, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(c.d_tidx.p, c.h_tidx.p, (size_t)nt * 8, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(c.d_tval.p, c.h_tval.p, (size_t)nt * 4, hipMemcpyHostToDevice, ctx->stream));
+        if (c.host) {
+            if (c.d_src.ensure((size_t)n_load) != hipSuccess) return fail(VT_E_NOMEM, "staging");
+            HIP_TRY(hipMemcpyAsync(c.d_src.p, c.h_src.p, (size_t)n_load * sizeof(float*), hipMemcpyHostToDevice,
+                                   ctx->stream));
+            HIP_TRY(launch_copy_bricks(c.d_src.p, c.d_slots.p, (int)n_load, c.slot_elems, c.d_pool, ctx->stream));
+        } else {
+            if (c.d_codes.ensure((size_t)n_load) != hipSuccess) return fail(VT_E_NOMEM, "staging");
+            HIP_TRY(hipMemcpyAsync(c.d_codes.p, c.h_codes.p, (size_t)n_load * 8, hipMemcpyHostToDevice, ctx->stream));
+            HIP_TRY(launch_bricks_from_volume(c.d_vol, c.vol_dims[0], c.vol_dims[1], c.vol_dims[2], c.d_codes.p,
+                                              c.d_slots.p, (int)n_load, c.level_c, ctx->n, c.cube, c.inc, c.d_pool,
+                                              ctx->stream));
+        }
+        HIP_TRY(launch_scatter_i32(c.d_table, c.d_tidx.p, c.d_tval.p, (int)nt, ctx->stream));
+        // the staging buffers are reused by the next call: finish the copies
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    c.stat_uploads += n_load;
+    c.stat_missed += count;
+    if (st) {
+        st->bricks_requested += count;
+        st->bricks_uploaded += n_load;
+        st->host_tier_misses += misses;
+    }
+    return VT_OK;
+}
+
+bool runs_well_formed(const uint64_t* E, int size, int level, std::string& why)
+{
+    if (size < 0 || (size & 1)) { why = "odd run-list size"; return false; }
+    uint64_t lo = 1ull << (3 * level), hi = (2ull << (3 * level)) - 1;
+    for (int j = 0; j < size; j += 2) {
+        if (E[j] > E[j + 1]) { why = "run start after its end"; return false; }
+        if (E[j] < lo || E[j + 1] > hi) { why = "code outside its level"; return false; }
+        if (j >= 2 && E[j] <= E[j - 1]) { why = "runs not sorted and disjoint"; return false; }
+    }
+    return true;
+}
+
+}  // namespace
+
+// ===================================================================== ABI
+
+extern "C" {
+
+int vt_last_error(vt_context*, char* buf, size_t len)
+{
+    if (!buf || len == 0) return VT_E_ARG;
+    snprintf(buf, len, "%s", g_err.c_str());
+    return VT_OK;
+}
+
+int vt_context_create(int device, vt_context** out)
+{
+    if (!out) return fail(VT_E_ARG, "out is NULL");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(VT_E_ARG, "no such device");
+    std::unique_ptr<vt_context> ctx(new vt_context);
+    ctx->device = device;
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&ctx->ev0));
+    HIP_TRY(hipEventCreate(&ctx->ev1));
+    HIP_TRY(hipMalloc((void**)&ctx->d_ctr, sizeof(FrameCounters)));
+    HIP_TRY(hipHostMalloc((void**)&ctx->h_ctr, sizeof(FrameCounters), hipHostMallocDefault));
+    *out = ctx.release();
+    return VT_OK;
+}
+
+int vt_context_destroy(vt_context* ctx)
+{
+    if (!ctx) return VT_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    delete ctx;
+    return VT_OK;
+}
+
+int vt_synchronize(vt_context* ctx)
+{
+    if (!ctx) return fail(VT_E_ARG, "ctx is NULL");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return VT_OK;
+}
+
+int vt_malloc(vt_context* ctx, size_t bytes, void** out)
+{
+    if (!ctx || !out) return fail(VT_E_ARG, "NULL argument");
+    if (int rc = set_device(ctx)) return rc;
+    HIP_TRY(hipMalloc(out, std::max<size_t>(bytes, 1)));
+    return VT_OK;
+}
+
+int vt_free(vt_context* ctx, void* p)
+{
+    if (!ctx) return fail(VT_E_ARG, "ctx is NULL");
+    if (!p) return VT_OK;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipFree(p));
+    return VT_OK;
+}
+
+int vt_memcpy_h2d(vt_context* ctx, void* d_dst, const void* src, size_t bytes)
+{
+    if (!ctx || (bytes && (!d_dst || !src))) return fail(VT_E_ARG, "NULL argument");
+    HIP_TRY(hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return VT_OK;
+}
+
+int vt_memcpy_d2h(vt_context* ctx, void* dst, const void* d_src, size_t bytes)
+{
+    if (!ctx || (bytes && (!dst || !d_src))) return fail(VT_E_ARG, "NULL argument");
+    HIP_TRY(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return VT_OK;
+}
+
+int vt_memset(vt_context* ctx, void* d_dst, int value, size_t bytes)
+{
+    if (!ctx || (bytes && !d_dst)) return fail(VT_E_ARG, "NULL argument");
+    HIP_TRY(hipMemsetAsync(d_dst, value, bytes, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return VT_OK;
+}
+
+int vt_screen_size(float distance, float fovH, float fovW, float* h, float* w)
+{
+    if (!h || !w) return fail(VT_E_ARG, "NULL argument");
+    // Screen.cpp:18-19
+    *h = 2 * distance * tanf((float)(fovH * (M_PI / 180.0)));
+    *w = 2 * distance * tanf((float)(fovW * (M_PI / 180.0)));
+    return VT_OK;
+}
+
+int vt_octree_upload(vt_context* ctx, const uint64_t* runs, const int32_t* sizes, int32_t nLevels,
+                     int32_t maxLevel)
+{
+    if (!ctx || !runs || !sizes) return fail(VT_E_ARG, "NULL argument");
+    if (nLevels < 1 || nLevels > 20) return fail(VT_E_ARG, "nLevels must be in [1,20]");
+    if (maxLevel < 0 || maxLevel > nLevels || maxLevel > 9)
+        return fail(VT_E_ARG, "maxLevel must be in [0, min(nLevels, 9)]");
+    if (int rc = set_device(ctx)) return rc;
+    // validate and expand the run lists into one membership byte per parent
+    std::vector<uint64_t> off(24, 0);
+    uint64_t total = 0;
+    for (int l = 1; l <= maxLevel; ++l) {
+        off[l] = total;
+        total += 1ull << (3 * (l - 1));
+    }
+    std::vector<uint8_t> bytes((size_t)std::max<uint64_t>(total, 1), 0);
+    const uint64_t* E = runs;
+    int root = 0;
+    for (int l = 0; l <= maxLevel; ++l) {
+        std::string why;
+        if (!runs_well_formed(E, sizes[l], l, why))
+            return fail(VT_E_FORMAT, "octree level " + std::to_string(l) + ": " + why);
+        if (l == 0) {
+            root = sizes[0] >= 2 && E[0] <= 1 && 1 <= E[1];
+        } else {
+            uint64_t base = 1ull << (3 * l);
+            for (int j = 0; j < sizes[l]; j += 2)
+                for (uint64_t c = E[j]; c <= E[j + 1]; ++c) {
+                    uint64_t parent_idx = (c >> 3) - (base >> 3);
+                    bytes[(size_t)(off[l] + parent_idx)] |= (uint8_t)(1u << (c & 7));
+                }
+        }
+        E += sizes[l];
+    }
+    if (ctx->d_child) (void)hipFree(ctx->d_child);
+    ctx->d_child = nullptr;
+    HIP_TRY(hipMalloc((void**)&ctx->d_child, bytes.size()));
+    HIP_TRY(hipMemcpy(ctx->d_child, bytes.data(), bytes.size(), hipMemcpyHostToDevice));
+    for (int l = 0; l < 24; ++l) ctx->level_off[l] = off[l];
+    ctx->n = nLevels;
+    ctx->max_level = maxLevel;
+    ctx->root_valid = root;
+    ctx->octree_ready = true;
+    return VT_OK;
+}
+
+int vt_octree_load_file(vt_context* ctx, const char* path, int32_t maxLevel, float* iso, int32_t* nLevels,
+                        int32_t realDim[3])
+{
+    if (!ctx || !path) return fail(VT_E_ARG, "NULL argument");
+    FILE* f = fopen(path, "rb");
+    if (!f) return fail(VT_E_IO, std::string("cannot open octree file ") + path);
+    int32_t magic = 0, dim = 0, rd[3] = {0, 0, 0}, n = 0;
+    float is = 0.0f;
+    bool ok = fread(&magic, 4, 1, f) == 1;
+    if (!ok || magic != 919278872) {   // Octree.cu:781
+        fclose(f);
+        return fail(VT_E_FORMAT, "invalid octree file format (magic)");
+    }
+    ok = fread(&is, 4, 1, f) == 1 && fread(&dim, 4, 1, f) == 1 && fread(rd, 4, 3, f) == 3 &&
+         fread(&n, 4, 1, f) == 1;
+    if (!ok || n < 1 || n > 20 || maxLevel > n) {
+        fclose(f);
+        return fail(VT_E_FORMAT, "truncated or inconsistent octree header");
+    }
+    std::vector<std::vector<uint64_t>> lv((size_t)n + 1);
+    for (int l = n; l >= 0; --l) {
+        int32_t ne = 0;
+        if (fread(&ne, 4, 1, f) != 1 || ne < 0) { fclose(f); return fail(VT_E_FORMAT, "truncated octree level"); }
+        lv[(size_t)l].resize((size_t)ne);
+        if (ne && fread(lv[(size_t)l].data(), 8, (size_t)ne, f) != (size_t)ne) {
+            fclose(f);
+            return fail(VT_E_FORMAT, "truncated octree level");
+        }
+    }
+    fclose(f);
+    std::vector<uint64_t> flat;
+    std::vector<int32_t> sizes((size_t)maxLevel + 1);
+    for (int l = 0; l <= maxLevel; ++l) {
+        sizes[(size_t)l] = (int32_t)lv[(size_t)l].size();
+        flat.insert(flat.end(), lv[(size_t)l].begin(), lv[(size_t)l].end());
+    }
+    int rc = vt_octree_upload(ctx, flat.data(), sizes.data(), n, maxLevel);
+    if (rc != VT_OK) return rc;
+    if (iso) *iso = is;
+    if (nLevels) *nLevels = n;
+    if (realDim) { realDim[0] = rd[0]; realDim[1] = rd[1]; realDim[2] = rd[2]; }
+    return VT_OK;
+}
+
+int vt_camera_rays(vt_context* ctx, const vt_camera* cam, float* d_dirs)
+{
+    if (!ctx || !d_dirs) return fail(VT_E_ARG, "NULL argument");
+    if (int rc = check_camera(cam)) return rc;
+    if (int rc = set_device(ctx)) return rc;
+    HIP_TRY(launch_camera_rays(to_cam(cam), d_dirs, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return VT_OK;
+}
+
+int vt_cache_create(vt_context* ctx, int32_t levelC, int32_t cubeDim, int32_t cubeInc, int64_t gpuSlots,
+                    int64_t cpuSlots)
+{
+    if (!ctx) return fail(VT_E_ARG, "ctx is NULL");
+    if (levelC < 0 || levelC > 9) return fail(VT_E_ARG, "levelC must be in [0,9]");
+    if (cubeDim <= 0 || cubeInc < 0 || gpuSlots <= 0 || cpuSlots < 0)
+        return fail(VT_E_ARG, "bad cache dimensions");
+    int64_t S = cubeDim + 2 * (int64_t)cubeInc;
+    if (S > 1024) return fail(VT_E_ARG, "brick too large");
+    if (int rc = set_device(ctx)) return rc;
+    std::unique_ptr<BrickCache> c(new BrickCache);
+    c->level_c = levelC;
+    c->cube = cubeDim;
+    c->inc = cubeInc;
+    c->S = (int)S;
+    c->slot_elems = (size_t)(S * S * S);
+    c->nbricks = 1ll << (3 * levelC);
+    c->gpu_slots = std::min<int64_t>(gpuSlots, c->nbricks);
+    c->cpu_slots = cpuSlots;
+    size_t pool_bytes = (size_t)c->gpu_slots * c->slot_elems * sizeof(float);
+    if (hipMalloc((void**)&c->d_pool, pool_bytes) != hipSuccess)
+        return fail(VT_E_NOMEM, "brick pool allocation failed (" + std::to_string(pool_bytes) + " bytes)");
+    HIP_TRY(hipMalloc((void**)&c->d_table, (size_t)c->nbricks * 4));
+    HIP_TRY(hipMemset(c->d_table, 0xff, (size_t)c->nbricks * 4));
+    HIP_TRY(hipMalloc((void**)&c->d_ref, (size_t)c->gpu_slots));
+    HIP_TRY(hipMemset(c->d_ref, 0, (size_t)c->gpu_slots));
+    c->table.assign((size_t)c->nbricks, -1);
+    c->slot_brick.assign((size_t)c->gpu_slots, -1);
+    c->ref.assign((size_t)c->gpu_slots, 0);
+    c->ref_tmp.assign((size_t)c->gpu_slots, 0);
+    c->pin.assign((size_t)c->gpu_slots, 0);
+    ctx->cache = std::move(c);
+    // request structures sized to the brick count
+    if (ctx->req_bits.ensure((size_t)((ctx->cache->nbricks + 31) / 32)) != hipSuccess ||
+        ctx->req_list.ensure((size_t)ctx->cache->nbricks) != hipSuccess ||
+        ctx->h_req.ensure((size_t)ctx->cache->nbricks) != hipSuccess)
+        return fail(VT_E_NOMEM, "request buffers");
+    return VT_OK;
+}
+
+int vt_cache_destroy(vt_context* ctx)
+{
+    if (!ctx) return fail(VT_E_ARG, "ctx is NULL");
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->cache.reset();
+    return VT_OK;
+}
+
+int vt_cache_attach_device_volume(vt_context* ctx, const float* d_vol, int32_t X, int32_t Y, int32_t Z)
+{
+    if (!ctx || !d_vol) return fail(VT_E_ARG, "NULL argument");
+    if (!ctx->cache) return fail(VT_E_STATE, "no cache");
+    if (X <= 0 || Y <= 0 || Z <= 0) return fail(VT_E_ARG, "bad volume dims");
+    ctx->cache->host.reset();
+    ctx->cache->d_vol = d_vol;
+    ctx->cache->vol_dims[0] = X;
+    ctx->cache->vol_dims[1] = Y;
+    ctx->cache->vol_dims[2] = Z;
+    return VT_OK;
+}
+
+int vt_cache_attach_hdf5(vt_context* ctx, const char* file, const char* dataset)
+{
+    if (!ctx || !file || !dataset) return fail(VT_E_ARG, "NULL argument");
+    if (!ctx->cache) return fail(VT_E_STATE, "no cache");
+    BrickCache& c = *ctx->cache;
+    if (c.cpu_slots <= 0) return fail(VT_E_ARG, "an HDF5 source needs a host tier (cpuSlots > 0)");
+    {
+        H5Volume probe;
+        std::string err;
+        if (!probe.open(file, dataset, 0, err)) return fail(VT_E_IO, err);
+    }
+    std::unique_ptr<HostTier> h(new HostTier);
+    h->level_c = c.level_c;
+    h->cube = c.cube;
+    h->inc = c.inc;
+    h->S = c.S;
+    h->slot_elems = c.slot_elems;
+    h->nbricks = c.nbricks;
+    h->capacity = std::min<int64_t>(c.cpu_slots, c.nbricks);
+    h->file = file;
+    h->dataset = dataset;
+    size_t bytes = (size_t)h->capacity * c.slot_elems * sizeof(float);
+    if (hipHostMalloc((void**)&h->pool, bytes, hipHostMallocMapped) != hipSuccess)
+        return fail(VT_E_NOMEM, "pinned host tier allocation failed (" + std::to_string(bytes) + " bytes)");
+    HIP_TRY(hipHostGetDevicePointer((void**)&h->pool_dev, h->pool, 0));
+    h->slot_of.assign((size_t)c.nbricks, -1);
+    h->brick_of.assign((size_t)h->capacity, -1);
+    h->ref.assign((size_t)h->capacity, 0);
+    h->pin.assign((size_t)h->capacity, 0);
+    c.d_vol = nullptr;
+    c.host = std::move(h);
+    return VT_OK;
+}
+
+int vt_cache_prefetch_host(vt_context* ctx, int wait)
+{
+    if (!ctx || !ctx->cache || !ctx->cache->host) return fail(VT_E_STATE, "no HDF5-backed cache");
+    if (!ctx->octree_ready) return fail(VT_E_STATE, "upload the octree first (brick geometry needs nLevels)");
+    HostTier& h = *ctx->cache->host;
+    if (!h.reader.joinable() && !h.full_mode) h.start(ctx->n);
+    if (wait && h.full_mode) {
+        std::unique_lock<std::mutex> lk(h.m);
+        h.cv.wait(lk, [&] { return h.reader_failed || h.slabs_done >= (int)h.side_bricks(); });
+        if (h.reader_failed) return fail(VT_E_IO, "host tier reader: " + h.reader_err);
+    }
+    return VT_OK;
+}
+
+int vt_cache_load_bricks(vt_context* ctx, const uint64_t* codes, int64_t count)
+{
+    if (!ctx || (count && !codes)) return fail(VT_E_ARG, "NULL argument");
+    if (!ctx->cache) return fail(VT_E_STATE, "no cache");
+    if (!ctx->octree_ready) return fail(VT_E_STATE, "no octree");
+    BrickCache& c = *ctx->cache;
+    if (!c.d_vol && !c.host) return fail(VT_E_STATE, "no brick source attached");
+    if (int rc = set_device(ctx)) return rc;
+    std::vector<uint32_t> req;
+    uint64_t base = 1ull << (3 * c.level_c);
+    for (int64_t i = 0; i < count; ++i) {
+        if (codes[i] < base || codes[i] >= 2 * base) return fail(VT_E_ARG, "brick code outside levelC");
+        uint64_t b = codes[i] - base;
+        if (c.table[(size_t)b] < 0) req.push_back((uint32_t)b);
+    }
+    std::sort(req.begin(), req.end());
+    req.erase(std::unique(req.begin(), req.end()), req.end());
+    return service_requests(ctx, req.data(), (int64_t)req.size(), nullptr);
+}
+
+int vt_cache_clear(vt_context* ctx)
+{
+    if (!ctx || !ctx->cache) return fail(VT_E_STATE, "no cache");
+    BrickCache& c = *ctx->cache;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemset(c.d_table, 0xff, (size_t)c.nbricks * 4));
+    HIP_TRY(hipMemset(c.d_ref, 0, (size_t)c.gpu_slots));
+    std::fill(c.table.begin(), c.table.end(), -1);
+    std::fill(c.slot_brick.begin(), c.slot_brick.end(), -1);
+    std::fill(c.ref.begin(), c.ref.end(), 0);
+    c.used = 0;
+    c.hand = 0;
+    return VT_OK;
+}
+
+int vt_cache_get_stats(vt_context* ctx, vt_cache_stats* out)
+{
+    if (!ctx || !out) return fail(VT_E_ARG, "NULL argument");
+    if (!ctx->cache) return fail(VT_E_STATE, "no cache");
+    BrickCache& c = *ctx->cache;
+    memset(out, 0, sizeof *out);
+    out->gpu_slots = c.gpu_slots;
+    out->gpu_resident = c.used;
+    out->lookups_missed = c.stat_missed;
+    out->uploads = c.stat_uploads;
+    out->evictions = c.stat_evictions;
+    if (c.host) {
+        std::lock_guard<std::mutex> g(c.host->m);
+        out->cpu_slots = c.host->capacity;
+        out->cpu_resident = c.host->full_mode ? (int64_t)c.host->slabs_done * c.host->side_bricks() * c.host->side_bricks()
+                                              : c.host->used;
+        out->file_reads = c.host->file_reads;
+    }
+    return VT_OK;
+}
+
+int vt_set_frame_options(vt_context* ctx, int collect_stats, int record_trace)
+{
+    if (!ctx) return fail(VT_E_ARG, "ctx is NULL");
+    ctx->collect_stats = collect_stats ? 1 : 0;
+    ctx->record_trace = record_trace ? 1 : 0;
+    return VT_OK;
+}
+
+int vt_get_brick_trace(vt_context* ctx, uint8_t* out, int64_t n)
+{
+    if (!ctx || !out) return fail(VT_E_ARG, "NULL argument");
+    if (!ctx->record_trace || ctx->trace_n == 0) return fail(VT_E_STATE, "trace recording is off");
+    if (n != ctx->trace_n) return fail(VT_E_ARG, "n must be 8^levelC");
+    HIP_TRY(hipMemcpyAsync(out, ctx->trace.p, (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return VT_OK;
+}
+
+int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_params* p, float* d_rgba,
+                    vt_hit* d_hits, vt_frame_stats* stats)
+{
+    double t_start = now_ms();
+    if (!ctx || !p || !d_rgba) return fail(VT_E_ARG, "NULL argument");
+    if (int rc = check_camera(cam)) return rc;
+    if (!ctx->octree_ready) return fail(VT_E_STATE, "no octree uploaded");
+    if (!ctx->cache) return fail(VT_E_STATE, "no brick cache");
+    BrickCache& c = *ctx->cache;
+    if (!c.d_vol && !c.host) return fail(VT_E_STATE, "no brick source attached");
+    if (p->levelO < 0 || p->levelO > ctx->max_level) return fail(VT_E_ARG, "levelO beyond the uploaded octree levels");
+    if (p->levelC != c.level_c || p->levelC > p->levelO) return fail(VT_E_ARG, "levelC must match the cache and be <= levelO");
+    if ((1 << (ctx->n - c.level_c)) != c.cube) return fail(VT_E_ARG, "cubeDim must equal 2^(nLevels-levelC)");
+    if (int rc = set_device(ctx)) return rc;
+    if (c.host && !c.host->full_mode && !c.host->reader.joinable()) c.host->start(ctx->n);
+
+    const uint32_t R = (uint32_t)cam->W * (uint32_t)cam->H;
+    if (ctx->ray_node.ensure(R) != hipSuccess || ctx->ray_phase.ensure(R) != hipSuccess ||
+        ctx->list[0].ensure(R) != hipSuccess || ctx->list[1].ensure(R) != hipSuccess)
+        return fail(VT_E_NOMEM, "ray state allocation failed");
+    HitRecord* hits = reinterpret_cast<HitRecord*>(d_hits);
+    if (!hits) {
+        if (ctx->hits_scratch.ensure(R) != hipSuccess) return fail(VT_E_NOMEM, "hit buffer");
+        hits = ctx->hits_scratch.p;
+    }
+    if (ctx->record_trace) {
+        if (ctx->trace.ensure((size_t)c.nbricks) != hipSuccess) return fail(VT_E_NOMEM, "trace buffer");
+        ctx->trace_n = c.nbricks;
+        HIP_TRY(hipMemsetAsync(ctx->trace.p, 0, (size_t)c.nbricks, ctx->stream));
+    }
+    HIP_TRY(hipMemsetAsync(ctx->d_ctr, 0, sizeof(FrameCounters), ctx->stream));
+    HIP_TRY(hipMemsetAsync(ctx->req_bits.p, 0, ctx->req_bits.n * 4, ctx->stream));
+
+    RenderArgs a;
+    memset(&a, 0, sizeof a);
+    a.cam = to_cam(cam);
+    a.oct = make_view(ctx, p->levelO);
+    a.level_c = c.level_c;
+    a.shift = 3 * (p->levelO - p->levelC);
+    a.brick_base = 1ull << (3 * c.level_c);
+    a.brick_slot = c.d_table;
+    a.pool = c.d_pool;
+    a.S = c.S;
+    a.inc = c.inc;
+    a.slot_elems = c.slot_elems;
+    a.iso = p->iso;
+    a.lx = p->light[0]; a.ly = p->light[1]; a.lz = p->light[2];
+    a.color_scale = p->color_scale;
+    a.ray_node = ctx->ray_node.p;
+    a.ray_phase = ctx->ray_phase.p;
+    a.req_bits = ctx->req_bits.p;
+    a.req_list = ctx->req_list.p;
+    a.slot_ref = c.d_ref;
+    a.brick_trace = ctx->record_trace ? ctx->trace.p : nullptr;
+    a.ctr = ctx->d_ctr;
+    a.rgba = reinterpret_cast<float4*>(d_rgba);
+    a.hits = hits;
+    a.tiles_x = (cam->W + 15) / 16;
+    a.tiles_total = a.tiles_x * ((cam->H + 15) / 16);
+    a.collect_stats = ctx->collect_stats;
+
+    vt_frame_stats st;
+    memset(&st, 0, sizeof st);
+    double ms_kernel = 0.0, ms_service = 0.0;
+    int cur = 0;
+    a.in_list = nullptr;
+    a.in_count = 0;
+    a.out_list = ctx->list[cur].p;
+    for (int pass = 0;; ++pass) {
+        HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+        HIP_TRY(launch_render_pass(a, ctx->stream));
+        HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(ctx->h_ctr, ctx->d_ctr, sizeof(FrameCounters), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ms_kernel += ms;
+        st.passes = pass + 1;
+        const uint32_t parked = ctx->h_ctr->parked_count;
+        const uint32_t nreq = ctx->h_ctr->req_count;
+        if (parked == 0) break;
+        st.rays_parked += parked;
+        double t0 = now_ms();
+        HIP_TRY(hipMemcpyAsync(ctx->h_req.p, ctx->req_list.p, (size_t)nreq * 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        int rc = service_requests(ctx, ctx->h_req.p, nreq, &st);
+        if (rc != VT_OK) return rc;
+        ms_service += now_ms() - t0;
+        HIP_TRY(hipMemsetAsync(ctx->req_bits.p, 0, ctx->req_bits.n * 4, ctx->stream));
+        HIP_TRY(hipMemsetAsync(ctx->d_ctr, 0, 8, ctx->stream));   // req_count, parked_count
+        a.in_list = ctx->list[cur].p;
+        a.in_count = parked;
+        cur ^= 1;
+        a.out_list = ctx->list[cur].p;
+        if (pass > 1000000) return fail(VT_E_STATE, "frame did not converge");
+    }
+    st.err = (int32_t)ctx->h_ctr->err;
+    st.samples = ctx->h_ctr->samples;
+    st.nodes = ctx->h_ctr->nodes;
+    st.ms_kernel = ms_kernel;
+    st.ms_service = ms_service;
+    st.ms_total = now_ms() - t_start;
+    if (stats) *stats = st;
+    if (st.err) return fail(VT_E_RANGE, "a sample fell outside its brick slot (halo too small for this geometry)");
+    return VT_OK;
+}
+
+int vt_trace_nodes(vt_context* ctx, const vt_camera* cam, int32_t finalLevel, int32_t maxNodes, int32_t capTotal,
+                   uint64_t* d_nodes, int32_t* d_counts)
+{
+    if (!ctx || !d_counts || (maxNodes > 0 && !d_nodes)) return fail(VT_E_ARG, "NULL argument");
+    if (int rc = check_camera(cam)) return rc;
+    if (!ctx->octree_ready) return fail(VT_E_STATE, "no octree uploaded");
+    if (finalLevel < 0 || finalLevel > ctx->max_level) return fail(VT_E_ARG, "finalLevel beyond uploaded levels");
+    if (maxNodes < 0 || capTotal < 0) return fail(VT_E_ARG, "negative sizes");
+    if (int rc = set_device(ctx)) return rc;
+    TraceArgs a;
+    a.cam = to_cam(cam);
+    a.oct = make_view(ctx, finalLevel);
+    a.max_nodes = maxNodes;
+    a.cap_total = capTotal;
+    a.nodes = d_nodes;
+    a.counts = d_counts;
+    HIP_TRY(launch_trace_nodes(a, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return VT_OK;
+}
+
+int vt_file_open(const char* file, const char* dataset, vt_file** out, int64_t dims[3])
+{
+    if (!file || !dataset || !out) return fail(VT_E_ARG, "NULL argument");
+    std::unique_ptr<vt_file> f(new vt_file);
+    std::string err;
+    if (!f->vol.open(file, dataset, 64ull << 20, err)) return fail(VT_E_IO, err);
+    if (dims)
+        for (int k = 0; k < 3; ++k) dims[k] = f->vol.dims()[k];
+    *out = f.release();
+    return VT_OK;
+}
+
+int vt_file_close(vt_file* f)
+{
+    delete f;
+    return VT_OK;
+}
+
+int vt_file_read_box(vt_file* f, const int32_t s[3], const int32_t e[3], float* out)
+{
+    if (!f || !s || !e || !out) return fail(VT_E_ARG, "NULL argument");
+    int ss[3] = {s[0], s[1], s[2]}, ee[3] = {e[0], e[1], e[2]};
+    std::string err;
+    if (!f->vol.read_box(ss, ee, out, err)) return fail(VT_E_IO, err);
+    return VT_OK;
+}
+
+int vt_file_create(const char* file, const char* dataset, const float* data, int64_t X, int64_t Y, int64_t Z,
+                   int32_t chunk)
+{
+    if (!file || !dataset || !data) return fail(VT_E_ARG, "NULL argument");
+    if (X <= 0 || Y <= 0 || Z <= 0 || chunk < 0) return fail(VT_E_ARG, "bad dimensions");
+    std::string err;
+    if (!h5_create_volume(file, dataset, data, X, Y, Z, chunk, err)) return fail(VT_E_IO, err);
+    return VT_OK;
+}
+
+}  // extern "C"
