@@ -1,0 +1,285 @@
+#!/usr/bin/env python3
+"""Benchmark of the isosurface ray-casting hot path on one MI355X.
+
+One step = one rendered frame (every primary ray of the frame traversed,
+marched and shaded, all passes and brick loads included).  Prints one JSON
+line: rays/sec over the timed frames, p50/p99 frame ms, the roofline of the
+render-pass kernel, a bounded CPU baseline of the reference algorithm, and a
+parity check of sampled rays against the CPU oracle.
+
+Workloads
+  c2  (default) 1024^3 value-noise fBm (6 octaves, lacunarity 2, gain 0.5),
+      iso 0, written to HDF5 in 32^3 chunks; 32^3 bricks with a 3-voxel halo
+      (levels 0..5, 32768 bricks); device brick cache capped at 25% (8192
+      slots), pinned host tier filled from the HDF5 file by the reader
+      thread before timing; 1920x1080, 45 deg fov, orbit at 2.2 extents
+      from the centre, 5.6 deg per frame.  Device misses are served over
+      PCIe from the pinned host tier inside the timed frames.
+  c1  256^3 Gaussian blobs (16, seeded), iso 0.5, 32^3 bricks (levels 0..3,
+      all 512 resident), 512x512, 45 deg fov, camera 2.5 extents from the
+      centre looking at it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import shutil
+import tempfile
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+import visualtur_amd as vt  # noqa: E402
+from visualtur_amd.volumes import C1_SEED, gaussian_blobs  # noqa: E402
+
+C2_SEED = 0x5EED0002
+METRIC = "rays/sec + p50/p99 frame ms on 1 MI355X, prefetch vs plain, identical hits"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def brick_bytes(cube: int, inc: int) -> int:
+    return (cube + 2 * inc) ** 3 * 4
+
+
+class Workload:
+    pass
+
+
+def setup_c1(ctx, args) -> Workload:
+    from oracle import vto  # the checker: builds the octree the reference way
+
+    w = Workload()
+    w.name = "c1"
+    w.desc = "256^3 gaussian blobs x16, iso 0.5, 32^3 bricks all resident, 512x512, 45deg, d=2.5 extents"
+    w.vol = gaussian_blobs(256, 16, C1_SEED)
+    w.iso, w.level_o, w.level_c, w.inc = 0.5, 3, 3, 3
+    t0 = time.perf_counter()
+    w.n, w.runs = vto.build_octree(w.vol, w.iso, literal=False)
+    log(f"octree built in {time.perf_counter() - t0:.2f}s (n={w.n})")
+    flat = np.concatenate(w.runs[: w.level_o + 1]).astype(np.uint64)
+    ctx.octree_upload(flat, [r.size for r in w.runs[: w.level_o + 1]], w.n, w.level_o)
+    w.d_vol = ctx.to_device(w.vol)
+    w.cube = 1 << (w.n - w.level_c)
+    w.slots = 1 << (3 * w.level_c)
+    ctx.cache_create(w.level_c, w.cube, w.inc, w.slots, 0)
+    ctx.cache_attach_device_volume(w.d_vol, w.vol.shape)
+    w.W, w.H = 512, 512
+    cam = vt.make_camera(w.W, w.H, (128.0, 128.0, 128.0 + 2.5 * 256.0), fov_v_deg=45.0)
+    w.cameras = lambda k: cam
+    w.prm = vt.render_params(w.level_o, w.level_c, w.iso)
+    w.oracle_volume = lambda: w.vol
+    w.oracle_runs = lambda: (w.n, w.runs)
+    w.extra = {"volume": "256^3", "bricks": 512, "cache_slots": 512, "frame": "512x512"}
+    return w
+
+
+def orbit(vt_mod, W, H, center, radius, deg, elev_deg=0.0, fov=45.0):
+    t, e = math.radians(deg), math.radians(elev_deg)
+    eye = np.array(center, np.float64) + radius * np.array([math.sin(t) * math.cos(e), math.sin(e),
+                                                            math.cos(t) * math.cos(e)])
+    look, up, right = vt_mod.look_at(eye, center)
+    return vt_mod.make_camera(W, H, eye, look, up, right, fov_v_deg=fov)
+
+
+def setup_c2(ctx, args) -> Workload:
+    w = Workload()
+    w.name = "c2"
+    w.desc = ("1024^3 value-noise fBm (6 oct, lac 2, gain 0.5), iso 0, HDF5 32^3 chunks, 32^3 bricks + 3-voxel "
+              "halo, device cache 25% (8192 of 32768 bricks), pinned host tier, 1920x1080, 45deg, orbit r=2.2 "
+              "extents, 5.6 deg/frame")
+    dims = (1024, 1024, 1024)
+    w.iso, w.level_o, w.level_c, w.inc = 0.0, 5, 5, 3
+    t0 = time.perf_counter()
+    d_vol = ctx.alloc(4 * 1024 ** 3)
+    ctx.volume_generate(vt.GEN_FBM, dims, C2_SEED, d_vol, octaves=6, base_freq=1.0 / 64.0, gain=0.5,
+                        lacunarity=2.0, scale=1.0)
+    log(f"volume generated in {time.perf_counter() - t0:.2f}s")
+    t0 = time.perf_counter()
+    w.n, sizes = ctx.octree_build(d_vol, dims, w.iso, False, w.level_o)
+    w.runs = [ctx.octree_runs(l, sizes[l]) for l in range(w.level_o + 1)]
+    log(f"octree built on the GPU in {time.perf_counter() - t0:.2f}s (n={w.n}, leaf runs={sizes[-1]})")
+    w.vol = d_vol.download(np.float32, dims) if not args.no_cpu else None
+    w.tmpdir = tempfile.mkdtemp(prefix="vt_c2_")
+    h5 = os.path.join(w.tmpdir, "c2.h5")
+    t0 = time.perf_counter()
+    ctx.volume_write_hdf5(d_vol, dims, h5, "value", 32)
+    log(f"hdf5 written in {time.perf_counter() - t0:.2f}s")
+    d_vol.free()
+    w.cube = 1 << (w.n - w.level_c)
+    nbricks = 1 << (3 * w.level_c)
+    w.slots = nbricks // 4
+    ctx.cache_create(w.level_c, w.cube, w.inc, w.slots, nbricks)
+    ctx.cache_attach_hdf5(h5, "value")
+    t0 = time.perf_counter()
+    ctx.cache_prefetch_host(wait=True)
+    log(f"host tier filled from hdf5 in {time.perf_counter() - t0:.2f}s")
+    w.W, w.H = 1920, 1080
+    w.cameras = lambda k: orbit(vt, w.W, w.H, (512.0, 512.0, 512.0), 2.2 * 1024.0, 5.6 * k)
+    w.prm = vt.render_params(w.level_o, w.level_c, w.iso)
+    w.oracle_volume = lambda: w.vol
+    w.oracle_runs = lambda: (w.n, w.runs)
+    w.extra = {"volume": "1024^3", "bricks": nbricks, "cache_slots": w.slots, "host_tier_slots": nbricks,
+               "frame": "1920x1080", "frames": "orbit"}
+    return w
+
+
+def cpu_baseline(w, cams, seconds: float):
+    """Reference algorithm (the C oracle) on this host's cores, over whole
+    frames of the same workload until `seconds` of wall time have elapsed."""
+    from oracle import vto
+
+    threads = len(os.sched_getaffinity(0))
+    V = vto.Volume(w.oracle_volume())
+    n, runs = w.oracle_runs()
+    O = vto.Octree(runs, n, w.level_o)
+    prm = vto.params(w.level_o, w.level_c, w.iso)
+    rays = 0
+    frames = 0
+    t0 = time.perf_counter()
+    while True:
+        cam = cams[frames % len(cams)]
+        vto.render(V, O, cam, prm, nthreads=threads)
+        rays += cam.W * cam.H
+        frames += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": rays / dt, "unit": "rays/s", "cores": threads, "kind": "port",
+            "sample": f"{frames} full {cams[0].W}x{cams[0].H} frames of the timed camera path, "
+                      f"{dt:.1f}s wall, OpenMP over 64-ray chunks"}, (V, O, prm)
+
+
+def parity_sample(w, ctx, cam, oracle_objs, nrays: int, seed: int):
+    """Render `cam` on the GPU and compare `nrays` random rays with the oracle."""
+    from oracle import vto
+
+    V, O, prm = oracle_objs
+    R = cam.W * cam.H
+    d_rgba, d_hits = ctx.alloc(R * 16), ctx.alloc(R * 32)
+    ctx.render_frame(cam, w.prm, d_rgba, d_hits)
+    rgba = d_rgba.download(np.float32, (R, 4))
+    hits = d_hits.download(vt.HIT_DTYPE, (R,))
+    d_rgba.free()
+    d_hits.free()
+    # three quarters of the sample among rays the GPU reports as hits (the
+    # rest uniform), so that marched rays dominate the check
+    rng = np.random.default_rng(seed)
+    hit_ids = np.nonzero(hits["node"] != 0)[0]
+    k_hit = min(hit_ids.size, (3 * nrays) // 4)
+    ids = np.unique(np.concatenate([rng.choice(hit_ids, size=k_hit, replace=False) if k_hit else
+                                    np.zeros(0, np.int64), rng.choice(R, size=min(nrays - k_hit, R), replace=False)]))
+    rc, hc = vto.render_list(V, O, cam, prm, ids)
+    hg, rg = hits[ids], rgba[ids]
+    exact = (hg["node"] == hc["node"]) & (hg["brick"] == hc["brick"]) & np.all(hg["voxel"] == hc["voxel"], axis=1)
+    dg, dc = hg["depth"].astype(np.float64), hc["depth"].astype(np.float64)
+    depth_ok = (np.isnan(dg) & np.isnan(dc)) | (np.abs(dg - dc) <= 1e-4)
+    rgba_ok = np.all((np.isnan(rg) & np.isnan(rc)) | (np.abs(rg.astype(np.float64) - rc) <= 1.0 / 255), axis=1)
+    bad = int((~(exact & depth_ok & rgba_ok)).sum())
+    return {"rays_checked": int(ids.size), "mismatches": bad,
+            "hit_fraction": float((hc["node"] != 0).mean())}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--parity-rays", type=int, default=4096)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one MI355X: the multi-GPU (image-space sharded) form is not built; "
+              "run with --gpus 1", file=sys.stderr)
+        return 2
+
+    ctx = vt.Context(0)
+    w = setup_c2(ctx, args) if args.config == "c2" else setup_c1(ctx, args)
+    cams = [w.cameras(k) for k in range(args.warmup + args.steps)]
+    R = w.W * w.H
+    d_rgba, d_hits = ctx.alloc(R * 16), ctx.alloc(R * 32)
+
+    for k in range(args.warmup):
+        ctx.render_frame(cams[k], w.prm, d_rgba, d_hits)
+    ctx.synchronize()
+
+    frame_ms, kernel_ms, passes, uploads = [], [], [], []
+    t0 = time.perf_counter()
+    for k in range(args.warmup, args.warmup + args.steps):
+        f0 = time.perf_counter()
+        st = ctx.render_frame(cams[k], w.prm, d_rgba, d_hits)
+        frame_ms.append((time.perf_counter() - f0) * 1e3)
+        kernel_ms.append(st.ms_kernel)
+        passes.append(st.passes)
+        uploads.append(st.bricks_uploaded)
+    ctx.synchronize()
+    total_s = time.perf_counter() - t0
+    rays_per_s = R * args.steps / total_s
+
+    # algorithmic bytes per frame: distinct bricks marched (trace replay,
+    # untimed) x brick payload + 48 B per ray + octree levels 0..levelO
+    ctx.set_frame_options(collect_stats=True, record_trace=True)
+    U, samples = [], []
+    for k in range(args.warmup, args.warmup + args.steps):
+        st = ctx.render_frame(cams[k], w.prm, d_rgba, d_hits)
+        U.append(int(ctx.brick_trace(w.level_c).sum()))
+        samples.append(int(st.samples))
+    ctx.set_frame_options(False, False)
+    octree_bytes = 8 * sum(r.size for r in w.runs[: w.level_o + 1])
+    Sb = brick_bytes(w.cube, w.inc)
+    B = [u * Sb + R * 48 + octree_bytes for u in U]
+    launches = sum(passes)
+    achieved_gbs = sum(B) / (sum(kernel_ms) * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved_gbs / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": "k_render_pass", "avg_launch_ms": round(sum(kernel_ms) / launches, 4),
+                "bytes_per_launch": int(sum(B) / launches),
+                "bricks_marched_per_frame": int(np.mean(U)), "brick_bytes": Sb,
+                "samples_per_frame": int(np.mean(samples))}
+
+    ms = np.array(frame_ms)
+    out = {
+        "metric": METRIC, "value": round(rays_per_s, 1), "unit": "rays/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(float(ms.mean()), 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": w.desc, "form": "plain", **w.extra},
+        "p50_ms": round(float(np.percentile(ms, 50)), 4), "p99_ms": round(float(np.percentile(ms, 99)), 4),
+        "kernel_ms_per_frame": round(float(np.mean(kernel_ms)), 4),
+        "passes_per_frame": float(np.mean(passes)), "bricks_uploaded_per_frame": float(np.mean(uploads)),
+        "bricks_marched_per_frame": float(np.mean(U)),
+        "device_miss_rate": round(float(sum(uploads)) / max(1, sum(U)), 5),
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        log("cpu baseline")
+        cb, objs = cpu_baseline(w, cams[args.warmup: args.warmup + args.steps], args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        log("parity sample")
+        out["parity"] = parity_sample(w, ctx, cams[-1], objs, args.parity_rays, seed=1)
+    else:
+        out["cpu_baseline"] = None
+    d_rgba.free()
+    d_hits.free()
+    ctx.close()
+    if getattr(w, "tmpdir", None):
+        shutil.rmtree(w.tmpdir, ignore_errors=True)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -98,7 +98,8 @@ typedef struct {
 
 /* ------------------------------------------------------------ context */
 
-/* Replaces the implicit CUDA context + cudaSetDevice (testVisualTur.cu:18-23). */
+/* Replaces the implicit CUDA context + cudaSetDevice (testVisualTur.cu:18-23);
+ * device -1 = the calling thread's current device. */
 int vt_context_create(int device, vt_context **out);
 int vt_context_destroy(vt_context *ctx);
 /* Message of the last failure on this thread (ctx may be NULL). */
@@ -184,6 +185,82 @@ int vt_get_brick_trace(vt_context *ctx, uint8_t *out, int64_t n);
  * capTotal) into d_counts[id]. */
 int vt_trace_nodes(vt_context *ctx, const vt_camera *cam, int32_t finalLevel,
                    int32_t maxNodes, int32_t capTotal, uint64_t *d_nodes, int32_t *d_counts);
+
+/* ------------------------------------------------------ staged protocol */
+
+/* visibleCube_t of the reference (config.hpp:13-19), 32 bytes. */
+typedef struct {
+    uint64_t id;           /* current node (levelO Morton code) */
+    float *data;           /* device pointer of the brick slot (set by vt_stage_cache) */
+    uint8_t state;         /* VT_NOCUBE / VT_NOCACHED / VT_CACHED / VT_PAINTED / VT_CUBE */
+    uint8_t pad_[7];
+    uint64_t cubeID;       /* brick code at levelC */
+} vt_visible_cube;
+enum { VT_NOCUBE = 0, VT_NOCACHED = 1, VT_CACHED = 2, VT_PAINTED = 4, VT_CUBE = 8 };
+
+/* Octree::resetState (Octree.cu:903-911): every ray restarts at the root. */
+int vt_stage_reset(vt_context *ctx, int64_t numRays);
+/* Octree::getBoxIntersected (Octree.cu:883-901) on device cubes: each
+ * NOCUBE ray advances to its next front-to-back node of level finalLevel
+ * (state CUBE) or is exhausted (id 0). */
+int vt_stage_traverse(vt_context *ctx, const vt_camera *cam, int32_t finalLevel, vt_visible_cube *d_cubes);
+/* lruCache::updateCache (lruCache.cu:298-313) on host cubes: rays in CUBE or
+ * NOCACHED get their brick (CACHED, data = slot) or stay NOCACHED when the
+ * pool is full of bricks this same pass uses.  levelO = octreeLevel,
+ * nLevels = the octree depth. */
+int vt_stage_cache(vt_context *ctx, vt_visible_cube *h_cubes, int64_t num, int32_t levelO, int32_t nLevels);
+/* rayCaster::renderCube / cuda_rayCasterCube (rayCaster.cu:267-357, 385-394):
+ * every pixel marched through one brick d_cube of (cubeDim+2*cubeInc)^3
+ * floats whose interior starts at voxel minBox (used by testRayCaster). */
+int vt_render_cube(vt_context *ctx, const vt_camera *cam, float iso, const float light[3],
+                   const float *d_cube, const int32_t minBox[3], const int32_t cubeDim[3], int32_t cubeInc,
+                   float *d_rgba);
+/* rayCaster::render (rayCaster.cu:375-384): NOCUBE rays are painted white;
+ * CACHED rays march their node in cube->data and become PAINTED (hit) or
+ * NOCUBE (no crossing). */
+int vt_stage_raycast(vt_context *ctx, const vt_camera *cam, const vt_render_params *p, int32_t nLevels,
+                     int32_t cubeDim, int32_t cubeInc, vt_visible_cube *d_cubes, float *d_rgba);
+
+/* ------------------------------------------------------- octree builder */
+
+/* Replaces OctreeM::OctreeM + createTree (CreateOctree.cu:300-334,
+ * 425-573) for a volume already in HBM (C order [X][Y][Z], zero outside):
+ * builds every level 0..nLevels on the device, keeps their run lists in the
+ * context, and installs levels 0..maxLevel as the traversal's octree.
+ * literal=1 reproduces the reference loop that never visits the last node
+ * of an inner level (CreateOctree.cu:550).  sizes_out (may be NULL) gets
+ * nLevels+1 run-list sizes (u64 entries per level); needs 2 <= nLevels <= 12. */
+int vt_octree_build(vt_context *ctx, const float *d_vol, int32_t X, int32_t Y, int32_t Z, float iso,
+                    int32_t literal, int32_t maxLevel, int32_t *nLevels_out, int64_t *sizes_out);
+/* Copy the run list of one level of the last build to host memory. */
+int vt_octree_get_runs(vt_context *ctx, int32_t level, uint64_t *out, int64_t count);
+/* OctreeM::writeToFile (CreateOctree.cu:342-371) for the last build. */
+int vt_octree_write_file(vt_context *ctx, const char *path);
+
+/* ---------------------------------------------------- volume generators */
+
+/* Seeded synthetic volumes of the benchmark configurations (no reference
+ * counterpart: the reference reads user-supplied HDF5 files). */
+enum { VT_GEN_BLOBS = 0, VT_GEN_FBM = 1, VT_GEN_SDF = 2, VT_GEN_FOURIER = 3 };
+typedef struct {
+    int32_t kind;
+    uint32_t seed;
+    int32_t n;              /* normalization extent */
+    int32_t count;          /* number of blobs / shapes / modes */
+    const float *shapes;    /* host array: blobs 5 floats (cx cy cz sigma amp, unit cube);
+                               sdf 8 floats (type cx cy cz r1 r2 axis 0, voxels);
+                               fourier 5 floats (kx ky kz phase amp, rad/voxel) */
+    int32_t octaves;        /* fBm part (fbm and sdf kinds) */
+    float base_freq, gain, lacunarity, scale;
+} vt_gen_params;
+/* Generate x-slab [x0, x0+nx) of an X*Y*Z volume into d_out (nx*Y*Z floats). */
+int vt_volume_generate(vt_context *ctx, const vt_gen_params *p, int64_t X, int64_t Y, int64_t Z,
+                       int64_t x0, int64_t nx, float *d_out);
+/* Write a device volume to an HDF5 dataset, x-slab by x-slab
+ * (FileManager::CreateFile semantics, FileManager.cu:169-243); chunk 0 =
+ * contiguous. */
+int vt_volume_write_hdf5(vt_context *ctx, const float *d_vol, int64_t X, int64_t Y, int64_t Z,
+                         const char *file, const char *dataset, int32_t chunk);
 
 /* ---------------------------------------------------------------- files */
 

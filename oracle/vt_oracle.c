@@ -617,6 +617,63 @@ int vto_render_list(const vto_volume *vol, const vto_octree *oct,
     return 0;
 }
 
+/* rayCaster.cu:267-357 (cuda_rayCasterCube) for every pixel: one brick with
+ * interior [minBox, minBox+dim)^3, sampled from the zero-padded volume
+ * (the brick is a box read of it).  The step counter is a float and the
+ * step count is (tfar-tnear)/0.5 without rounding up; colour uses dim. */
+int vto_render_cube(const vto_volume *vol, const vto_camera *cam, float iso, const float light[3],
+                    const int32_t minBox[3], int dim, float *rgba)
+{
+    vto_params p;
+    memset(&p, 0, sizeof p);
+    p.iso = iso;
+    p.light[0] = light[0]; p.light[1] = light[1]; p.light[2] = light[2];
+    p.color_scale = (float)dim;
+    int64_t R = (int64_t)cam->W * cam->H;
+    for (int64_t id = 0; id < R; ++id) {
+        float d[3], *px = rgba + 4 * id;
+        vto_ray_dir(cam, (int)(id / cam->W), (int)(id % cam->W), d);
+        const float *o = cam->pos;
+        int32_t mx[3] = {minBox[0] + dim, minBox[1] + dim, minBox[2] + dim};
+        float tnear, tfar;
+        int hit = 0;
+        float Xnew[3] = {0, 0, 0};
+        if (vto_slab(o, d, minBox, mx, &tnear, &tfar)) {
+            float X[3], Xfar[3], vStep[3];
+            for (int k = 0; k < 3; ++k) X[k] = o[k] + tnear * d[k];
+            for (int k = 0; k < 3; ++k) { Xfar[k] = X[k]; Xnew[k] = X[k]; }
+            for (int k = 0; k < 3; ++k) vStep[k] = 0.5f * d[k];
+            float maxStep = (float)((double)(tfar - tnear) / 0.5);
+            if (maxStep > (float)(4 * dim + 8)) maxStep = (float)(4 * dim + 8);
+            int first = 1;
+            float ant = 0.0f, sig = 0.0f, steps = 0.0f;
+            while (steps <= maxStep) {
+                if (first) {
+                    first = 0;
+                    ant = vto_sample(vol, X[0], X[1], X[2]);
+                    for (int k = 0; k < 3; ++k) Xfar[k] = X[k];
+                } else {
+                    sig = vto_sample(vol, X[0], X[1], X[2]);
+                    if (((iso - ant) < 0.0 && (iso - sig) < 0.0) || ((iso - ant) > 0.0 && (iso - sig) > 0.0)) {
+                        ant = sig;
+                        for (int k = 0; k < 3; ++k) Xfar[k] = X[k];
+                    } else {
+                        float a = (iso - ant) / (sig - ant);
+                        for (int k = 0; k < 3; ++k) Xnew[k] = Xfar[k] * (1.0f - a) + X[k] * a;
+                        hit = 1;
+                        steps = maxStep;
+                    }
+                }
+                for (int k = 0; k < 3; ++k) X[k] = X[k] + vStep[k];
+                steps++;
+            }
+        }
+        if (hit) shade(vol, Xnew, &p, px);
+        else px[0] = px[1] = px[2] = px[3] = 1.0f;
+    }
+    return 0;
+}
+
 /* =============================================================== builder */
 
 typedef struct {

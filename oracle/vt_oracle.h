@@ -129,6 +129,10 @@ int vto_render_list(const vto_volume *vol, const vto_octree *oct,
                     const int64_t *ids, int64_t count,
                     float *rgba, vto_hit *hits, int nthreads);
 
+/* rayCaster::renderCube (rayCaster.cu:267-357): one brick, every pixel. */
+int vto_render_cube(const vto_volume *vol, const vto_camera *cam, float iso, const float light[3],
+                    const int32_t minBox[3], int dim, float *rgba);
+
 /* --------------------------------------------------------------- builder */
 /* CreateOctree restatement.  literal=1 reproduces the inner-level loop that
  * skips the last node of each level (CreateOctree.cu:550); literal=0 visits

@@ -63,6 +63,7 @@ _SIGS = {
     "vto_build_octree": (ctypes.c_int, [ctypes.POINTER(OVolume), _F, ctypes.c_int, ctypes.POINTER(_P), _P,
                                         ctypes.c_int]),
     "vto_free": (None, [_P]),
+    "vto_render_cube": (ctypes.c_int, [ctypes.POINTER(OVolume), ctypes.POINTER(OCamera), _F, _P, _P, ctypes.c_int, _P]),
     "vto_octree_write": (ctypes.c_int, [ctypes.c_char_p, _F, _I32, _P, _I32, _P, _P]),
     "vto_octree_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_F), ctypes.POINTER(_I32), _P,
                                        ctypes.POINTER(_I32), ctypes.POINTER(_P), _P, ctypes.c_int]),
@@ -312,3 +313,13 @@ def render_list(vol: Volume, oct: Octree, cam, prm: OParams, ids, nthreads=0):
     if rc != 0:
         raise ValueError("oracle render: bad parameters")
     return rgba, hits
+
+
+def render_cube(vol: Volume, cam, iso, light, min_box, dim):
+    """rayCaster::renderCube restatement; returns rgba[R,4]."""
+    oc = camera_from(cam)
+    rgba = np.zeros((oc.W * oc.H, 4), np.float32)
+    li, mb = _f3(light), _i3(min_box)
+    lib().vto_render_cube(ctypes.byref(vol.c), ctypes.byref(oc), float(iso), li.ctypes.data, mb.ctypes.data,
+                          int(dim), rgba.ctypes.data)
+    return rgba

@@ -38,15 +38,5 @@ def ctx(vt):
 
 def blobs_volume(n: int, count: int, seed: int, dims=None) -> np.ndarray:
     """Sum of isotropic Gaussian blobs in normalized coordinates x=(i+0.5)/n."""
-    rng = np.random.default_rng(seed)
-    dims = dims or (n, n, n)
-    c = rng.uniform(0.0, 1.0, (count, 3))
-    s = rng.uniform(0.03, 0.12, count)
-    a = rng.uniform(0.5, 1.0, count)
-    ax = [(np.arange(d, dtype=np.float64) + 0.5) / n for d in dims]
-    X, Y, Z = np.meshgrid(*ax, indexing="ij")
-    v = np.zeros(dims, np.float64)
-    for k in range(count):
-        r2 = (X - c[k, 0]) ** 2 + (Y - c[k, 1]) ** 2 + (Z - c[k, 2]) ** 2
-        v += a[k] * np.exp(-r2 / (2 * s[k] ** 2))
-    return v.astype(np.float32)
+    from visualtur_amd.volumes import gaussian_blobs
+    return gaussian_blobs(n, count, seed, dims)

@@ -57,6 +57,15 @@ class CacheStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class GenParams(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("seed", ctypes.c_uint32), ("n", ctypes.c_int32),
+                ("count", ctypes.c_int32), ("shapes", ctypes.c_void_p), ("octaves", ctypes.c_int32),
+                ("base_freq", ctypes.c_float), ("gain", ctypes.c_float), ("lacunarity", ctypes.c_float),
+                ("scale", ctypes.c_float)]
+
+
+GEN_BLOBS, GEN_FBM, GEN_SDF, GEN_FOURIER = 0, 1, 2, 3
+
 HIT_DTYPE = np.dtype([("node", "<u8"), ("brick", "<u8"), ("voxel", "<i4", (3,)), ("depth", "<f4")])
 assert HIT_DTYPE.itemsize == 32
 
@@ -92,6 +101,14 @@ _SIGS = {
     "vt_get_brick_trace": (ctypes.c_int, [_P, _P, ctypes.c_int64]),
     "vt_trace_nodes": (ctypes.c_int, [_P, ctypes.POINTER(Camera), ctypes.c_int32, ctypes.c_int32,
                                       ctypes.c_int32, _P, _P]),
+    "vt_octree_build": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                                       ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), _P]),
+    "vt_octree_get_runs": (ctypes.c_int, [_P, ctypes.c_int32, _P, ctypes.c_int64]),
+    "vt_octree_write_file": (ctypes.c_int, [_P, ctypes.c_char_p]),
+    "vt_volume_generate": (ctypes.c_int, [_P, ctypes.POINTER(GenParams), ctypes.c_int64, ctypes.c_int64,
+                                          ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _P]),
+    "vt_volume_write_hdf5": (ctypes.c_int, [_P, _P, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_char_p,
+                                            ctypes.c_char_p, ctypes.c_int32]),
     "vt_file_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_P),
                                     ctypes.POINTER(ctypes.c_int64)]),
     "vt_file_close": (ctypes.c_int, [_P]),
@@ -241,6 +258,41 @@ class Context:
         self._c(lib().vt_octree_load_file(self.handle, str(path).encode(), max_level, ctypes.byref(iso),
                                           ctypes.byref(n), rd))
         return iso.value, n.value, tuple(rd)
+
+    def octree_build(self, d_vol: "DeviceBuffer", dims, iso: float, literal: bool, max_level: int):
+        """GPU octree builder; returns (nLevels, [run-list size per level])."""
+        n = ctypes.c_int32()
+        sizes = np.zeros(32, np.int64)
+        self._c(lib().vt_octree_build(self.handle, d_vol.ptr, *[int(d) for d in dims], iso, int(literal), max_level,
+                                      ctypes.byref(n), sizes.ctypes.data))
+        return n.value, [int(v) for v in sizes[: n.value + 1]]
+
+    def octree_runs(self, level: int, size: int) -> np.ndarray:
+        out = np.zeros(max(size, 1), np.uint64)
+        self._c(lib().vt_octree_get_runs(self.handle, level, out.ctypes.data, size))
+        return out[:size]
+
+    def octree_write_file(self, path) -> None:
+        self._c(lib().vt_octree_write_file(self.handle, str(path).encode()))
+
+    def volume_generate(self, kind: int, dims, seed: int, d_out: "DeviceBuffer", shapes=None, n: int = 0,
+                        octaves: int = 0, base_freq: float = 0.0, gain: float = 0.5, lacunarity: float = 2.0,
+                        scale: float = 1.0, x0: int = 0, nx: int | None = None) -> None:
+        X, Y, Z = (int(d) for d in dims)
+        p = GenParams()
+        p.kind, p.seed, p.n = kind, seed & 0xFFFFFFFF, n
+        sh = np.ascontiguousarray(shapes if shapes is not None else np.zeros(0), np.float32)
+        stride = 8 if kind == GEN_SDF else 5
+        p.count = sh.size // stride if sh.size else 0
+        p.shapes = sh.ctypes.data if sh.size else None
+        p.octaves, p.base_freq, p.gain, p.lacunarity, p.scale = octaves, base_freq, gain, lacunarity, scale
+        nx = X - x0 if nx is None else nx
+        self._c(lib().vt_volume_generate(self.handle, ctypes.byref(p), X, Y, Z, x0, nx, d_out.ptr))
+
+    def volume_write_hdf5(self, d_vol: "DeviceBuffer", dims, path, dataset: str = "value", chunk: int = 0) -> None:
+        X, Y, Z = (int(d) for d in dims)
+        self._c(lib().vt_volume_write_hdf5(self.handle, d_vol.ptr, X, Y, Z, str(path).encode(), dataset.encode(),
+                                           chunk))
 
     # ---- cache
     def cache_create(self, level_c: int, cube_dim: int, cube_inc: int, gpu_slots: int, cpu_slots: int = 0):

@@ -28,7 +28,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unus
 HIP = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 INCS = [f"-I{INCLUDE}", f"-I{CSRC}", f"-I{HDF5_INC}"]
 
-HIP_SOURCES = ["vt_kernels.hip", "vt_runtime.cpp"]
+HIP_SOURCES = ["vt_kernels.hip", "vt_build.hip", "vt_runtime.cpp"]
 HOST_SOURCES = ["vt_hdf5.cpp"]
 CLASS_SOURCES = ["vt_classes.cpp"]
 
@@ -80,6 +80,10 @@ def build(verbose: bool = False) -> Path:
     _run([HIPCC, "-shared", "-fPIC", "-o", str(cls), *[str(obj / (s + ".o")) for s in classes],
           f"-L{BUILD}", "-lvt_hip", "-Wl,-rpath,$ORIGIN", f"-L{deps}", "-lhdf5",
           "-Wl,-rpath,$ORIGIN/deps"], verbose)
+    # C++ drivers of the class surface used by the tests
+    for drv in sorted((ROOT / "tests" / "cpp").glob("*.cpp")):
+        _run([HIPCC, *COMMON, *INCS, str(drv), "-o", str(BUILD / drv.stem), f"-L{BUILD}", "-lvisualtur",
+              "-lvt_hip", "-Wl,-rpath,$ORIGIN", f"-L{deps}", "-lhdf5", "-Wl,-rpath,$ORIGIN/deps"], verbose)
     return lib
 
 

@@ -88,15 +88,15 @@ __device__ __forceinline__ void make_ray(const Camera& c, int i, int j, Ray& r)
 }
 
 // ------------------------------------------------------------------- slab
-// Octree.cu:258-330 (_cuda_RayAABB2) == rayCaster.cu:12-79
-__device__ __forceinline__ bool slab(const Ray& r, int mnx, int mny, int mnz, int side,
-                                     float& tnear, float& tfar)
+// Octree.cu:258-330 (_cuda_RayAABB2) == rayCaster.cu:12-79, box [mn, mx]
+__device__ __forceinline__ bool slab_box(const Ray& r, int mnx, int mny, int mnz, int mxx, int mxy, int mxz,
+                                         float& tnear, float& tfar)
 {
     bool hit = true;
     float tmin, tmax, tymin, tymax, tzmin, tzmax;
-    float fx0 = (float)mnx, fx1 = (float)(mnx + side);
-    float fy0 = (float)mny, fy1 = (float)(mny + side);
-    float fz0 = (float)mnz, fz1 = (float)(mnz + side);
+    float fx0 = (float)mnx, fx1 = (float)mxx;
+    float fy0 = (float)mny, fy1 = (float)mxy;
+    float fz0 = (float)mnz, fz1 = (float)mxz;
     if (r.ix >= 0) { tmin = (fx0 - r.ox) * r.ix; tmax = (fx1 - r.ox) * r.ix; }
     else           { tmin = (fx1 - r.ox) * r.ix; tmax = (fx0 - r.ox) * r.ix; }
     if (r.iy >= 0) { tymin = (fy0 - r.oy) * r.iy; tymax = (fy1 - r.oy) * r.iy; }
@@ -112,6 +112,11 @@ __device__ __forceinline__ bool slab(const Ray& r, int mnx, int mny, int mnz, in
     tnear = (tmin < 0.0f) ? 0.0f : tmin;
     tfar = tmax;
     return hit;
+}
+
+__device__ __forceinline__ bool slab(const Ray& r, int mnx, int mny, int mnz, int side, float& tnear, float& tfar)
+{
+    return slab_box(r, mnx, mny, mnz, mnx + side, mny + side, mnz + side, tnear, tfar);
 }
 
 // --------------------------------------------------------------- octree

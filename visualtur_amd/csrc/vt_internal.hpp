@@ -68,7 +68,49 @@ struct TraceArgs {
     int32_t* counts;             // [R]
 };
 
+// visibleCube_t of the reference (config.hpp:13-19), 32 bytes on x86-64
+struct VisibleCube {
+    uint64_t id;
+    float* data;
+    uint8_t state;
+    uint8_t pad[7];
+    uint64_t cubeID;
+};
+static_assert(sizeof(VisibleCube) == 32, "visibleCube_t layout");
+
+enum CubeState : uint8_t { ST_NOCUBE = 0, ST_NOCACHED = 1, ST_CACHED = 2, ST_PAINTED = 4, ST_CUBE = 8 };
+enum TravState : uint8_t { TRAV_FRESH = 0, TRAV_ACTIVE = 1, TRAV_EXHAUSTED = 2 };
+
+struct StageTraverseArgs {
+    Camera cam;
+    OctreeView oct;
+    VisibleCube* cubes;
+    uint8_t* trav;            // per-ray traversal state (the reference's stackActual)
+};
+
+struct StageRaycastArgs {
+    Camera cam;
+    int n, level_o, level_c, inc, S;
+    float iso, lx, ly, lz, color_scale;
+    VisibleCube* cubes;
+    float4* rgba;
+    uint32_t* err;
+};
+
+struct CubeArgs {
+    Camera cam;
+    float iso, lx, ly, lz;
+    const float* cube;        // (dim+2*inc)^3 floats, z fastest
+    int mnx, mny, mnz;        // interior min corner (voxels)
+    int dim, inc;
+    float4* rgba;
+    uint32_t* err;
+};
+
 // launchers (vt_kernels.hip)
+hipError_t launch_render_cube(const CubeArgs& a, hipStream_t s);
+hipError_t launch_stage_traverse(const StageTraverseArgs& a, hipStream_t s);
+hipError_t launch_stage_raycast(const StageRaycastArgs& a, hipStream_t s);
 hipError_t launch_render_pass(const RenderArgs& a, hipStream_t s);
 hipError_t launch_trace_nodes(const TraceArgs& a, hipStream_t s);
 // Assemble bricks with halo from a device volume [X][Y][Z] (zero outside).

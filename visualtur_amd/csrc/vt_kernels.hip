@@ -201,6 +201,150 @@ hipError_t launch_trace_nodes(const TraceArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
+// ------------------------------------------------------- staged protocol
+// The reference's three per-pass stages on visibleCube_t arrays, for callers
+// that drive Octree / lruCache / rayCaster themselves (visualTur.cu:175-220).
+
+// Octree.cu:620-739 (cuda_getFirtsVoxel): next node of each NOCUBE ray.
+// The resume point is the ray's current node code plus a 1-byte state in
+// place of the reference's 32-entry stack.
+__global__ __launch_bounds__(kBlock) void k_stage_traverse(StageTraverseArgs a)
+{
+    uint32_t id = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t R = (uint32_t)a.cam.W * (uint32_t)a.cam.H;
+    if (id >= R) return;
+    Ray r;
+    make_ray(a.cam, (int)(id / (uint32_t)a.cam.W), (int)(id % (uint32_t)a.cam.W), r);
+    VisibleCube* c = a.cubes + id;
+    if (!root_ok(r, a.oct)) {   // Octree.cu:729-733
+        c->id = 0;
+        return;
+    }
+    if (c->state != ST_NOCUBE) return;
+    uint8_t ts = a.trav[id];
+    if (ts == TRAV_EXHAUSTED) return;
+    uint64_t node = (ts == TRAV_FRESH) ? next_node(r, a.oct, 1ull, true) : next_node(r, a.oct, c->id, false);
+    if (node != 0) {
+        c->id = node;
+        c->state = ST_CUBE;
+        a.trav[id] = TRAV_ACTIVE;
+    } else {
+        c->id = 0;
+        a.trav[id] = TRAV_EXHAUSTED;
+    }
+}
+
+hipError_t launch_stage_traverse(const StageTraverseArgs& a, hipStream_t s)
+{
+    uint32_t R = (uint32_t)a.cam.W * (uint32_t)a.cam.H;
+    hipLaunchKernelGGL(k_stage_traverse, dim3((R + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+// rayCaster.cu:139-265 (cuda_rayCaster) on the staged state.
+__global__ __launch_bounds__(kBlock) void k_stage_raycast(StageRaycastArgs a)
+{
+    uint32_t id = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t R = (uint32_t)a.cam.W * (uint32_t)a.cam.H;
+    if (id >= R) return;
+    VisibleCube* c = a.cubes + id;
+    if (c->state == ST_NOCUBE) {
+        a.rgba[id] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+        c->state = ST_PAINTED;
+        return;
+    }
+    if (c->state != ST_CACHED) return;
+    Ray r;
+    make_ray(a.cam, (int)(id / (uint32_t)a.cam.W), (int)(id % (uint32_t)a.cam.W), r);
+    int nx, ny, nz, bx, by, bz;
+    node_min(c->id, a.level_o, a.n, nx, ny, nz);
+    int side = 1 << (a.n - a.level_o);
+    float tn, tf;
+    if (!slab(r, nx, ny, nz, side, tn, tf)) return;   // state stays CACHED, as in the reference
+    node_min(c->cubeID, a.level_c, a.n, bx, by, bz);
+    bx -= a.inc; by -= a.inc; bz -= a.inc;
+    uint32_t err = 0, ns = 0;
+    float Xx, Xy, Xz;
+    if (march_node(r, nx, ny, nz, side, c->data, a.S, bx, by, bz, a.iso, Xx, Xy, Xz, err, ns)) {
+        float4 col;
+        shade(c->data, a.S, bx, by, bz, Xx, Xy, Xz, a.lx, a.ly, a.lz, a.color_scale, col, err);
+        a.rgba[id] = col;
+        c->state = ST_PAINTED;
+    } else {
+        c->state = ST_NOCUBE;
+    }
+    if (err) atomicOr(a.err, err);
+}
+
+hipError_t launch_stage_raycast(const StageRaycastArgs& a, hipStream_t s)
+{
+    uint32_t R = (uint32_t)a.cam.W * (uint32_t)a.cam.H;
+    hipLaunchKernelGGL(k_stage_raycast, dim3((R + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+// rayCaster.cu:267-357 (cuda_rayCasterCube): one brick, every pixel.  Its
+// loop counter is a float and maxStep is not rounded up, and the colour
+// ramp divides by the brick extent instead of 256.
+__global__ __launch_bounds__(kBlock) void k_render_cube(CubeArgs a)
+{
+    uint32_t id = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t R = (uint32_t)a.cam.W * (uint32_t)a.cam.H;
+    if (id >= R) return;
+    Ray r;
+    make_ray(a.cam, (int)(id / (uint32_t)a.cam.W), (int)(id % (uint32_t)a.cam.W), r);
+    uint32_t err = 0;
+    bool hit = false;
+    float Xx = 0.0f, Xy = 0.0f, Xz = 0.0f;
+    const int S = a.dim + 2 * a.inc;
+    const int bx = a.mnx - a.inc, by = a.mny - a.inc, bz = a.mnz - a.inc;
+    float tnear, tfar;
+    if (slab_box(r, a.mnx, a.mny, a.mnz, a.mnx + a.dim, a.mny + a.dim, a.mnz + a.dim, tnear, tfar)) {
+        float X0 = r.ox + tnear * r.dx, X1 = r.oy + tnear * r.dy, X2 = r.oz + tnear * r.dz;
+        float F0 = X0, F1 = X1, F2 = X2;
+        float s0 = 0.5f * r.dx, s1 = 0.5f * r.dy, s2 = 0.5f * r.dz;
+        float maxStep = (tfar - tnear) * 2.0f;
+        const float cap = (float)(4 * a.dim + 8);
+        if (maxStep > cap) maxStep = cap;
+        bool first = true;
+        float ant = 0.0f;
+        for (float steps = 0.0f; steps <= maxStep; steps += 1.0f) {
+            float v = sample(a.cube, S, bx, by, bz, X0, X1, X2, err);
+            if (first) {
+                first = false;
+                ant = v;
+                F0 = X0; F1 = X1; F2 = X2;
+            } else if (((a.iso - ant) < 0.0f && (a.iso - v) < 0.0f) || ((a.iso - ant) > 0.0f && (a.iso - v) > 0.0f)) {
+                ant = v;
+                F0 = X0; F1 = X1; F2 = X2;
+            } else {
+                float t = (a.iso - ant) / (v - ant);
+                Xx = F0 * (1.0f - t) + X0 * t;
+                Xy = F1 * (1.0f - t) + X1 * t;
+                Xz = F2 * (1.0f - t) + X2 * t;
+                hit = true;
+                break;
+            }
+            X0 = X0 + s0; X1 = X1 + s1; X2 = X2 + s2;
+        }
+    }
+    if (hit) {
+        float4 col;
+        shade(a.cube, S, bx, by, bz, Xx, Xy, Xz, a.lx, a.ly, a.lz, (float)a.dim, col, err);
+        a.rgba[id] = col;
+    } else {
+        a.rgba[id] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    }
+    if (err) atomicOr(a.err, err);
+}
+
+hipError_t launch_render_cube(const CubeArgs& a, hipStream_t s)
+{
+    uint32_t R = (uint32_t)a.cam.W * (uint32_t)a.cam.H;
+    hipLaunchKernelGGL(k_render_cube, dim3((R + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------ camera rays
 __global__ void k_camera_rays(Camera cam, float* dirs)
 {

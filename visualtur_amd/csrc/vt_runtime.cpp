@@ -18,6 +18,7 @@
 #include <thread>
 #include <vector>
 
+#include "vt_build.hpp"
 #include "vt_hdf5.hpp"
 #include "vt_internal.hpp"
 
@@ -256,6 +257,7 @@ struct HostTier {
 // ----------------------------------------------------------- brick cache
 struct BrickCache {
     int level_c = 0, cube = 0, inc = 0, S = 0;
+    int n = 0;   // octree depth the brick codes refer to (set on first use)
     size_t slot_elems = 0;
     int64_t nbricks = 0, gpu_slots = 0, cpu_slots = 0;
     float* d_pool = nullptr;
@@ -339,13 +341,20 @@ struct vt_context {
     PinBuf<uint32_t> h_req;
     int collect_stats = 0, record_trace = 0;
     int64_t trace_n = 0;
+    DevBuf<uint8_t> stage_trav;
+    int64_t stage_rays = 0;
+    // last octree build (all levels, host copies)
+    std::vector<std::vector<uint64_t>> built_runs;
+    float built_iso = 0.0f;
+    int built_n = 0, built_dim = 0;
+    int built_real[3] = {0, 0, 0};
 
     ~vt_context()
     {
         cache.reset();
         if (d_child) (void)hipFree(d_child);
         ray_node.release(); ray_phase.release(); list[0].release(); list[1].release();
-        hits_scratch.release(); req_bits.release(); req_list.release(); trace.release();
+        hits_scratch.release(); req_bits.release(); req_list.release(); trace.release(); stage_trav.release();
         if (d_ctr) (void)hipFree(d_ctr);
         if (h_ctr) (void)hipHostFree(h_ctr);
         h_req.release();
@@ -389,33 +398,57 @@ int check_camera(const vt_camera* cam)
     return VT_OK;
 }
 
+// Fold the device reference bits (set by the render kernel on every brick
+// it marches) into the host clock.
+int fold_ref_bits(vt_context* ctx)
+{
+    BrickCache& c = *ctx->cache;
+    if (c.used == 0) return VT_OK;
+    HIP_TRY(hipMemcpyAsync(c.ref_tmp.data(), c.d_ref, (size_t)c.gpu_slots, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int64_t s = 0; s < c.gpu_slots; ++s) c.ref[(size_t)s] |= c.ref_tmp[(size_t)s];
+    HIP_TRY(hipMemsetAsync(c.d_ref, 0, (size_t)c.gpu_slots, ctx->stream));
+    return VT_OK;
+}
+
+int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t* misses, int32_t* slots_out);
+
 // Load `count` requested bricks (indices relative to 8^levelC) into the
 // device pool as far as capacity allows.  Bricks loaded here are pinned
 // against eviction until the next call.
 int service_requests(vt_context* ctx, const uint32_t* reqs, int64_t count, vt_frame_stats* st)
 {
     BrickCache& c = *ctx->cache;
-    // fold the device reference bits into the clock
-    if (c.used > 0) {
-        HIP_TRY(hipMemcpyAsync(c.ref_tmp.data(), c.d_ref, (size_t)c.gpu_slots, hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-        for (int64_t s = 0; s < c.gpu_slots; ++s) c.ref[s] |= c.ref_tmp[s];
-        HIP_TRY(hipMemsetAsync(c.d_ref, 0, (size_t)c.gpu_slots, ctx->stream));
-    }
+    if (int rc = fold_ref_bits(ctx)) return rc;
     std::vector<uint32_t> sorted(reqs, reqs + count);
     std::sort(sorted.begin(), sorted.end());
     int64_t n_load = std::min<int64_t>((int64_t)sorted.size(), c.gpu_slots);
     if (c.host && !c.host->full_mode) n_load = std::min<int64_t>(n_load, c.host->capacity);
     c.epoch++;
-    uint64_t host_epoch = c.epoch;
+    int64_t misses = 0;
+    if (int rc = load_bricks(ctx, sorted.data(), n_load, &misses, nullptr)) return rc;
+    c.stat_missed += count;
+    if (st) {
+        st->bricks_requested += count;
+        st->bricks_uploaded += n_load;
+        st->host_tier_misses += misses;
+    }
+    return VT_OK;
+}
+
+// Place `n_load` non-resident bricks into victim slots (skipping slots
+// pinned in the current epoch), then fill them from the brick source.
+int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t* misses, int32_t* slots_out)
+{
+    BrickCache& c = *ctx->cache;
+    if (n_load <= 0) return VT_OK;
     if (c.h_src.ensure((size_t)n_load) != hipSuccess || c.h_slots.ensure((size_t)n_load) != hipSuccess ||
         c.h_codes.ensure((size_t)n_load) != hipSuccess || c.h_tidx.ensure((size_t)(2 * n_load)) != hipSuccess ||
         c.h_tval.ensure((size_t)(2 * n_load)) != hipSuccess)
         return fail(VT_E_NOMEM, "pinned staging allocation failed");
     int64_t nt = 0;
-    int64_t misses = 0;
     for (int64_t q = 0; q < n_load; ++q) {
-        int64_t b = sorted[(size_t)q];
+        int64_t b = bricks[q];
         int64_t slot = c.pick_victim();
         if (slot < 0) return fail(VT_E_STATE, "brick cache: no evictable slot");
         int64_t old = c.slot_brick[(size_t)slot];
@@ -431,43 +464,36 @@ int service_requests(vt_context* ctx, const uint32_t* reqs, int64_t count, vt_fr
         c.ref[(size_t)slot] = 1;
         c.h_slots.p[q] = (int32_t)slot;
         c.h_codes.p[q] = b + (int64_t)(1ull << (3 * c.level_c));
+        if (slots_out) slots_out[q] = (int32_t)slot;
         if (c.host) {
             const float* src = nullptr;
-            int rc = c.host->get(b, host_epoch, &src, &misses);
+            int rc = c.host->get(b, c.epoch, &src, misses);
             if (rc != VT_OK) return rc;
             c.h_src.p[q] = src;
         }
     }
-    if (n_load > 0) {
-        if (c.d_slots.ensure((size_t)n_load) != hipSuccess || c.d_tidx.ensure((size_t)nt) != hipSuccess ||
-            c.d_tval.ensure((size_t)nt) != hipSuccess)
-            return fail(VT_E_NOMEM, "device staging allocation failed");
-        HIP_TRY(hipMemcpyAsync(c.d_slots.p, c.h_slots.p, (size_t)n_load * 4, hipMemcpyHostToDevice, ctx->stream));
-        HIP_TRY(hipMemcpyAsync(c.d_tidx.p, c.h_tidx.p, (size_t)nt * 8, hipMemcpyHostToDevice, ctx->stream));
-        HIP_TRY(hipMemcpyAsync(c.d_tval.p, c.h_tval.p, (size_t)nt * 4, hipMemcpyHostToDevice, ctx->stream));
-        if (c.host) {
-            if (c.d_src.ensure((size_t)n_load) != hipSuccess) return fail(VT_E_NOMEM, "staging");
-            HIP_TRY(hipMemcpyAsync(c.d_src.p, c.h_src.p, (size_t)n_load * sizeof(float*), hipMemcpyHostToDevice,
-                                   ctx->stream));
-            HIP_TRY(launch_copy_bricks(c.d_src.p, c.d_slots.p, (int)n_load, c.slot_elems, c.d_pool, ctx->stream));
-        } else {
-            if (c.d_codes.ensure((size_t)n_load) != hipSuccess) return fail(VT_E_NOMEM, "staging");
-            HIP_TRY(hipMemcpyAsync(c.d_codes.p, c.h_codes.p, (size_t)n_load * 8, hipMemcpyHostToDevice, ctx->stream));
-            HIP_TRY(launch_bricks_from_volume(c.d_vol, c.vol_dims[0], c.vol_dims[1], c.vol_dims[2], c.d_codes.p,
-                                              c.d_slots.p, (int)n_load, c.level_c, ctx->n, c.cube, c.inc, c.d_pool,
-                                              ctx->stream));
-        }
-        HIP_TRY(launch_scatter_i32(c.d_table, c.d_tidx.p, c.d_tval.p, (int)nt, ctx->stream));
-        // the staging buffers are reused by the next call: finish the copies
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (c.d_slots.ensure((size_t)n_load) != hipSuccess || c.d_tidx.ensure((size_t)nt) != hipSuccess ||
+        c.d_tval.ensure((size_t)nt) != hipSuccess)
+        return fail(VT_E_NOMEM, "device staging allocation failed");
+    HIP_TRY(hipMemcpyAsync(c.d_slots.p, c.h_slots.p, (size_t)n_load * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(c.d_tidx.p, c.h_tidx.p, (size_t)nt * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(c.d_tval.p, c.h_tval.p, (size_t)nt * 4, hipMemcpyHostToDevice, ctx->stream));
+    if (c.host) {
+        if (c.d_src.ensure((size_t)n_load) != hipSuccess) return fail(VT_E_NOMEM, "staging");
+        HIP_TRY(hipMemcpyAsync(c.d_src.p, c.h_src.p, (size_t)n_load * sizeof(float*), hipMemcpyHostToDevice,
+                               ctx->stream));
+        HIP_TRY(launch_copy_bricks(c.d_src.p, c.d_slots.p, (int)n_load, c.slot_elems, c.d_pool, ctx->stream));
+    } else {
+        if (c.d_codes.ensure((size_t)n_load) != hipSuccess) return fail(VT_E_NOMEM, "staging");
+        HIP_TRY(hipMemcpyAsync(c.d_codes.p, c.h_codes.p, (size_t)n_load * 8, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(launch_bricks_from_volume(c.d_vol, c.vol_dims[0], c.vol_dims[1], c.vol_dims[2], c.d_codes.p,
+                                          c.d_slots.p, (int)n_load, c.level_c, c.n, c.cube, c.inc, c.d_pool,
+                                          ctx->stream));
     }
+    HIP_TRY(launch_scatter_i32(c.d_table, c.d_tidx.p, c.d_tval.p, (int)nt, ctx->stream));
+    // the staging buffers are reused by the next call: finish the copies
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
     c.stat_uploads += n_load;
-    c.stat_missed += count;
-    if (st) {
-        st->bricks_requested += count;
-        st->bricks_uploaded += n_load;
-        st->host_tier_misses += misses;
-    }
     return VT_OK;
 }
 
@@ -501,6 +527,7 @@ int vt_context_create(int device, vt_context** out)
     if (!out) return fail(VT_E_ARG, "out is NULL");
     int ndev = 0;
     HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0) HIP_TRY(hipGetDevice(&device));   // -1: the calling thread's current device
     if (device < 0 || device >= ndev) return fail(VT_E_ARG, "no such device");
     std::unique_ptr<vt_context> ctx(new vt_context);
     ctx->device = device;
@@ -780,6 +807,7 @@ int vt_cache_prefetch_host(vt_context* ctx, int wait)
     if (!ctx || !ctx->cache || !ctx->cache->host) return fail(VT_E_STATE, "no HDF5-backed cache");
     if (!ctx->octree_ready) return fail(VT_E_STATE, "upload the octree first (brick geometry needs nLevels)");
     HostTier& h = *ctx->cache->host;
+    ctx->cache->n = ctx->n;
     if (!h.reader.joinable() && !h.full_mode) h.start(ctx->n);
     if (wait && h.full_mode) {
         std::unique_lock<std::mutex> lk(h.m);
@@ -797,6 +825,8 @@ int vt_cache_load_bricks(vt_context* ctx, const uint64_t* codes, int64_t count)
     BrickCache& c = *ctx->cache;
     if (!c.d_vol && !c.host) return fail(VT_E_STATE, "no brick source attached");
     if (int rc = set_device(ctx)) return rc;
+    c.n = ctx->n;
+    if (c.host && !c.host->full_mode && !c.host->reader.joinable()) c.host->start(c.n);
     std::vector<uint32_t> req;
     uint64_t base = 1ull << (3 * c.level_c);
     for (int64_t i = 0; i < count; ++i) {
@@ -877,7 +907,8 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
     if (p->levelC != c.level_c || p->levelC > p->levelO) return fail(VT_E_ARG, "levelC must match the cache and be <= levelO");
     if ((1 << (ctx->n - c.level_c)) != c.cube) return fail(VT_E_ARG, "cubeDim must equal 2^(nLevels-levelC)");
     if (int rc = set_device(ctx)) return rc;
-    if (c.host && !c.host->full_mode && !c.host->reader.joinable()) c.host->start(ctx->n);
+    c.n = ctx->n;
+    if (c.host && !c.host->full_mode && !c.host->reader.joinable()) c.host->start(c.n);
 
     const uint32_t R = (uint32_t)cam->W * (uint32_t)cam->H;
     if (ctx->ray_node.ensure(R) != hipSuccess || ctx->ray_phase.ensure(R) != hipSuccess ||
@@ -1029,3 +1060,372 @@ int vt_file_create(const char* file, const char* dataset, const float* data, int
 }
 
 }  // extern "C"
+
+// ============================================================ builder ABI
+
+namespace {
+
+// CreateOctree.cu:308-319 (nLevels from the largest extent, float log2)
+int tree_levels(int X, int Y, int Z)
+{
+    int dimension;
+    if (X > Y && X > Z) dimension = X;
+    else if (Y > Z) dimension = Y;
+    else dimension = Z;
+    float aux = logf((float)dimension) / logf(2.0f);
+    float aux2 = aux - floorf(aux);
+    return aux2 > 0.0 ? (int)(aux + 1) : (int)aux;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vt_octree_build(vt_context* ctx, const float* d_vol, int32_t X, int32_t Y, int32_t Z, float iso,
+                    int32_t literal, int32_t maxLevel, int32_t* nLevels_out, int64_t* sizes_out)
+{
+    if (!ctx || !d_vol) return fail(VT_E_ARG, "NULL argument");
+    if (X <= 0 || Y <= 0 || Z <= 0) return fail(VT_E_ARG, "bad volume dims");
+    const int n = tree_levels(X, Y, Z);
+    if (n < 2 || n > 12) return fail(VT_E_ARG, "the GPU builder needs 2 <= nLevels <= 12");
+    if (maxLevel < 0 || maxLevel > n || maxLevel > 9) return fail(VT_E_ARG, "bad maxLevel");
+    if (int rc = set_device(ctx)) return rc;
+    hipStream_t s = ctx->stream;
+    // bitmaps of levels 2..n, 8^l bits each
+    std::vector<uint64_t> woff(n + 1, 0);
+    uint64_t total = 0;
+    for (int l = 2; l <= n; ++l) { woff[l] = total; total += (1ull << (3 * l)) / 64; }
+    DevBuf<uint64_t> bits;
+    if (bits.ensure(total) != hipSuccess) return fail(VT_E_NOMEM, "octree bitmap allocation failed");
+    HIP_TRY(launch_leaf_bits(d_vol, X, Y, Z, iso, (1ull << (3 * n)) / 64, bits.p + woff[n], s));
+    for (int l = n - 1; l >= 2; --l) {
+        uint64_t nw = (1ull << (3 * l)) / 64;
+        HIP_TRY(launch_parent_bits(bits.p + woff[l + 1], nw, bits.p + woff[l], s));
+        if (literal) HIP_TRY(launch_clear_top_bit(bits.p + woff[l] + nw - 1, s));
+    }
+    // levels 1 and 0 from the single word of level 2
+    uint64_t w2 = 0;
+    HIP_TRY(hipMemcpyAsync(&w2, bits.p + woff[2], 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    uint32_t lvl1 = 0;
+    for (int k = 0; k < 8; ++k)
+        if ((w2 >> (8 * k)) & 0xffull) lvl1 |= 1u << k;
+    if (literal && n - 1 >= 1) lvl1 &= 0x7fu;   // last node of level 1 (code 15)
+    const bool root = lvl1 != 0;                // CreateOctree.cu:532-540
+    // run-length encode every level
+    std::vector<std::vector<uint64_t>> runs((size_t)n + 1);
+    if (root) runs[0] = {1, 1};
+    for (int k = 0; k < 8;) {   // level 1 (8 nodes) on the host
+        if (!((lvl1 >> k) & 1)) { ++k; continue; }
+        int e = k;
+        while (e + 1 < 8 && ((lvl1 >> (e + 1)) & 1)) ++e;
+        runs[1].push_back(8 + (uint64_t)k);
+        runs[1].push_back(8 + (uint64_t)e);
+        k = e + 1;
+    }
+    DevBuf<uint32_t> bsum;
+    DevBuf<uint64_t> boff, dout;
+    for (int l = 2; l <= n; ++l) {
+        uint64_t nbits = 1ull << (3 * l), nw = nbits / 64, nb = run_blocks(nw);
+        if (bsum.ensure(nb) != hipSuccess || boff.ensure(nb) != hipSuccess)
+            return fail(VT_E_NOMEM, "scan buffers");
+        HIP_TRY(launch_run_counts(bits.p + woff[l], nw, nbits, bsum.p, s));
+        std::vector<uint32_t> hs(nb);
+        std::vector<uint64_t> ho(nb);
+        HIP_TRY(hipMemcpyAsync(hs.data(), bsum.p, nb * 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        uint64_t acc = 0;
+        for (uint64_t b = 0; b < nb; ++b) { ho[b] = acc; acc += hs[b]; }
+        if (acc == 0) continue;
+        if (dout.ensure(acc) != hipSuccess) return fail(VT_E_NOMEM, "run buffer");
+        HIP_TRY(hipMemcpyAsync(boff.p, ho.data(), nb * 8, hipMemcpyHostToDevice, s));
+        HIP_TRY(launch_run_write(bits.p + woff[l], nw, nbits, 1ull << (3 * l), boff.p, dout.p, s));
+        runs[(size_t)l].resize(acc);
+        HIP_TRY(hipMemcpyAsync(runs[(size_t)l].data(), dout.p, acc * 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    bits.release();
+    bsum.release();
+    boff.release();
+    dout.release();
+    // install levels 0..maxLevel as the traversal octree
+    std::vector<uint64_t> flat;
+    std::vector<int32_t> sizes((size_t)maxLevel + 1);
+    for (int l = 0; l <= maxLevel; ++l) {
+        sizes[(size_t)l] = (int32_t)runs[(size_t)l].size();
+        flat.insert(flat.end(), runs[(size_t)l].begin(), runs[(size_t)l].end());
+    }
+    int rc = vt_octree_upload(ctx, flat.data(), sizes.data(), n, maxLevel);
+    if (rc != VT_OK) return rc;
+    ctx->built_runs = std::move(runs);
+    ctx->built_iso = iso;
+    ctx->built_n = n;
+    ctx->built_dim = 1 << n;
+    ctx->built_real[0] = X; ctx->built_real[1] = Y; ctx->built_real[2] = Z;
+    if (nLevels_out) *nLevels_out = n;
+    if (sizes_out)
+        for (int l = 0; l <= n; ++l) sizes_out[l] = (int64_t)ctx->built_runs[(size_t)l].size();
+    return VT_OK;
+}
+
+int vt_octree_get_runs(vt_context* ctx, int32_t level, uint64_t* out, int64_t count)
+{
+    if (!ctx || (count && !out)) return fail(VT_E_ARG, "NULL argument");
+    if (ctx->built_runs.empty()) return fail(VT_E_STATE, "no octree built");
+    if (level < 0 || level > ctx->built_n) return fail(VT_E_ARG, "no such level");
+    const auto& r = ctx->built_runs[(size_t)level];
+    if ((size_t)count != r.size()) return fail(VT_E_ARG, "count must equal the level's run-list size");
+    if (count) memcpy(out, r.data(), r.size() * 8);
+    return VT_OK;
+}
+
+int vt_octree_write_file(vt_context* ctx, const char* path)
+{
+    if (!ctx || !path) return fail(VT_E_ARG, "NULL argument");
+    if (ctx->built_runs.empty()) return fail(VT_E_STATE, "no octree built");
+    FILE* f = fopen(path, "wb");
+    if (!f) return fail(VT_E_IO, std::string("cannot create ") + path);
+    int32_t magic = 919278872, n = ctx->built_n, dim = ctx->built_dim;
+    bool ok = fwrite(&magic, 4, 1, f) == 1 && fwrite(&ctx->built_iso, 4, 1, f) == 1 &&
+              fwrite(&dim, 4, 1, f) == 1 && fwrite(ctx->built_real, 4, 3, f) == 3 && fwrite(&n, 4, 1, f) == 1;
+    for (int l = n; ok && l >= 0; --l) {
+        const auto& r = ctx->built_runs[(size_t)l];
+        int32_t ne = (int32_t)r.size();
+        ok = fwrite(&ne, 4, 1, f) == 1 && (ne == 0 || fwrite(r.data(), 8, r.size(), f) == r.size());
+    }
+    ok = (fclose(f) == 0) && ok;
+    if (!ok) return fail(VT_E_IO, std::string("write failed: ") + path);
+    return VT_OK;
+}
+
+int vt_volume_generate(vt_context* ctx, const vt_gen_params* p, int64_t X, int64_t Y, int64_t Z, int64_t x0,
+                       int64_t nx, float* d_out)
+{
+    if (!ctx || !p || !d_out) return fail(VT_E_ARG, "NULL argument");
+    if (X <= 0 || Y <= 0 || Z <= 0 || x0 < 0 || nx <= 0 || x0 + nx > X) return fail(VT_E_ARG, "bad extents");
+    if (p->kind < VT_GEN_BLOBS || p->kind > VT_GEN_FOURIER) return fail(VT_E_ARG, "unknown generator");
+    if (p->count < 0 || (p->count > 0 && !p->shapes)) return fail(VT_E_ARG, "shapes missing");
+    if (p->kind != VT_GEN_BLOBS && p->kind != VT_GEN_FOURIER && (p->octaves < 0 || p->octaves > 16))
+        return fail(VT_E_ARG, "octaves must be in [0,16]");
+    if (int rc = set_device(ctx)) return rc;
+    int stride = p->kind == VT_GEN_SDF ? 8 : 5;
+    DevBuf<float> shp;
+    if (shp.ensure((size_t)std::max(1, p->count * stride)) != hipSuccess) return fail(VT_E_NOMEM, "shapes");
+    if (p->count)
+        HIP_TRY(hipMemcpyAsync(shp.p, p->shapes, (size_t)p->count * stride * 4, hipMemcpyHostToDevice, ctx->stream));
+    GenParams g;
+    g.kind = p->kind;
+    g.seed = p->seed;
+    g.X = X; g.Y = Y; g.Z = Z;
+    g.n = p->n > 0 ? p->n : (int)X;
+    g.count = p->count;
+    g.shapes = shp.p;
+    g.octaves = p->octaves;
+    g.base_freq = p->base_freq;
+    g.gain = p->gain;
+    g.lacunarity = p->lacunarity;
+    g.scale = p->scale;
+    HIP_TRY(launch_generate(d_out, g, x0, nx, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    shp.release();
+    return VT_OK;
+}
+
+int vt_volume_write_hdf5(vt_context* ctx, const float* d_vol, int64_t X, int64_t Y, int64_t Z, const char* file,
+                         const char* dataset, int32_t chunk)
+{
+    if (!ctx || !d_vol || !file || !dataset) return fail(VT_E_ARG, "NULL argument");
+    if (X <= 0 || Y <= 0 || Z <= 0 || chunk < 0) return fail(VT_E_ARG, "bad dimensions");
+    if (int rc = set_device(ctx)) return rc;
+    std::string err;
+    H5Writer w;
+    if (!w.create(file, dataset, X, Y, Z, chunk, err)) return fail(VT_E_IO, err);
+    // slabs of whole chunk rows, ~256 MB each, double-buffered in pinned memory
+    int64_t step = chunk > 0 ? chunk : 1;
+    int64_t per = std::max<int64_t>(step, ((256ll << 20) / (Y * Z * 4)) / step * step);
+    PinBuf<float> hb[2];
+    if (hb[0].ensure((size_t)(per * Y * Z)) != hipSuccess || hb[1].ensure((size_t)(per * Y * Z)) != hipSuccess)
+        return fail(VT_E_NOMEM, "pinned slab buffers");
+    int cur = 0;
+    int64_t x0 = 0, nx0 = std::min(per, X);
+    HIP_TRY(hipMemcpyAsync(hb[cur].p, d_vol, (size_t)(nx0 * Y * Z) * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    while (x0 < X) {
+        int64_t nx = std::min(per, X - x0);
+        int64_t x1 = x0 + nx;
+        if (x1 < X) {   // next slab's copy overlaps this slab's file write
+            int64_t nn = std::min(per, X - x1);
+            HIP_TRY(hipMemcpyAsync(hb[cur ^ 1].p, d_vol + (size_t)x1 * Y * Z, (size_t)(nn * Y * Z) * 4,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+        }
+        if (!w.write_slab(x0, nx, hb[cur].p, err)) return fail(VT_E_IO, err);
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        cur ^= 1;
+        x0 = x1;
+    }
+    if (!w.close(err)) return fail(VT_E_IO, err);
+    return VT_OK;
+}
+
+}  // extern "C"
+
+// ======================================================= staged protocol
+
+extern "C" {
+
+int vt_stage_reset(vt_context* ctx, int64_t numRays)
+{
+    if (!ctx || numRays <= 0) return fail(VT_E_ARG, "bad arguments");
+    if (int rc = set_device(ctx)) return rc;
+    if (ctx->stage_trav.ensure((size_t)numRays) != hipSuccess) return fail(VT_E_NOMEM, "traversal state");
+    HIP_TRY(hipMemsetAsync(ctx->stage_trav.p, 0, (size_t)numRays, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->stage_rays = numRays;
+    return VT_OK;
+}
+
+int vt_stage_traverse(vt_context* ctx, const vt_camera* cam, int32_t finalLevel, vt_visible_cube* d_cubes)
+{
+    if (!ctx || !d_cubes) return fail(VT_E_ARG, "NULL argument");
+    if (int rc = check_camera(cam)) return rc;
+    if (!ctx->octree_ready) return fail(VT_E_STATE, "no octree uploaded");
+    if (finalLevel < 0 || finalLevel > ctx->max_level) return fail(VT_E_ARG, "finalLevel beyond uploaded levels");
+    int64_t R = (int64_t)cam->W * cam->H;
+    if (ctx->stage_rays < R) {
+        if (int rc = vt_stage_reset(ctx, R)) return rc;
+    }
+    if (int rc = set_device(ctx)) return rc;
+    StageTraverseArgs a;
+    a.cam = to_cam(cam);
+    a.oct = make_view(ctx, finalLevel);
+    a.cubes = reinterpret_cast<VisibleCube*>(d_cubes);
+    a.trav = ctx->stage_trav.p;
+    HIP_TRY(launch_stage_traverse(a, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return VT_OK;
+}
+
+int vt_stage_cache(vt_context* ctx, vt_visible_cube* h_cubes, int64_t num, int32_t levelO, int32_t nLevels)
+{
+    if (!ctx || (num && !h_cubes)) return fail(VT_E_ARG, "NULL argument");
+    if (!ctx->cache) return fail(VT_E_STATE, "no cache");
+    BrickCache& c = *ctx->cache;
+    if (!c.d_vol && !c.host) return fail(VT_E_STATE, "no brick source attached");
+    if (levelO < c.level_c || nLevels < levelO || nLevels > 20) return fail(VT_E_ARG, "bad levels");
+    if ((1 << (nLevels - c.level_c)) != c.cube) return fail(VT_E_ARG, "cubeDim must equal 2^(nLevels-levelC)");
+    if (int rc = set_device(ctx)) return rc;
+    c.n = nLevels;
+    if (c.host && !c.host->full_mode && !c.host->reader.joinable()) c.host->start(c.n);
+    const int shift = 3 * (levelO - c.level_c);
+    const uint64_t base = 1ull << (3 * c.level_c);
+    std::vector<uint32_t> need;
+    for (int64_t i = 0; i < num; ++i) {
+        uint8_t st = h_cubes[i].state;
+        if (st != VT_CUBE && st != VT_NOCACHED) continue;
+        uint64_t b = h_cubes[i].id >> shift;
+        if (b < base || b >= 2 * base) return fail(VT_E_ARG, "node code outside the octree");
+        need.push_back((uint32_t)(b - base));
+    }
+    std::sort(need.begin(), need.end());
+    need.erase(std::unique(need.begin(), need.end()), need.end());
+    if (int rc = fold_ref_bits(ctx)) return rc;
+    // pin every brick this pass already holds, so that loading the missing
+    // ones can never evict a brick another ray of the pass points at
+    // (lruCache.cu:254-266 could)
+    c.epoch++;
+    int64_t pinned = 0;
+    std::vector<uint32_t> missing;
+    for (uint32_t b : need) {
+        int32_t s = c.table[b];
+        if (s >= 0) {
+            c.pin[(size_t)s] = c.epoch;
+            c.ref[(size_t)s] = 1;
+            pinned++;
+        } else {
+            missing.push_back(b);
+        }
+    }
+    int64_t avail = c.gpu_slots - pinned;
+    int64_t n_load = std::min<int64_t>((int64_t)missing.size(), avail);
+    if (c.host && !c.host->full_mode) n_load = std::min<int64_t>(n_load, c.host->capacity);
+    int64_t misses = 0;
+    if (int rc = load_bricks(ctx, missing.data(), n_load, &misses, nullptr)) return rc;
+    c.stat_missed += (int64_t)missing.size();
+    for (int64_t i = 0; i < num; ++i) {
+        vt_visible_cube& v = h_cubes[i];
+        if (v.state != VT_CUBE && v.state != VT_NOCACHED) continue;
+        uint64_t brick = v.id >> shift;
+        int32_t s = c.table[(size_t)(brick - base)];
+        if (s >= 0 && c.pin[(size_t)s] == c.epoch) {
+            v.data = c.d_pool + (size_t)s * c.slot_elems;
+            v.state = VT_CACHED;
+            v.cubeID = brick;
+        } else {
+            v.state = VT_NOCACHED;   // retried next pass (lruCache.cu:184-191)
+            v.cubeID = 0;
+        }
+    }
+    return VT_OK;
+}
+
+int vt_stage_raycast(vt_context* ctx, const vt_camera* cam, const vt_render_params* p, int32_t nLevels,
+                     int32_t cubeDim, int32_t cubeInc, vt_visible_cube* d_cubes, float* d_rgba)
+{
+    if (!ctx || !p || !d_cubes || !d_rgba) return fail(VT_E_ARG, "NULL argument");
+    if (int rc = check_camera(cam)) return rc;
+    if (nLevels < 1 || nLevels > 20 || p->levelO > nLevels || p->levelC > p->levelO || cubeDim <= 0 || cubeInc < 0)
+        return fail(VT_E_ARG, "bad levels or brick geometry");
+    if ((1 << (nLevels - p->levelC)) != cubeDim) return fail(VT_E_ARG, "cubeDim must equal 2^(nLevels-levelC)");
+    if (int rc = set_device(ctx)) return rc;
+    StageRaycastArgs a;
+    a.cam = to_cam(cam);
+    a.n = nLevels;
+    a.level_o = p->levelO;
+    a.level_c = p->levelC;
+    a.inc = cubeInc;
+    a.S = cubeDim + 2 * cubeInc;
+    a.iso = p->iso;
+    a.lx = p->light[0]; a.ly = p->light[1]; a.lz = p->light[2];
+    a.color_scale = p->color_scale;
+    a.cubes = reinterpret_cast<VisibleCube*>(d_cubes);
+    a.rgba = reinterpret_cast<float4*>(d_rgba);
+    a.err = &ctx->d_ctr->err;
+    HIP_TRY(hipMemsetAsync(&ctx->d_ctr->err, 0, 4, ctx->stream));
+    HIP_TRY(launch_stage_raycast(a, ctx->stream));
+    uint32_t err = 0;
+    HIP_TRY(hipMemcpyAsync(&ctx->h_ctr->err, &ctx->d_ctr->err, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    err = ctx->h_ctr->err;
+    if (err) return fail(VT_E_RANGE, "a sample fell outside its brick slot (halo too small for this geometry)");
+    return VT_OK;
+}
+
+}  // extern "C"
+
+extern "C" int vt_render_cube(vt_context* ctx, const vt_camera* cam, float iso, const float light[3],
+                              const float* d_cube, const int32_t minBox[3], const int32_t cubeDim[3],
+                              int32_t cubeInc, float* d_rgba)
+{
+    if (!ctx || !light || !d_cube || !minBox || !cubeDim || !d_rgba) return fail(VT_E_ARG, "NULL argument");
+    if (int rc = check_camera(cam)) return rc;
+    // posToIndex (rayCaster.cu:9) strides every axis by dim.x: only cubic bricks are laid out consistently
+    if (cubeDim[0] <= 0 || cubeDim[0] != cubeDim[1] || cubeDim[0] != cubeDim[2] || cubeInc < 0)
+        return fail(VT_E_ARG, "renderCube needs a cubic brick");
+    if (int rc = set_device(ctx)) return rc;
+    CubeArgs a;
+    a.cam = to_cam(cam);
+    a.iso = iso;
+    a.lx = light[0]; a.ly = light[1]; a.lz = light[2];
+    a.cube = d_cube;
+    a.mnx = minBox[0]; a.mny = minBox[1]; a.mnz = minBox[2];
+    a.dim = cubeDim[0];
+    a.inc = cubeInc;
+    a.rgba = reinterpret_cast<float4*>(d_rgba);
+    a.err = &ctx->d_ctr->err;
+    HIP_TRY(hipMemsetAsync(&ctx->d_ctr->err, 0, 4, ctx->stream));
+    HIP_TRY(launch_render_cube(a, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(&ctx->h_ctr->err, &ctx->d_ctr->err, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->h_ctr->err) return fail(VT_E_RANGE, "a sample fell outside the brick (halo too small)");
+    return VT_OK;
+}
