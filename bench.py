@@ -199,6 +199,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--parity-rays", type=int, default=4096)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run a 256 MiB streaming copy kernel first (HBM counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -207,6 +209,13 @@ def main() -> int:
         return 2
 
     ctx = vt.Context(0)
+    if args.copy_probe:
+        nb = 256 << 20
+        a_, b_ = ctx.alloc(nb), ctx.alloc(nb)
+        for _ in range(3):
+            ctx.copy_probe(a_, b_, nb)
+        a_.free()
+        b_.free()
     w = setup_c2(ctx, args) if args.config == "c2" else setup_c1(ctx, args)
     cams = [w.cameras(k) for k in range(args.warmup + args.steps)]
     R = w.W * w.H

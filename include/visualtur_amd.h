@@ -262,6 +262,39 @@ int vt_volume_generate(vt_context *ctx, const vt_gen_params *p, int64_t X, int64
 int vt_volume_write_hdf5(vt_context *ctx, const float *d_vol, int64_t X, int64_t Y, int64_t Z,
                          const char *file, const char *dataset, int32_t chunk);
 
+/* ------------------------------------------- learned brick prefetcher */
+
+/* Brick-visibility MLP (no reference counterpart): 12 features -> 256 ->
+ * 256 -> 256 -> 1 logit, ReLU, BF16 weights and activations with FP32
+ * accumulation on MFMA, FP32 master weights trained with Adam.
+ * Parameters (vt_mlp_num_params floats, this order): W1[12][256], b1[256],
+ * W2[256][256], b2[256], W3[256][256], b3[256], w4[256], b4[1]; layer
+ * k computes h_k = relu(h_{k-1} W_k + b_k) and the logit is h_3 . w4 + b4. */
+typedef struct vt_mlp vt_mlp;
+int vt_mlp_num_params(void);
+int vt_mlp_create(vt_context *ctx, uint64_t seed, int64_t max_batch, vt_mlp **out);
+int vt_mlp_destroy(vt_mlp *m);
+int vt_mlp_get_params(vt_mlp *m, float *out);
+int vt_mlp_set_params(vt_mlp *m, const float *in);
+/* logits of N feature rows d_x[N][12] (device) into d_logits[N] (device) */
+int vt_mlp_forward(vt_mlp *m, const float *d_x, int64_t N, float *d_logits);
+/* gradient of the mean weighted BCE-with-logits (labels d_y, positives
+ * weighted by pos_weight) into host grad_out[vt_mlp_num_params()] */
+int vt_mlp_gradients(vt_mlp *m, const float *d_x, const float *d_y, int64_t N, float pos_weight, float *grad_out,
+                     float *loss_out);
+/* one Adam step (beta 0.9/0.999, eps 1e-8) on the batch; loss_out may be NULL */
+int vt_mlp_train_step(vt_mlp *m, const float *d_x, const float *d_y, int64_t N, float lr, float pos_weight,
+                      float *loss_out);
+/* features of (camera, brick) pairs: camera position and brick centre in
+ * cube units minus 0.5, view direction, levelC/10, distance / cube side,
+ * cosine of the angle between the view direction and the brick. */
+int vt_mlp_features(vt_mlp *m, const vt_camera *cam, int32_t nLevels, int32_t levelC, const uint64_t *d_codes,
+                    int64_t N, float *d_x);
+
+/* Streaming device copy (no reference counterpart): a kernel of known
+ * traffic, read bytes + written bytes, that calibrates HBM byte counters. */
+int vt_copy_probe(vt_context *ctx, const void *d_src, void *d_dst, size_t bytes);
+
 /* ---------------------------------------------------------------- files */
 
 /* FileManager::FileManager (FileManager.cu:11-37), read-only. */

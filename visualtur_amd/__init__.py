@@ -109,6 +109,19 @@ _SIGS = {
                                           ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _P]),
     "vt_volume_write_hdf5": (ctypes.c_int, [_P, _P, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_char_p,
                                             ctypes.c_char_p, ctypes.c_int32]),
+    "vt_copy_probe": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
+    "vt_mlp_num_params": (ctypes.c_int, []),
+    "vt_mlp_create": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(_P)]),
+    "vt_mlp_destroy": (ctypes.c_int, [_P]),
+    "vt_mlp_get_params": (ctypes.c_int, [_P, _P]),
+    "vt_mlp_set_params": (ctypes.c_int, [_P, _P]),
+    "vt_mlp_forward": (ctypes.c_int, [_P, _P, ctypes.c_int64, _P]),
+    "vt_mlp_gradients": (ctypes.c_int, [_P, _P, _P, ctypes.c_int64, ctypes.c_float, _P,
+                                        ctypes.POINTER(ctypes.c_float)]),
+    "vt_mlp_train_step": (ctypes.c_int, [_P, _P, _P, ctypes.c_int64, ctypes.c_float, ctypes.c_float,
+                                         ctypes.POINTER(ctypes.c_float)]),
+    "vt_mlp_features": (ctypes.c_int, [_P, ctypes.POINTER(Camera), ctypes.c_int32, ctypes.c_int32, _P,
+                                       ctypes.c_int64, _P]),
     "vt_file_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_P),
                                     ctypes.POINTER(ctypes.c_int64)]),
     "vt_file_close": (ctypes.c_int, [_P]),
@@ -243,6 +256,9 @@ class Context:
         a = np.ascontiguousarray(arr)
         return DeviceBuffer(self, a.nbytes).upload(a)
 
+    def copy_probe(self, src: DeviceBuffer, dst: DeviceBuffer, nbytes: int) -> None:
+        self._c(lib().vt_copy_probe(self.handle, src.ptr, dst.ptr, nbytes))
+
     def synchronize(self) -> None:
         self._c(lib().vt_synchronize(self.handle))
 
@@ -358,6 +374,50 @@ class Context:
             return d.download(np.float32, (R, 3))
         finally:
             d.free()
+
+
+class MLP:
+    """The brick-visibility MLP (vt_mlp_*)."""
+
+    def __init__(self, ctx: Context, seed: int = 1, max_batch: int = 8192):
+        h = ctypes.c_void_p()
+        _check(lib().vt_mlp_create(ctx.handle, seed, max_batch, ctypes.byref(h)), ctx.handle)
+        self.ctx, self.handle, self.max_batch = ctx, h.value, max_batch
+        self.nparams = lib().vt_mlp_num_params()
+
+    def close(self) -> None:
+        if self.handle:
+            lib().vt_mlp_destroy(self.handle)
+            self.handle = None
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.nparams, np.float32)
+        _check(lib().vt_mlp_get_params(self.handle, out.ctypes.data))
+        return out
+
+    def set_params(self, p) -> None:
+        a = np.ascontiguousarray(p, np.float32)
+        assert a.size == self.nparams
+        _check(lib().vt_mlp_set_params(self.handle, a.ctypes.data))
+
+    def forward(self, d_x: DeviceBuffer, n: int, d_out: DeviceBuffer) -> None:
+        _check(lib().vt_mlp_forward(self.handle, d_x.ptr, n, d_out.ptr))
+
+    def gradients(self, d_x: DeviceBuffer, d_y: DeviceBuffer, n: int, pos_weight: float = 1.0):
+        g = np.zeros(self.nparams, np.float32)
+        loss = ctypes.c_float()
+        _check(lib().vt_mlp_gradients(self.handle, d_x.ptr, d_y.ptr, n, pos_weight, g.ctypes.data,
+                                      ctypes.byref(loss)))
+        return g, loss.value
+
+    def train_step(self, d_x: DeviceBuffer, d_y: DeviceBuffer, n: int, lr: float, pos_weight: float = 1.0) -> float:
+        loss = ctypes.c_float()
+        _check(lib().vt_mlp_train_step(self.handle, d_x.ptr, d_y.ptr, n, lr, pos_weight, ctypes.byref(loss)))
+        return loss.value
+
+    def features(self, cam: Camera, n_levels: int, level_c: int, d_codes: DeviceBuffer, n: int,
+                 d_x: DeviceBuffer) -> None:
+        _check(lib().vt_mlp_features(self.handle, ctypes.byref(cam), n_levels, level_c, d_codes.ptr, n, d_x.ptr))
 
 
 def render_params(level_o: int, level_c: int, iso: float, light=(0.0, 512.0, 0.0),

@@ -1,0 +1,356 @@
+// vt_context.hpp -- private state of a vt_context (host runtime).
+#pragma once
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "visualtur_amd.h"
+#include "vt_hdf5.hpp"
+#include "vt_internal.hpp"
+
+using namespace vt;
+
+static_assert(sizeof(vt_camera) == sizeof(vt::Camera), "camera layout");
+static_assert(sizeof(vt_hit) == sizeof(vt::HitRecord), "hit layout");
+
+namespace vt_rt {
+int fail(int code, const std::string& msg);
+
+#define HIP_TRY(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return fail(VT_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+inline double now_ms()
+{
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t count)
+    {
+        if (count <= n && p) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T));
+        if (e == hipSuccess) n = count;
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+template <class T>
+struct PinBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t count)
+    {
+        if (count <= n && p) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipHostMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T), hipHostMallocDefault);
+        if (e == hipSuccess) n = count;
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+// ------------------------------------------------------------ host tier
+// Pinned, device-visible brick slots.  When it can hold every brick of the
+// cube, a reader thread fills it slab by slab (x-slabs of bricks, one HDF5
+// hyperslab read per slab); otherwise bricks are read on demand into
+// clock-evicted slots.
+struct HostTier {
+    int level_c = 0, n = 0, cube = 0, inc = 0, S = 0;
+    int64_t nbricks = 0, capacity = 0;
+    size_t slot_elems = 0;
+    float* pool = nullptr;        // pinned host memory
+    float* pool_dev = nullptr;    // the same memory as seen by kernels
+    std::string file, dataset;
+    H5Volume sync_reader;         // on-demand reads (service thread)
+    std::vector<int32_t> slot_of; // brick index -> host slot
+    std::vector<int64_t> brick_of;
+    std::vector<uint8_t> ref;
+    std::vector<uint64_t> pin;
+    int64_t hand = 0, used = 0;
+    // prefetch state
+    bool full_mode = false;
+    std::thread reader;
+    std::mutex m;
+    std::condition_variable cv;
+    int slabs_done = 0;           // x-slabs of bricks fully loaded
+    bool reader_failed = false;
+    std::string reader_err;
+    int64_t file_reads = 0;
+
+    ~HostTier()
+    {
+        if (reader.joinable()) reader.join();
+        if (pool) (void)hipHostFree(pool);
+    }
+
+    int64_t side_bricks() const { return 1ll << level_c; }
+
+    // linear x-major brick index of (bx,by,bz) brick coordinates
+    static uint64_t brick_code(int bx, int by, int bz, int level_c)
+    {
+        return (1ull << (3 * level_c)) | (dilate3((uint32_t)bx) << 2) | (dilate3((uint32_t)by) << 1) |
+               dilate3((uint32_t)bz);
+    }
+
+    void run_prefetch()
+    {
+        H5Volume vol;
+        std::string err;
+        if (!vol.open(file, dataset, 0, err)) {
+            std::lock_guard<std::mutex> g(m);
+            reader_failed = true;
+            reader_err = err;
+            cv.notify_all();
+            return;
+        }
+        const int64_t B = side_bricks();
+        const int64_t ext = B * cube;   // cube extent in voxels
+        const int64_t SY = ext + 2 * inc, SZ = ext + 2 * inc;
+        std::vector<float> slab((size_t)S * SY * SZ);
+        for (int64_t bx = 0; bx < B; ++bx) {
+            int s[3] = {(int)(bx * cube - inc), -inc, -inc};
+            int e[3] = {(int)(bx * cube - inc + S), (int)(ext + inc), (int)(ext + inc)};
+            if (!vol.read_box(s, e, slab.data(), err)) {
+                std::lock_guard<std::mutex> g(m);
+                reader_failed = true;
+                reader_err = err;
+                cv.notify_all();
+                return;
+            }
+            for (int64_t by = 0; by < B; ++by)
+                for (int64_t bz = 0; bz < B; ++bz) {
+                    uint64_t code = brick_code((int)bx, (int)by, (int)bz, level_c);
+                    int64_t bidx = (int64_t)(code - (1ull << (3 * level_c)));
+                    float* dst = pool + (size_t)slot_of[bidx] * slot_elems;
+                    for (int x = 0; x < S; ++x)
+                        for (int y = 0; y < S; ++y) {
+                            const float* src = slab.data() + ((size_t)x * SY + (size_t)(by * cube + y)) * SZ +
+                                               (size_t)(bz * cube);
+                            memcpy(dst + ((size_t)x * S + y) * S, src, (size_t)S * sizeof(float));
+                        }
+                }
+            {
+                std::lock_guard<std::mutex> g(m);
+                slabs_done = (int)bx + 1;
+                file_reads += B * B;
+            }
+            cv.notify_all();
+        }
+    }
+
+    int start(int n_levels)
+    {
+        n = n_levels;
+        if (capacity >= nbricks) {
+            full_mode = true;
+            for (int64_t b = 0; b < nbricks; ++b) { slot_of[b] = (int32_t)b; brick_of[b] = b; }
+            used = nbricks;
+            reader = std::thread([this] { run_prefetch(); });
+        }
+        return VT_OK;
+    }
+
+    // device-visible pointer of brick `bidx`, reading it if needed.
+    int get(int64_t bidx, uint64_t epoch, const float** out, int64_t* misses)
+    {
+        if (full_mode) {
+            int32_t code_bx;
+            {
+                int x, y, z;
+                node_min(bidx + (1ull << (3 * level_c)), level_c, level_c, x, y, z);
+                code_bx = x;
+            }
+            std::unique_lock<std::mutex> lk(m);
+            cv.wait(lk, [&] { return reader_failed || slabs_done > code_bx; });
+            if (reader_failed) return fail(VT_E_IO, "host tier reader: " + reader_err);
+            *out = pool_dev + (size_t)slot_of[bidx] * slot_elems;
+            return VT_OK;
+        }
+        int32_t s = slot_of[bidx];
+        if (s >= 0) {
+            ref[s] = 1;
+            pin[s] = epoch;
+            *out = pool_dev + (size_t)s * slot_elems;
+            return VT_OK;
+        }
+        // choose a host slot: unused first, then clock skipping pinned
+        int64_t slot = -1;
+        if (used < capacity) {
+            slot = used++;
+        } else {
+            for (int64_t it = 0; it < 3 * capacity; ++it) {
+                int64_t c = hand;
+                hand = (hand + 1) % capacity;
+                if (pin[c] == epoch) continue;
+                if (ref[c]) { ref[c] = 0; continue; }
+                slot = c;
+                break;
+            }
+            if (slot < 0) return fail(VT_E_STATE, "host tier: every slot is pinned");
+            slot_of[brick_of[slot]] = -1;
+        }
+        if (!sync_reader.is_open()) {
+            std::string err;
+            if (!sync_reader.open(file, dataset, 64ull << 20, err)) return fail(VT_E_IO, err);
+        }
+        int x, y, z;
+        node_min(bidx + (1ull << (3 * level_c)), level_c, n, x, y, z);
+        int s3[3] = {x - inc, y - inc, z - inc};
+        int e3[3] = {x - inc + S, y - inc + S, z - inc + S};
+        std::string err;
+        if (!sync_reader.read_box(s3, e3, pool + (size_t)slot * slot_elems, err)) return fail(VT_E_IO, err);
+        brick_of[slot] = bidx;
+        slot_of[bidx] = (int32_t)slot;
+        ref[slot] = 1;
+        pin[slot] = epoch;
+        file_reads++;
+        (*misses)++;
+        *out = pool_dev + (size_t)slot * slot_elems;
+        return VT_OK;
+    }
+};
+
+// ----------------------------------------------------------- brick cache
+struct BrickCache {
+    int level_c = 0, cube = 0, inc = 0, S = 0;
+    int n = 0;   // octree depth the brick codes refer to (set on first use)
+    size_t slot_elems = 0;
+    int64_t nbricks = 0, gpu_slots = 0, cpu_slots = 0;
+    float* d_pool = nullptr;
+    int32_t* d_table = nullptr;
+    uint8_t* d_ref = nullptr;
+    std::vector<int32_t> table;
+    std::vector<int64_t> slot_brick;
+    std::vector<uint8_t> ref, ref_tmp;
+    std::vector<uint64_t> pin;
+    uint64_t epoch = 0;
+    int64_t hand = 0, used = 0;
+    // sources
+    const float* d_vol = nullptr;
+    int vol_dims[3] = {0, 0, 0};
+    std::unique_ptr<HostTier> host;
+    // staging
+    PinBuf<const float*> h_src;
+    PinBuf<int32_t> h_slots;
+    PinBuf<int64_t> h_codes;
+    PinBuf<int64_t> h_tidx;
+    PinBuf<int32_t> h_tval;
+    DevBuf<const float*> d_src;
+    DevBuf<int32_t> d_slots;
+    DevBuf<int64_t> d_codes;
+    DevBuf<int64_t> d_tidx;
+    DevBuf<int32_t> d_tval;
+    // stats
+    int64_t stat_missed = 0, stat_uploads = 0, stat_evictions = 0;
+
+    ~BrickCache()
+    {
+        host.reset();
+        if (d_pool) (void)hipFree(d_pool);
+        if (d_table) (void)hipFree(d_table);
+        if (d_ref) (void)hipFree(d_ref);
+        h_src.release(); h_slots.release(); h_codes.release(); h_tidx.release(); h_tval.release();
+        d_src.release(); d_slots.release(); d_codes.release(); d_tidx.release(); d_tval.release();
+    }
+
+    int64_t pick_victim()
+    {
+        if (used < gpu_slots) return used++;
+        for (int64_t it = 0; it < 3 * gpu_slots; ++it) {
+            int64_t c = hand;
+            hand = (hand + 1) % gpu_slots;
+            if (pin[c] == epoch) continue;
+            if (ref[c]) { ref[c] = 0; continue; }
+            return c;
+        }
+        return -1;
+    }
+};
+
+}  // namespace vt_rt
+
+using vt_rt::fail;
+using namespace vt_rt;
+
+struct vt_file {
+    H5Volume vol;
+};
+
+struct vt_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // octree
+    bool octree_ready = false;
+    int n = 0, max_level = 0;
+    uint8_t* d_child = nullptr;
+    uint64_t level_off[24] = {0};
+    int root_valid = 0;
+    // cache
+    std::unique_ptr<BrickCache> cache;
+    // ray state
+    DevBuf<uint64_t> ray_node;
+    DevBuf<uint8_t> ray_phase;
+    DevBuf<uint32_t> list[2];
+    DevBuf<HitRecord> hits_scratch;
+    DevBuf<uint32_t> req_bits, req_list;
+    DevBuf<uint8_t> trace;
+    FrameCounters* d_ctr = nullptr;
+    FrameCounters* h_ctr = nullptr;
+    PinBuf<uint32_t> h_req;
+    int collect_stats = 0, record_trace = 0;
+    int64_t trace_n = 0;
+    DevBuf<uint8_t> stage_trav;
+    int64_t stage_rays = 0;
+    // last octree build (all levels, host copies)
+    std::vector<std::vector<uint64_t>> built_runs;
+    float built_iso = 0.0f;
+    int built_n = 0, built_dim = 0;
+    int built_real[3] = {0, 0, 0};
+
+    ~vt_context()
+    {
+        cache.reset();
+        if (d_child) (void)hipFree(d_child);
+        ray_node.release(); ray_phase.release(); list[0].release(); list[1].release();
+        hits_scratch.release(); req_bits.release(); req_list.release(); trace.release(); stage_trav.release();
+        if (d_ctr) (void)hipFree(d_ctr);
+        if (h_ctr) (void)hipHostFree(h_ctr);
+        h_req.release();
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+

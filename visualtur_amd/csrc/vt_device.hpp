@@ -141,8 +141,13 @@ __device__ __forceinline__ uint32_t child_mask(const OctreeView& o, uint64_t par
 // Literal restatement of the bubble sort of Octree.cu:665-706 on <= 8 keys.
 // Only used when a key is NaN (degenerate rays); otherwise the rank form
 // below gives the identical stable order.
-__device__ __noinline__ uint32_t bubble_order(const float* tn_in, const float* tf_in, uint32_t valid)
+// Keys arrive by value so that the common path keeps them in registers.
+__device__ __noinline__ uint32_t bubble_order(float n0, float n1, float n2, float n3, float n4, float n5,
+                                              float n6, float n7, float f0, float f1, float f2, float f3,
+                                              float f4, float f5, float f6, float f7, uint32_t valid)
 {
+    const float tn_in[8] = {n0, n1, n2, n3, n4, n5, n6, n7};
+    const float tf_in[8] = {f0, f1, f2, f3, f4, f5, f6, f7};
     float tn[8], tf[8];
     int ks[8];
     int nv = 0;
@@ -196,7 +201,9 @@ __device__ __forceinline__ uint32_t sorted_children(const Ray& r, const OctreeVi
         }
     }
     if (valid == 0) return 0;
-    if (has_nan) return bubble_order(tn, tf, valid);
+    if (has_nan)
+        return bubble_order(tn[0], tn[1], tn[2], tn[3], tn[4], tn[5], tn[6], tn[7], tf[0], tf[1], tf[2], tf[3],
+                            tf[4], tf[5], tf[6], tf[7], valid);
     // Without NaN the comparison (tnear, then tfar) is a strict weak order and
     // bubble sort is stable, so position = number of valid children that sort
     // strictly before, plus equal keys with a smaller k.

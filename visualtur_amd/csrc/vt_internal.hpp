@@ -124,6 +124,8 @@ hipError_t launch_copy_bricks(const float* const* src, const int32_t* dst_slots,
 // table[idx[i]] = val[i]
 hipError_t launch_scatter_i32(int32_t* table, const int64_t* idx, const int32_t* val, int count,
                               hipStream_t s);
+// streaming copy of `bytes` (multiple of 16) used to calibrate HBM counters
+hipError_t launch_copy_probe(const float* src, float* dst, size_t bytes, hipStream_t s);
 // camera rays (Camera.cu:39-74)
 hipError_t launch_camera_rays(const Camera& cam, float* dirs, hipStream_t s);
 

@@ -428,6 +428,22 @@ hipError_t launch_copy_bricks(const float* const* src, const int32_t* dst_slots,
     return hipGetLastError();
 }
 
+// Plain streaming copy: reference point for the HBM byte counters.
+__global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                    size_t n4)
+{
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n4; e += (size_t)gridDim.x * blockDim.x)
+        dst[e] = src[e];
+}
+
+hipError_t launch_copy_probe(const float* src, float* dst, size_t bytes, hipStream_t s)
+{
+    size_t n4 = bytes / 16;
+    hipLaunchKernelGGL(k_copy_probe, dim3(2048), dim3(256), 0, s, reinterpret_cast<const float4*>(src),
+                       reinterpret_cast<float4*>(dst), n4);
+    return hipGetLastError();
+}
+
 __global__ void k_scatter_i32(int32_t* table, const int64_t* idx, const int32_t* val, int count)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;

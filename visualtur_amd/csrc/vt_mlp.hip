@@ -1,0 +1,328 @@
+// vt_mlp.hip -- BF16 MFMA kernels of the brick-visibility MLP (no reference
+// counterpart).  Network: 12 features -> 256 -> 256 -> 256 -> 1 logit, ReLU
+// hidden layers; BF16 weights and activations, FP32 accumulation, FP32
+// master weights updated by Adam.
+//
+// GEMM: C[MxN] = op(A)[MxK] * op(B)[KxN] with v_mfma_f32_16x16x32_bf16.
+// A 256-thread workgroup owns a 64x64 tile of C; each wave64 owns 32x32 of
+// it as 2x2 MFMA blocks.  K advances 32 at a time through LDS.  MFMA operand
+// layout (16x16x32): lane l supplies A[l%16][8*(l/16) .. +7] and
+// B[8*(l/16) .. +7][l%16]; it receives D[4*(l/16) .. +3][l%16].
+#include "vt_mlp.hpp"
+
+namespace vt {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int TM = 64, TN = 64, TK = 32, LDSK = TK + 8;   // +8: 80-byte rows, 16-byte aligned
+
+template <bool TA, bool TB, int EPI>
+__global__ __launch_bounds__(256) void k_gemm(const bf16* __restrict__ A, int lda, const bf16* __restrict__ B,
+                                              int ldb, float* __restrict__ Cf, bf16* __restrict__ Cb, int ldc,
+                                              int M, int N, int K, int kchunk, const float* __restrict__ bias,
+                                              const bf16* __restrict__ mask, int ldm)
+{
+    __shared__ __attribute__((aligned(16))) bf16 As[TM][LDSK];   // As[m][k]
+    __shared__ __attribute__((aligned(16))) bf16 Bs[TN][LDSK];   // Bs[n][k] (B transposed)
+    const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+    const int kbeg = blockIdx.z * kchunk;
+    const int kend = min(K, kbeg + kchunk);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const bf16 zero = (bf16)0.0f;
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    for (int k0 = kbeg; k0 < kend; k0 += TK) {
+        // stage the A and B tiles; the fastest-varying thread index walks the
+        // contiguous dimension of each operand in global memory
+        for (int e = tid; e < TM * TK; e += 256) {
+            int r, c;
+            if (TA) { r = e % TM; c = e / TM; } else { r = e / TK; c = e % TK; }
+            int gm = m0 + r, gk = k0 + c;
+            bf16 v = zero;
+            if (gm < M && gk < kend) v = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
+            As[r][c] = v;
+        }
+        for (int e = tid; e < TN * TK; e += 256) {
+            int r, c;
+            if (TB) { r = e / TK; c = e % TK; } else { r = e % TN; c = e / TN; }
+            int gn = n0 + r, gk = k0 + c;
+            bf16 v = zero;
+            if (gn < N && gk < kend) v = TB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn];
+            Bs[r][c] = v;
+        }
+        __syncthreads();
+        const int kk = 8 * (lane >> 4);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            bf16x8 a = *reinterpret_cast<const bf16x8*>(&As[wm + 16 * i + (lane & 15)][kk]);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                bf16x8 b = *reinterpret_cast<const bf16x8*>(&Bs[wn + 16 * j + (lane & 15)][kk]);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i][j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                int row = m0 + wm + 16 * i + 4 * (lane >> 4) + v;
+                int col = n0 + wn + 16 * j + (lane & 15);
+                if (row >= M || col >= N) continue;
+                float x = acc[i][j][v];
+                size_t o = (size_t)row * ldc + col;
+                if (EPI == EPI_F32) {
+                    Cf[o] = x;
+                } else if (EPI == EPI_ATOMIC_F32) {
+                    atomicAdd(&Cf[o], x);
+                } else if (EPI == EPI_BIAS_RELU_BF16) {
+                    x = x + bias[col];
+                    Cb[o] = (bf16)(x > 0.0f ? x : 0.0f);
+                } else if (EPI == EPI_MASK_BF16) {
+                    float mk = (float)mask[(size_t)row * ldm + col];
+                    Cb[o] = (bf16)(mk > 0.0f ? x : 0.0f);
+                }
+            }
+}
+
+// logit z[i] = H[i,:] . w + b, one wave per row
+__global__ __launch_bounds__(256) void k_logit(const bf16* __restrict__ H, const float* __restrict__ w,
+                                               const float* __restrict__ b, int N, int width, float* __restrict__ z)
+{
+    int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    int lane = threadIdx.x & 63;
+    if (row >= N) return;
+    float s = 0.0f;
+    for (int j = lane; j < width; j += 64) s += (float)H[(size_t)row * width + j] * w[j];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane == 0) z[row] = s + b[0];
+}
+
+// weighted BCE-with-logits: loss_i = w_i (softplus(z) - y z), dz_i = w_i (sigmoid(z) - y) / N
+__global__ void k_bce(const float* __restrict__ z, const float* __restrict__ y, int N, float pos_weight,
+                      float* __restrict__ dz, float* __restrict__ loss_sum)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.0f;
+    if (i < N) {
+        float zi = z[i], yi = y[i];
+        float wi = yi > 0.5f ? pos_weight : 1.0f;
+        float sp = zi > 0.0f ? zi + log1pf(expf(-zi)) : log1pf(expf(zi));
+        l = wi * (sp - yi * zi);
+        float sg = 1.0f / (1.0f + expf(-zi));
+        dz[i] = wi * (sg - yi) / (float)N;
+    }
+    for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(loss_sum, l);
+}
+
+// dH[i][j] = dz[i] * w[j] where H[i][j] > 0 (ReLU derivative), as BF16
+__global__ void k_outer_mask(const float* __restrict__ dz, const float* __restrict__ w, const bf16* __restrict__ H,
+                             int N, int width, bf16* __restrict__ dH)
+{
+    size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (e >= (size_t)N * width) return;
+    int i = (int)(e / width), j = (int)(e % width);
+    float h = (float)H[e];
+    dH[e] = (bf16)(h > 0.0f ? dz[i] * w[j] : 0.0f);
+}
+
+// column sums out[j] (+)= sum_i X[i][j] * (s ? s[i] : 1), one thread per column chunk
+__global__ void k_colsum(const bf16* __restrict__ X, const float* __restrict__ s, int N, int width,
+                         float* __restrict__ out)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int r0 = blockIdx.y * 256, r1 = min(N, r0 + 256);
+    if (j >= width) return;
+    float acc = 0.0f;
+    for (int i = r0; i < r1; ++i) acc += (float)X[(size_t)i * width + j] * (s ? s[i] : 1.0f);
+    atomicAdd(&out[j], acc);
+}
+
+__global__ void k_sum(const float* __restrict__ x, int N, float* __restrict__ out)
+{
+    float a = 0.0f;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) a += x[i];
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, a);
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, bf16* __restrict__ pb, int n, float lr, float b1, float b2,
+                       float eps, float c1, float c2)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float gi = g[i];
+    float mi = b1 * m[i] + (1.0f - b1) * gi;
+    float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    p[i] = pi;
+    if (pb) pb[i] = (bf16)pi;
+}
+
+__global__ void k_to_bf16(const float* __restrict__ x, bf16* __restrict__ y, size_t n)
+{
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) y[i] = (bf16)x[i];
+}
+
+__global__ void k_sigmoid(float* __restrict__ z, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) z[i] = 1.0f / (1.0f + expf(-z[i]));
+}
+
+__global__ void k_pack_x(const float* __restrict__ x, int N, bf16* __restrict__ X)
+{
+    size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (e >= (size_t)N * 32) return;
+    int i = (int)(e / 32), j = (int)(e % 32);
+    X[e] = (bf16)(j < 12 ? x[(size_t)i * 12 + j] : 0.0f);
+}
+
+// 12 features of (camera, brick): camera position and brick centre in
+// units of the cube (centred), view direction, brick level, distance, and
+// the cosine between the view direction and the direction to the brick
+__global__ void k_features(Camera cam, int n, int level_c, const uint64_t* __restrict__ codes, int N,
+                           float* __restrict__ x)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const float D = (float)(1 << n);
+    int bx, by, bz;
+    node_min(codes[i], level_c, n, bx, by, bz);
+    const float half = 0.5f * (float)(1 << (n - level_c));
+    float c0 = (float)bx + half, c1 = (float)by + half, c2 = (float)bz + half;
+    float d0 = c0 - cam.pos[0], d1 = c1 - cam.pos[1], d2 = c2 - cam.pos[2];
+    float dist = sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
+    float inv = dist > 0.0f ? 1.0f / dist : 0.0f;
+    float* o = x + (size_t)i * 12;
+    o[0] = cam.pos[0] / D - 0.5f;
+    o[1] = cam.pos[1] / D - 0.5f;
+    o[2] = cam.pos[2] / D - 0.5f;
+    o[3] = cam.look[0];
+    o[4] = cam.look[1];
+    o[5] = cam.look[2];
+    o[6] = c0 / D - 0.5f;
+    o[7] = c1 / D - 0.5f;
+    o[8] = c2 / D - 0.5f;
+    o[9] = (float)level_c / 10.0f;
+    o[10] = dist / D;
+    o[11] = (d0 * cam.look[0] + d1 * cam.look[1] + d2 * cam.look[2]) * inv;
+}
+
+}  // namespace
+
+hipError_t launch_pack_x(const float* x, int N, bf16* X, hipStream_t s)
+{
+    size_t total = (size_t)N * 32;
+    hipLaunchKernelGGL(k_pack_x, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, x, N, X);
+    return hipGetLastError();
+}
+
+hipError_t launch_features(const Camera& cam, int n, int level_c, const uint64_t* codes, int N, float* x,
+                           hipStream_t s)
+{
+    hipLaunchKernelGGL(k_features, dim3((N + 255) / 256), dim3(256), 0, s, cam, n, level_c, codes, N, x);
+    return hipGetLastError();
+}
+
+template <bool TA, bool TB, int EPI>
+static hipError_t gemm_t(const bf16* A, int lda, const bf16* B, int ldb, float* Cf, bf16* Cb, int ldc, int M, int N,
+                         int K, int splitk, const float* bias, const bf16* mask, int ldm, hipStream_t s)
+{
+    int kchunk = (K + splitk - 1) / splitk;
+    kchunk = (kchunk + TK - 1) / TK * TK;
+    int nz = (K + kchunk - 1) / kchunk;
+    dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, nz);
+    hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), grid, dim3(256), 0, s, A, lda, B, ldb, Cf, Cb, ldc, M, N, K, kchunk,
+                       bias, mask, ldm);
+    return hipGetLastError();
+}
+
+hipError_t launch_gemm(bool ta, bool tb, int epi, const bf16* A, int lda, const bf16* B, int ldb, float* Cf,
+                       bf16* Cb, int ldc, int M, int N, int K, int splitk, const float* bias, const bf16* mask,
+                       int ldm, hipStream_t s)
+{
+#define VT_GEMM_CASE(TA_, TB_, E_) \
+    if (ta == TA_ && tb == TB_ && epi == E_) \
+        return gemm_t<TA_, TB_, E_>(A, lda, B, ldb, Cf, Cb, ldc, M, N, K, splitk, bias, mask, ldm, s);
+    VT_GEMM_CASE(false, false, EPI_F32)
+    VT_GEMM_CASE(false, false, EPI_BIAS_RELU_BF16)
+    VT_GEMM_CASE(false, true, EPI_F32)
+    VT_GEMM_CASE(false, true, EPI_MASK_BF16)
+    VT_GEMM_CASE(true, false, EPI_F32)
+    VT_GEMM_CASE(true, false, EPI_ATOMIC_F32)
+#undef VT_GEMM_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_logit(const bf16* H, const float* w, const float* b, int N, int width, float* z, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_logit, dim3((N + 3) / 4), dim3(256), 0, s, H, w, b, N, width, z);
+    return hipGetLastError();
+}
+
+hipError_t launch_bce(const float* z, const float* y, int N, float pos_weight, float* dz, float* loss_sum,
+                      hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bce, dim3((N + 255) / 256), dim3(256), 0, s, z, y, N, pos_weight, dz, loss_sum);
+    return hipGetLastError();
+}
+
+hipError_t launch_outer_mask(const float* dz, const float* w, const bf16* H, int N, int width, bf16* dH,
+                             hipStream_t s)
+{
+    size_t total = (size_t)N * width;
+    hipLaunchKernelGGL(k_outer_mask, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, dz, w, H, N, width, dH);
+    return hipGetLastError();
+}
+
+hipError_t launch_colsum(const bf16* X, const float* sc, int N, int width, float* out, hipStream_t s)
+{
+    dim3 grid((width + 255) / 256, (N + 255) / 256);
+    hipLaunchKernelGGL(k_colsum, grid, dim3(256), 0, s, X, sc, N, width, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum(const float* x, int N, float* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sum, dim3(64), dim3(256), 0, s, x, N, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(float* p, const float* g, float* m, float* v, bf16* pb, int n, float lr, float b1, float b2,
+                       float eps, int step, hipStream_t s)
+{
+    float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+    hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, s, p, g, m, v, pb, n, lr, b1, b2, eps, c1, c2);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_bf16(const float* x, bf16* y, size_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_sigmoid(float* z, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sigmoid, dim3((n + 255) / 256), dim3(256), 0, s, z, n);
+    return hipGetLastError();
+}
+
+}  // namespace vt

@@ -1,0 +1,260 @@
+// vt_prefetch.cpp -- the learned brick-visibility prefetcher (no reference
+// counterpart): MLP parameters, training step, inference, and the prefetch
+// of the highest-scoring bricks of an upcoming camera into the brick cache.
+#include <math.h>
+
+#include <random>
+
+#include "vt_context.hpp"
+#include "vt_mlp.hpp"
+
+namespace {
+
+constexpr int kIn = 12, kInPad = 32, kHid = 256;
+// internal fp32 parameter layout (W1 padded to 32 input rows)
+constexpr size_t kW1 = 0, kB1 = kW1 + (size_t)kInPad * kHid, kW2 = kB1 + kHid, kB2 = kW2 + (size_t)kHid * kHid,
+                 kW3 = kB2 + kHid, kB3 = kW3 + (size_t)kHid * kHid, kW4 = kB3 + kHid, kB4 = kW4 + kHid,
+                 kParams = kB4 + 1;
+// exposed layout (W1 with its 12 real input rows)
+constexpr size_t kExposed = kParams - (size_t)(kInPad - kIn) * kHid;
+
+}  // namespace
+
+struct vt_mlp {
+    vt_context* ctx = nullptr;
+    int64_t max_batch = 0;
+    int step = 0;
+    DevBuf<float> P, G, Mo, Vo;   // master params, grads, Adam moments
+    DevBuf<bf16> Pb;              // BF16 copies (same layout)
+    DevBuf<bf16> X, H1, H2, H3, dH1, dH2, dH3;
+    DevBuf<float> z, dz, y, loss;
+    DevBuf<float> feats;
+    DevBuf<uint64_t> codes;
+    ~vt_mlp()
+    {
+        P.release(); G.release(); Mo.release(); Vo.release(); Pb.release();
+        X.release(); H1.release(); H2.release(); H3.release(); dH1.release(); dH2.release(); dH3.release();
+        z.release(); dz.release(); y.release(); loss.release(); feats.release(); codes.release();
+    }
+};
+
+namespace {
+
+int mlp_check(vt_mlp* m, int64_t N)
+{
+    if (!m) return fail(VT_E_ARG, "mlp is NULL");
+    if (N <= 0 || N > m->max_batch) return fail(VT_E_ARG, "batch size must be in [1, max_batch]");
+    return VT_OK;
+}
+
+// expose <-> internal parameter layouts
+void to_internal(const float* in, std::vector<float>& out)
+{
+    out.assign(kParams, 0.0f);
+    memcpy(out.data() + kW1, in, (size_t)kIn * kHid * 4);
+    memcpy(out.data() + kB1, in + (size_t)kIn * kHid, (kParams - kB1) * 4);
+}
+
+void to_exposed(const std::vector<float>& in, float* out)
+{
+    memcpy(out, in.data() + kW1, (size_t)kIn * kHid * 4);
+    memcpy(out + (size_t)kIn * kHid, in.data() + kB1, (kParams - kB1) * 4);
+}
+
+int refresh_bf16(vt_mlp* m)
+{
+    HIP_TRY(launch_to_bf16(m->P.p, m->Pb.p, kParams, m->ctx->stream));
+    return VT_OK;
+}
+
+int forward(vt_mlp* m, const float* d_x, int N)
+{
+    hipStream_t s = m->ctx->stream;
+    const bf16* Wb = m->Pb.p;
+    const float* P = m->P.p;
+    HIP_TRY(launch_pack_x(d_x, N, m->X.p, s));
+    HIP_TRY(launch_gemm(false, false, EPI_BIAS_RELU_BF16, m->X.p, kInPad, Wb + kW1, kHid, nullptr, m->H1.p, kHid, N,
+                        kHid, kInPad, 1, P + kB1, nullptr, 0, s));
+    HIP_TRY(launch_gemm(false, false, EPI_BIAS_RELU_BF16, m->H1.p, kHid, Wb + kW2, kHid, nullptr, m->H2.p, kHid, N,
+                        kHid, kHid, 1, P + kB2, nullptr, 0, s));
+    HIP_TRY(launch_gemm(false, false, EPI_BIAS_RELU_BF16, m->H2.p, kHid, Wb + kW3, kHid, nullptr, m->H3.p, kHid, N,
+                        kHid, kHid, 1, P + kB3, nullptr, 0, s));
+    HIP_TRY(launch_logit(m->H3.p, P + kW4, P + kB4, N, kHid, m->z.p, s));
+    return VT_OK;
+}
+
+// gradients of the mean weighted BCE into m->G (zeroed here); loss into m->loss[0]
+int backward(vt_mlp* m, const float* d_y, int N, float pos_weight)
+{
+    hipStream_t s = m->ctx->stream;
+    const bf16* Wb = m->Pb.p;
+    const float* P = m->P.p;
+    float* G = m->G.p;
+    HIP_TRY(hipMemsetAsync(G, 0, kParams * 4, s));
+    HIP_TRY(hipMemsetAsync(m->loss.p, 0, 4, s));
+    HIP_TRY(launch_bce(m->z.p, d_y, N, pos_weight, m->dz.p, m->loss.p, s));
+    HIP_TRY(launch_colsum(m->H3.p, m->dz.p, N, kHid, G + kW4, s));
+    HIP_TRY(launch_sum(m->dz.p, N, G + kB4, s));
+    HIP_TRY(launch_outer_mask(m->dz.p, P + kW4, m->H3.p, N, kHid, m->dH3.p, s));
+    const int split = std::max(1, N / 512);
+    // layer 3
+    HIP_TRY(launch_gemm(true, false, EPI_ATOMIC_F32, m->H2.p, kHid, m->dH3.p, kHid, G + kW3, nullptr, kHid, kHid, kHid,
+                        N, split, nullptr, nullptr, 0, s));
+    HIP_TRY(launch_colsum(m->dH3.p, nullptr, N, kHid, G + kB3, s));
+    HIP_TRY(launch_gemm(false, true, EPI_MASK_BF16, m->dH3.p, kHid, Wb + kW3, kHid, nullptr, m->dH2.p, kHid, N, kHid,
+                        kHid, 1, nullptr, m->H2.p, kHid, s));
+    // layer 2
+    HIP_TRY(launch_gemm(true, false, EPI_ATOMIC_F32, m->H1.p, kHid, m->dH2.p, kHid, G + kW2, nullptr, kHid, kHid, kHid,
+                        N, split, nullptr, nullptr, 0, s));
+    HIP_TRY(launch_colsum(m->dH2.p, nullptr, N, kHid, G + kB2, s));
+    HIP_TRY(launch_gemm(false, true, EPI_MASK_BF16, m->dH2.p, kHid, Wb + kW2, kHid, nullptr, m->dH1.p, kHid, N, kHid,
+                        kHid, 1, nullptr, m->H1.p, kHid, s));
+    // layer 1
+    HIP_TRY(launch_gemm(true, false, EPI_ATOMIC_F32, m->X.p, kInPad, m->dH1.p, kHid, G + kW1, nullptr, kHid, kInPad,
+                        kHid, N, split, nullptr, nullptr, 0, s));
+    HIP_TRY(launch_colsum(m->dH1.p, nullptr, N, kHid, G + kB1, s));
+    return VT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vt_mlp_num_params(void) { return (int)kExposed; }
+
+int vt_mlp_create(vt_context* ctx, uint64_t seed, int64_t max_batch, vt_mlp** out)
+{
+    if (!ctx || !out) return fail(VT_E_ARG, "NULL argument");
+    if (max_batch <= 0 || max_batch > (1 << 24)) return fail(VT_E_ARG, "max_batch must be in [1, 2^24]");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::unique_ptr<vt_mlp> m(new vt_mlp);
+    m->ctx = ctx;
+    m->max_batch = max_batch;
+    size_t N = (size_t)max_batch;
+    bool ok = m->P.ensure(kParams) == hipSuccess && m->G.ensure(kParams) == hipSuccess &&
+              m->Mo.ensure(kParams) == hipSuccess && m->Vo.ensure(kParams) == hipSuccess &&
+              m->Pb.ensure(kParams) == hipSuccess && m->X.ensure(N * kInPad) == hipSuccess &&
+              m->H1.ensure(N * kHid) == hipSuccess && m->H2.ensure(N * kHid) == hipSuccess &&
+              m->H3.ensure(N * kHid) == hipSuccess && m->dH1.ensure(N * kHid) == hipSuccess &&
+              m->dH2.ensure(N * kHid) == hipSuccess && m->dH3.ensure(N * kHid) == hipSuccess &&
+              m->z.ensure(N) == hipSuccess && m->dz.ensure(N) == hipSuccess && m->y.ensure(N) == hipSuccess &&
+              m->loss.ensure(1) == hipSuccess && m->feats.ensure(N * kIn) == hipSuccess &&
+              m->codes.ensure(N) == hipSuccess;
+    if (!ok) return fail(VT_E_NOMEM, "mlp buffers");
+    // He-uniform initialisation of the weights, zero biases
+    std::mt19937_64 rng(seed);
+    std::vector<float> exposed(kExposed, 0.0f);
+    auto fill = [&](size_t off, int fan_in, size_t count) {
+        std::uniform_real_distribution<float> u(-sqrtf(6.0f / fan_in), sqrtf(6.0f / fan_in));
+        for (size_t i = 0; i < count; ++i) exposed[off + i] = u(rng);
+    };
+    const size_t e_w1 = 0, e_w2 = (size_t)kIn * kHid + kHid, e_w3 = e_w2 + (size_t)kHid * kHid + kHid,
+                 e_w4 = e_w3 + (size_t)kHid * kHid + kHid;
+    fill(e_w1, kIn, (size_t)kIn * kHid);
+    fill(e_w2, kHid, (size_t)kHid * kHid);
+    fill(e_w3, kHid, (size_t)kHid * kHid);
+    fill(e_w4, kHid, kHid);
+    std::vector<float> internal;
+    to_internal(exposed.data(), internal);
+    HIP_TRY(hipMemcpy(m->P.p, internal.data(), kParams * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(m->Mo.p, 0, kParams * 4));
+    HIP_TRY(hipMemset(m->Vo.p, 0, kParams * 4));
+    if (int rc = refresh_bf16(m.get())) return rc;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    *out = m.release();
+    return VT_OK;
+}
+
+int vt_mlp_destroy(vt_mlp* m)
+{
+    if (m) {
+        (void)hipStreamSynchronize(m->ctx->stream);
+        delete m;
+    }
+    return VT_OK;
+}
+
+int vt_mlp_get_params(vt_mlp* m, float* out)
+{
+    if (!m || !out) return fail(VT_E_ARG, "NULL argument");
+    std::vector<float> internal(kParams);
+    HIP_TRY(hipMemcpyAsync(internal.data(), m->P.p, kParams * 4, hipMemcpyDeviceToHost, m->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    to_exposed(internal, out);
+    return VT_OK;
+}
+
+int vt_mlp_set_params(vt_mlp* m, const float* in)
+{
+    if (!m || !in) return fail(VT_E_ARG, "NULL argument");
+    std::vector<float> internal;
+    to_internal(in, internal);
+    HIP_TRY(hipMemcpyAsync(m->P.p, internal.data(), kParams * 4, hipMemcpyHostToDevice, m->ctx->stream));
+    HIP_TRY(hipMemsetAsync(m->Mo.p, 0, kParams * 4, m->ctx->stream));
+    HIP_TRY(hipMemsetAsync(m->Vo.p, 0, kParams * 4, m->ctx->stream));
+    m->step = 0;
+    if (int rc = refresh_bf16(m)) return rc;
+    HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    return VT_OK;
+}
+
+int vt_mlp_forward(vt_mlp* m, const float* d_x, int64_t N, float* d_logits)
+{
+    if (int rc = mlp_check(m, N)) return rc;
+    if (!d_x || !d_logits) return fail(VT_E_ARG, "NULL argument");
+    if (int rc = forward(m, d_x, (int)N)) return rc;
+    HIP_TRY(hipMemcpyAsync(d_logits, m->z.p, (size_t)N * 4, hipMemcpyDeviceToDevice, m->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    return VT_OK;
+}
+
+int vt_mlp_gradients(vt_mlp* m, const float* d_x, const float* d_y, int64_t N, float pos_weight, float* grad_out,
+                     float* loss_out)
+{
+    if (int rc = mlp_check(m, N)) return rc;
+    if (!d_x || !d_y || !grad_out) return fail(VT_E_ARG, "NULL argument");
+    if (int rc = forward(m, d_x, (int)N)) return rc;
+    if (int rc = backward(m, d_y, (int)N, pos_weight)) return rc;
+    std::vector<float> internal(kParams);
+    float loss = 0.0f;
+    HIP_TRY(hipMemcpyAsync(internal.data(), m->G.p, kParams * 4, hipMemcpyDeviceToHost, m->ctx->stream));
+    HIP_TRY(hipMemcpyAsync(&loss, m->loss.p, 4, hipMemcpyDeviceToHost, m->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    to_exposed(internal, grad_out);
+    if (loss_out) *loss_out = loss / (float)N;
+    return VT_OK;
+}
+
+int vt_mlp_train_step(vt_mlp* m, const float* d_x, const float* d_y, int64_t N, float lr, float pos_weight,
+                      float* loss_out)
+{
+    if (int rc = mlp_check(m, N)) return rc;
+    if (!d_x || !d_y) return fail(VT_E_ARG, "NULL argument");
+    if (int rc = forward(m, d_x, (int)N)) return rc;
+    if (int rc = backward(m, d_y, (int)N, pos_weight)) return rc;
+    m->step++;
+    HIP_TRY(launch_adam(m->P.p, m->G.p, m->Mo.p, m->Vo.p, m->Pb.p, (int)kParams, lr, 0.9f, 0.999f, 1e-8f, m->step,
+                        m->ctx->stream));
+    if (loss_out) {
+        float loss = 0.0f;
+        HIP_TRY(hipMemcpyAsync(&loss, m->loss.p, 4, hipMemcpyDeviceToHost, m->ctx->stream));
+        HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+        *loss_out = loss / (float)N;
+    }
+    return VT_OK;
+}
+
+int vt_mlp_features(vt_mlp* m, const vt_camera* cam, int32_t nLevels, int32_t levelC, const uint64_t* d_codes,
+                    int64_t N, float* d_x)
+{
+    if (int rc = mlp_check(m, N)) return rc;
+    if (!cam || !d_codes || !d_x) return fail(VT_E_ARG, "NULL argument");
+    if (levelC < 0 || levelC > nLevels || nLevels > 20) return fail(VT_E_ARG, "bad levels");
+    Camera c;
+    memcpy(&c, cam, sizeof c);
+    HIP_TRY(launch_features(c, nLevels, levelC, d_codes, (int)N, d_x, m->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    return VT_OK;
+}
+
+}  // extern "C"
