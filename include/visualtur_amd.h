@@ -291,6 +291,55 @@ int vt_mlp_train_step(vt_mlp *m, const float *d_x, const float *d_y, int64_t N, 
 int vt_mlp_features(vt_mlp *m, const vt_camera *cam, int32_t nLevels, int32_t levelC, const uint64_t *d_codes,
                     int64_t N, float *d_x);
 
+/* Score every candidate brick (the bricks holding octree nodes, i.e. the
+ * level-levelC run list) for the camera with the MLP, then load the
+ * highest-scoring non-resident ones with probability >= min_prob, at most
+ * `budget`, into free slots or in place of resident bricks that score
+ * lower.  Hits are never affected: the render still decides every brick
+ * it marches.  *loaded (may be NULL) receives the number of bricks loaded. */
+int vt_prefetch(vt_context *ctx, vt_mlp *m, const vt_camera *cam, int64_t budget, float min_prob, int64_t *loaded);
+
+/* --------------------------------------------------------- serving loop */
+
+/* Render-request serving (no reference counterpart).  Scenes keep every
+ * brick resident in HBM plus a min/max pyramid over the octree levels, so a
+ * request may carry its own isovalue (a node is valid for iso t exactly
+ * when min < t <= max over its zero-padded samples).  vt_server_run admits
+ * requests as they arrive while their outputs (48 B per ray) fit the HBM
+ * budget, and renders 16x16 tiles of all admitted requests in one launch
+ * (a hipGraph of fixed shape, replayed per batch): continuous batching.
+ * Mode 1 renders one request per launch in arrival order (the plain
+ * per-request baseline). */
+typedef struct vt_server vt_server;
+typedef struct {
+    int32_t scene;          /* from vt_server_add_scene */
+    vt_camera cam;
+    float iso;
+    float light[3];
+    float color_scale;
+    double arrival_s;       /* arrival time, seconds after the run starts */
+    float *h_rgba;          /* optional host copy of the W*H float4 pixels */
+    vt_hit *h_hits;         /* optional host copy of the hit records */
+} vt_request;
+typedef struct {
+    double admit_s, done_s; /* seconds after the run starts */
+    double latency_ms;      /* done - arrival */
+    int32_t launches;       /* launches the request took part in */
+    int32_t pad_;
+} vt_request_result;
+typedef struct {
+    int64_t requests, launches, tiles, rays;
+    double wall_s, busy_s;
+} vt_serve_stats;
+int vt_server_create(vt_context *ctx, int64_t hbm_budget_bytes, int32_t max_tiles, vt_server **out);
+int vt_server_destroy(vt_server *sv);
+/* A volume in HBM (C order [X][Y][Z]) becomes a scene: octree levels
+ * 0..levelO, bricks of level levelC with cubeInc halo, all resident. */
+int vt_server_add_scene(vt_server *sv, const float *d_vol, int32_t X, int32_t Y, int32_t Z, int32_t levelO,
+                        int32_t levelC, int32_t cubeInc, int32_t *scene_out);
+int vt_server_run(vt_server *sv, const vt_request *reqs, int64_t n, int32_t mode, vt_request_result *results,
+                  vt_serve_stats *stats);
+
 /* Streaming device copy (no reference counterpart): a kernel of known
  * traffic, read bytes + written bytes, that calibrates HBM byte counters. */
 int vt_copy_probe(vt_context *ctx, const void *d_src, void *d_dst, size_t bytes);

@@ -1,0 +1,68 @@
+"""The serving loop: every request's result is the same alone, in a batched
+mix, and as the oracle renders it; per-request isovalues come from the
+min/max pyramid and equal the builder's octree for that isovalue."""
+import numpy as np
+import pytest
+
+from conftest import blobs_volume
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mixed_requests_same_alone_and_in_the_mix(vt, vto, ctx):
+    from visualtur_amd.prefetch import random_walk
+
+    vol_a = blobs_volume(64, 16, seed=5)
+    vol_b = blobs_volume(64, 10, seed=6, dims=(64, 48, 40)) - 0.2
+    sv = vt.Server(ctx, hbm_budget=64 << 20, max_tiles=96)
+    da, db = ctx.to_device(vol_a), ctx.to_device(vol_b)
+    sa = sv.add_scene(da, vol_a.shape, 4, 2)
+    sb = sv.add_scene(db, vol_b.shape, 3, 3)
+    reqs = []
+    sizes = [(64, 64), (96, 80), (37, 23), (160, 120), (17, 200)]
+    rng = np.random.default_rng(3)
+    for k in range(14):
+        W, H = sizes[k % len(sizes)]
+        scene = sa if k % 3 else sb
+        iso = [0.2, 0.45, 0.3][k % 3] if scene == sa else [0.0, 0.1][k % 2]
+        cam = random_walk(W, H, 64, 1, seed=100 + k)[0]
+        reqs.append({"scene": scene, "cam": cam, "iso": iso, "arrival_s": 0.001 * (k // 5), "keep": True})
+    mix, st_mix, out_mix = sv.run(reqs, mode=0)
+    alone, st_alone, out_alone = sv.run(reqs, mode=1)
+    assert st_mix["launches"] < st_alone["launches"]   # requests shared launches
+    assert max(r["launches"] for r in mix) >= 1
+    for k, q in enumerate(reqs):
+        (rg, hg), (ra, ha) = out_mix[k], out_alone[k]
+        assert np.array_equal(hg, ha), k
+        assert np.array_equal(rg.view(np.uint32), ra.view(np.uint32)), k
+        # the oracle, with the reference builder's octree for this isovalue
+        vol = vol_a if q["scene"] == sa else vol_b
+        lo, lc = (4, 2) if q["scene"] == sa else (3, 3)
+        n, runs = vto.build_octree(vol, q["iso"], literal=False)
+        rc, hc, _ = vto.render(vto.Volume(vol), vto.Octree(runs, n, lo), q["cam"], vto.params(lo, lc, q["iso"]))
+        assert np.array_equal(hg["node"], hc["node"]) and np.array_equal(hg["voxel"], hc["voxel"]), k
+        nan = np.isnan(rc)
+        assert np.array_equal(np.isnan(rg), nan)
+        assert np.all(np.abs(rg[~nan] - rc[~nan]) <= 1.0 / 255), k
+    sv.close()
+    da.free()
+    db.free()
+
+
+def test_admission_respects_the_budget(vt, ctx):
+    vol = blobs_volume(32, 6, seed=1)
+    sv = vt.Server(ctx, hbm_budget=2 << 20, max_tiles=64)
+    d = ctx.to_device(vol)
+    s = sv.add_scene(d, vol.shape, 3, 2)
+    from visualtur_amd.prefetch import random_walk
+    # 128x128 needs 786 KB of outputs: at most two fit a 2 MB budget at once
+    reqs = [{"scene": s, "cam": random_walk(128, 128, 32, 1, seed=k)[0], "iso": 0.3, "arrival_s": 0.0}
+            for k in range(6)]
+    res, st, _ = sv.run(reqs, mode=0)
+    spans = sorted((r["admit_s"], r["done_s"]) for r in res)
+    for t in np.linspace(0, max(e for _, e in spans), 200):
+        assert sum(1 for a, e in spans if a <= t < e) <= 2
+    with pytest.raises(vt.VTError):
+        sv.run([{"scene": s, "cam": random_walk(1024, 1024, 32, 1, seed=0)[0], "iso": 0.3}], mode=0)
+    sv.close()
+    d.free()

@@ -66,6 +66,25 @@ class GenParams(ctypes.Structure):
 
 GEN_BLOBS, GEN_FBM, GEN_SDF, GEN_FOURIER = 0, 1, 2, 3
 
+class Request(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_int32), ("cam", Camera), ("iso", ctypes.c_float), ("light", ctypes.c_float * 3),
+                ("color_scale", ctypes.c_float), ("arrival_s", ctypes.c_double), ("h_rgba", ctypes.c_void_p),
+                ("h_hits", ctypes.c_void_p)]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [("admit_s", ctypes.c_double), ("done_s", ctypes.c_double), ("latency_ms", ctypes.c_double),
+                ("launches", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("requests", ctypes.c_int64), ("launches", ctypes.c_int64), ("tiles", ctypes.c_int64),
+                ("rays", ctypes.c_int64), ("wall_s", ctypes.c_double), ("busy_s", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 HIT_DTYPE = np.dtype([("node", "<u8"), ("brick", "<u8"), ("voxel", "<i4", (3,)), ("depth", "<f4")])
 assert HIT_DTYPE.itemsize == 32
 
@@ -110,7 +129,14 @@ _SIGS = {
     "vt_volume_write_hdf5": (ctypes.c_int, [_P, _P, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_char_p,
                                             ctypes.c_char_p, ctypes.c_int32]),
     "vt_copy_probe": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
+    "vt_server_create": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(_P)]),
+    "vt_server_destroy": (ctypes.c_int, [_P]),
+    "vt_server_add_scene": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
+    "vt_server_run": (ctypes.c_int, [_P, _P, ctypes.c_int64, ctypes.c_int32, _P, ctypes.POINTER(ServeStats)]),
     "vt_mlp_num_params": (ctypes.c_int, []),
+    "vt_prefetch": (ctypes.c_int, [_P, _P, ctypes.POINTER(Camera), ctypes.c_int64, ctypes.c_float,
+                                   ctypes.POINTER(ctypes.c_int64)]),
     "vt_mlp_create": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(_P)]),
     "vt_mlp_destroy": (ctypes.c_int, [_P]),
     "vt_mlp_get_params": (ctypes.c_int, [_P, _P]),
@@ -415,9 +441,62 @@ class MLP:
         _check(lib().vt_mlp_train_step(self.handle, d_x.ptr, d_y.ptr, n, lr, pos_weight, ctypes.byref(loss)))
         return loss.value
 
+    def prefetch(self, cam: Camera, budget: int, min_prob: float = 0.5) -> int:
+        """Load the bricks this model expects `cam` to need (vt_prefetch)."""
+        n = ctypes.c_int64()
+        _check(lib().vt_prefetch(self.ctx.handle, self.handle, ctypes.byref(cam), budget, min_prob,
+                                 ctypes.byref(n)), self.ctx.handle)
+        return n.value
+
     def features(self, cam: Camera, n_levels: int, level_c: int, d_codes: DeviceBuffer, n: int,
                  d_x: DeviceBuffer) -> None:
         _check(lib().vt_mlp_features(self.handle, ctypes.byref(cam), n_levels, level_c, d_codes.ptr, n, d_x.ptr))
+
+
+class Server:
+    """Render-request serving loop (vt_server_*)."""
+
+    def __init__(self, ctx: Context, hbm_budget: int = 1 << 30, max_tiles: int = 16384):
+        h = ctypes.c_void_p()
+        _check(lib().vt_server_create(ctx.handle, hbm_budget, max_tiles, ctypes.byref(h)), ctx.handle)
+        self.ctx, self.handle = ctx, h.value
+
+    def add_scene(self, d_vol: DeviceBuffer, dims, level_o: int, level_c: int, cube_inc: int = 3) -> int:
+        sid = ctypes.c_int32()
+        _check(lib().vt_server_add_scene(self.handle, d_vol.ptr, *[int(d) for d in dims], level_o, level_c,
+                                         cube_inc, ctypes.byref(sid)), self.ctx.handle)
+        return sid.value
+
+    def run(self, requests, mode: int = 0):
+        """requests: list of dicts {scene, cam, iso, arrival_s, keep (bool)}.  Returns
+        (results as a list of dicts, stats dict, outputs {index: (rgba, hits)})."""
+        n = len(requests)
+        arr = (Request * max(n, 1))()
+        outs = {}
+        for k, q in enumerate(requests):
+            r = arr[k]
+            r.scene = q["scene"]
+            r.cam = q["cam"]
+            r.iso = q["iso"]
+            r.light[:] = list(q.get("light", (0.0, 512.0, 0.0)))
+            r.color_scale = q.get("color_scale", 256.0)
+            r.arrival_s = q.get("arrival_s", 0.0)
+            if q.get("keep"):
+                R = q["cam"].W * q["cam"].H
+                rgba, hits = np.zeros((R, 4), np.float32), np.zeros(R, HIT_DTYPE)
+                outs[k] = (rgba, hits)
+                r.h_rgba, r.h_hits = rgba.ctypes.data, hits.ctypes.data
+        res = (RequestResult * max(n, 1))()
+        st = ServeStats()
+        _check(lib().vt_server_run(self.handle, arr, n, mode, res, ctypes.byref(st)), self.ctx.handle)
+        results = [{"admit_s": res[k].admit_s, "done_s": res[k].done_s, "latency_ms": res[k].latency_ms,
+                    "launches": res[k].launches} for k in range(n)]
+        return results, st.as_dict(), outs
+
+    def close(self) -> None:
+        if self.handle:
+            lib().vt_server_destroy(self.handle)
+            self.handle = None
 
 
 def render_params(level_o: int, level_c: int, iso: float, light=(0.0, 512.0, 0.0),

@@ -28,7 +28,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unus
 HIP = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 INCS = [f"-I{INCLUDE}", f"-I{CSRC}", f"-I{HDF5_INC}"]
 
-HIP_SOURCES = ["vt_kernels.hip", "vt_build.hip", "vt_mlp.hip", "vt_runtime.cpp", "vt_prefetch.cpp"]
+HIP_SOURCES = ["vt_kernels.hip", "vt_build.hip", "vt_mlp.hip", "vt_runtime.cpp", "vt_prefetch.cpp",
+               "vt_serve.cpp"]
 HOST_SOURCES = ["vt_hdf5.cpp"]
 CLASS_SOURCES = ["vt_classes.cpp"]
 

@@ -16,6 +16,7 @@
 // The level bitmaps double as the traversal's membership tables: the bitmap
 // of level l viewed as bytes is one byte per level-(l-1) parent.
 #include "vt_build.hpp"
+#include "vt_internal.hpp"
 
 #pragma clang fp contract(off)
 
@@ -244,7 +245,44 @@ __global__ __launch_bounds__(kScanBlock) void k_run_write(const uint64_t* __rest
 
 __global__ void k_clear_top_bit(uint64_t* w) { *w &= ~(1ull << 63); }
 
+__global__ __launch_bounds__(256) void k_node_minmax(const float* __restrict__ vol, int X, int Y, int Z, int n,
+                                                     int level, float* __restrict__ mn, float* __restrict__ mx)
+{
+    __shared__ float smin[4], smax[4];
+    uint64_t idx = blockIdx.x;   // node index within the level
+    uint64_t code = (1ull << (3 * level)) | idx;
+    int x0, y0, z0;
+    node_min(code, level, n, x0, y0, z0);
+    const int s1 = (1 << (n - level)) + 1;
+    const int total = s1 * s1 * s1;
+    float lo = 3.0e38f, hi = -3.0e38f;
+    for (int e = threadIdx.x; e < total; e += 256) {
+        int z = e % s1, y = (e / s1) % s1, x = e / (s1 * s1);
+        float v = vox(vol, X, Y, Z, x0 + x, y0 + y, z0 + z);
+        lo = fminf(lo, v);
+        hi = fmaxf(hi, v);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = fminf(lo, __shfl_xor(lo, off));
+        hi = fmaxf(hi, __shfl_xor(hi, off));
+    }
+    if ((threadIdx.x & 63) == 0) { smin[threadIdx.x >> 6] = lo; smax[threadIdx.x >> 6] = hi; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        mn[idx] = fminf(fminf(smin[0], smin[1]), fminf(smin[2], smin[3]));
+        mx[idx] = fmaxf(fmaxf(smax[0], smax[1]), fmaxf(smax[2], smax[3]));
+    }
+}
+
 }  // namespace
+
+hipError_t launch_node_minmax(const float* vol, int X, int Y, int Z, int n, int level, float* mn, float* mx,
+                              hipStream_t s)
+{
+    hipLaunchKernelGGL(k_node_minmax, dim3((unsigned)(1ull << (3 * level))), dim3(256), 0, s, vol, X, Y, Z, n, level,
+                       mn, mx);
+    return hipGetLastError();
+}
 
 hipError_t launch_clear_top_bit(uint64_t* word, hipStream_t s)
 {

@@ -333,6 +333,12 @@ struct vt_context {
     int64_t trace_n = 0;
     DevBuf<uint8_t> stage_trav;
     int64_t stage_rays = 0;
+    // host copies of the uploaded run lists (levels 0..max_level), and the
+    // candidate brick codes derived from them for the prefetcher
+    std::vector<std::vector<uint64_t>> up_runs;
+    int cand_level = -1;
+    std::vector<uint64_t> cand_codes;
+    DevBuf<uint64_t> d_cand;
     // last octree build (all levels, host copies)
     std::vector<std::vector<uint64_t>> built_runs;
     float built_iso = 0.0f;
@@ -345,6 +351,7 @@ struct vt_context {
         if (d_child) (void)hipFree(d_child);
         ray_node.release(); ray_phase.release(); list[0].release(); list[1].release();
         hits_scratch.release(); req_bits.release(); req_list.release(); trace.release(); stage_trav.release();
+        d_cand.release();
         if (d_ctr) (void)hipFree(d_ctr);
         if (h_ctr) (void)hipHostFree(h_ctr);
         h_req.release();

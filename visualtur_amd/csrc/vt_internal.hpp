@@ -16,6 +16,22 @@ struct HitRecord {        // 32 bytes, one per ray
 };
 static_assert(sizeof(HitRecord) == 32, "hit record layout");
 
+// the brick pool as seen by the march
+struct BrickView {
+    const int32_t* brick_slot;   // brick index -> slot, -1 when absent
+    const float* pool;
+    int S, inc;
+    size_t slot_elems;
+    int level_c, shift;          // shift = 3*(levelO - levelC)
+    uint64_t brick_base;         // 8^levelC
+};
+
+struct Shading {
+    float iso, lx, ly, lz, color_scale;
+};
+
+enum WalkResult : int { WALK_MISS = 0, WALK_HIT = 1, WALK_ABSENT = 2 };
+
 enum RayPhase : uint8_t { PHASE_START = 0, PHASE_NEED = 1, PHASE_DONE = 2 };
 
 // per-frame device counters (one 64-byte block)
@@ -107,7 +123,34 @@ struct CubeArgs {
     uint32_t* err;
 };
 
+// one admitted render request of the serving loop
+struct ServeJob {
+    Camera cam;
+    OctreeView oct;      // membership tables for this request's isovalue
+    BrickView bv;
+    Shading sh;
+    float4* rgba;
+    HitRecord* hits;
+    int tiles_x;
+};
+
+struct ServeTile {
+    uint32_t job;
+    uint32_t tile;
+};
+
+struct ServeCounters {
+    uint32_t nwork;      // tiles in this launch
+    uint32_t err;
+};
+
 // launchers (vt_kernels.hip)
+hipError_t launch_serve_pass(const ServeJob* jobs, const ServeTile* work, ServeCounters* ctr, int max_tiles,
+                             hipStream_t s);
+// vt_build.hip: min and max over the (s+1)^3 zero-padded samples of every
+// node of one level (s = 2^(n-level)); one workgroup per node
+hipError_t launch_node_minmax(const float* vol, int X, int Y, int Z, int n, int level, float* mn, float* mx,
+                              hipStream_t s);
 hipError_t launch_render_cube(const CubeArgs& a, hipStream_t s);
 hipError_t launch_stage_traverse(const StageTraverseArgs& a, hipStream_t s);
 hipError_t launch_stage_raycast(const StageRaycastArgs& a, hipStream_t s);

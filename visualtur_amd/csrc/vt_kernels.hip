@@ -31,18 +31,56 @@ __device__ __forceinline__ uint32_t wave_atomic_add(uint32_t* ctr, bool pred)
     return base + (uint32_t)__popcll(lower);
 }
 
-__device__ __forceinline__ void write_miss(const RenderArgs& a, uint32_t id)
-{
-    a.rgba[id] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);   // rayCaster.cu:146-153
-    HitRecord h;
-    h.node = 0; h.brick = 0; h.voxel[0] = h.voxel[1] = h.voxel[2] = 0; h.depth = 0.0f;
-    a.hits[id] = h;
-    a.ray_phase[id] = PHASE_DONE;
-}
-
 __device__ __forceinline__ int floor_to_int(float x) { return cvt_rzi(floorf(x)); }
 
 }  // namespace
+
+// Walks one ray from `node` (the next node to march; 0 = exhausted) until it
+// hits, runs out of nodes, or needs a brick that is not resident.
+//   WALK_HIT    : out/h hold the shaded hit
+//   WALK_MISS   : out/h hold the white background and an empty record
+//   WALK_ABSENT : node = the node whose brick (index bidx) is missing
+__device__ __forceinline__ int walk_ray(const Ray& r, const OctreeView& oct, const BrickView& bv, const Shading& sh,
+                                        uint64_t& node, uint8_t* slot_ref, uint8_t* brick_trace, float4& out,
+                                        HitRecord& h, uint64_t& bidx, uint32_t& err, uint32_t& nsamples,
+                                        uint32_t& nnodes)
+{
+    const int n = oct.n, levelO = oct.final_level;
+    const int side = 1 << (n - levelO);
+    for (;;) {
+        if (node == 0) {
+            out = make_float4(1.0f, 1.0f, 1.0f, 1.0f);   // rayCaster.cu:146-153
+            h.node = 0; h.brick = 0; h.voxel[0] = h.voxel[1] = h.voxel[2] = 0; h.depth = 0.0f;
+            return WALK_MISS;
+        }
+        uint64_t brick = node >> bv.shift;
+        bidx = brick - bv.brick_base;
+        int slot = bv.brick_slot[bidx];
+        if (slot < 0) return WALK_ABSENT;
+        if (slot_ref) slot_ref[slot] = 1;
+        if (brick_trace) brick_trace[bidx] = 1;
+        int nx, ny, nz, bx, by, bz;
+        node_min(node, levelO, n, nx, ny, nz);
+        node_min(brick, bv.level_c, n, bx, by, bz);
+        bx -= bv.inc; by -= bv.inc; bz -= bv.inc;
+        const float* sp = bv.pool + (size_t)slot * bv.slot_elems;
+        float Xx, Xy, Xz;
+        nnodes += 1;
+        if (march_node(r, nx, ny, nz, side, sp, bv.S, bx, by, bz, sh.iso, Xx, Xy, Xz, err, nsamples)) {
+            shade(sp, bv.S, bx, by, bz, Xx, Xy, Xz, sh.lx, sh.ly, sh.lz, sh.color_scale, out, err);
+            nsamples += 6;
+            h.node = node;
+            h.brick = brick;
+            h.voxel[0] = floor_to_int(Xx);
+            h.voxel[1] = floor_to_int(Xy);
+            h.voxel[2] = floor_to_int(Xz);
+            float dx = Xx - r.ox, dy = Xy - r.oy, dz = Xz - r.oz;
+            h.depth = sqrtf(dx * dx + dy * dy + dz * dz);
+            return WALK_HIT;
+        }
+        node = next_node(r, oct, node, false);
+    }
+}
 
 __global__ __launch_bounds__(kBlock) void k_render_pass(RenderArgs a)
 {
@@ -84,57 +122,25 @@ __global__ __launch_bounds__(kBlock) void k_render_pass(RenderArgs a)
         } else {
             node = a.ray_node[id];   // parked on this node, not yet marched
         }
-        const int n = a.oct.n, levelO = a.oct.final_level;
-        const int side = 1 << (n - levelO);
-        bool done = false;
-        while (!done) {
-            if (node == 0) {
-                write_miss(a, id);
-                done = true;
-                break;
-            }
-            uint64_t brick = node >> a.shift;
-            uint64_t bidx = brick - a.brick_base;
-            int slot = a.brick_slot[bidx];
-            if (slot < 0) {
-                // request the brick once per pass and park the ray on `node`
-                uint32_t bit = 1u << (bidx & 31);
-                uint32_t old = atomicOr(&a.req_bits[bidx >> 5], bit);
-                new_req = !(old & bit);
-                req_idx = (uint32_t)bidx;
-                a.ray_node[id] = node;
-                a.ray_phase[id] = PHASE_NEED;
-                park = true;
-                break;
-            }
-            a.slot_ref[slot] = 1;
-            if (a.brick_trace) a.brick_trace[bidx] = 1;
-            int nx, ny, nz, bx, by, bz;
-            node_min(node, levelO, n, nx, ny, nz);
-            node_min(brick, a.level_c, n, bx, by, bz);
-            bx -= a.inc; by -= a.inc; bz -= a.inc;
-            const float* sp = a.pool + (size_t)slot * a.slot_elems;
-            float Xx, Xy, Xz;
-            nnodes += 1;
-            if (march_node(r, nx, ny, nz, side, sp, a.S, bx, by, bz, a.iso, Xx, Xy, Xz, err, nsamples)) {
-                float4 c;
-                shade(sp, a.S, bx, by, bz, Xx, Xy, Xz, a.lx, a.ly, a.lz, a.color_scale, c, err);
-                nsamples += 6;
-                a.rgba[id] = c;
-                HitRecord h;
-                h.node = node;
-                h.brick = brick;
-                h.voxel[0] = floor_to_int(Xx);
-                h.voxel[1] = floor_to_int(Xy);
-                h.voxel[2] = floor_to_int(Xz);
-                float dx = Xx - r.ox, dy = Xy - r.oy, dz = Xz - r.oz;
-                h.depth = sqrtf(dx * dx + dy * dy + dz * dz);
-                a.hits[id] = h;
-                a.ray_phase[id] = PHASE_DONE;
-                done = true;
-                break;
-            }
-            node = next_node(r, a.oct, node, false);
+        BrickView bv{a.brick_slot, a.pool, a.S, a.inc, a.slot_elems, a.level_c, a.shift, a.brick_base};
+        Shading sh{a.iso, a.lx, a.ly, a.lz, a.color_scale};
+        float4 c;
+        HitRecord h;
+        uint64_t bidx = 0;
+        int st = walk_ray(r, a.oct, bv, sh, node, a.slot_ref, a.brick_trace, c, h, bidx, err, nsamples, nnodes);
+        if (st == WALK_ABSENT) {
+            // request the brick once per pass and park the ray on `node`
+            uint32_t bit = 1u << (bidx & 31);
+            uint32_t old = atomicOr(&a.req_bits[bidx >> 5], bit);
+            new_req = !(old & bit);
+            req_idx = (uint32_t)bidx;
+            a.ray_node[id] = node;
+            a.ray_phase[id] = PHASE_NEED;
+            park = true;
+        } else {
+            a.rgba[id] = c;
+            a.hits[id] = h;
+            a.ray_phase[id] = PHASE_DONE;
         }
     }
 
@@ -168,6 +174,49 @@ hipError_t launch_render_pass(const RenderArgs& a, hipStream_t s)
         grid = (unsigned)(per_xcd * 8);
     }
     hipLaunchKernelGGL(k_render_pass, dim3(grid), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+// --------------------------------------------------------- serving pass
+// One workgroup per 16x16 tile of any admitted request: requests of
+// different sizes, scenes and isovalues share one launch.  Every brick of a
+// serving scene is resident, so each ray finishes in the launch.  The grid
+// has a fixed size (the graph replays it); blocks past the tile count exit.
+__global__ __launch_bounds__(kBlock) void k_serve_pass(const ServeJob* __restrict__ jobs,
+                                                       const ServeTile* __restrict__ work, ServeCounters* ctr)
+{
+    if (blockIdx.x >= ctr->nwork) return;
+    const ServeTile t = work[blockIdx.x];
+    const ServeJob& J = jobs[t.job];
+    const int W = J.cam.W, H = J.cam.H;
+    int ty = (int)t.tile / J.tiles_x, tx = (int)t.tile % J.tiles_x;
+    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int j = tx * 16 + (w & 1) * 8 + (lane & 7);
+    int i = ty * 16 + (w >> 1) * 8 + (lane >> 3);
+    if (i >= H || j >= W) return;
+    uint32_t id = (uint32_t)(i * W + j);
+    Ray r;
+    make_ray(J.cam, i, j, r);
+    uint64_t node = root_ok(r, J.oct) ? next_node(r, J.oct, 1ull, true) : 0ull;
+    float4 c;
+    HitRecord h;
+    uint64_t bidx;
+    uint32_t err = 0, ns = 0, nn = 0;
+    int st = walk_ray(r, J.oct, J.bv, J.sh, node, nullptr, nullptr, c, h, bidx, err, ns, nn);
+    if (st == WALK_ABSENT) {   // cannot happen with a fully resident scene
+        err |= 2u;
+        c = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+        h.node = 0; h.brick = 0; h.voxel[0] = h.voxel[1] = h.voxel[2] = 0; h.depth = 0.0f;
+    }
+    J.rgba[id] = c;
+    J.hits[id] = h;
+    if (err) atomicOr(&ctr->err, err);
+}
+
+hipError_t launch_serve_pass(const ServeJob* jobs, const ServeTile* work, ServeCounters* ctr, int max_tiles,
+                             hipStream_t s)
+{
+    hipLaunchKernelGGL(k_serve_pass, dim3(max_tiles), dim3(kBlock), 0, s, jobs, work, ctr);
     return hipGetLastError();
 }
 

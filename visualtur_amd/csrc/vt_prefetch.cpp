@@ -38,6 +38,9 @@ struct vt_mlp {
     }
 };
 
+int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t* misses, int32_t* slots_out,
+                const int64_t* victims);
+
 namespace {
 
 int mlp_check(vt_mlp* m, int64_t N)
@@ -258,3 +261,95 @@ int vt_mlp_features(vt_mlp* m, const vt_camera* cam, int32_t nLevels, int32_t le
 }
 
 }  // extern "C"
+
+// ====================================================== learned prefetch
+
+int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t* misses, int32_t* slots_out,
+                const int64_t* victims);
+int fold_ref_bits(vt_context* ctx);
+
+extern "C" int vt_prefetch(vt_context* ctx, vt_mlp* m, const vt_camera* cam, int64_t budget, float min_prob,
+                           int64_t* loaded)
+{
+    if (!ctx || !m || !cam) return fail(VT_E_ARG, "NULL argument");
+    if (!ctx->octree_ready) return fail(VT_E_STATE, "no octree uploaded");
+    if (!ctx->cache) return fail(VT_E_STATE, "no cache");
+    BrickCache& c = *ctx->cache;
+    if (!c.d_vol && !c.host) return fail(VT_E_STATE, "no brick source attached");
+    if (c.level_c > ctx->max_level) return fail(VT_E_STATE, "the octree must be uploaded down to levelC");
+    if (budget < 0 || min_prob < 0.0f || min_prob >= 1.0f) return fail(VT_E_ARG, "bad budget or threshold");
+    HIP_TRY(hipSetDevice(ctx->device));
+    c.n = ctx->n;
+    if (c.host && !c.host->full_mode && !c.host->reader.joinable()) c.host->start(c.n);
+    hipStream_t s = ctx->stream;
+    // candidates: the bricks that hold octree nodes (the level-levelC run list)
+    if (ctx->cand_level != c.level_c) {
+        ctx->cand_codes.clear();
+        const auto& R = ctx->up_runs[(size_t)c.level_c];
+        for (size_t j = 0; j + 1 < R.size(); j += 2)
+            for (uint64_t code = R[j]; code <= R[j + 1]; ++code) ctx->cand_codes.push_back(code);
+        if (ctx->d_cand.ensure(std::max<size_t>(1, ctx->cand_codes.size())) != hipSuccess)
+            return fail(VT_E_NOMEM, "candidate list");
+        if (!ctx->cand_codes.empty())
+            HIP_TRY(hipMemcpy(ctx->d_cand.p, ctx->cand_codes.data(), ctx->cand_codes.size() * 8,
+                              hipMemcpyHostToDevice));
+        ctx->cand_level = c.level_c;
+    }
+    const int64_t NC = (int64_t)ctx->cand_codes.size();
+    int64_t nloaded = 0;
+    if (NC > 0 && budget > 0) {
+        // score every candidate (chunks of the MLP batch size)
+        Camera k;
+        memcpy(&k, cam, sizeof k);
+        std::vector<float> score((size_t)NC);
+        for (int64_t o = 0; o < NC; o += m->max_batch) {
+            int nb = (int)std::min<int64_t>(m->max_batch, NC - o);
+            HIP_TRY(launch_features(k, ctx->n, c.level_c, ctx->d_cand.p + o, nb, m->feats.p, s));
+            if (int rc = forward(m, m->feats.p, nb)) return rc;
+            HIP_TRY(hipMemcpyAsync(score.data() + o, m->z.p, (size_t)nb * 4, hipMemcpyDeviceToHost, s));
+        }
+        HIP_TRY(hipStreamSynchronize(s));
+        if (int rc = fold_ref_bits(ctx)) return rc;
+        const uint64_t base = 1ull << (3 * c.level_c);
+        const float zmin = logf(min_prob / (1.0f - min_prob)) ;   // logit threshold (-inf at 0)
+        std::vector<float> slot_score((size_t)c.gpu_slots, -INFINITY);
+        std::vector<int64_t> want;   // non-resident candidates, best first
+        for (int64_t i = 0; i < NC; ++i) {
+            int64_t b = (int64_t)(ctx->cand_codes[(size_t)i] - base);
+            int32_t sl = c.table[(size_t)b];
+            if (sl >= 0) slot_score[(size_t)sl] = score[(size_t)i];
+            else if (score[(size_t)i] >= zmin) want.push_back(i);
+        }
+        std::sort(want.begin(), want.end(), [&](int64_t a, int64_t b) { return score[(size_t)a] > score[(size_t)b]; });
+        // victims: free slots first, then resident bricks by ascending score;
+        // a brick is only replaced by a better-scoring one
+        std::vector<int64_t> victims;
+        for (int64_t sl = c.used; sl < c.gpu_slots; ++sl) victims.push_back(sl);
+        std::vector<int64_t> resident;
+        for (int64_t sl = 0; sl < c.used; ++sl) resident.push_back(sl);
+        std::sort(resident.begin(), resident.end(),
+                  [&](int64_t a, int64_t b) { return slot_score[(size_t)a] < slot_score[(size_t)b]; });
+        size_t free_n = victims.size();
+        victims.insert(victims.end(), resident.begin(), resident.end());
+        std::vector<uint32_t> load;
+        std::vector<int64_t> vslots;
+        c.epoch++;
+        for (size_t q = 0; q < want.size() && (int64_t)load.size() < budget && q < victims.size(); ++q) {
+            int64_t v = victims[q];
+            float ws = score[(size_t)want[q]];
+            if (q >= free_n && !(ws > slot_score[(size_t)v])) break;
+            load.push_back((uint32_t)(ctx->cand_codes[(size_t)want[q]] - base));
+            vslots.push_back(v);
+        }
+        // free slots are handed out in order, so `used` must advance with them
+        int64_t nfree = 0;
+        for (int64_t v : vslots)
+            if (v >= c.used) nfree++;
+        c.used += nfree;
+        int64_t misses = 0;
+        if (int rc = load_bricks(ctx, load.data(), (int64_t)load.size(), &misses, nullptr, vslots.data())) return rc;
+        nloaded = (int64_t)load.size();
+    }
+    if (loaded) *loaded = nloaded;
+    return VT_OK;
+}

@@ -67,6 +67,8 @@ int check_camera(const vt_camera* cam)
     return VT_OK;
 }
 
+}  // namespace
+
 // Fold the device reference bits (set by the render kernel on every brick
 // it marches) into the host clock.
 int fold_ref_bits(vt_context* ctx)
@@ -80,7 +82,8 @@ int fold_ref_bits(vt_context* ctx)
     return VT_OK;
 }
 
-int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t* misses, int32_t* slots_out);
+int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t* misses, int32_t* slots_out,
+                const int64_t* victims = nullptr);
 
 // Load `count` requested bricks (indices relative to 8^levelC) into the
 // device pool as far as capacity allows.  Bricks loaded here are pinned
@@ -107,7 +110,8 @@ int service_requests(vt_context* ctx, const uint32_t* reqs, int64_t count, vt_fr
 
 // Place `n_load` non-resident bricks into victim slots (skipping slots
 // pinned in the current epoch), then fill them from the brick source.
-int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t* misses, int32_t* slots_out)
+int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t* misses, int32_t* slots_out,
+                const int64_t* victims)
 {
     BrickCache& c = *ctx->cache;
     if (n_load <= 0) return VT_OK;
@@ -118,7 +122,7 @@ int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t
     int64_t nt = 0;
     for (int64_t q = 0; q < n_load; ++q) {
         int64_t b = bricks[q];
-        int64_t slot = c.pick_victim();
+        int64_t slot = victims ? victims[q] : c.pick_victim();
         if (slot < 0) return fail(VT_E_STATE, "brick cache: no evictable slot");
         int64_t old = c.slot_brick[(size_t)slot];
         if (old >= 0) {
@@ -165,6 +169,8 @@ int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t
     c.stat_uploads += n_load;
     return VT_OK;
 }
+
+namespace {
 
 bool runs_well_formed(const uint64_t* E, int size, int level, std::string& why)
 {
@@ -310,6 +316,15 @@ int vt_octree_upload(vt_context* ctx, const uint64_t* runs, const int32_t* sizes
         }
         E += sizes[l];
     }
+    ctx->up_runs.assign((size_t)maxLevel + 1, {});
+    {
+        const uint64_t* R = runs;
+        for (int l = 0; l <= maxLevel; ++l) {
+            ctx->up_runs[(size_t)l].assign(R, R + sizes[l]);
+            R += sizes[l];
+        }
+    }
+    ctx->cand_level = -1;
     if (ctx->d_child) (void)hipFree(ctx->d_child);
     ctx->d_child = nullptr;
     HIP_TRY(hipMalloc((void**)&ctx->d_child, bytes.size()));

@@ -1,0 +1,397 @@
+// vt_serve.cpp -- render-request serving loop (no reference counterpart).
+//
+// Scenes are volumes whose every brick is resident in HBM, with a min/max
+// pyramid over the octree levels so that any isovalue gets its membership
+// tables without rebuilding the octree (a node is valid for iso t exactly
+// when min < t <= max over its zero-padded samples, the closed form of the
+// reference builder).  Requests arrive over time; the loop admits them while
+// their output buffers fit the HBM budget and tile slots are free, and every
+// launch renders 16x16 tiles of all admitted requests together.  The launch
+// is a captured hipGraph of fixed shape, replayed once per batch with a new
+// tile list.  Mode 1 renders one request per launch, in arrival order (the
+// plain per-request baseline).
+#include <math.h>
+
+#include <map>
+#include <thread>
+
+#include "vt_build.hpp"
+#include "vt_context.hpp"
+
+namespace {
+
+struct Scene {
+    int n = 0, level_o = 0, level_c = 0, inc = 0, S = 0;
+    size_t slot_elems = 0;
+    float* d_pool = nullptr;
+    int32_t* d_slot = nullptr;
+    std::vector<std::vector<float>> mn, mx;   // per level 0..levelO, 8^l nodes
+    std::map<uint32_t, uint8_t*> tables;      // isovalue bits -> membership bytes
+    std::map<uint32_t, int> root;
+    std::vector<uint64_t> level_off;
+    size_t table_bytes = 0;
+};
+
+struct Arena {   // first-fit allocator over one device allocation
+    char* base = nullptr;
+    size_t size = 0;
+    std::map<size_t, size_t> free_;   // offset -> length
+    bool init(size_t bytes)
+    {
+        if (hipMalloc((void**)&base, bytes) != hipSuccess) return false;
+        size = bytes;
+        free_[0] = bytes;
+        return true;
+    }
+    size_t alloc(size_t bytes)
+    {
+        bytes = (bytes + 255) & ~(size_t)255;
+        for (auto it = free_.begin(); it != free_.end(); ++it)
+            if (it->second >= bytes) {
+                size_t off = it->first, len = it->second;
+                free_.erase(it);
+                if (len > bytes) free_[off + bytes] = len - bytes;
+                return off;
+            }
+        return (size_t)-1;
+    }
+    void release(size_t off, size_t bytes)
+    {
+        bytes = (bytes + 255) & ~(size_t)255;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+}  // namespace
+
+struct vt_server {
+    vt_context* ctx = nullptr;
+    int64_t budget = 0;
+    int max_tiles = 0;
+    int max_jobs = 256;
+    std::vector<std::unique_ptr<Scene>> scenes;
+    Arena arena;
+    DevBuf<ServeJob> d_jobs;
+    DevBuf<ServeTile> d_work;
+    ServeCounters* d_ctr = nullptr;
+    PinBuf<ServeTile> h_work;
+    PinBuf<ServeCounters> h_ctr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~vt_server()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto& s : scenes) {
+            if (s->d_pool) (void)hipFree(s->d_pool);
+            if (s->d_slot) (void)hipFree(s->d_slot);
+            for (auto& t : s->tables) (void)hipFree(t.second);
+        }
+        if (arena.base) (void)hipFree(arena.base);
+        if (d_ctr) (void)hipFree(d_ctr);
+        d_jobs.release();
+        d_work.release();
+        h_work.release();
+        h_ctr.release();
+    }
+};
+
+namespace {
+
+int tree_levels3(int X, int Y, int Z)
+{
+    int d = std::max(X, std::max(Y, Z)), n = 0;
+    while ((1 << n) < d) ++n;
+    return n;
+}
+
+// membership bytes of levels 1..levelO for isovalue t (one byte per parent)
+int scene_table(vt_server* sv, Scene& sc, float iso, uint8_t** out, int* root)
+{
+    uint32_t key;
+    memcpy(&key, &iso, 4);
+    auto it = sc.tables.find(key);
+    if (it != sc.tables.end()) {
+        *out = it->second;
+        *root = sc.root[key];
+        return VT_OK;
+    }
+    std::vector<uint8_t> bytes(std::max<size_t>(1, sc.table_bytes), 0);
+    for (int l = 1; l <= sc.level_o; ++l) {
+        const auto& mn = sc.mn[(size_t)l];
+        const auto& mx = sc.mx[(size_t)l];
+        for (size_t c = 0; c < mn.size(); ++c)
+            if (mn[c] < iso && iso <= mx[c]) bytes[(size_t)(sc.level_off[(size_t)l] + (c >> 3))] |= (uint8_t)(1u << (c & 7));
+    }
+    int r = sc.mn[0][0] < iso && iso <= sc.mx[0][0];
+    uint8_t* d = nullptr;
+    HIP_TRY(hipMalloc((void**)&d, bytes.size()));
+    HIP_TRY(hipMemcpyAsync(d, bytes.data(), bytes.size(), hipMemcpyHostToDevice, sv->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(sv->ctx->stream));
+    sc.tables[key] = d;
+    sc.root[key] = r;
+    *out = d;
+    *root = r;
+    return VT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vt_server_create(vt_context* ctx, int64_t hbm_budget_bytes, int32_t max_tiles, vt_server** out)
+{
+    if (!ctx || !out) return fail(VT_E_ARG, "NULL argument");
+    if (hbm_budget_bytes < (1 << 20) || max_tiles < 1 || max_tiles > (1 << 20)) return fail(VT_E_ARG, "bad sizes");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::unique_ptr<vt_server> sv(new vt_server);
+    sv->ctx = ctx;
+    sv->budget = hbm_budget_bytes;
+    sv->max_tiles = max_tiles;
+    if (!sv->arena.init((size_t)hbm_budget_bytes)) return fail(VT_E_NOMEM, "serving arena");
+    if (sv->d_jobs.ensure((size_t)sv->max_jobs) != hipSuccess || sv->d_work.ensure((size_t)max_tiles) != hipSuccess ||
+        sv->h_work.ensure((size_t)max_tiles) != hipSuccess || sv->h_ctr.ensure(1) != hipSuccess)
+        return fail(VT_E_NOMEM, "serving buffers");
+    HIP_TRY(hipMalloc((void**)&sv->d_ctr, sizeof(ServeCounters)));
+    HIP_TRY(hipMemset(sv->d_ctr, 0, sizeof(ServeCounters)));
+    // the batch launch, captured once and replayed: the tile list and count
+    // change between replays, the shape never does
+    HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t le = launch_serve_pass(sv->d_jobs.p, sv->d_work.p, sv->d_ctr, max_tiles, ctx->stream);
+    hipError_t ce = hipStreamEndCapture(ctx->stream, &sv->graph);
+    if (le != hipSuccess) return fail(VT_E_HIP, std::string("capture: ") + hipGetErrorString(le));
+    if (ce != hipSuccess) return fail(VT_E_HIP, std::string("capture: ") + hipGetErrorString(ce));
+    HIP_TRY(hipGraphInstantiate(&sv->exec, sv->graph, nullptr, nullptr, 0));
+    *out = sv.release();
+    return VT_OK;
+}
+
+int vt_server_destroy(vt_server* sv)
+{
+    if (sv) {
+        (void)hipStreamSynchronize(sv->ctx->stream);
+        delete sv;
+    }
+    return VT_OK;
+}
+
+int vt_server_add_scene(vt_server* sv, const float* d_vol, int32_t X, int32_t Y, int32_t Z, int32_t levelO,
+                        int32_t levelC, int32_t cubeInc, int32_t* scene_out)
+{
+    if (!sv || !d_vol || !scene_out) return fail(VT_E_ARG, "NULL argument");
+    if (X <= 0 || Y <= 0 || Z <= 0) return fail(VT_E_ARG, "bad volume dims");
+    const int n = tree_levels3(X, Y, Z);
+    if (levelC < 0 || levelC > levelO || levelO > n || levelO > 8 || cubeInc < 0)
+        return fail(VT_E_ARG, "bad levels");
+    hipStream_t s = sv->ctx->stream;
+    std::unique_ptr<Scene> sc(new Scene);
+    sc->n = n;
+    sc->level_o = levelO;
+    sc->level_c = levelC;
+    sc->inc = cubeInc;
+    const int cube = 1 << (n - levelC);
+    sc->S = cube + 2 * cubeInc;
+    sc->slot_elems = (size_t)sc->S * sc->S * sc->S;
+    // min/max of every level-levelO node on the device, then up the pyramid
+    const size_t NO = (size_t)1 << (3 * levelO);
+    DevBuf<float> dmn, dmx;
+    if (dmn.ensure(NO) != hipSuccess || dmx.ensure(NO) != hipSuccess) return fail(VT_E_NOMEM, "pyramid");
+    HIP_TRY(launch_node_minmax(d_vol, X, Y, Z, n, levelO, dmn.p, dmx.p, s));
+    sc->mn.assign((size_t)levelO + 1, {});
+    sc->mx.assign((size_t)levelO + 1, {});
+    sc->mn[(size_t)levelO].resize(NO);
+    sc->mx[(size_t)levelO].resize(NO);
+    HIP_TRY(hipMemcpyAsync(sc->mn[(size_t)levelO].data(), dmn.p, NO * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(sc->mx[(size_t)levelO].data(), dmx.p, NO * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int l = levelO - 1; l >= 0; --l) {
+        size_t N = (size_t)1 << (3 * l);
+        sc->mn[(size_t)l].assign(N, 3.0e38f);
+        sc->mx[(size_t)l].assign(N, -3.0e38f);
+        for (size_t c = 0; c < N * 8; ++c) {
+            sc->mn[(size_t)l][c >> 3] = std::min(sc->mn[(size_t)l][c >> 3], sc->mn[(size_t)l + 1][c]);
+            sc->mx[(size_t)l][c >> 3] = std::max(sc->mx[(size_t)l][c >> 3], sc->mx[(size_t)l + 1][c]);
+        }
+    }
+    sc->level_off.assign(24, 0);
+    uint64_t off = 0;
+    for (int l = 1; l <= levelO; ++l) {
+        sc->level_off[(size_t)l] = off;
+        off += 1ull << (3 * (l - 1));
+    }
+    sc->table_bytes = (size_t)off;
+    // every brick resident: slot = brick index
+    const int64_t nb = 1ll << (3 * levelC);
+    size_t pool_bytes = (size_t)nb * sc->slot_elems * 4;
+    if (hipMalloc((void**)&sc->d_pool, pool_bytes) != hipSuccess)
+        return fail(VT_E_NOMEM, "scene brick pool (" + std::to_string(pool_bytes) + " bytes)");
+    std::vector<int32_t> ident((size_t)nb);
+    std::vector<int64_t> codes((size_t)nb);
+    for (int64_t b = 0; b < nb; ++b) { ident[(size_t)b] = (int32_t)b; codes[(size_t)b] = b + nb; }
+    HIP_TRY(hipMalloc((void**)&sc->d_slot, (size_t)nb * 4));
+    HIP_TRY(hipMemcpy(sc->d_slot, ident.data(), (size_t)nb * 4, hipMemcpyHostToDevice));
+    DevBuf<int64_t> dcodes;
+    if (dcodes.ensure((size_t)nb) != hipSuccess) return fail(VT_E_NOMEM, "codes");
+    HIP_TRY(hipMemcpy(dcodes.p, codes.data(), (size_t)nb * 8, hipMemcpyHostToDevice));
+    for (int64_t o = 0; o < nb; o += 65536) {
+        int cnt = (int)std::min<int64_t>(65536, nb - o);
+        HIP_TRY(launch_bricks_from_volume(d_vol, X, Y, Z, dcodes.p + o, sc->d_slot + o, cnt, levelC, n, cube, cubeInc,
+                                          sc->d_pool, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    *scene_out = (int32_t)sv->scenes.size();
+    sv->scenes.push_back(std::move(sc));
+    return VT_OK;
+}
+
+int vt_server_run(vt_server* sv, const vt_request* reqs, int64_t nreq, int32_t mode, vt_request_result* results,
+                  vt_serve_stats* stats)
+{
+    if (!sv || (nreq && (!reqs || !results))) return fail(VT_E_ARG, "NULL argument");
+    if (mode != 0 && mode != 1) return fail(VT_E_ARG, "mode must be 0 (batched) or 1 (one request per launch)");
+    HIP_TRY(hipSetDevice(sv->ctx->device));
+    hipStream_t s = sv->ctx->stream;
+    // validate and order by arrival
+    std::vector<int64_t> order((size_t)nreq);
+    for (int64_t i = 0; i < nreq; ++i) {
+        const vt_request& q = reqs[i];
+        if (q.scene < 0 || q.scene >= (int32_t)sv->scenes.size()) return fail(VT_E_ARG, "unknown scene");
+        if (q.cam.W <= 0 || q.cam.H <= 0 || (int64_t)q.cam.W * q.cam.H > (1 << 26)) return fail(VT_E_ARG, "bad screen");
+        size_t need = (size_t)q.cam.W * q.cam.H * 48 + 512;
+        if ((int64_t)need > sv->budget) return fail(VT_E_ARG, "a request's outputs exceed the HBM budget");
+        order[(size_t)i] = i;
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t a, int64_t b) { return reqs[a].arrival_s < reqs[b].arrival_s; });
+    struct Live {
+        int64_t req;
+        int job;
+        size_t off, bytes;
+        int tiles, next_tile;
+        int launches;
+    };
+    std::vector<Live> live;
+    std::vector<int> free_jobs;
+    for (int j = sv->max_jobs - 1; j >= 0; --j) free_jobs.push_back(j);
+    size_t next = 0;
+    int64_t done = 0, launches = 0, tiles_total = 0;
+    uint64_t rays = 0;
+    double busy_ms = 0.0;
+    const double t0 = now_ms();
+    auto elapsed_s = [&]() { return (now_ms() - t0) * 1e-3; };
+    while (done < nreq) {
+        // admit arrived requests while budget, job slots (and, in mode 1, an idle device) allow
+        while (next < order.size() && reqs[order[next]].arrival_s <= elapsed_s() && !free_jobs.empty() &&
+               (mode == 0 || live.empty())) {
+            const vt_request& q = reqs[order[next]];
+            size_t bytes = (size_t)q.cam.W * q.cam.H * 48;
+            size_t off = sv->arena.alloc(bytes);
+            if (off == (size_t)-1) break;   // wait for outputs to drain
+            Scene& sc = *sv->scenes[(size_t)q.scene];
+            uint8_t* table = nullptr;
+            int root = 0;
+            if (int rc = scene_table(sv, sc, q.iso, &table, &root)) return rc;
+            ServeJob J;
+            memcpy(&J.cam, &q.cam, sizeof J.cam);
+            J.oct.child_bytes = table;
+            for (int l = 0; l < 24; ++l) J.oct.level_off[l] = sc.level_off[(size_t)l];
+            J.oct.n = sc.n;
+            J.oct.final_level = sc.level_o;
+            J.oct.root_valid = root;
+            J.bv = BrickView{sc.d_slot, sc.d_pool, sc.S, sc.inc, sc.slot_elems, sc.level_c,
+                             3 * (sc.level_o - sc.level_c), 1ull << (3 * sc.level_c)};
+            J.sh = Shading{q.iso, q.light[0], q.light[1], q.light[2], q.color_scale};
+            J.rgba = reinterpret_cast<float4*>(sv->arena.base + off);
+            J.hits = reinterpret_cast<HitRecord*>(sv->arena.base + off + (size_t)q.cam.W * q.cam.H * 16);
+            J.tiles_x = (q.cam.W + 15) / 16;
+            int job = free_jobs.back();
+            free_jobs.pop_back();
+            HIP_TRY(hipMemcpyAsync(sv->d_jobs.p + job, &J, sizeof J, hipMemcpyHostToDevice, s));
+            Live L;
+            L.req = order[next];
+            L.job = job;
+            L.off = off;
+            L.bytes = bytes;
+            L.tiles = J.tiles_x * ((q.cam.H + 15) / 16);
+            L.next_tile = 0;
+            L.launches = 0;
+            live.push_back(L);
+            results[L.req].admit_s = elapsed_s();
+            ++next;
+        }
+        if (live.empty()) {
+            if (next < order.size()) {   // idle until the next arrival
+                double wait = reqs[order[next]].arrival_s - elapsed_s();
+                if (wait > 0) std::this_thread::sleep_for(std::chrono::microseconds((int64_t)(wait * 1e6)));
+            }
+            continue;
+        }
+        // fill the launch with tiles, oldest request first
+        int nw = 0;
+        for (auto& L : live) {
+            while (L.next_tile < L.tiles && nw < sv->max_tiles) {
+                sv->h_work.p[nw].job = (uint32_t)L.job;
+                sv->h_work.p[nw].tile = (uint32_t)L.next_tile++;
+                ++nw;
+            }
+            if (nw == sv->max_tiles) break;
+        }
+        sv->h_ctr.p->nwork = (uint32_t)nw;
+        sv->h_ctr.p->err = 0;
+        double l0 = now_ms();
+        HIP_TRY(hipMemcpyAsync(sv->d_work.p, sv->h_work.p, (size_t)nw * sizeof(ServeTile), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(sv->d_ctr, sv->h_ctr.p, sizeof(ServeCounters), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipGraphLaunch(sv->exec, s));
+        HIP_TRY(hipMemcpyAsync(sv->h_ctr.p, sv->d_ctr, sizeof(ServeCounters), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        busy_ms += now_ms() - l0;
+        if (sv->h_ctr.p->err) return fail(VT_E_RANGE, "serving pass: a sample fell outside its brick slot");
+        ++launches;
+        tiles_total += nw;
+        // retire finished requests
+        for (size_t k = 0; k < live.size();) {
+            Live& L = live[k];
+            L.launches++;
+            if (L.next_tile < L.tiles) { ++k; continue; }
+            const vt_request& q = reqs[L.req];
+            size_t R = (size_t)q.cam.W * q.cam.H;
+            if (q.h_rgba)
+                HIP_TRY(hipMemcpyAsync(q.h_rgba, sv->arena.base + L.off, R * 16, hipMemcpyDeviceToHost, s));
+            if (q.h_hits)
+                HIP_TRY(hipMemcpyAsync(q.h_hits, sv->arena.base + L.off + R * 16, R * 32, hipMemcpyDeviceToHost, s));
+            if (q.h_rgba || q.h_hits) HIP_TRY(hipStreamSynchronize(s));
+            vt_request_result& res = results[L.req];
+            res.done_s = elapsed_s();
+            res.latency_ms = (res.done_s - q.arrival_s) * 1e3;
+            res.launches = L.launches;
+            rays += R;
+            sv->arena.release(L.off, L.bytes);
+            free_jobs.push_back(L.job);
+            live.erase(live.begin() + (long)k);
+            ++done;
+        }
+    }
+    if (stats) {
+        stats->requests = nreq;
+        stats->launches = launches;
+        stats->tiles = tiles_total;
+        stats->rays = (int64_t)rays;
+        stats->wall_s = elapsed_s();
+        stats->busy_s = busy_ms * 1e-3;
+    }
+    return VT_OK;
+}
+
+}  // extern "C"
