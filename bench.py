@@ -134,6 +134,119 @@ def setup_c2(ctx, args) -> Workload:
     return w
 
 
+def prefetch_leg(ctx, w, cams, args, d_rgba, d_hits, plain_ms, plain_uploads, U, plain_passes):
+    """Learned prefetching against the plain run just timed: train the MLP on
+    the visited-brick traces of other camera paths, clear the device cache,
+    replay the same frames with a prefetch before each, and compare."""
+    from visualtur_amd import prefetch as pf
+
+    t0 = time.perf_counter()
+    cand = pf.candidate_codes(w.runs[w.level_c])
+    tcams = [c for s in range(args.train_paths)
+             for c in pf.random_walk(w.W // 4, w.H // 4, 1 << w.n, args.train_frames, seed=9000 + s,
+                                     radius=(1.9, 2.6))]
+    traces = pf.collect_traces(ctx, tcams, w.prm, w.level_c)
+    net = vt.MLP(ctx, seed=11, max_batch=16384)
+    X, Y = pf.build_dataset(ctx, net, tcams, traces, cand, w.n, w.level_c, neg_per_pos=3, seed=2)
+    losses, pos_w = pf.train(ctx, net, X, Y, steps=args.train_steps, batch=8192, lr=1e-3)
+    train_s = time.perf_counter() - t0
+    log(f"prefetcher trained in {train_s:.1f}s on {len(tcams)} frames / {X.shape[0]} samples, "
+        f"loss {losses[:10].mean():.3f} -> {losses[-10:].mean():.3f}")
+    R = w.W * w.H
+    ctx.cache_clear()
+    for k in range(args.warmup):
+        net.prefetch(cams[k], w.slots, 0.5)
+        ctx.render_frame(cams[k], w.prm, d_rgba, d_hits)
+    ctx.synchronize()
+    ms, pre_ms, uploads, pre_loaded, passes = [], [], [], [], []
+    t0 = time.perf_counter()
+    for k in range(args.warmup, args.warmup + args.steps):
+        f0 = time.perf_counter()
+        pre_loaded.append(net.prefetch(cams[k], w.slots, 0.5))
+        f1 = time.perf_counter()
+        st = ctx.render_frame(cams[k], w.prm, d_rgba, d_hits)
+        f2 = time.perf_counter()
+        ms.append((f2 - f0) * 1e3)
+        pre_ms.append((f1 - f0) * 1e3)
+        uploads.append(st.bricks_uploaded)
+        passes.append(st.passes)
+    ctx.synchronize()
+    total = time.perf_counter() - t0
+    # the last frame must be identical to the plain render of the same camera
+    hits_pf = d_hits.download(vt.HIT_DTYPE, (R,))
+    rgba_pf = d_rgba.download(np.float32, (R, 4))
+    ctx.cache_clear()
+    ctx.render_frame(cams[args.warmup + args.steps - 1], w.prm, d_rgba, d_hits)
+    same = bool(np.array_equal(hits_pf, d_hits.download(vt.HIT_DTYPE, (R,))) and
+                np.array_equal(rgba_pf.view(np.uint32), d_rgba.download(np.float32, (R, 4)).view(np.uint32)))
+    net.close()
+    m = np.array(ms)
+    return {"value": round(R * args.steps / total, 1), "unit": "rays/s",
+            "p50_ms": round(float(np.percentile(m, 50)), 4), "p99_ms": round(float(np.percentile(m, 99)), 4),
+            "prefetch_ms_per_frame": round(float(np.mean(pre_ms)), 4),
+            "demand_misses_per_frame": float(np.mean(uploads)),
+            "prefetched_per_frame": float(np.mean(pre_loaded)),
+            "device_miss_rate": round(float(sum(uploads)) / max(1, sum(U)), 5),
+            "plain_device_miss_rate": round(float(sum(plain_uploads)) / max(1, sum(U)), 5),
+            "passes_per_frame": float(np.mean(passes)), "plain_passes_per_frame": float(np.mean(plain_passes)),
+            "plain_value": round(R * len(plain_ms) / (sum(plain_ms) * 1e-3), 1),
+            "identical_last_frame": same, "train_seconds": round(train_s, 2),
+            "train_frames": len(tcams), "train_samples": int(X.shape[0]),
+            "final_loss": round(float(losses[-10:].mean()), 4)}
+
+
+def serving_sweep(ctx, args):
+    """Config 4: the c1 and c2 volumes resident as two scenes; 32 viewers on
+    seeded random walks issue requests as a Poisson stream; resolution
+    uniform over {256^2, 512^2, 1024^2, 1920x1080}; iso uniform over
+    {-0.2, 0, 0.2} on the c1 volume, 0 on the c2 volume.  Batched serving
+    (mode 0) against one request per launch (mode 1), per arrival rate."""
+    from visualtur_amd.prefetch import random_walk
+
+    sv = vt.Server(ctx, hbm_budget=4 << 30, max_tiles=65536)
+    v1 = gaussian_blobs(256, 16, C1_SEED)
+    d1 = ctx.to_device(v1)
+    s1 = sv.add_scene(d1, v1.shape, 3, 3)
+    d1.free()
+    d2 = ctx.alloc(4 * 1024 ** 3)
+    ctx.volume_generate(vt.GEN_FBM, (1024,) * 3, C2_SEED, d2, octaves=6, base_freq=1.0 / 64.0, gain=0.5,
+                        lacunarity=2.0, scale=1.0)
+    s2 = sv.add_scene(d2, (1024,) * 3, 5, 5)
+    d2.free()
+    res_set = [(256, 256), (512, 512), (1024, 1024), (1920, 1080)]
+    out = {}
+    for rate in args.rates:
+        rng = np.random.default_rng(int(rate))
+        n = max(16, int(rate * args.serve_seconds))
+        walks = {}
+        reqs = []
+        t = 0.0
+        for k in range(n):
+            t += rng.exponential(1.0 / rate)
+            viewer = int(rng.integers(32))
+            W, H = res_set[int(rng.integers(4))]
+            scene = s1 if viewer % 2 == 0 else s2
+            D = 256 if scene == s1 else 1024
+            step = walks.get(viewer, 0)
+            walks[viewer] = step + 1
+            cam = random_walk(W, H, D, step + 1, seed=7000 + viewer)[-1]
+            iso = float(rng.choice([-0.2, 0.0, 0.2])) if scene == s1 else 0.0
+            reqs.append({"scene": scene, "cam": cam, "iso": iso, "arrival_s": t})
+        row = {"requests": n}
+        for mode, name in ((0, "batched"), (1, "per_request")):
+            results, st, _ = sv.run(reqs, mode=mode)
+            lat = np.array([r["latency_ms"] for r in results])
+            row[name] = {"p50_ms": round(float(np.percentile(lat, 50)), 3),
+                         "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                         "rays_per_s": round(st["rays"] / st["wall_s"], 1),
+                         "busy_rays_per_s": round(st["rays"] / max(st["busy_s"], 1e-9), 1),
+                         "launches": st["launches"], "wall_s": round(st["wall_s"], 3)}
+            log(f"serving rate {rate}/s {name}: {row[name]}")
+        out[str(rate)] = row
+    sv.close()
+    return out
+
+
 def cpu_baseline(w, cams, seconds: float):
     """Reference algorithm (the C oracle) on this host's cores, over whole
     frames of the same workload until `seconds` of wall time have elapsed."""
@@ -195,10 +308,16 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c4"])
+    ap.add_argument("--rates", type=float, nargs="*", default=[50, 100, 200, 400, 800])
+    ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--parity-rays", type=int, default=4096)
+    ap.add_argument("--prefetch", type=int, default=1, help="c2: also time the learned-prefetch form")
+    ap.add_argument("--train-paths", type=int, default=24)
+    ap.add_argument("--train-frames", type=int, default=6)
+    ap.add_argument("--train-steps", type=int, default=600)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 256 MiB streaming copy kernel first (HBM counter calibration)")
     args = ap.parse_args()
@@ -209,6 +328,19 @@ def main() -> int:
         return 2
 
     ctx = vt.Context(0)
+    if args.config == "c4":
+        sweep = serving_sweep(ctx, args)
+        best = max(sweep.values(), key=lambda r: r["batched"]["rays_per_s"])
+        out = {"metric": METRIC, "value": best["batched"]["rays_per_s"], "unit": "rays/s", "n_gpus": 1,
+               "steps": sum(r["requests"] for r in sweep.values()), "warmup": 0, "ms_per_step": None,
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+               "data": "synthetic",
+               "config": {"workload": "serving: c1+c2 volumes resident, 32 random-walk viewers, Poisson arrivals, "
+                                      "mixed resolutions, per-request iso", "form": "batched vs per-request"},
+               "serving": sweep}
+        ctx.close()
+        print(json.dumps(out), flush=True)
+        return 0
     if args.copy_probe:
         nb = 256 << 20
         a_, b_ = ctx.alloc(nb), ctx.alloc(nb)
@@ -273,6 +405,8 @@ def main() -> int:
         "device_miss_rate": round(float(sum(uploads)) / max(1, sum(U)), 5),
         "roofline": roofline,
     }
+    if args.prefetch and w.name == "c2":
+        out["prefetch"] = prefetch_leg(ctx, w, cams, args, d_rgba, d_hits, frame_ms, uploads, U, passes)
     if not args.no_cpu:
         log("cpu baseline")
         cb, objs = cpu_baseline(w, cams[args.warmup: args.warmup + args.steps], args.cpu_seconds)

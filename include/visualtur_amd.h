@@ -175,6 +175,10 @@ int vt_render_frame(vt_context *ctx, const vt_camera *cam, const vt_render_param
                     float *d_rgba, vt_hit *d_hits, vt_frame_stats *stats);
 /* Enable per-frame sample/node counters and the visited-brick trace. */
 int vt_set_frame_options(vt_context *ctx, int collect_stats, int record_trace);
+/* March option (no reference counterpart): skip_empty=1 (default) skips
+ * the evaluation of samples whose 4x4x4-cell sub-block lies strictly on one
+ * side of iso; results are identical either way. */
+int vt_set_march_options(vt_context *ctx, int skip_empty);
 /* Visited-brick trace of the last frame: out[i] = 1 if brick
  * (i + 8^levelC) was marched by at least one ray.  n = 8^levelC. */
 int vt_get_brick_trace(vt_context *ctx, uint8_t *out, int64_t n);

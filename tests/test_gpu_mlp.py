@@ -33,7 +33,8 @@ def test_forward_matches_oracle(vt, ctx, net, n):
     # against the BF16-faithful oracle: FP32 accumulation only
     assert np.max(np.abs(z - z_ref)) <= 2e-3 * (1 + np.max(np.abs(z_ref)))
     # against pure FP64: BF16 rounding of weights and activations
-    assert np.corrcoef(z, z_64)[0, 1] > 0.999
+    if n > 1:
+        assert np.corrcoef(z, z_64)[0, 1] > 0.999
     d_x.free()
     d_z.free()
 

@@ -199,3 +199,28 @@ def test_c1_full_frame(vt, vto, ctx):
     assert_same(hg, rg, hc, rc)
     assert bitexact_fraction(hg, rg, hc, rc) == 1.0
     assert (hc["node"] != 0).mean() > 0.02   # the 16 blobs at iso 0.5 cover ~5% of the frame
+
+
+@pytest.mark.parametrize("iso,seed", [(0.25, 41), (0.6, 42), (0.05, 43)])
+def test_sample_skipping_is_exact(vt, vto, ctx, iso, seed):
+    """The sub-block skip evaluates fewer samples but gives bit-identical
+    frames (and equals the oracle)."""
+    vol = blobs_volume(64, 14, seed=seed)
+    sc = Scene(vto, vol, iso, 3, 2)
+    sc.setup_gpu(vt, ctx, slots=64)
+    cam = orbit_camera(vt, 96, 72, (32, 32, 32), 100.0, 20.0 * seed)
+    ctx.set_frame_options(collect_stats=True)
+    try:
+        ctx.set_march_options(skip_empty=False)
+        r0, h0, s0 = sc.render_gpu(vt, ctx, cam)
+        ctx.set_march_options(skip_empty=True)
+        r1, h1, s1 = sc.render_gpu(vt, ctx, cam)
+    finally:
+        ctx.set_march_options(skip_empty=True)
+        ctx.set_frame_options(False, False)
+    assert np.array_equal(h0, h1)
+    assert np.array_equal(r0.view(np.uint32), r1.view(np.uint32))
+    rc, hc = sc.render_cpu(vto, cam)
+    assert_same(h1, r1, hc, rc)
+    assert s1.samples < s0.samples
+    print(f"samples without skip {s0.samples}, with skip {s1.samples}", flush=True)

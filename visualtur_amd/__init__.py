@@ -118,6 +118,7 @@ _SIGS = {
                                        ctypes.POINTER(FrameStats)]),
     "vt_set_frame_options": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
     "vt_get_brick_trace": (ctypes.c_int, [_P, _P, ctypes.c_int64]),
+    "vt_set_march_options": (ctypes.c_int, [_P, ctypes.c_int]),
     "vt_trace_nodes": (ctypes.c_int, [_P, ctypes.POINTER(Camera), ctypes.c_int32, ctypes.c_int32,
                                       ctypes.c_int32, _P, _P]),
     "vt_octree_build": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
@@ -371,6 +372,9 @@ class Context:
         self._c(lib().vt_render_frame(self.handle, ctypes.byref(cam), ctypes.byref(params), d_rgba.ptr,
                                       d_hits.ptr if d_hits is not None else None, ctypes.byref(st)))
         return st
+
+    def set_march_options(self, skip_empty: bool = True) -> None:
+        self._c(lib().vt_set_march_options(self.handle, int(skip_empty)))
 
     def brick_trace(self, level_c: int) -> np.ndarray:
         n = 1 << (3 * level_c)

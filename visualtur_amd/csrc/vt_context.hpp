@@ -250,6 +250,8 @@ struct BrickCache {
     float* d_pool = nullptr;
     int32_t* d_table = nullptr;
     uint8_t* d_ref = nullptr;
+    float2* d_submm = nullptr;   // G^3 sub-block (min, max) per slot, null when bricks are not 4k wide
+    int G = 0;
     std::vector<int32_t> table;
     std::vector<int64_t> slot_brick;
     std::vector<uint8_t> ref, ref_tmp;
@@ -280,6 +282,7 @@ struct BrickCache {
         if (d_pool) (void)hipFree(d_pool);
         if (d_table) (void)hipFree(d_table);
         if (d_ref) (void)hipFree(d_ref);
+        if (d_submm) (void)hipFree(d_submm);
         h_src.release(); h_slots.release(); h_codes.release(); h_tidx.release(); h_tval.release();
         d_src.release(); d_slots.release(); d_codes.release(); d_tidx.release(); d_tval.release();
     }
@@ -330,6 +333,7 @@ struct vt_context {
     FrameCounters* h_ctr = nullptr;
     PinBuf<uint32_t> h_req;
     int collect_stats = 0, record_trace = 0;
+    int skip_empty = 1;   // sub-block sample skipping in the march
     int64_t trace_n = 0;
     DevBuf<uint8_t> stage_trav;
     int64_t stage_rays = 0;

@@ -357,6 +357,109 @@ __device__ __forceinline__ bool march_node(const Ray& r, int nx, int ny, int nz,
     return false;
 }
 
+// ----------------------------------------------- march with sample skipping
+// Same sample sequence, positions, crossing test and secant as march_node,
+// but a sample is not evaluated when the min/max of the 4x4x4-cell
+// sub-block holding its cell proves that the interpolated value lies
+// strictly on one side of iso: then only its side matters to the crossing
+// test.  The margin 2^-16 * max(|lo|,|hi|,|iso|) is far above the rounding
+// error of the trilinear form (convex weights for non-negative positions),
+// so the proven side is the side the float evaluation would give.  A value
+// the secant needs (the last sample before a crossing) is evaluated exactly.
+
+enum Side : int { SIDE_NONE = 0, SIDE_ABOVE = 1, SIDE_BELOW = 2 };   // value above / below iso
+
+__device__ __forceinline__ int side_of(float iso, float v)
+{
+    return (iso - v) < 0.0f ? SIDE_ABOVE : ((iso - v) > 0.0f ? SIDE_BELOW : SIDE_NONE);
+}
+
+struct SubGrid {
+    const float2* mm;   // G^3 (min, max) of this brick, x-major, or null
+    int G;              // sub-blocks per axis (cube / 4)
+    int cx0, cy0, cz0;  // global voxel of the brick's first core cell
+};
+
+__device__ __forceinline__ int proven_side(const SubGrid& g, float iso, float px, float py, float pz, int& last_sb,
+                                           float2& last_mm)
+{
+    if (!g.mm || !(px >= 0.0f) || !(py >= 0.0f) || !(pz >= 0.0f) || !coord_ok(px) || !coord_ok(py) || !coord_ok(pz))
+        return SIDE_NONE;
+    int cx = (int)px - g.cx0, cy = (int)py - g.cy0, cz = (int)pz - g.cz0;
+    int lim = 4 * g.G;
+    if ((unsigned)cx >= (unsigned)lim || (unsigned)cy >= (unsigned)lim || (unsigned)cz >= (unsigned)lim)
+        return SIDE_NONE;
+    int sb = ((cx >> 2) * g.G + (cy >> 2)) * g.G + (cz >> 2);
+    if (sb != last_sb) {
+        last_mm = g.mm[sb];
+        last_sb = sb;
+    }
+    float lo = last_mm.x, hi = last_mm.y;
+    float M = fmaxf(fmaxf(fabsf(lo), fabsf(hi)), fabsf(iso));
+    float m = M * 1.52587890625e-05f + 1e-30f;   // 2^-16 * M
+    if (lo > iso && (lo - iso) > m) return SIDE_ABOVE;
+    if (hi < iso && (iso - hi) > m) return SIDE_BELOW;
+    return SIDE_NONE;
+}
+
+__device__ __forceinline__ bool march_node_skip(const Ray& r, int nx, int ny, int nz, int side,
+                                                const float* __restrict__ slot, int S, int bx, int by, int bz,
+                                                const SubGrid& g, float iso, float& Xx, float& Xy, float& Xz,
+                                                uint32_t& err, uint32_t& nsamples)
+{
+    float tnear, tfar;
+    if (!slab(r, nx, ny, nz, side, tnear, tfar)) return false;
+    float X0 = r.ox + tnear * r.dx, X1 = r.oy + tnear * r.dy, X2 = r.oz + tnear * r.dz;
+    const float s0 = 0.5f * r.dx, s1 = 0.5f * r.dy, s2 = 0.5f * r.dz;
+    int maxStep = cvt_rzi(ceilf((tfar - tnear) * 2.0f));
+    if (maxStep > 4 * side + 8) maxStep = 4 * side + 8;
+    if (maxStep < 0) return false;
+    int last_sb = -1;
+    float2 last_mm = make_float2(0.0f, 0.0f);
+    // the previous sample: its position F*, its value ant (valid only when
+    // ant_known) and its side
+    float F0 = X0, F1 = X1, F2 = X2;
+    float ant = 0.0f;
+    bool ant_known;
+    int ant_side = proven_side(g, iso, X0, X1, X2, last_sb, last_mm);
+    if (ant_side == SIDE_NONE) {
+        ant = sample(slot, S, bx, by, bz, X0, X1, X2, err);
+        nsamples += 1;
+        ant_known = true;
+        ant_side = side_of(iso, ant);
+    } else {
+        ant_known = false;
+    }
+    for (int steps = 1; steps <= maxStep; ++steps) {
+        X0 = X0 + s0; X1 = X1 + s1; X2 = X2 + s2;
+        int sd = proven_side(g, iso, X0, X1, X2, last_sb, last_mm);
+        if (sd != SIDE_NONE && sd == ant_side) {   // same strict side: the reference's "ant = sig"
+            F0 = X0; F1 = X1; F2 = X2;
+            ant_known = false;
+            continue;
+        }
+        float sig = sample(slot, S, bx, by, bz, X0, X1, X2, err);
+        nsamples += 1;
+        int sig_side = side_of(iso, sig);
+        if (sig_side != SIDE_NONE && sig_side == ant_side) {
+            ant = sig;
+            ant_known = true;
+            F0 = X0; F1 = X1; F2 = X2;
+            continue;
+        }
+        if (!ant_known) {   // the secant needs the previous value itself
+            ant = sample(slot, S, bx, by, bz, F0, F1, F2, err);
+            nsamples += 1;
+        }
+        float a = (iso - ant) / (sig - ant);
+        Xx = F0 * (1.0f - a) + X0 * a;
+        Xy = F1 * (1.0f - a) + X1 * a;
+        Xz = F2 * (1.0f - a) + X2 * a;
+        return true;
+    }
+    return false;
+}
+
 // rayCaster.cu:131-137 and 238-251
 __device__ __forceinline__ void shade(const float* __restrict__ slot, int S, int bx, int by, int bz,
                                       float Xx, float Xy, float Xz,

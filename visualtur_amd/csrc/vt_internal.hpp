@@ -24,6 +24,8 @@ struct BrickView {
     size_t slot_elems;
     int level_c, shift;          // shift = 3*(levelO - levelC)
     uint64_t brick_base;         // 8^levelC
+    const float2* submm;         // per slot G^3 sub-block (min, max), or null
+    int G;                       // sub-blocks per axis (cubeDim / 4)
 };
 
 struct Shading {
@@ -67,6 +69,8 @@ struct RenderArgs {
     uint32_t* req_list;
     uint8_t* slot_ref;
     uint8_t* brick_trace;        // optional, 1 byte per brick
+    const float2* submm;         // sub-block min/max per slot, or null (no skipping)
+    int G;
     FrameCounters* ctr;
     float4* rgba;
     HitRecord* hits;
@@ -161,6 +165,9 @@ hipError_t launch_bricks_from_volume(const float* vol, int X, int Y, int Z,
                                      const int64_t* brick_codes, const int32_t* slots, int count,
                                      int level_c, int n, int cube_dim, int inc,
                                      float* pool, hipStream_t s);
+// (min, max) of every 4x4x4-cell sub-block (5^3 samples) of the given slots
+hipError_t launch_submm(const float* pool, const int32_t* slots, int count, int S, int inc, int G, float2* submm,
+                        hipStream_t s);
 // Copy whole brick slots from (device-visible) host memory into the pool.
 hipError_t launch_copy_bricks(const float* const* src, const int32_t* dst_slots, int count,
                               size_t slot_elems, float* pool, hipStream_t s);

@@ -66,7 +66,14 @@ __device__ __forceinline__ int walk_ray(const Ray& r, const OctreeView& oct, con
         const float* sp = bv.pool + (size_t)slot * bv.slot_elems;
         float Xx, Xy, Xz;
         nnodes += 1;
-        if (march_node(r, nx, ny, nz, side, sp, bv.S, bx, by, bz, sh.iso, Xx, Xy, Xz, err, nsamples)) {
+        bool hit;
+        if (bv.submm) {
+            SubGrid g{bv.submm + (size_t)slot * bv.G * bv.G * bv.G, bv.G, bx + bv.inc, by + bv.inc, bz + bv.inc};
+            hit = march_node_skip(r, nx, ny, nz, side, sp, bv.S, bx, by, bz, g, sh.iso, Xx, Xy, Xz, err, nsamples);
+        } else {
+            hit = march_node(r, nx, ny, nz, side, sp, bv.S, bx, by, bz, sh.iso, Xx, Xy, Xz, err, nsamples);
+        }
+        if (hit) {
             shade(sp, bv.S, bx, by, bz, Xx, Xy, Xz, sh.lx, sh.ly, sh.lz, sh.color_scale, out, err);
             nsamples += 6;
             h.node = node;
@@ -122,7 +129,7 @@ __global__ __launch_bounds__(kBlock) void k_render_pass(RenderArgs a)
         } else {
             node = a.ray_node[id];   // parked on this node, not yet marched
         }
-        BrickView bv{a.brick_slot, a.pool, a.S, a.inc, a.slot_elems, a.level_c, a.shift, a.brick_base};
+        BrickView bv{a.brick_slot, a.pool, a.S, a.inc, a.slot_elems, a.level_c, a.shift, a.brick_base, a.submm, a.G};
         Shading sh{a.iso, a.lx, a.ly, a.lz, a.color_scale};
         float4 c;
         HitRecord h;
@@ -490,6 +497,40 @@ hipError_t launch_copy_probe(const float* src, float* dst, size_t bytes, hipStre
     size_t n4 = bytes / 16;
     hipLaunchKernelGGL(k_copy_probe, dim3(2048), dim3(256), 0, s, reinterpret_cast<const float4*>(src),
                        reinterpret_cast<float4*>(dst), n4);
+    return hipGetLastError();
+}
+
+// one workgroup per (slot, sub-block row): min and max of the 5^3 samples
+// of each 4x4x4-cell sub-block of a brick's core
+__global__ __launch_bounds__(64) void k_submm(const float* __restrict__ pool, const int32_t* __restrict__ slots,
+                                              int S, int inc, int G, float2* __restrict__ submm)
+{
+    const int b = blockIdx.x;   // brick in the list
+    const int slot = slots[b];
+    const float* p = pool + (size_t)slot * S * S * S;
+    float2* out = submm + (size_t)slot * G * G * G;
+    for (int sb = threadIdx.x; sb < G * G * G; sb += blockDim.x) {
+        int gx = sb / (G * G), gy = (sb / G) % G, gz = sb % G;
+        float lo = 3.0e38f, hi = -3.0e38f;
+        bool nan = false;
+        for (int x = 0; x <= 4; ++x)
+            for (int y = 0; y <= 4; ++y)
+                for (int z = 0; z <= 4; ++z) {
+                    float v = p[((size_t)(inc + 4 * gx + x) * S + (inc + 4 * gy + y)) * S + (inc + 4 * gz + z)];
+                    nan |= (v != v);
+                    lo = fminf(lo, v);
+                    hi = fmaxf(hi, v);
+                }
+        // a NaN sample makes every comparison false, so nothing is skipped
+        out[sb] = nan ? make_float2(__builtin_nanf(""), __builtin_nanf("")) : make_float2(lo, hi);
+    }
+}
+
+hipError_t launch_submm(const float* pool, const int32_t* slots, int count, int S, int inc, int G, float2* submm,
+                        hipStream_t s)
+{
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_submm, dim3(count), dim3(64), 0, s, pool, slots, S, inc, G, submm);
     return hipGetLastError();
 }
 

@@ -163,6 +163,8 @@ int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t
                                           c.d_slots.p, (int)n_load, c.level_c, c.n, c.cube, c.inc, c.d_pool,
                                           ctx->stream));
     }
+    if (c.d_submm)
+        HIP_TRY(launch_submm(c.d_pool, c.d_slots.p, (int)n_load, c.S, c.inc, c.G, c.d_submm, ctx->stream));
     HIP_TRY(launch_scatter_i32(c.d_table, c.d_tidx.p, c.d_tval.p, (int)nt, ctx->stream));
     // the staging buffers are reused by the next call: finish the copies
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -417,6 +419,11 @@ int vt_cache_create(vt_context* ctx, int32_t levelC, int32_t cubeDim, int32_t cu
     HIP_TRY(hipMemset(c->d_table, 0xff, (size_t)c->nbricks * 4));
     HIP_TRY(hipMalloc((void**)&c->d_ref, (size_t)c->gpu_slots));
     HIP_TRY(hipMemset(c->d_ref, 0, (size_t)c->gpu_slots));
+    if (cubeDim % 4 == 0 && cubeInc >= 1) {   // sub-blocks need the +1 sample of the halo
+        c->G = cubeDim / 4;
+        size_t mm = (size_t)c->gpu_slots * c->G * c->G * c->G * sizeof(float2);
+        if (hipMalloc((void**)&c->d_submm, mm) != hipSuccess) return fail(VT_E_NOMEM, "sub-block min/max");
+    }
     c->table.assign((size_t)c->nbricks, -1);
     c->slot_brick.assign((size_t)c->gpu_slots, -1);
     c->ref.assign((size_t)c->gpu_slots, 0);
@@ -567,6 +574,13 @@ int vt_set_frame_options(vt_context* ctx, int collect_stats, int record_trace)
     return VT_OK;
 }
 
+int vt_set_march_options(vt_context* ctx, int skip_empty)
+{
+    if (!ctx) return fail(VT_E_ARG, "ctx is NULL");
+    ctx->skip_empty = skip_empty ? 1 : 0;
+    return VT_OK;
+}
+
 int vt_get_brick_trace(vt_context* ctx, uint8_t* out, int64_t n)
 {
     if (!ctx || !out) return fail(VT_E_ARG, "NULL argument");
@@ -632,6 +646,8 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
     a.req_list = ctx->req_list.p;
     a.slot_ref = c.d_ref;
     a.brick_trace = ctx->record_trace ? ctx->trace.p : nullptr;
+    a.submm = ctx->skip_empty ? c.d_submm : nullptr;
+    a.G = c.G;
     a.ctr = ctx->d_ctr;
     a.rgba = reinterpret_cast<float4*>(d_rgba);
     a.hits = hits;

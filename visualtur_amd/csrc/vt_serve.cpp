@@ -25,6 +25,8 @@ struct Scene {
     size_t slot_elems = 0;
     float* d_pool = nullptr;
     int32_t* d_slot = nullptr;
+    float2* d_submm = nullptr;
+    int G = 0;
     std::vector<std::vector<float>> mn, mx;   // per level 0..levelO, 8^l nodes
     std::map<uint32_t, uint8_t*> tables;      // isovalue bits -> membership bytes
     std::map<uint32_t, int> root;
@@ -88,6 +90,7 @@ struct vt_server {
     ServeCounters* d_ctr = nullptr;
     PinBuf<ServeTile> h_work;
     PinBuf<ServeCounters> h_ctr;
+    PinBuf<ServeJob> h_jobs;   // pinned staging, one entry per job slot (async copies read it later)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     ~vt_server()
@@ -97,6 +100,7 @@ struct vt_server {
         for (auto& s : scenes) {
             if (s->d_pool) (void)hipFree(s->d_pool);
             if (s->d_slot) (void)hipFree(s->d_slot);
+            if (s->d_submm) (void)hipFree(s->d_submm);
             for (auto& t : s->tables) (void)hipFree(t.second);
         }
         if (arena.base) (void)hipFree(arena.base);
@@ -105,6 +109,7 @@ struct vt_server {
         d_work.release();
         h_work.release();
         h_ctr.release();
+        h_jobs.release();
     }
 };
 
@@ -162,7 +167,8 @@ int vt_server_create(vt_context* ctx, int64_t hbm_budget_bytes, int32_t max_tile
     sv->max_tiles = max_tiles;
     if (!sv->arena.init((size_t)hbm_budget_bytes)) return fail(VT_E_NOMEM, "serving arena");
     if (sv->d_jobs.ensure((size_t)sv->max_jobs) != hipSuccess || sv->d_work.ensure((size_t)max_tiles) != hipSuccess ||
-        sv->h_work.ensure((size_t)max_tiles) != hipSuccess || sv->h_ctr.ensure(1) != hipSuccess)
+        sv->h_work.ensure((size_t)max_tiles) != hipSuccess || sv->h_ctr.ensure(1) != hipSuccess ||
+        sv->h_jobs.ensure((size_t)sv->max_jobs) != hipSuccess)
         return fail(VT_E_NOMEM, "serving buffers");
     HIP_TRY(hipMalloc((void**)&sv->d_ctr, sizeof(ServeCounters)));
     HIP_TRY(hipMemset(sv->d_ctr, 0, sizeof(ServeCounters)));
@@ -245,10 +251,17 @@ int vt_server_add_scene(vt_server* sv, const float* d_vol, int32_t X, int32_t Y,
     DevBuf<int64_t> dcodes;
     if (dcodes.ensure((size_t)nb) != hipSuccess) return fail(VT_E_NOMEM, "codes");
     HIP_TRY(hipMemcpy(dcodes.p, codes.data(), (size_t)nb * 8, hipMemcpyHostToDevice));
+    if (cube % 4 == 0 && cubeInc >= 1) {
+        sc->G = cube / 4;
+        size_t mm = (size_t)nb * sc->G * sc->G * sc->G * sizeof(float2);
+        if (hipMalloc((void**)&sc->d_submm, mm) != hipSuccess) return fail(VT_E_NOMEM, "sub-block min/max");
+    }
     for (int64_t o = 0; o < nb; o += 65536) {
         int cnt = (int)std::min<int64_t>(65536, nb - o);
         HIP_TRY(launch_bricks_from_volume(d_vol, X, Y, Z, dcodes.p + o, sc->d_slot + o, cnt, levelC, n, cube, cubeInc,
                                           sc->d_pool, s));
+        if (sc->d_submm)
+            HIP_TRY(launch_submm(sc->d_pool, sc->d_slot + o, cnt, sc->S, cubeInc, sc->G, sc->d_submm, s));
     }
     HIP_TRY(hipStreamSynchronize(s));
     *scene_out = (int32_t)sv->scenes.size();
@@ -303,7 +316,9 @@ int vt_server_run(vt_server* sv, const vt_request* reqs, int64_t nreq, int32_t m
             uint8_t* table = nullptr;
             int root = 0;
             if (int rc = scene_table(sv, sc, q.iso, &table, &root)) return rc;
-            ServeJob J;
+            const int job = free_jobs.back();
+            free_jobs.pop_back();
+            ServeJob& J = sv->h_jobs.p[job];
             memcpy(&J.cam, &q.cam, sizeof J.cam);
             J.oct.child_bytes = table;
             for (int l = 0; l < 24; ++l) J.oct.level_off[l] = sc.level_off[(size_t)l];
@@ -311,13 +326,11 @@ int vt_server_run(vt_server* sv, const vt_request* reqs, int64_t nreq, int32_t m
             J.oct.final_level = sc.level_o;
             J.oct.root_valid = root;
             J.bv = BrickView{sc.d_slot, sc.d_pool, sc.S, sc.inc, sc.slot_elems, sc.level_c,
-                             3 * (sc.level_o - sc.level_c), 1ull << (3 * sc.level_c)};
+                             3 * (sc.level_o - sc.level_c), 1ull << (3 * sc.level_c), sc.d_submm, sc.G};
             J.sh = Shading{q.iso, q.light[0], q.light[1], q.light[2], q.color_scale};
             J.rgba = reinterpret_cast<float4*>(sv->arena.base + off);
             J.hits = reinterpret_cast<HitRecord*>(sv->arena.base + off + (size_t)q.cam.W * q.cam.H * 16);
             J.tiles_x = (q.cam.W + 15) / 16;
-            int job = free_jobs.back();
-            free_jobs.pop_back();
             HIP_TRY(hipMemcpyAsync(sv->d_jobs.p + job, &J, sizeof J, hipMemcpyHostToDevice, s));
             Live L;
             L.req = order[next];
