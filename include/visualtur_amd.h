@@ -175,10 +175,15 @@ int vt_render_frame(vt_context *ctx, const vt_camera *cam, const vt_render_param
                     float *d_rgba, vt_hit *d_hits, vt_frame_stats *stats);
 /* Enable per-frame sample/node counters and the visited-brick trace. */
 int vt_set_frame_options(vt_context *ctx, int collect_stats, int record_trace);
-/* March option (no reference counterpart): skip_empty=1 (default) skips
- * the evaluation of samples whose 4x4x4-cell sub-block lies strictly on one
- * side of iso; results are identical either way. */
-int vt_set_march_options(vt_context *ctx, int skip_empty);
+/* Frame options with no reference counterpart; results are identical for
+ * every setting.  skip_empty=1 (default) skips the evaluation of samples
+ * whose 4x4x4-cell sub-block lies strictly on one side of iso.  max_defer
+ * (0..16, default 4) is how many nodes with an absent brick a ray may walk
+ * past in one pass (their bricks are requested, and they are marched in a
+ * later pass before the ray's tentative result is accepted); 0 parks a ray
+ * at its first absent brick, as the reference's staged loop does
+ * (visualTur.cu:175-220). */
+int vt_set_march_options(vt_context *ctx, int skip_empty, int max_defer);
 /* Visited-brick trace of the last frame: out[i] = 1 if brick
  * (i + 8^levelC) was marched by at least one ray.  n = 8^levelC. */
 int vt_get_brick_trace(vt_context *ctx, uint8_t *out, int64_t n);

@@ -74,15 +74,20 @@ def test_render_all_resident(vt, vto, ctx, level_o, level_c, inc):
         assert (hc["node"] != 0).mean() > 0.1   # the view actually hits the surface
 
 
-@pytest.mark.parametrize("slots", [1, 2, 5, 17])
-def test_capacity_invariance(vt, vto, ctx, slots):
+@pytest.mark.parametrize("slots,defer", [(1, 0), (2, 0), (5, 0), (17, 0), (1, 4), (2, 16), (5, 1), (17, 4)])
+def test_capacity_invariance(vt, vto, ctx, slots, defer):
     """Hits do not depend on the device cache capacity (the reference's GPU
-    tier could overwrite a brick another ray still used, lruCache.cu:254-266)."""
+    tier could overwrite a brick another ray still used, lruCache.cu:254-266)
+    nor on how many absent-brick nodes a ray walks past per pass."""
     vol = blobs_volume(64, 12, seed=11)
     sc = Scene(vto, vol, 0.25, 3, 2)
     sc.setup_gpu(vt, ctx, slots=slots)
     cam = orbit_camera(vt, 64, 64, (32, 32, 32), 100.0, 30.0)
-    rg, hg, st = sc.render_gpu(vt, ctx, cam)
+    ctx.set_march_options(skip_empty=True, max_defer=defer)
+    try:
+        rg, hg, st = sc.render_gpu(vt, ctx, cam)
+    finally:
+        ctx.set_march_options()
     rc, hc = sc.render_cpu(vto, cam)
     assert_same(hg, rg, hc, rc)
     assert st.passes >= 2
@@ -224,3 +229,36 @@ def test_sample_skipping_is_exact(vt, vto, ctx, iso, seed):
     assert_same(h1, r1, hc, rc)
     assert s1.samples < s0.samples
     print(f"samples without skip {s0.samples}, with skip {s1.samples}", flush=True)
+
+
+@pytest.mark.parametrize("level_o,slots", [(3, 40), (4, 64), (4, 512)])
+def test_deferred_nodes_are_exact(vt, vto, ctx, level_o, slots):
+    """Walking past absent bricks (marching them in a later pass, before the
+    tentative result is accepted) changes the pass count, never a byte of the
+    frame.  512 slots hold every brick: then only the cold start defers."""
+    vol = blobs_volume(64, 16, seed=51)
+    sc = Scene(vto, vol, 0.3, level_o, 3)
+    sc.setup_gpu(vt, ctx, slots=slots)
+    ctx.set_frame_options(collect_stats=True)
+    try:
+        for theta in (10.0, 140.0):
+            cam = orbit_camera(vt, 96, 80, (32, 32, 32), 110.0, theta, 25.0)
+            rc, hc = sc.render_cpu(vto, cam)
+            ref, passes = None, {}
+            for defer in (0, 1, 4, 16):
+                log(f"defer={defer} levelO={level_o} slots={slots} theta={theta}")
+                ctx.set_march_options(skip_empty=True, max_defer=defer)
+                ctx.cache_clear()
+                rg, hg, st = sc.render_gpu(vt, ctx, cam)
+                assert st.err == 0
+                passes[defer] = st.passes
+                if ref is None:
+                    assert_same(hg, rg, hc, rc)
+                    ref = (hg.tobytes(), rg.tobytes())
+                else:
+                    assert (hg.tobytes(), rg.tobytes()) == ref, defer
+            print(f"passes by max_defer: {passes}", flush=True)
+            assert passes[16] < passes[0]
+    finally:
+        ctx.set_march_options()
+        ctx.set_frame_options(False, False)

@@ -118,7 +118,7 @@ _SIGS = {
                                        ctypes.POINTER(FrameStats)]),
     "vt_set_frame_options": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
     "vt_get_brick_trace": (ctypes.c_int, [_P, _P, ctypes.c_int64]),
-    "vt_set_march_options": (ctypes.c_int, [_P, ctypes.c_int]),
+    "vt_set_march_options": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
     "vt_trace_nodes": (ctypes.c_int, [_P, ctypes.POINTER(Camera), ctypes.c_int32, ctypes.c_int32,
                                       ctypes.c_int32, _P, _P]),
     "vt_octree_build": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
@@ -373,8 +373,8 @@ class Context:
                                       d_hits.ptr if d_hits is not None else None, ctypes.byref(st)))
         return st
 
-    def set_march_options(self, skip_empty: bool = True) -> None:
-        self._c(lib().vt_set_march_options(self.handle, int(skip_empty)))
+    def set_march_options(self, skip_empty: bool = True, max_defer: int = 4) -> None:
+        self._c(lib().vt_set_march_options(self.handle, int(skip_empty), int(max_defer)))
 
     def brick_trace(self, level_c: int) -> np.ndarray:
         n = 1 << (3 * level_c)

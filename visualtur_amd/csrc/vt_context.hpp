@@ -325,6 +325,7 @@ struct vt_context {
     // ray state
     DevBuf<uint64_t> ray_node;
     DevBuf<uint8_t> ray_phase;
+    DevBuf<uint64_t> ray_defer;   // [max_defer][R] deferred nodes of parked rays
     DevBuf<uint32_t> list[2];
     DevBuf<HitRecord> hits_scratch;
     DevBuf<uint32_t> req_bits, req_list;
@@ -334,6 +335,7 @@ struct vt_context {
     PinBuf<uint32_t> h_req;
     int collect_stats = 0, record_trace = 0;
     int skip_empty = 1;   // sub-block sample skipping in the march
+    int max_defer = 4;    // absent-brick nodes a ray walks past per pass
     int64_t trace_n = 0;
     DevBuf<uint8_t> stage_trav;
     int64_t stage_rays = 0;
@@ -353,7 +355,7 @@ struct vt_context {
     {
         cache.reset();
         if (d_child) (void)hipFree(d_child);
-        ray_node.release(); ray_phase.release(); list[0].release(); list[1].release();
+        ray_node.release(); ray_phase.release(); ray_defer.release(); list[0].release(); list[1].release();
         hits_scratch.release(); req_bits.release(); req_list.release(); trace.release(); stage_trav.release();
         d_cand.release();
         if (d_ctr) (void)hipFree(d_ctr);

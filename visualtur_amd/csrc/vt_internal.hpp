@@ -34,7 +34,11 @@ struct Shading {
 
 enum WalkResult : int { WALK_MISS = 0, WALK_HIT = 1, WALK_ABSENT = 2 };
 
-enum RayPhase : uint8_t { PHASE_START = 0, PHASE_NEED = 1, PHASE_DONE = 2 };
+// Parked-ray phase byte: bits 0-4 = entries in the ray's deferral list,
+// bit 7 = the output already holds the ray's tentative result (the walk
+// ended past its deferred nodes; a deferred node that crosses overrides it).
+enum RayPhase : uint8_t { PHASE_NMASK = 0x1f, PHASE_TENT = 0x80 };
+constexpr int kMaxDefer = 16;
 
 // per-frame device counters (one 64-byte block)
 struct FrameCounters {
@@ -60,8 +64,10 @@ struct RenderArgs {
     size_t slot_elems;           // S^3
     float iso, lx, ly, lz, color_scale;
     // ray state
-    uint64_t* ray_node;
+    uint64_t* ray_node;          // resume node of a parked ray
     uint8_t* ray_phase;
+    uint64_t* ray_defer;         // [max_defer][R] deferred nodes, or null
+    int max_defer;               // 0: park at the first absent brick
     const uint32_t* in_list;     // null: first pass over the whole frame
     uint32_t in_count;
     uint32_t* out_list;

@@ -35,6 +35,50 @@ __device__ __forceinline__ int floor_to_int(float x) { return cvt_rzi(floorf(x))
 
 }  // namespace
 
+__device__ __forceinline__ void set_miss(float4& out, HitRecord& h)
+{
+    out = make_float4(1.0f, 1.0f, 1.0f, 1.0f);   // rayCaster.cu:146-153
+    h.node = 0; h.brick = 0; h.voxel[0] = h.voxel[1] = h.voxel[2] = 0; h.depth = 0.0f;
+}
+
+// Marches node `node` of brick `brick`, resident in `slot`; on a crossing,
+// shades it into out/h and returns true.
+__device__ __forceinline__ bool visit_node(const Ray& r, const OctreeView& oct, const BrickView& bv, const Shading& sh,
+                                           uint64_t node, uint64_t brick, int slot, uint8_t* slot_ref,
+                                           uint8_t* brick_trace, float4& out, HitRecord& h, uint32_t& err,
+                                           uint32_t& nsamples, uint32_t& nnodes)
+{
+    const int n = oct.n, levelO = oct.final_level;
+    const int side = 1 << (n - levelO);
+    if (slot_ref) slot_ref[slot] = 1;
+    if (brick_trace) brick_trace[brick - bv.brick_base] = 1;
+    int nx, ny, nz, bx, by, bz;
+    node_min(node, levelO, n, nx, ny, nz);
+    node_min(brick, bv.level_c, n, bx, by, bz);
+    bx -= bv.inc; by -= bv.inc; bz -= bv.inc;
+    const float* sp = bv.pool + (size_t)slot * bv.slot_elems;
+    float Xx, Xy, Xz;
+    nnodes += 1;
+    bool hit;
+    if (bv.submm) {
+        SubGrid g{bv.submm + (size_t)slot * bv.G * bv.G * bv.G, bv.G, bx + bv.inc, by + bv.inc, bz + bv.inc};
+        hit = march_node_skip(r, nx, ny, nz, side, sp, bv.S, bx, by, bz, g, sh.iso, Xx, Xy, Xz, err, nsamples);
+    } else {
+        hit = march_node(r, nx, ny, nz, side, sp, bv.S, bx, by, bz, sh.iso, Xx, Xy, Xz, err, nsamples);
+    }
+    if (!hit) return false;
+    shade(sp, bv.S, bx, by, bz, Xx, Xy, Xz, sh.lx, sh.ly, sh.lz, sh.color_scale, out, err);
+    nsamples += 6;
+    h.node = node;
+    h.brick = brick;
+    h.voxel[0] = floor_to_int(Xx);
+    h.voxel[1] = floor_to_int(Xy);
+    h.voxel[2] = floor_to_int(Xz);
+    float dx = Xx - r.ox, dy = Xy - r.oy, dz = Xz - r.oz;
+    h.depth = sqrtf(dx * dx + dy * dy + dz * dz);
+    return true;
+}
+
 // Walks one ray from `node` (the next node to march; 0 = exhausted) until it
 // hits, runs out of nodes, or needs a brick that is not resident.
 //   WALK_HIT    : out/h hold the shaded hit
@@ -45,53 +89,66 @@ __device__ __forceinline__ int walk_ray(const Ray& r, const OctreeView& oct, con
                                         HitRecord& h, uint64_t& bidx, uint32_t& err, uint32_t& nsamples,
                                         uint32_t& nnodes)
 {
-    const int n = oct.n, levelO = oct.final_level;
-    const int side = 1 << (n - levelO);
     for (;;) {
         if (node == 0) {
-            out = make_float4(1.0f, 1.0f, 1.0f, 1.0f);   // rayCaster.cu:146-153
-            h.node = 0; h.brick = 0; h.voxel[0] = h.voxel[1] = h.voxel[2] = 0; h.depth = 0.0f;
+            set_miss(out, h);
             return WALK_MISS;
         }
         uint64_t brick = node >> bv.shift;
         bidx = brick - bv.brick_base;
         int slot = bv.brick_slot[bidx];
         if (slot < 0) return WALK_ABSENT;
-        if (slot_ref) slot_ref[slot] = 1;
-        if (brick_trace) brick_trace[bidx] = 1;
-        int nx, ny, nz, bx, by, bz;
-        node_min(node, levelO, n, nx, ny, nz);
-        node_min(brick, bv.level_c, n, bx, by, bz);
-        bx -= bv.inc; by -= bv.inc; bz -= bv.inc;
-        const float* sp = bv.pool + (size_t)slot * bv.slot_elems;
-        float Xx, Xy, Xz;
-        nnodes += 1;
-        bool hit;
-        if (bv.submm) {
-            SubGrid g{bv.submm + (size_t)slot * bv.G * bv.G * bv.G, bv.G, bx + bv.inc, by + bv.inc, bz + bv.inc};
-            hit = march_node_skip(r, nx, ny, nz, side, sp, bv.S, bx, by, bz, g, sh.iso, Xx, Xy, Xz, err, nsamples);
-        } else {
-            hit = march_node(r, nx, ny, nz, side, sp, bv.S, bx, by, bz, sh.iso, Xx, Xy, Xz, err, nsamples);
+        if (visit_node(r, oct, bv, sh, node, brick, slot, slot_ref, brick_trace, out, h, err, nsamples, nnodes))
+            return WALK_HIT;
+        node = next_node(r, oct, node, false);
+    }
+}
+
+// Adds brick `bidx` to this pass's request set (first request appends it).
+__device__ __forceinline__ void request_brick(const RenderArgs& a, uint64_t bidx)
+{
+    uint32_t bit = 1u << (bidx & 31);
+    uint32_t old = atomicOr(&a.req_bits[bidx >> 5], bit);
+    if (!(old & bit)) a.req_list[atomicAdd(&a.ctr->req_count, 1u)] = (uint32_t)bidx;
+}
+
+// walk_ray of the frame passes: a node whose brick is absent is deferred
+// (appended to the ray's deferral list, its brick requested) while the list
+// has room, and the walk goes on past it.  The first crossing in a deferred
+// node, marched in a later pass, precedes every crossing found beyond it,
+// so a walk that ends with nd > 0 gives a tentative result.
+//   WALK_HIT / WALK_MISS : the walk ended (tentative when nd > 0)
+//   WALK_ABSENT          : the list is full and `node`'s brick (bidx) is absent
+__device__ __forceinline__ int walk_defer(const Ray& r, const RenderArgs& a, const BrickView& bv, const Shading& sh,
+                                          uint64_t& node, uint32_t id, uint32_t R, int& nd, float4& out,
+                                          HitRecord& h, uint64_t& bidx, uint32_t& err, uint32_t& nsamples,
+                                          uint32_t& nnodes)
+{
+    for (;;) {
+        if (node == 0) {
+            set_miss(out, h);
+            return WALK_MISS;
         }
-        if (hit) {
-            shade(sp, bv.S, bx, by, bz, Xx, Xy, Xz, sh.lx, sh.ly, sh.lz, sh.color_scale, out, err);
-            nsamples += 6;
-            h.node = node;
-            h.brick = brick;
-            h.voxel[0] = floor_to_int(Xx);
-            h.voxel[1] = floor_to_int(Xy);
-            h.voxel[2] = floor_to_int(Xz);
-            float dx = Xx - r.ox, dy = Xy - r.oy, dz = Xz - r.oz;
-            h.depth = sqrtf(dx * dx + dy * dy + dz * dz);
+        uint64_t brick = node >> bv.shift;
+        bidx = brick - bv.brick_base;
+        int slot = bv.brick_slot[bidx];
+        if (slot < 0) {
+            if (nd >= a.max_defer) return WALK_ABSENT;
+            a.ray_defer[(size_t)nd * R + id] = node;
+            nd += 1;
+            request_brick(a, bidx);
+        } else if (visit_node(r, a.oct, bv, sh, node, brick, slot, a.slot_ref, a.brick_trace, out, h, err, nsamples,
+                              nnodes)) {
             return WALK_HIT;
         }
-        node = next_node(r, oct, node, false);
+        node = next_node(r, a.oct, node, false);
     }
 }
 
 __global__ __launch_bounds__(kBlock) void k_render_pass(RenderArgs a)
 {
     const int W = a.cam.W, H = a.cam.H;
+    const uint32_t R = (uint32_t)W * (uint32_t)H;
     uint32_t id;
     bool active = true;
     if (a.in_list) {
@@ -116,46 +173,76 @@ __global__ __launch_bounds__(kBlock) void k_render_pass(RenderArgs a)
 
     uint32_t err = 0, nsamples = 0, nnodes = 0;
     bool park = false;
-    uint32_t req_idx = 0;
-    bool new_req = false;
 
     if (active) {
         int i = (int)(id / (uint32_t)W), j = (int)(id % (uint32_t)W);
         Ray r;
         make_ray(a.cam, i, j, r);
-        uint64_t node;
-        if (a.in_list == nullptr) {
-            node = root_ok(r, a.oct) ? next_node(r, a.oct, 1ull, true) : 0ull;
-        } else {
-            node = a.ray_node[id];   // parked on this node, not yet marched
-        }
         BrickView bv{a.brick_slot, a.pool, a.S, a.inc, a.slot_elems, a.level_c, a.shift, a.brick_base, a.submm, a.G};
         Shading sh{a.iso, a.lx, a.ly, a.lz, a.color_scale};
         float4 c;
         HitRecord h;
-        uint64_t bidx = 0;
-        int st = walk_ray(r, a.oct, bv, sh, node, a.slot_ref, a.brick_trace, c, h, bidx, err, nsamples, nnodes);
-        if (st == WALK_ABSENT) {
-            // request the brick once per pass and park the ray on `node`
-            uint32_t bit = 1u << (bidx & 31);
-            uint32_t old = atomicOr(&a.req_bits[bidx >> 5], bit);
-            new_req = !(old & bit);
-            req_idx = (uint32_t)bidx;
-            a.ray_node[id] = node;
-            a.ray_phase[id] = PHASE_NEED;
-            park = true;
+        uint64_t bidx = 0, node = 0;
+        int nd = 0;
+        // what the pass leaves: WALK_HIT/WALK_MISS = c/h to write, WALK_ABSENT =
+        // park on `node`, kKeep = the output already holds the ray's result
+        // (final when nd == 0), kBlocked = a deferred node is still absent
+        constexpr int kKeep = 3, kBlocked = 4;
+        int st;
+        if (a.in_list == nullptr) {
+            node = root_ok(r, a.oct) ? next_node(r, a.oct, 1ull, true) : 0ull;
+            st = walk_defer(r, a, bv, sh, node, id, R, nd, c, h, bidx, err, nsamples, nnodes);
         } else {
+            const uint8_t ph = a.ray_phase[id];
+            const int nprev = ph & PHASE_NMASK;
+            int k = 0;
+            bool hit = false;
+            for (; k < nprev; ++k) {   // deferred nodes, in walk order
+                uint64_t s = a.ray_defer[(size_t)k * R + id];
+                uint64_t brick = s >> bv.shift;
+                bidx = brick - bv.brick_base;
+                int slot = bv.brick_slot[bidx];
+                if (slot < 0) break;
+                if (visit_node(r, a.oct, bv, sh, s, brick, slot, a.slot_ref, a.brick_trace, c, h, err, nsamples,
+                               nnodes)) {
+                    hit = true;
+                    break;
+                }
+            }
+            if (hit) {
+                st = WALK_HIT;
+            } else if (k < nprev) {
+                for (int q = k; q < nprev; ++q) a.ray_defer[(size_t)(q - k) * R + id] = a.ray_defer[(size_t)q * R + id];
+                nd = nprev - k;
+                request_brick(a, bidx);
+                a.ray_phase[id] = (uint8_t)((ph & PHASE_TENT) | nd);   // resume node and tentative result kept
+                park = true;
+                st = kBlocked;
+            } else if (ph & PHASE_TENT) {
+                st = kKeep;   // every deferred node missed: the tentative result stands
+            } else {
+                node = a.ray_node[id];   // parked on this node, not yet marched
+                st = walk_defer(r, a, bv, sh, node, id, R, nd, c, h, bidx, err, nsamples, nnodes);
+            }
+        }
+        if (st == WALK_ABSENT) {
+            request_brick(a, bidx);
+            a.ray_node[id] = node;
+            a.ray_phase[id] = (uint8_t)nd;
+            park = true;
+        } else if (st == WALK_HIT || st == WALK_MISS) {
             a.rgba[id] = c;
             a.hits[id] = h;
-            a.ray_phase[id] = PHASE_DONE;
+            if (nd > 0) {
+                a.ray_phase[id] = (uint8_t)(PHASE_TENT | nd);
+                park = true;
+            }
         }
     }
 
-    // parked rays and first-time requests: one atomic per wave each
+    // parked rays: one atomic per wave
     uint32_t pslot = wave_atomic_add(&a.ctr->parked_count, park);
     if (park) a.out_list[pslot] = id;
-    uint32_t rslot = wave_atomic_add(&a.ctr->req_count, new_req);
-    if (new_req) a.req_list[rslot] = req_idx;
     if (err) atomicOr(&a.ctr->err, err);
     if (a.collect_stats) {
         unsigned long long s = nsamples, nn = nnodes;
@@ -459,29 +546,48 @@ hipError_t launch_bricks_from_volume(const float* vol, int X, int Y, int Z,
     return hipGetLastError();
 }
 
-// Whole-slot copies from device-visible pinned host memory (one workgroup
-// per brick, 16-byte accesses; the slot size is a multiple of 4 floats when
-// S is even, otherwise the tail loop finishes it).
+// Whole-slot copies from device-visible pinned host memory.  Each brick is
+// split over kCopySplit workgroups (blockIdx.x = part, blockIdx.y = brick)
+// and every lane keeps four 16-byte reads in flight, so that enough PCIe
+// reads are outstanding to cover their latency.  The slot size is a
+// multiple of 4 floats when S is even; otherwise part 0 copies the tail.
+constexpr int kCopySplit = 8;
+
 __global__ __launch_bounds__(256) void k_copy_bricks(const float* const* __restrict__ src,
                                                      const int32_t* __restrict__ dst_slots,
                                                      size_t slot_elems, float* __restrict__ pool)
 {
-    int b = blockIdx.x;
+    const int b = blockIdx.y;
     const float* s = src[b];
     float* d = pool + (size_t)dst_slots[b] * slot_elems;
-    size_t n4 = slot_elems / 4;
-    const float4* s4 = reinterpret_cast<const float4*>(s);
-    float4* d4 = reinterpret_cast<float4*>(d);
-    for (size_t e = threadIdx.x; e < n4; e += blockDim.x) d4[e] = s4[e];
-    for (size_t e = n4 * 4 + threadIdx.x; e < slot_elems; e += blockDim.x) d[e] = s[e];
+    const size_t n4 = slot_elems / 4;
+    const size_t per = (n4 + kCopySplit - 1) / kCopySplit;
+    const size_t beg = per * blockIdx.x, end = beg + per < n4 ? beg + per : n4;
+    const float4* __restrict__ s4 = reinterpret_cast<const float4*>(s);
+    float4* __restrict__ d4 = reinterpret_cast<float4*>(d);
+    size_t e = beg + threadIdx.x;
+    for (; e + 768 < end; e += 1024) {
+        float4 v0 = s4[e], v1 = s4[e + 256], v2 = s4[e + 512], v3 = s4[e + 768];
+        d4[e] = v0;
+        d4[e + 256] = v1;
+        d4[e + 512] = v2;
+        d4[e + 768] = v3;
+    }
+    for (; e < end; e += 256) d4[e] = s4[e];
+    if (blockIdx.x == 0)
+        for (size_t q = n4 * 4 + threadIdx.x; q < slot_elems; q += 256) d[q] = s[q];
 }
 
 hipError_t launch_copy_bricks(const float* const* src, const int32_t* dst_slots, int count,
                               size_t slot_elems, float* pool, hipStream_t s)
 {
-    if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_copy_bricks, dim3(count), dim3(256), 0, s, src, dst_slots, slot_elems, pool);
-    return hipGetLastError();
+    for (int o = 0; o < count; o += 65535) {   // grid y is at most 65535
+        const int m = count - o < 65535 ? count - o : 65535;
+        hipLaunchKernelGGL(k_copy_bricks, dim3(kCopySplit, m), dim3(256), 0, s, src + o, dst_slots + o, slot_elems,
+                           pool);
+        if (hipError_t e = hipGetLastError()) return e;
+    }
+    return hipSuccess;
 }
 
 // Plain streaming copy: reference point for the HBM byte counters.
