@@ -258,7 +258,8 @@ def test_deferred_nodes_are_exact(vt, vto, ctx, level_o, slots):
                 else:
                     assert (hg.tobytes(), rg.tobytes()) == ref, defer
             print(f"passes by max_defer: {passes}", flush=True)
-            assert passes[16] < passes[0]
+            if slots == 512:   # room for every request: deferral saves passes
+                assert passes[16] < passes[0]
     finally:
         ctx.set_march_options()
         ctx.set_frame_options(False, False)
