@@ -319,7 +319,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--parity-rays", type=int, default=4096)
     ap.add_argument("--prefetch", type=int, default=1, help="c2: also time the learned-prefetch form")
-    ap.add_argument("--max-defer", type=int, default=4,
+    ap.add_argument("--overlap", type=int, default=1, help="load bricks while the requesting pass runs")
+    ap.add_argument("--reserve", type=int, default=-1, help="free device slots kept between frames (-1: 1/8)")
+    ap.add_argument("--max-defer", type=int, default=2,
                     help="absent-brick nodes a ray may walk past per pass (0 = park at the first)")
     ap.add_argument("--train-paths", type=int, default=24)
     ap.add_argument("--train-frames", type=int, default=6)
@@ -356,6 +358,7 @@ def main() -> int:
         b_.free()
     w = setup_c2(ctx, args) if args.config == "c2" else setup_c1(ctx, args)
     ctx.set_march_options(skip_empty=True, max_defer=args.max_defer)
+    ctx.set_load_options(overlap_loads=bool(args.overlap), reserve_slots=args.reserve)
     cams = [w.cameras(k) for k in range(args.warmup + args.steps)]
     R = w.W * w.H
     d_rgba, d_hits = ctx.alloc(R * 16), ctx.alloc(R * 32)
@@ -364,7 +367,7 @@ def main() -> int:
         ctx.render_frame(cams[k], w.prm, d_rgba, d_hits)
     ctx.synchronize()
 
-    frame_ms, kernel_ms, passes, uploads, parked = [], [], [], [], []
+    frame_ms, kernel_ms, passes, uploads, parked, overl = [], [], [], [], [], []
     t0 = time.perf_counter()
     for k in range(args.warmup, args.warmup + args.steps):
         f0 = time.perf_counter()
@@ -374,6 +377,7 @@ def main() -> int:
         passes.append(st.passes)
         uploads.append(st.bricks_uploaded)
         parked.append(st.rays_parked)
+        overl.append(st.bricks_overlapped)
     ctx.synchronize()
     total_s = time.perf_counter() - t0
     rays_per_s = R * args.steps / total_s
@@ -410,6 +414,8 @@ def main() -> int:
         "kernel_ms_per_frame": round(float(np.mean(kernel_ms)), 4),
         "passes_per_frame": float(np.mean(passes)), "bricks_uploaded_per_frame": float(np.mean(uploads)),
         "rays_parked_per_frame": float(np.mean(parked)), "max_defer": args.max_defer,
+        "overlap_loads": args.overlap, "reserve_slots": args.reserve,
+        "bricks_overlapped_per_frame": float(np.mean(overl)),
         "bricks_marched_per_frame": float(np.mean(U)),
         "device_miss_rate": round(float(sum(uploads)) / max(1, sum(U)), 5),
         "roofline": roofline,

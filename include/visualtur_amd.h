@@ -85,6 +85,7 @@ typedef struct {
     double ms_total;            /* frame wall time */
     double ms_kernel;           /* device time of the render passes */
     double ms_service;          /* host time spent servicing misses */
+    int64_t bricks_overlapped;  /* of bricks_uploaded: loaded while a pass was running */
 } vt_frame_stats;
 
 typedef struct {
@@ -178,12 +179,20 @@ int vt_set_frame_options(vt_context *ctx, int collect_stats, int record_trace);
 /* Frame options with no reference counterpart; results are identical for
  * every setting.  skip_empty=1 (default) skips the evaluation of samples
  * whose 4x4x4-cell sub-block lies strictly on one side of iso.  max_defer
- * (0..16, default 4) is how many nodes with an absent brick a ray may walk
+ * (0..16, default 2) is how many nodes with an absent brick a ray may walk
  * past in one pass (their bricks are requested, and they are marched in a
  * later pass before the ray's tentative result is accepted); 0 parks a ray
  * at its first absent brick, as the reference's staged loop does
  * (visualTur.cu:175-220). */
 int vt_set_march_options(vt_context *ctx, int skip_empty, int max_defer);
+/* Brick-loading option with no reference counterpart; results are identical
+ * for every setting.  overlap_loads=1 (default) loads requested bricks into
+ * free device slots on a second stream while the render pass that requested
+ * them still runs (the reference loads between passes, visualTur.cu:175-220).
+ * reserve_slots is how many free slots vt_render_frame keeps for that at the
+ * end of a frame, evicting by the clock (-1 = gpuSlots/8; never more than
+ * half; nothing is evicted while every brick fits). */
+int vt_set_load_options(vt_context *ctx, int overlap_loads, int64_t reserve_slots);
 /* Visited-brick trace of the last frame: out[i] = 1 if brick
  * (i + 8^levelC) was marched by at least one ray.  n = 8^levelC. */
 int vt_get_brick_trace(vt_context *ctx, uint8_t *out, int64_t n);

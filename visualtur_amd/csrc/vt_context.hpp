@@ -62,13 +62,17 @@ template <class T>
 struct PinBuf {
     T* p = nullptr;
     size_t n = 0;
-    hipError_t ensure(size_t count)
+    T* dev = nullptr;   // the same memory as seen by kernels
+    hipError_t ensure(size_t count, unsigned flags = hipHostMallocDefault)
     {
         if (count <= n && p) return hipSuccess;
         if (p) (void)hipHostFree(p);
         p = nullptr;
+        dev = nullptr;
         n = 0;
-        hipError_t e = hipHostMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T), hipHostMallocDefault);
+        hipError_t e = hipHostMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T), flags | hipHostMallocMapped);
+        if (e != hipSuccess) return e;
+        e = hipHostGetDevicePointer((void**)&dev, p, 0);
         if (e == hipSuccess) n = count;
         return e;
     }
@@ -76,6 +80,7 @@ struct PinBuf {
     {
         if (p) (void)hipHostFree(p);
         p = nullptr;
+        dev = nullptr;
         n = 0;
     }
 };
@@ -241,6 +246,33 @@ struct HostTier {
     }
 };
 
+// Pinned, device-visible staging of the loads issued on the copy stream (the
+// copy, sub-block and table-scatter kernels read it in place).  Entries are
+// appended while the copy stream works and reset only when it is idle.
+struct AsyncStage {
+    PinBuf<const float*> src;
+    PinBuf<int32_t> slots;
+    PinBuf<int64_t> codes;
+    PinBuf<int64_t> tidx;
+    PinBuf<int32_t> tval;
+    int64_t used = 0, cap = 0;
+    hipError_t ensure(int64_t count)
+    {
+        hipError_t e;
+        if ((e = src.ensure((size_t)count)) || (e = slots.ensure((size_t)count)) ||
+            (e = codes.ensure((size_t)count)) || (e = tidx.ensure((size_t)count)) || (e = tval.ensure((size_t)count)))
+            return e;
+        cap = count;
+        used = 0;
+        return hipSuccess;
+    }
+    void release()
+    {
+        src.release(); slots.release(); codes.release(); tidx.release(); tval.release();
+        used = cap = 0;
+    }
+};
+
 // ----------------------------------------------------------- brick cache
 struct BrickCache {
     int level_c = 0, cube = 0, inc = 0, S = 0;
@@ -257,7 +289,11 @@ struct BrickCache {
     std::vector<uint8_t> ref, ref_tmp;
     std::vector<uint64_t> pin;
     uint64_t epoch = 0;
-    int64_t hand = 0, used = 0;
+    int64_t hand = 0, used = 0;   // slots [used, gpu_slots) were never filled
+    // Slots below `used` that hold no brick.  Loads issued while a render
+    // pass runs take only these (or never-filled slots), since no kernel can
+    // be reading them; evictions happen only while no render kernel runs.
+    std::vector<int64_t> free_list;
     // sources
     const float* d_vol = nullptr;
     int vol_dims[3] = {0, 0, 0};
@@ -287,17 +323,39 @@ struct BrickCache {
         d_src.release(); d_slots.release(); d_codes.release(); d_tidx.release(); d_tval.release();
     }
 
-    int64_t pick_victim()
+    int64_t resident() const { return used - (int64_t)free_list.size(); }
+    int64_t available() const { return (int64_t)free_list.size() + (gpu_slots - used); }
+
+    // a slot that holds no brick, or -1
+    int64_t take_free()
     {
-        if (used < gpu_slots) return used++;
+        if (!free_list.empty()) {
+            int64_t s = free_list.back();
+            free_list.pop_back();
+            return s;
+        }
+        return used < gpu_slots ? used++ : -1;
+    }
+
+    // clock over the slots that hold bricks, skipping those pinned in the
+    // current epoch and giving referenced ones a second chance
+    int64_t clock_victim()
+    {
         for (int64_t it = 0; it < 3 * gpu_slots; ++it) {
             int64_t c = hand;
             hand = (hand + 1) % gpu_slots;
-            if (pin[c] == epoch) continue;
-            if (ref[c]) { ref[c] = 0; continue; }
+            if (c >= used || slot_brick[(size_t)c] < 0) continue;
+            if (pin[(size_t)c] == epoch) continue;
+            if (ref[(size_t)c]) { ref[(size_t)c] = 0; continue; }
             return c;
         }
         return -1;
+    }
+
+    int64_t pick_victim()
+    {
+        int64_t s = take_free();
+        return s >= 0 ? s : clock_victim();
     }
 };
 
@@ -313,6 +371,7 @@ struct vt_file {
 struct vt_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t cstream = nullptr;   // brick loads that overlap a render pass
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // octree
     bool octree_ready = false;
@@ -328,14 +387,22 @@ struct vt_context {
     DevBuf<uint64_t> ray_defer;   // [max_defer][R] deferred nodes of parked rays
     DevBuf<uint32_t> list[2];
     DevBuf<HitRecord> hits_scratch;
-    DevBuf<uint32_t> req_bits, req_list;
+    DevBuf<uint32_t> req_bits;
+    // request list of the running pass: brick index + 1 per entry, written by
+    // the render kernel with system-scope stores into coherent host memory
+    // so that the host can start loads while the pass still runs; 0 = not
+    // yet written
+    PinBuf<uint32_t> req_list;
+    AsyncStage astage;
+    std::vector<uint32_t> req_left;   // requests of a pass left for the synchronous service
+    int async_loads = 1;
+    int64_t reserve_slots = -1;       // free slots kept between frames (-1: gpu_slots/8)
     DevBuf<uint8_t> trace;
     FrameCounters* d_ctr = nullptr;
     FrameCounters* h_ctr = nullptr;
-    PinBuf<uint32_t> h_req;
     int collect_stats = 0, record_trace = 0;
     int skip_empty = 1;   // sub-block sample skipping in the march
-    int max_defer = 4;    // absent-brick nodes a ray walks past per pass
+    int max_defer = 2;    // absent-brick nodes a ray walks past per pass
     int64_t trace_n = 0;
     DevBuf<uint8_t> stage_trav;
     int64_t stage_rays = 0;
@@ -356,14 +423,16 @@ struct vt_context {
         cache.reset();
         if (d_child) (void)hipFree(d_child);
         ray_node.release(); ray_phase.release(); ray_defer.release(); list[0].release(); list[1].release();
+        if (cstream) (void)hipStreamSynchronize(cstream);
         hits_scratch.release(); req_bits.release(); req_list.release(); trace.release(); stage_trav.release();
+        astage.release();
         d_cand.release();
         if (d_ctr) (void)hipFree(d_ctr);
         if (h_ctr) (void)hipHostFree(h_ctr);
-        h_req.release();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
+        if (cstream) (void)hipStreamDestroy(cstream);
     }
 };
 

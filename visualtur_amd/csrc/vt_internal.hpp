@@ -72,7 +72,7 @@ struct RenderArgs {
     uint32_t in_count;
     uint32_t* out_list;
     uint32_t* req_bits;
-    uint32_t* req_list;
+    uint32_t* req_list;          // host-visible; entry = brick index + 1
     uint8_t* slot_ref;
     uint8_t* brick_trace;        // optional, 1 byte per brick
     const float2* submm;         // sub-block min/max per slot, or null (no skipping)

@@ -104,12 +104,17 @@ __device__ __forceinline__ int walk_ray(const Ray& r, const OctreeView& oct, con
     }
 }
 
-// Adds brick `bidx` to this pass's request set (first request appends it).
+// Adds brick `bidx` to this pass's request set.  The first request appends
+// bidx + 1 to the list in coherent host memory with a system-scope store,
+// so the host can load the brick while this pass is still running.
 __device__ __forceinline__ void request_brick(const RenderArgs& a, uint64_t bidx)
 {
     uint32_t bit = 1u << (bidx & 31);
     uint32_t old = atomicOr(&a.req_bits[bidx >> 5], bit);
-    if (!(old & bit)) a.req_list[atomicAdd(&a.ctr->req_count, 1u)] = (uint32_t)bidx;
+    if (!(old & bit)) {
+        uint32_t q = atomicAdd(&a.ctr->req_count, 1u);
+        __hip_atomic_store(&a.req_list[q], (uint32_t)bidx + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 
 // walk_ray of the frame passes: a node whose brick is absent is deferred

@@ -38,9 +38,6 @@ struct vt_mlp {
     }
 };
 
-int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t* misses, int32_t* slots_out,
-                const int64_t* victims);
-
 namespace {
 
 int mlp_check(vt_mlp* m, int64_t N)
@@ -264,9 +261,9 @@ int vt_mlp_features(vt_mlp* m, const vt_camera* cam, int32_t nLevels, int32_t le
 
 // ====================================================== learned prefetch
 
-int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t* misses, int32_t* slots_out,
-                const int64_t* victims);
-int fold_ref_bits(vt_context* ctx);
+int load_async(vt_context* ctx, const uint32_t* req, size_t n, std::vector<uint32_t>& left, int64_t* loaded,
+               int64_t* misses);
+int copy_stream_idle(vt_context* ctx);
 
 extern "C" int vt_prefetch(vt_context* ctx, vt_mlp* m, const vt_camera* cam, int64_t budget, float min_prob,
                            int64_t* loaded)
@@ -309,46 +306,61 @@ extern "C" int vt_prefetch(vt_context* ctx, vt_mlp* m, const vt_camera* cam, int
             HIP_TRY(hipMemcpyAsync(score.data() + o, m->z.p, (size_t)nb * 4, hipMemcpyDeviceToHost, s));
         }
         HIP_TRY(hipStreamSynchronize(s));
-        if (int rc = fold_ref_bits(ctx)) return rc;
         const uint64_t base = 1ull << (3 * c.level_c);
-        const float zmin = logf(min_prob / (1.0f - min_prob)) ;   // logit threshold (-inf at 0)
-        std::vector<float> slot_score((size_t)c.gpu_slots, -INFINITY);
-        std::vector<int64_t> want;   // non-resident candidates, best first
+        const float zmin = logf(min_prob / (1.0f - min_prob));   // logit threshold (-inf at 0)
+        std::vector<int64_t> want;   // absent candidates above the threshold, best first
         for (int64_t i = 0; i < NC; ++i) {
             int64_t b = (int64_t)(ctx->cand_codes[(size_t)i] - base);
-            int32_t sl = c.table[(size_t)b];
-            if (sl >= 0) slot_score[(size_t)sl] = score[(size_t)i];
-            else if (score[(size_t)i] >= zmin) want.push_back(i);
+            if (c.table[(size_t)b] < 0 && score[(size_t)i] >= zmin) want.push_back(i);
         }
         std::sort(want.begin(), want.end(), [&](int64_t a, int64_t b) { return score[(size_t)a] > score[(size_t)b]; });
-        // victims: free slots first, then resident bricks by ascending score;
-        // a brick is only replaced by a better-scoring one
-        std::vector<int64_t> victims;
-        for (int64_t sl = c.used; sl < c.gpu_slots; ++sl) victims.push_back(sl);
-        std::vector<int64_t> resident;
-        for (int64_t sl = 0; sl < c.used; ++sl) resident.push_back(sl);
-        std::sort(resident.begin(), resident.end(),
-                  [&](int64_t a, int64_t b) { return slot_score[(size_t)a] < slot_score[(size_t)b]; });
-        size_t free_n = victims.size();
-        victims.insert(victims.end(), resident.begin(), resident.end());
-        std::vector<uint32_t> load;
-        std::vector<int64_t> vslots;
-        c.epoch++;
-        for (size_t q = 0; q < want.size() && (int64_t)load.size() < budget && q < victims.size(); ++q) {
-            int64_t v = victims[q];
-            float ws = score[(size_t)want[q]];
-            if (q >= free_n && !(ws > slot_score[(size_t)v])) break;
-            load.push_back((uint32_t)(ctx->cand_codes[(size_t)want[q]] - base));
-            vslots.push_back(v);
+        int64_t n = std::min<int64_t>((int64_t)want.size(), budget);
+        if (n > c.available()) {
+            // make room: evict resident bricks that score below the candidate
+            // they make room for (bricks that are not candidates score -inf)
+            if (int rc = copy_stream_idle(ctx)) return rc;
+            std::vector<float> slot_score((size_t)c.gpu_slots, -INFINITY);
+            for (int64_t i = 0; i < NC; ++i) {
+                int32_t sl = c.table[(size_t)(ctx->cand_codes[(size_t)i] - base)];
+                if (sl >= 0) slot_score[(size_t)sl] = score[(size_t)i];
+            }
+            std::vector<int64_t> res;
+            for (int64_t sl = 0; sl < c.used; ++sl)
+                if (c.slot_brick[(size_t)sl] >= 0) res.push_back(sl);
+            std::sort(res.begin(), res.end(),
+                      [&](int64_t x, int64_t y) { return slot_score[(size_t)x] < slot_score[(size_t)y]; });
+            std::vector<int64_t> idx;
+            std::vector<int32_t> val;
+            for (size_t j = 0; j < res.size() && c.available() < n; ++j) {
+                const int64_t v = res[j];
+                if (!(score[(size_t)want[(size_t)c.available()]] > slot_score[(size_t)v])) break;
+                const int64_t old = c.slot_brick[(size_t)v];
+                c.table[(size_t)old] = -1;
+                c.slot_brick[(size_t)v] = -1;
+                c.ref[(size_t)v] = 0;
+                c.free_list.push_back(v);
+                c.stat_evictions++;
+                idx.push_back(old);
+                val.push_back(-1);
+            }
+            if (!idx.empty()) {
+                DevBuf<int64_t> di;
+                DevBuf<int32_t> dv;
+                if (di.ensure(idx.size()) != hipSuccess || dv.ensure(val.size()) != hipSuccess)
+                    return fail(VT_E_NOMEM, "eviction staging");
+                HIP_TRY(hipMemcpyAsync(di.p, idx.data(), idx.size() * 8, hipMemcpyHostToDevice, s));
+                HIP_TRY(hipMemcpyAsync(dv.p, val.data(), val.size() * 4, hipMemcpyHostToDevice, s));
+                HIP_TRY(launch_scatter_i32(c.d_table, di.p, dv.p, (int)idx.size(), s));
+                HIP_TRY(hipStreamSynchronize(s));
+            }
+            n = std::min<int64_t>(n, c.available());
         }
-        // free slots are handed out in order, so `used` must advance with them
-        int64_t nfree = 0;
-        for (int64_t v : vslots)
-            if (v >= c.used) nfree++;
-        c.used += nfree;
+        // into free slots, on the copy stream, without waiting: the next
+        // render pass runs while these bricks arrive
+        std::vector<uint32_t> load((size_t)n), left;
+        for (int64_t q = 0; q < n; ++q) load[(size_t)q] = (uint32_t)(ctx->cand_codes[(size_t)want[(size_t)q]] - base);
         int64_t misses = 0;
-        if (int rc = load_bricks(ctx, load.data(), (int64_t)load.size(), &misses, nullptr, vslots.data())) return rc;
-        nloaded = (int64_t)load.size();
+        if (int rc = load_async(ctx, load.data(), load.size(), left, &nloaded, &misses)) return rc;
     }
     if (loaded) *loaded = nloaded;
     return VT_OK;

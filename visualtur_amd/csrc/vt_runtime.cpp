@@ -84,6 +84,7 @@ int fold_ref_bits(vt_context* ctx)
 
 int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t* misses, int32_t* slots_out,
                 const int64_t* victims = nullptr);
+int copy_stream_idle(vt_context* ctx);
 
 // Load `count` requested bricks (indices relative to 8^levelC) into the
 // device pool as far as capacity allows.  Bricks loaded here are pinned
@@ -92,8 +93,12 @@ int service_requests(vt_context* ctx, const uint32_t* reqs, int64_t count, vt_fr
 {
     BrickCache& c = *ctx->cache;
     if (int rc = fold_ref_bits(ctx)) return rc;
-    std::vector<uint32_t> sorted(reqs, reqs + count);
+    std::vector<uint32_t> sorted;   // bricks still absent (a load on the copy stream may have placed some)
+    sorted.reserve((size_t)count);
+    for (int64_t i = 0; i < count; ++i)
+        if (c.table[(size_t)reqs[i]] < 0) sorted.push_back(reqs[i]);
     std::sort(sorted.begin(), sorted.end());
+    sorted.erase(std::unique(sorted.begin(), sorted.end()), sorted.end());
     int64_t n_load = std::min<int64_t>((int64_t)sorted.size(), c.gpu_slots);
     if (c.host && !c.host->full_mode) n_load = std::min<int64_t>(n_load, c.host->capacity);
     c.epoch++;
@@ -101,7 +106,6 @@ int service_requests(vt_context* ctx, const uint32_t* reqs, int64_t count, vt_fr
     if (int rc = load_bricks(ctx, sorted.data(), n_load, &misses, nullptr)) return rc;
     c.stat_missed += count;
     if (st) {
-        st->bricks_requested += count;
         st->bricks_uploaded += n_load;
         st->host_tier_misses += misses;
     }
@@ -115,6 +119,7 @@ int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t
 {
     BrickCache& c = *ctx->cache;
     if (n_load <= 0) return VT_OK;
+    if (int rc = copy_stream_idle(ctx)) return rc;   // evictions below must not hit a slot still being filled
     if (c.h_src.ensure((size_t)n_load) != hipSuccess || c.h_slots.ensure((size_t)n_load) != hipSuccess ||
         c.h_codes.ensure((size_t)n_load) != hipSuccess || c.h_tidx.ensure((size_t)(2 * n_load)) != hipSuccess ||
         c.h_tval.ensure((size_t)(2 * n_load)) != hipSuccess)
@@ -172,6 +177,110 @@ int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t
     return VT_OK;
 }
 
+// Loads the absent bricks of `req` that fit in free slots on the copy
+// stream, without waiting; the others are appended to `left`.  No kernel
+// reads a free slot, and the table entry of a brick is written after its
+// copy, so a render pass may run meanwhile: its rays see either the old
+// entry (and defer or park) or the complete brick.  Host-tier slots read
+// here stay pinned in the current epoch, which ends only after the copy
+// stream has been synchronized.
+int load_async(vt_context* ctx, const uint32_t* req, size_t n, std::vector<uint32_t>& left, int64_t* loaded,
+               int64_t* misses)
+{
+    BrickCache& c = *ctx->cache;
+    AsyncStage& A = ctx->astage;
+    const int64_t base = A.used;
+    int64_t k = 0;
+    bool full = false;
+    for (size_t i = 0; i < n; ++i) {
+        const uint32_t b = req[i];
+        if (c.table[(size_t)b] >= 0) continue;
+        int64_t slot = full || base + k >= A.cap ? -1 : c.take_free();
+        if (slot < 0) {
+            full = true;
+            left.push_back(b);
+            continue;
+        }
+        const float* src = nullptr;
+        if (c.host) {
+            int rc = c.host->get(b, c.epoch, &src, misses);
+            if (rc == VT_E_STATE) {   // every host slot pinned in this epoch: leave it to the synchronous service
+                c.free_list.push_back(slot);
+                full = true;
+                left.push_back(b);
+                continue;
+            }
+            if (rc != VT_OK) return rc;
+        }
+        c.slot_brick[(size_t)slot] = b;
+        c.table[(size_t)b] = (int32_t)slot;
+        c.pin[(size_t)slot] = c.epoch;
+        c.ref[(size_t)slot] = 1;
+        A.src.p[base + k] = src;
+        A.slots.p[base + k] = (int32_t)slot;
+        A.codes.p[base + k] = (int64_t)b + (int64_t)(1ull << (3 * c.level_c));
+        A.tidx.p[base + k] = b;
+        A.tval.p[base + k] = (int32_t)slot;
+        k++;
+    }
+    if (k == 0) return VT_OK;
+    hipStream_t s = ctx->cstream;
+    if (c.host) {
+        HIP_TRY(launch_copy_bricks(A.src.dev + base, A.slots.dev + base, (int)k, c.slot_elems, c.d_pool, s));
+    } else {
+        HIP_TRY(launch_bricks_from_volume(c.d_vol, c.vol_dims[0], c.vol_dims[1], c.vol_dims[2], A.codes.dev + base,
+                                          A.slots.dev + base, (int)k, c.level_c, c.n, c.cube, c.inc, c.d_pool, s));
+    }
+    if (c.d_submm) HIP_TRY(launch_submm(c.d_pool, A.slots.dev + base, (int)k, c.S, c.inc, c.G, c.d_submm, s));
+    HIP_TRY(launch_scatter_i32(c.d_table, A.tidx.dev + base, A.tval.dev + base, (int)k, s));
+    A.used += k;
+    c.stat_uploads += k;
+    *loaded += k;
+    return VT_OK;
+}
+
+// Waits for the copy stream; its staging can then be reused.
+int copy_stream_idle(vt_context* ctx)
+{
+    HIP_TRY(hipStreamSynchronize(ctx->cstream));
+    ctx->astage.used = 0;
+    return VT_OK;
+}
+
+// Between frames (no kernel running, copy stream idle): evict by the clock
+// until `reserve_slots` slots are free for the loads of the next pass.  A
+// cache that can hold every brick never evicts.
+int replenish_free_slots(vt_context* ctx)
+{
+    BrickCache& c = *ctx->cache;
+    if (c.gpu_slots >= c.nbricks) return VT_OK;
+    int64_t target = ctx->reserve_slots >= 0 ? ctx->reserve_slots : c.gpu_slots / 8;
+    target = std::min(target, c.gpu_slots / 2);
+    int64_t need = target - c.available();
+    if (need <= 0) return VT_OK;
+    if (int rc = fold_ref_bits(ctx)) return rc;
+    if (c.h_tidx.ensure((size_t)need) != hipSuccess || c.h_tval.ensure((size_t)need) != hipSuccess)
+        return fail(VT_E_NOMEM, "pinned staging allocation failed");
+    c.epoch++;
+    int64_t nt = 0;
+    for (; nt < need; ++nt) {
+        int64_t v = c.clock_victim();
+        if (v < 0) break;
+        int64_t old = c.slot_brick[(size_t)v];
+        c.table[(size_t)old] = -1;
+        c.slot_brick[(size_t)v] = -1;
+        c.ref[(size_t)v] = 0;
+        c.free_list.push_back(v);
+        c.h_tidx.p[nt] = old;
+        c.h_tval.p[nt] = -1;
+        c.stat_evictions++;
+    }
+    if (nt == 0) return VT_OK;
+    HIP_TRY(launch_scatter_i32(c.d_table, c.h_tidx.dev, c.h_tval.dev, (int)nt, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return VT_OK;
+}
+
 namespace {
 
 bool runs_well_formed(const uint64_t* E, int size, int level, std::string& why)
@@ -210,6 +319,7 @@ int vt_context_create(int device, vt_context** out)
     ctx->device = device;
     HIP_TRY(hipSetDevice(device));
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&ctx->cstream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreate(&ctx->ev0));
     HIP_TRY(hipEventCreate(&ctx->ev1));
     HIP_TRY(hipMalloc((void**)&ctx->d_ctr, sizeof(FrameCounters)));
@@ -432,9 +542,10 @@ int vt_cache_create(vt_context* ctx, int32_t levelC, int32_t cubeDim, int32_t cu
     ctx->cache = std::move(c);
     // request structures sized to the brick count
     if (ctx->req_bits.ensure((size_t)((ctx->cache->nbricks + 31) / 32)) != hipSuccess ||
-        ctx->req_list.ensure((size_t)ctx->cache->nbricks) != hipSuccess ||
-        ctx->h_req.ensure((size_t)ctx->cache->nbricks) != hipSuccess)
+        ctx->req_list.ensure((size_t)ctx->cache->nbricks, hipHostMallocCoherent) != hipSuccess ||
+        ctx->astage.ensure(ctx->cache->gpu_slots) != hipSuccess)
         return fail(VT_E_NOMEM, "request buffers");
+    memset(ctx->req_list.p, 0, (size_t)ctx->cache->nbricks * 4);
     return VT_OK;
 }
 
@@ -442,6 +553,7 @@ int vt_cache_destroy(vt_context* ctx)
 {
     if (!ctx) return fail(VT_E_ARG, "ctx is NULL");
     (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->cstream);
     ctx->cache.reset();
     return VT_OK;
 }
@@ -535,6 +647,7 @@ int vt_cache_clear(vt_context* ctx)
     if (!ctx || !ctx->cache) return fail(VT_E_STATE, "no cache");
     BrickCache& c = *ctx->cache;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (int rc = copy_stream_idle(ctx)) return rc;
     HIP_TRY(hipMemset(c.d_table, 0xff, (size_t)c.nbricks * 4));
     HIP_TRY(hipMemset(c.d_ref, 0, (size_t)c.gpu_slots));
     std::fill(c.table.begin(), c.table.end(), -1);
@@ -542,6 +655,7 @@ int vt_cache_clear(vt_context* ctx)
     std::fill(c.ref.begin(), c.ref.end(), 0);
     c.used = 0;
     c.hand = 0;
+    c.free_list.clear();
     return VT_OK;
 }
 
@@ -552,7 +666,7 @@ int vt_cache_get_stats(vt_context* ctx, vt_cache_stats* out)
     BrickCache& c = *ctx->cache;
     memset(out, 0, sizeof *out);
     out->gpu_slots = c.gpu_slots;
-    out->gpu_resident = c.used;
+    out->gpu_resident = c.resident();
     out->lookups_missed = c.stat_missed;
     out->uploads = c.stat_uploads;
     out->evictions = c.stat_evictions;
@@ -580,6 +694,15 @@ int vt_set_march_options(vt_context* ctx, int skip_empty, int max_defer)
     if (max_defer < 0 || max_defer > kMaxDefer) return fail(VT_E_ARG, "max_defer must be in [0, 16]");
     ctx->skip_empty = skip_empty ? 1 : 0;
     ctx->max_defer = max_defer;
+    return VT_OK;
+}
+
+int vt_set_load_options(vt_context* ctx, int overlap_loads, int64_t reserve_slots)
+{
+    if (!ctx) return fail(VT_E_ARG, "ctx is NULL");
+    if (reserve_slots < -1) return fail(VT_E_ARG, "reserve_slots must be >= -1");
+    ctx->async_loads = overlap_loads ? 1 : 0;
+    ctx->reserve_slots = reserve_slots;
     return VT_OK;
 }
 
@@ -667,25 +790,67 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
     a.in_list = nullptr;
     a.in_count = 0;
     a.out_list = ctx->list[cur].p;
+    const bool overlap = ctx->async_loads != 0;
+    uint32_t* rq = ctx->req_list.p;
     for (int pass = 0;; ++pass) {
         HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
         HIP_TRY(launch_render_pass(a, ctx->stream));
         HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
         HIP_TRY(hipMemcpyAsync(ctx->h_ctr, ctx->d_ctr, sizeof(FrameCounters), hipMemcpyDeviceToHost, ctx->stream));
+        // while the pass runs: load requested bricks into free slots
+        std::vector<uint32_t>& left = ctx->req_left;
+        left.clear();
+        uint32_t consumed = 0;
+        int64_t overlapped = 0, misses = 0;
+        if (overlap) {
+            std::vector<uint32_t> batch;
+            double last = now_ms();
+            for (;;) {
+                hipError_t q = hipEventQuery(ctx->ev1);
+                if (q != hipSuccess && q != hipErrorNotReady) HIP_TRY(q);
+                while (consumed < (uint32_t)c.nbricks) {
+                    uint32_t v = __atomic_load_n(&rq[consumed], __ATOMIC_ACQUIRE);
+                    if (v == 0) break;
+                    batch.push_back(v - 1);
+                    consumed++;
+                }
+                const double t = now_ms();
+                if (!batch.empty() && (q == hipSuccess || batch.size() >= 64 || t - last > 0.02)) {
+                    if (int rc = load_async(ctx, batch.data(), batch.size(), left, &overlapped, &misses)) return rc;
+                    batch.clear();
+                    last = t;
+                }
+                if (q == hipSuccess) break;
+            }
+        }
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         float ms = 0.0f;
         HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         ms_kernel += ms;
         st.passes = pass + 1;
+        st.bricks_overlapped += overlapped;
+        st.bricks_uploaded += overlapped;
+        st.host_tier_misses += misses;
         const uint32_t parked = ctx->h_ctr->parked_count;
         const uint32_t nreq = ctx->h_ctr->req_count;
+        if (nreq > (uint32_t)c.nbricks) return fail(VT_E_STATE, "request list overflow");
+        st.bricks_requested += nreq;
+        // every entry below nreq was stored before the kernel ended
+        for (uint32_t i = consumed; i < nreq; ++i) {
+            uint32_t v = __atomic_load_n(&rq[i], __ATOMIC_ACQUIRE);
+            if (v == 0) return fail(VT_E_STATE, "request entry not visible after the pass");
+            left.push_back(v - 1);
+        }
+        memset(rq, 0, (size_t)nreq * 4);
+        c.stat_missed += (int64_t)nreq - (int64_t)left.size();   // the rest is counted by service_requests
         if (parked == 0) break;
         st.rays_parked += parked;
         double t0 = now_ms();
-        HIP_TRY(hipMemcpyAsync(ctx->h_req.p, ctx->req_list.p, (size_t)nreq * 4, hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-        int rc = service_requests(ctx, ctx->h_req.p, nreq, &st);
-        if (rc != VT_OK) return rc;
+        if (int rc = copy_stream_idle(ctx)) return rc;
+        if (!left.empty()) {
+            int rc = service_requests(ctx, left.data(), (int64_t)left.size(), &st);
+            if (rc != VT_OK) return rc;
+        }
         ms_service += now_ms() - t0;
         HIP_TRY(hipMemsetAsync(ctx->req_bits.p, 0, ctx->req_bits.n * 4, ctx->stream));
         HIP_TRY(hipMemsetAsync(ctx->d_ctr, 0, 8, ctx->stream));   // req_count, parked_count
@@ -694,6 +859,12 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
         cur ^= 1;
         a.out_list = ctx->list[cur].p;
         if (pass > 1000000) return fail(VT_E_STATE, "frame did not converge");
+    }
+    if (int rc = copy_stream_idle(ctx)) return rc;
+    if (overlap) {
+        double t0 = now_ms();
+        if (int rc = replenish_free_slots(ctx)) return rc;
+        ms_service += now_ms() - t0;
     }
     st.err = (int32_t)ctx->h_ctr->err;
     st.samples = ctx->h_ctr->samples;
