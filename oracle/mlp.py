@@ -70,18 +70,26 @@ def loss_and_grad(p, x, y, pos_weight=1.0, emulate_bf16=False):
     return loss, pack(g)
 
 
-def features(cam_pos, look, brick_min, brick_side, n, level_c):
-    """12 features of vt_mlp_features for bricks with min corners brick_min[N,3]."""
+def features(cam, brick_min, brick_side, n, level_c):
+    """12 features of vt_mlp_features for bricks with min corners brick_min[N,3]:
+    camera position and look (the pose), brick centre, levelC/10, and the
+    brick centre's image-plane position (u, v), 2*distance*(d.right, d.up) /
+    ((d.look) * (w, h)), clamped to [-4, 4], 4 behind the camera."""
     D = float(1 << n)
     c = np.asarray(brick_min, np.float64) + 0.5 * brick_side
-    p = np.asarray(cam_pos, np.float64)
+    p = np.asarray(cam.pos[:], np.float64)
+    look, up, right = (np.asarray(v[:], np.float64) for v in (cam.look, cam.up, cam.right))
     d = c - p
-    dist = np.linalg.norm(d, axis=1)
+    dl, dr, du = d @ look, d @ right, d @ up
+    front = dl > 0
+    safe = np.where(front, dl, 1.0)
+    u = np.where(front, np.clip(dr / safe * (2.0 * cam.distance / cam.w), -4, 4), 4.0)
+    v = np.where(front, np.clip(du / safe * (2.0 * cam.distance / cam.h), -4, 4), 4.0)
     out = np.zeros((c.shape[0], 12))
     out[:, 0:3] = p / D - 0.5
-    out[:, 3:6] = np.asarray(look, np.float64)
+    out[:, 3:6] = look
     out[:, 6:9] = c / D - 0.5
     out[:, 9] = level_c / 10.0
-    out[:, 10] = dist / D
-    out[:, 11] = (d @ np.asarray(look, np.float64)) / np.where(dist > 0, dist, 1.0)
+    out[:, 10] = u
+    out[:, 11] = v
     return out

@@ -84,11 +84,26 @@ def test_training_reduces_loss(vt, ctx):
 def test_features_match_oracle(vt, vto, ctx, net):
     n_levels, level_c = 8, 3
     codes = np.array([vto.encode(x * 32, y * 32, z * 32, level_c, n_levels)
-                      for x, y, z in [(0, 0, 0), (7, 7, 7), (3, 1, 6), (2, 5, 0)]], np.uint64)
-    cam = vt.make_camera(8, 8, (128.0, 100.0, 700.0))
-    d_c, d_x = ctx.to_device(codes), ctx.alloc(codes.size * 48)
-    net.features(cam, n_levels, level_c, d_c, codes.size, d_x)
-    got = d_x.download(np.float32, (codes.size, 12))
+                      for x in range(8) for y in range(8) for z in range(8)], np.uint64)
     mins = np.array([vto.decode(c, level_c, n_levels) for c in codes])
-    ref = mlp.features(np.array(cam.pos), np.array(cam.look), mins, 32, n_levels, level_c)
-    assert np.allclose(got, ref, atol=1e-5)
+    for cam in (vt.make_camera(8, 8, (128.0, 100.0, 700.0)),
+                vt.make_camera(64, 48, (300.0, 20.0, -90.0), *vt.look_at((300.0, 20.0, -90.0), (120, 140, 100)))):
+        d_c, d_x = ctx.to_device(codes), ctx.alloc(codes.size * 48)
+        net.features(cam, n_levels, level_c, d_c, codes.size, d_x)
+        got = d_x.download(np.float32, (codes.size, 12)).astype(np.float64)
+        ref = mlp.features(cam, mins, 32, n_levels, level_c)
+        assert np.allclose(got, ref, atol=2e-5)
+        # (u, v) invert the camera's ray construction: a centre in front with
+        # |u|, |v| < 4 lies on the ray through the screen point (u, v)
+        pos = np.array(cam.pos[:], np.float64)
+        look, up, right = (np.array(v[:], np.float64) for v in (cam.look, cam.up, cam.right))
+        c = mins + 16.0
+        u, v = got[:, 10], got[:, 11]
+        inside = (np.abs(u) < 3.99) & (np.abs(v) < 3.99)
+        A = (look[None] * cam.distance + right[None] * (u[:, None] * cam.w / 2) + up[None] * (v[:, None] * cam.h / 2))
+        dvec = c - pos
+        cosang = np.sum(A * dvec, 1) / np.linalg.norm(A, axis=1) / np.linalg.norm(dvec, axis=1)
+        assert inside.sum() > 20 and np.all(cosang[inside] > 1 - 1e-8)
+        assert np.all(got[(dvec @ look) <= 0][:, 10:12] == 4.0)
+        d_c.free()
+        d_x.free()

@@ -87,8 +87,9 @@ struct PinBuf {
 
 // ------------------------------------------------------------ host tier
 // Pinned, device-visible brick slots.  When it can hold every brick of the
-// cube, a reader thread fills it slab by slab (x-slabs of bricks, one HDF5
-// hyperslab read per slab); otherwise bricks are read on demand into
+// cube, or every brick that holds octree nodes (the only ones a ray can
+// request), a reader thread fills it slab by slab (x-slabs of bricks, one
+// HDF5 hyperslab read per slab); otherwise bricks are read on demand into
 // clock-evicted slots.
 struct HostTier {
     int level_c = 0, n = 0, cube = 0, inc = 0, S = 0;
@@ -105,10 +106,12 @@ struct HostTier {
     int64_t hand = 0, used = 0;
     // prefetch state
     bool full_mode = false;
+    bool started = false;
     std::thread reader;
     std::mutex m;
     std::condition_variable cv;
     int slabs_done = 0;           // x-slabs of bricks fully loaded
+    int64_t filled = 0;           // bricks the reader has placed
     bool reader_failed = false;
     std::string reader_err;
     int64_t file_reads = 0;
@@ -153,10 +156,13 @@ struct HostTier {
                 cv.notify_all();
                 return;
             }
+            int64_t placed = 0;
             for (int64_t by = 0; by < B; ++by)
                 for (int64_t bz = 0; bz < B; ++bz) {
                     uint64_t code = brick_code((int)bx, (int)by, (int)bz, level_c);
                     int64_t bidx = (int64_t)(code - (1ull << (3 * level_c)));
+                    if (slot_of[bidx] < 0) continue;   // holds no octree node
+                    placed++;
                     float* dst = pool + (size_t)slot_of[bidx] * slot_elems;
                     for (int x = 0; x < S; ++x)
                         for (int y = 0; y < S; ++y) {
@@ -168,21 +174,38 @@ struct HostTier {
             {
                 std::lock_guard<std::mutex> g(m);
                 slabs_done = (int)bx + 1;
-                file_reads += B * B;
+                file_reads += placed;
+                filled += placed;
             }
             cv.notify_all();
         }
     }
 
-    int start(int n_levels)
+    // runs_c: the octree's level-levelC run list, or null when it is not
+    // uploaded (then only a tier that holds every brick is filled ahead)
+    int start(int n_levels, const std::vector<uint64_t>* runs_c)
     {
+        started = true;
         n = n_levels;
         if (capacity >= nbricks) {
             full_mode = true;
             for (int64_t b = 0; b < nbricks; ++b) { slot_of[b] = (int32_t)b; brick_of[b] = b; }
             used = nbricks;
-            reader = std::thread([this] { run_prefetch(); });
+        } else if (runs_c) {
+            const uint64_t base = 1ull << (3 * level_c);
+            int64_t count = 0;
+            for (size_t j = 0; j + 1 < runs_c->size(); j += 2) count += (int64_t)((*runs_c)[j + 1] - (*runs_c)[j] + 1);
+            if (count > capacity) return VT_OK;   // on demand
+            full_mode = true;
+            int64_t k = 0;
+            for (size_t j = 0; j + 1 < runs_c->size(); j += 2)
+                for (uint64_t code = (*runs_c)[j]; code <= (*runs_c)[j + 1]; ++code, ++k) {
+                    slot_of[(size_t)(code - base)] = (int32_t)k;
+                    brick_of[(size_t)k] = (int64_t)(code - base);
+                }
+            used = count;
         }
+        if (full_mode) reader = std::thread([this] { run_prefetch(); });
         return VT_OK;
     }
 
@@ -196,6 +219,7 @@ struct HostTier {
                 node_min(bidx + (1ull << (3 * level_c)), level_c, level_c, x, y, z);
                 code_bx = x;
             }
+            if (slot_of[bidx] < 0) return fail(VT_E_ARG, "requested brick holds no octree node");
             std::unique_lock<std::mutex> lk(m);
             cv.wait(lk, [&] { return reader_failed || slabs_done > code_bx; });
             if (reader_failed) return fail(VT_E_IO, "host tier reader: " + reader_err);

@@ -208,8 +208,17 @@ __global__ void k_features(Camera cam, int n, int level_c, const uint64_t* __res
     const float half = 0.5f * (float)(1 << (n - level_c));
     float c0 = (float)bx + half, c1 = (float)by + half, c2 = (float)bz + half;
     float d0 = c0 - cam.pos[0], d1 = c1 - cam.pos[1], d2 = c2 - cam.pos[2];
-    float dist = sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
-    float inv = dist > 0.0f ? 1.0f / dist : 0.0f;
+    // image-plane position of the brick centre: (u, v) in [-1, 1] on screen
+    // (the ray of pixel (i, j) has u = 2(j+0.5)/W - 1, v = 2(i+0.5)/H - 1),
+    // clamped to [-4, 4]; 4 for a centre behind the camera
+    float dl = d0 * cam.look[0] + d1 * cam.look[1] + d2 * cam.look[2];
+    float dr = d0 * cam.right[0] + d1 * cam.right[1] + d2 * cam.right[2];
+    float du = d0 * cam.up[0] + d1 * cam.up[1] + d2 * cam.up[2];
+    float u = 4.0f, v = 4.0f;
+    if (dl > 0.0f) {
+        u = fminf(fmaxf(dr / dl * (2.0f * cam.distance / cam.w), -4.0f), 4.0f);
+        v = fminf(fmaxf(du / dl * (2.0f * cam.distance / cam.h), -4.0f), 4.0f);
+    }
     float* o = x + (size_t)i * 12;
     o[0] = cam.pos[0] / D - 0.5f;
     o[1] = cam.pos[1] / D - 0.5f;
@@ -221,8 +230,8 @@ __global__ void k_features(Camera cam, int n, int level_c, const uint64_t* __res
     o[7] = c1 / D - 0.5f;
     o[8] = c2 / D - 0.5f;
     o[9] = (float)level_c / 10.0f;
-    o[10] = dist / D;
-    o[11] = (d0 * cam.look[0] + d1 * cam.look[1] + d2 * cam.look[2]) * inv;
+    o[10] = u;
+    o[11] = v;
 }
 
 }  // namespace

@@ -264,6 +264,7 @@ int vt_mlp_features(vt_mlp* m, const vt_camera* cam, int32_t nLevels, int32_t le
 int load_async(vt_context* ctx, const uint32_t* req, size_t n, std::vector<uint32_t>& left, int64_t* loaded,
                int64_t* misses);
 int copy_stream_idle(vt_context* ctx);
+void host_tier_start(vt_context* ctx);
 
 extern "C" int vt_prefetch(vt_context* ctx, vt_mlp* m, const vt_camera* cam, int64_t budget, float min_prob,
                            int64_t* loaded)
@@ -277,7 +278,7 @@ extern "C" int vt_prefetch(vt_context* ctx, vt_mlp* m, const vt_camera* cam, int
     if (budget < 0 || min_prob < 0.0f || min_prob >= 1.0f) return fail(VT_E_ARG, "bad budget or threshold");
     HIP_TRY(hipSetDevice(ctx->device));
     c.n = ctx->n;
-    if (c.host && !c.host->full_mode && !c.host->reader.joinable()) c.host->start(c.n);
+    host_tier_start(ctx);
     hipStream_t s = ctx->stream;
     // candidates: the bricks that hold octree nodes (the level-levelC run list)
     if (ctx->cand_level != c.level_c) {

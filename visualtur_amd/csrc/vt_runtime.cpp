@@ -86,6 +86,17 @@ int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t
                 const int64_t* victims = nullptr);
 int copy_stream_idle(vt_context* ctx);
 
+// Starts the host tier once the brick geometry is known; with the octree
+// uploaded down to levelC, a tier that holds every brick with octree nodes
+// is filled ahead by the reader thread.
+void host_tier_start(vt_context* ctx)
+{
+    BrickCache& c = *ctx->cache;
+    if (!c.host || c.host->started) return;
+    const bool have = ctx->octree_ready && ctx->max_level >= c.level_c && (int)ctx->up_runs.size() > c.level_c;
+    c.host->start(ctx->n, have ? &ctx->up_runs[(size_t)c.level_c] : nullptr);
+}
+
 // Load `count` requested bricks (indices relative to 8^levelC) into the
 // device pool as far as capacity allows.  Bricks loaded here are pinned
 // against eviction until the next call.
@@ -611,7 +622,7 @@ int vt_cache_prefetch_host(vt_context* ctx, int wait)
     if (!ctx->octree_ready) return fail(VT_E_STATE, "upload the octree first (brick geometry needs nLevels)");
     HostTier& h = *ctx->cache->host;
     ctx->cache->n = ctx->n;
-    if (!h.reader.joinable() && !h.full_mode) h.start(ctx->n);
+    host_tier_start(ctx);
     if (wait && h.full_mode) {
         std::unique_lock<std::mutex> lk(h.m);
         h.cv.wait(lk, [&] { return h.reader_failed || h.slabs_done >= (int)h.side_bricks(); });
@@ -629,7 +640,7 @@ int vt_cache_load_bricks(vt_context* ctx, const uint64_t* codes, int64_t count)
     if (!c.d_vol && !c.host) return fail(VT_E_STATE, "no brick source attached");
     if (int rc = set_device(ctx)) return rc;
     c.n = ctx->n;
-    if (c.host && !c.host->full_mode && !c.host->reader.joinable()) c.host->start(c.n);
+    host_tier_start(ctx);
     std::vector<uint32_t> req;
     uint64_t base = 1ull << (3 * c.level_c);
     for (int64_t i = 0; i < count; ++i) {
@@ -673,7 +684,7 @@ int vt_cache_get_stats(vt_context* ctx, vt_cache_stats* out)
     if (c.host) {
         std::lock_guard<std::mutex> g(c.host->m);
         out->cpu_slots = c.host->capacity;
-        out->cpu_resident = c.host->full_mode ? (int64_t)c.host->slabs_done * c.host->side_bricks() * c.host->side_bricks()
+        out->cpu_resident = c.host->full_mode ? c.host->filled
                                               : c.host->used;
         out->file_reads = c.host->file_reads;
     }
@@ -731,7 +742,7 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
     if ((1 << (ctx->n - c.level_c)) != c.cube) return fail(VT_E_ARG, "cubeDim must equal 2^(nLevels-levelC)");
     if (int rc = set_device(ctx)) return rc;
     c.n = ctx->n;
-    if (c.host && !c.host->full_mode && !c.host->reader.joinable()) c.host->start(c.n);
+    host_tier_start(ctx);
 
     const uint32_t R = (uint32_t)cam->W * (uint32_t)cam->H;
     if (ctx->ray_node.ensure(R) != hipSuccess || ctx->ray_phase.ensure(R) != hipSuccess ||
@@ -1191,7 +1202,7 @@ int vt_stage_cache(vt_context* ctx, vt_visible_cube* h_cubes, int64_t num, int32
     if ((1 << (nLevels - c.level_c)) != c.cube) return fail(VT_E_ARG, "cubeDim must equal 2^(nLevels-levelC)");
     if (int rc = set_device(ctx)) return rc;
     c.n = nLevels;
-    if (c.host && !c.host->full_mode && !c.host->reader.joinable()) c.host->start(c.n);
+    host_tier_start(ctx);
     const int shift = 3 * (levelO - c.level_c);
     const uint64_t base = 1ull << (3 * c.level_c);
     std::vector<uint32_t> need;
