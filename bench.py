@@ -251,6 +251,145 @@ def serving_sweep(ctx, args):
     return out
 
 
+def run_c3(ctx, args):
+    """Config 3: 2048^3 signed-distance field (seeded union of 200 spheres and
+    tori, radii U[0.01, 0.08] of the extent, plus 0.02-amplitude fBm), iso 0,
+    32^3 bricks (levels 0..6, 262144 bricks), HDF5 in 32^3 chunks; device
+    cache 8 GB, pinned host tier 32 GB; a seeded random-walk camera path at
+    1920x1080.  The prefetcher is trained on the visited-brick traces of
+    other seeded paths (rendered at 480x270 from the volume in HBM before
+    the file is written), then the path is rendered plain and with a
+    prefetch before every frame, each from a cold device cache."""
+    from visualtur_amd import prefetch as pf
+    from visualtur_amd.volumes import C3_SEED, sdf_shapes
+
+    D, level_c, inc, iso = 2048, 6, 3, 0.0
+    dims = (D, D, D)
+    radius = (1.3, 2.2)
+    Sb = brick_bytes(32, inc)
+    t0 = time.perf_counter()
+    d_vol = ctx.alloc(4 * D ** 3)
+    ctx.volume_generate(vt.GEN_SDF, dims, C3_SEED, d_vol, shapes=sdf_shapes(200, C3_SEED, D), n=D, octaves=5,
+                        base_freq=1.0 / 32.0, gain=0.5, lacunarity=2.0, scale=0.02)
+    log(f"c3 volume generated in {time.perf_counter() - t0:.2f}s")
+    t0 = time.perf_counter()
+    n, sizes = ctx.octree_build(d_vol, dims, iso, False, level_c)
+    runs = [ctx.octree_runs(l, sizes[l]) for l in range(level_c + 1)]
+    cand = pf.candidate_codes(runs[level_c])
+    log(f"c3 octree built in {time.perf_counter() - t0:.2f}s (n={n}, bricks with nodes {cand.size})")
+    prm = vt.render_params(level_c, level_c, iso)
+    # training traces from the volume in HBM (every brick with nodes fits)
+    t0 = time.perf_counter()
+    ctx.cache_create(level_c, 32, inc, int(cand.size), 0)
+    ctx.cache_attach_device_volume(d_vol, dims)
+    tcams = [c for s in range(args.c3_train_paths)
+             for c in pf.random_walk(480, 270, D, args.c3_train_frames, seed=30000 + s, radius=radius)]
+    traces = pf.collect_traces(ctx, tcams, prm, level_c)
+    net = vt.MLP(ctx, seed=13, max_batch=16384)
+    X, Y = pf.build_dataset(ctx, net, tcams, traces, cand, n, level_c, neg_per_pos=3, seed=3, max_pos=1024)
+    losses, _ = pf.train(ctx, net, X, Y, steps=args.c3_train_steps, batch=8192, lr=1e-3)
+    train_s = time.perf_counter() - t0
+    log(f"c3 prefetcher trained in {train_s:.1f}s: {len(tcams)} frames of {args.c3_train_paths} paths, "
+        f"{X.shape[0]} samples, loss {losses[:10].mean():.3f} -> {losses[-10:].mean():.3f}")
+    del X, Y
+    tmpdir = tempfile.mkdtemp(prefix="vt_c3_")
+    h5 = os.path.join(tmpdir, "c3.h5")
+    t0 = time.perf_counter()
+    ctx.volume_write_hdf5(d_vol, dims, h5, "value", 32)
+    log(f"c3 hdf5 written in {time.perf_counter() - t0:.2f}s")
+    vol_host = None if args.no_cpu else d_vol.download(np.float32, dims)
+    ctx.cache_destroy()
+    d_vol.free()
+    gpu_slots, cpu_slots = int(8e9 // Sb), int(32e9 // Sb)
+    W, H = 1920, 1080
+    R = W * H
+    cams = pf.random_walk(W, H, D, args.warmup + args.steps, seed=C3_SEED, radius=radius)
+    d_rgba, d_hits = ctx.alloc(R * 16), ctx.alloc(R * 32)
+    legs, last = {}, {}
+    for form in ("plain", "prefetch"):
+        ctx.cache_create(level_c, 32, inc, gpu_slots, cpu_slots)
+        ctx.cache_attach_hdf5(h5, "value")
+        ctx.set_march_options(skip_empty=True, max_defer=args.max_defer)
+        ctx.set_load_options(overlap_loads=bool(args.overlap), reserve_slots=args.reserve)
+        t0 = time.perf_counter()
+        ctx.cache_prefetch_host(wait=True)
+        log(f"c3 {form}: host tier filled from hdf5 in {time.perf_counter() - t0:.2f}s, "
+            f"{ctx.cache_stats()['cpu_resident']} bricks")
+        use = net if form == "prefetch" else None
+        ms, pre_ms, uploads, passes, pre_n = [], [], [], [], []
+        for k in range(args.warmup + args.steps):
+            f0 = time.perf_counter()
+            pre_n.append(use.prefetch(cams[k], gpu_slots, 0.5) if use is not None else 0)
+            f1 = time.perf_counter()
+            st = ctx.render_frame(cams[k], prm, d_rgba, d_hits)
+            f2 = time.perf_counter()
+            if k >= args.warmup:
+                ms.append((f2 - f0) * 1e3)
+                pre_ms.append((f1 - f0) * 1e3)
+                uploads.append(st.bricks_uploaded)
+                passes.append(st.passes)
+        ctx.synchronize()
+        m = np.array(ms)
+        legs[form] = {"value": round(R * len(ms) / (m.sum() * 1e-3), 1), "p50_ms": round(float(np.percentile(m, 50)), 3),
+                      "p99_ms": round(float(np.percentile(m, 99)), 3),
+                      "demand_misses_per_frame": float(np.mean(uploads)), "passes_per_frame": float(np.mean(passes)),
+                      "prefetched_per_frame": float(np.mean(pre_n[args.warmup:])),
+                      "prefetch_ms_per_frame": round(float(np.mean(pre_ms)), 3)}
+        log(f"c3 {form}: {legs[form]}")
+        last[form] = (d_hits.download(vt.HIT_DTYPE, (R,)), d_rgba.download(np.float32, (R, 4)))
+        ctx.cache_destroy()
+    # visited bricks of the timed frames (trace replay, untimed) for the miss rates
+    ctx.cache_create(level_c, 32, inc, gpu_slots, cpu_slots)
+    ctx.cache_attach_hdf5(h5, "value")
+    ctx.cache_prefetch_host(wait=True)
+    ctx.set_frame_options(collect_stats=False, record_trace=True)
+    U = []
+    for k in range(args.warmup, args.warmup + args.steps):
+        ctx.render_frame(cams[k], prm, d_rgba, d_hits)
+        U.append(int(ctx.brick_trace(level_c).sum()))
+    ctx.set_frame_options(False, False)
+    for form in legs:
+        legs[form]["device_miss_rate"] = round(legs[form]["demand_misses_per_frame"] / max(1.0, float(np.mean(U))), 5)
+    same = (last["plain"][0].tobytes() == last["prefetch"][0].tobytes() and
+            last["plain"][1].tobytes() == last["prefetch"][1].tobytes())
+    out = {"metric": METRIC, "value": legs["prefetch"]["value"], "unit": "rays/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(float(R / legs["prefetch"]["value"] * 1e3), 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "2048^3 SDF (200 spheres/tori + 0.02 fBm), iso 0, HDF5 32^3 chunks, 32^3 bricks + "
+                                  "3-voxel halo, device cache 8 GB, pinned host tier 32 GB, 1920x1080 random walk",
+                      "form": "prefetch (plain beside)", "bricks_with_nodes": int(cand.size),
+                      "gpu_slots": gpu_slots, "cpu_slots": cpu_slots},
+           "prefetch": legs["prefetch"], "plain": legs["plain"], "speedup": round(legs["prefetch"]["value"] /
+                                                                                   legs["plain"]["value"], 3),
+           "bricks_marched_per_frame": float(np.mean(U)), "identical_last_frame": same,
+           "train": {"paths": args.c3_train_paths, "frames": len(tcams), "seconds": round(train_s, 2),
+                     "final_loss": round(float(losses[-10:].mean()), 4)}}
+    if vol_host is not None:
+        from oracle import vto
+
+        w = Workload()
+        w.prm = prm
+        objs = (vto.Volume(vol_host), vto.Octree(runs, n, level_c), vto.params(level_c, level_c, iso))
+        out["parity"] = parity_sample(w, ctx, cams[-1], objs, args.parity_rays, seed=3)
+        # the reference algorithm on this host's cores: rays of the last frame, about cpu_seconds of work
+        threads = len(os.sched_getaffinity(0))
+        ids = np.arange(0, R, max(1, R // 65536), dtype=np.int64)
+        t0 = time.perf_counter()
+        done = 0
+        while time.perf_counter() - t0 < args.cpu_seconds:
+            vto.render_list(*objs[:2], cams[-1], objs[2], ids)
+            done += ids.size
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": done / dt, "unit": "rays/s", "cores": threads, "kind": "port",
+                               "sample": f"{done} rays ({ids.size} evenly spaced rays of the last frame, repeated), "
+                                         f"{dt:.1f}s wall, all bricks in memory"}
+    d_rgba.free()
+    d_hits.free()
+    net.close()
+    shutil.rmtree(tmpdir, ignore_errors=True)
+    return out
+
+
 def cpu_baseline(w, cams, seconds: float):
     """Reference algorithm (the C oracle) on this host's cores, over whole
     frames of the same workload until `seconds` of wall time have elapsed."""
@@ -312,7 +451,10 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c4"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--c3-train-paths", type=int, default=512)
+    ap.add_argument("--c3-train-frames", type=int, default=4)
+    ap.add_argument("--c3-train-steps", type=int, default=1500)
     ap.add_argument("--rates", type=float, nargs="*", default=[50, 100, 200, 400, 800])
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -346,6 +488,11 @@ def main() -> int:
                "config": {"workload": "serving: c1+c2 volumes resident, 32 random-walk viewers, Poisson arrivals, "
                                       "mixed resolutions, per-request iso", "form": "batched vs per-request"},
                "serving": sweep}
+        ctx.close()
+        print(json.dumps(out), flush=True)
+        return 0
+    if args.config == "c3":
+        out = run_c3(ctx, args)
         ctx.close()
         print(json.dumps(out), flush=True)
         return 0

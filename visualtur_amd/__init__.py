@@ -343,6 +343,9 @@ class Context:
     def cache_create(self, level_c: int, cube_dim: int, cube_inc: int, gpu_slots: int, cpu_slots: int = 0):
         self._c(lib().vt_cache_create(self.handle, level_c, cube_dim, cube_inc, gpu_slots, cpu_slots))
 
+    def cache_destroy(self) -> None:
+        self._c(lib().vt_cache_destroy(self.handle))
+
     def cache_attach_device_volume(self, vol: DeviceBuffer, dims) -> None:
         self._keep.append(vol)
         self._c(lib().vt_cache_attach_device_volume(self.handle, vol.ptr, *[int(d) for d in dims]))

@@ -279,12 +279,15 @@ struct AsyncStage {
     PinBuf<int64_t> codes;
     PinBuf<int64_t> tidx;
     PinBuf<int32_t> tval;
+    PinBuf<void*> dst, hsrc;   // DMA batch arguments
+    PinBuf<size_t> size;
     int64_t used = 0, cap = 0;
     hipError_t ensure(int64_t count)
     {
         hipError_t e;
         if ((e = src.ensure((size_t)count)) || (e = slots.ensure((size_t)count)) ||
-            (e = codes.ensure((size_t)count)) || (e = tidx.ensure((size_t)count)) || (e = tval.ensure((size_t)count)))
+            (e = codes.ensure((size_t)count)) || (e = tidx.ensure((size_t)count)) || (e = tval.ensure((size_t)count)) ||
+            (e = dst.ensure((size_t)count)) || (e = hsrc.ensure((size_t)count)) || (e = size.ensure((size_t)count)))
             return e;
         cap = count;
         used = 0;
@@ -293,6 +296,7 @@ struct AsyncStage {
     void release()
     {
         src.release(); slots.release(); codes.release(); tidx.release(); tval.release();
+        dst.release(); hsrc.release(); size.release();
         used = cap = 0;
     }
 };

@@ -237,7 +237,16 @@ int load_async(vt_context* ctx, const uint32_t* req, size_t n, std::vector<uint3
     if (k == 0) return VT_OK;
     hipStream_t s = ctx->cstream;
     if (c.host) {
-        HIP_TRY(launch_copy_bricks(A.src.dev + base, A.slots.dev + base, (int)k, c.slot_elems, c.d_pool, s));
+        // DMA engines, not a copy kernel: the render pass keeps every CU
+        // (the argument arrays live in the staging until the stream is idle)
+        for (int64_t q = base; q < base + k; ++q) {
+            A.dst.p[q] = c.d_pool + (size_t)A.slots.p[q] * c.slot_elems;
+            A.hsrc.p[q] = const_cast<float*>(A.src.p[q]);
+            A.size.p[q] = c.slot_elems * sizeof(float);
+        }
+        size_t fail_idx = 0;
+        HIP_TRY(hipMemcpyBatchAsync(A.dst.p + base, A.hsrc.p + base, A.size.p + base, (size_t)k, nullptr, nullptr, 0,
+                                    &fail_idx, s));
     } else {
         HIP_TRY(launch_bricks_from_volume(c.d_vol, c.vol_dims[0], c.vol_dims[1], c.vol_dims[2], A.codes.dev + base,
                                           A.slots.dev + base, (int)k, c.level_c, c.n, c.cube, c.inc, c.d_pool, s));
@@ -260,11 +269,17 @@ int copy_stream_idle(vt_context* ctx)
 
 // Between frames (no kernel running, copy stream idle): evict by the clock
 // until `reserve_slots` slots are free for the loads of the next pass.  A
-// cache that can hold every brick never evicts.
+// cache that can hold every brick with octree nodes never evicts.
 int replenish_free_slots(vt_context* ctx)
 {
     BrickCache& c = *ctx->cache;
-    if (c.gpu_slots >= c.nbricks) return VT_OK;
+    int64_t needed = c.nbricks;   // bricks a ray can request: those holding octree nodes
+    if (ctx->octree_ready && ctx->max_level >= c.level_c && (int)ctx->up_runs.size() > c.level_c) {
+        const std::vector<uint64_t>& r = ctx->up_runs[(size_t)c.level_c];
+        needed = 0;
+        for (size_t j = 0; j + 1 < r.size(); j += 2) needed += (int64_t)(r[j + 1] - r[j] + 1);
+    }
+    if (c.gpu_slots >= needed) return VT_OK;
     int64_t target = ctx->reserve_slots >= 0 ? ctx->reserve_slots : c.gpu_slots / 8;
     target = std::min(target, c.gpu_slots / 2);
     int64_t need = target - c.available();

@@ -65,9 +65,11 @@ def collect_traces(ctx: Context, cams, prm, level_c: int):
     return out
 
 
-def build_dataset(ctx: Context, net: MLP, cams, traces, candidates, n_levels, level_c, neg_per_pos=4, seed=0):
+def build_dataset(ctx: Context, net: MLP, cams, traces, candidates, n_levels, level_c, neg_per_pos=4, seed=0,
+                  max_pos=None):
     """Features (computed by vt_mlp_features) and labels: every visited brick
-    is a positive, plus neg_per_pos sampled non-visited candidates each."""
+    (or max_pos of them, sampled, per frame) is a positive, plus neg_per_pos
+    sampled non-visited candidates each."""
     rng = np.random.default_rng(seed)
     xs, ys = [], []
     d_codes = ctx.alloc(max(1, candidates.size) * 8)
@@ -75,6 +77,8 @@ def build_dataset(ctx: Context, net: MLP, cams, traces, candidates, n_levels, le
     try:
         for cam, pos in zip(cams, traces):
             neg_pool = np.setdiff1d(candidates, pos, assume_unique=False)
+            if max_pos is not None and pos.size > max_pos:
+                pos = rng.choice(pos, size=max_pos, replace=False)
             k = min(neg_pool.size, max(8, neg_per_pos * pos.size))
             neg = rng.choice(neg_pool, size=k, replace=False) if k else np.zeros(0, np.uint64)
             codes = np.concatenate([pos, neg]).astype(np.uint64)
