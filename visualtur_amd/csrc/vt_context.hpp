@@ -317,6 +317,7 @@ struct BrickCache {
     std::vector<uint8_t> ref, ref_tmp;
     std::vector<uint64_t> pin;
     uint64_t epoch = 0;
+    uint64_t protect_from = 0;    // bricks loaded in epochs >= this (since the last frame ended) go last
     int64_t hand = 0, used = 0;   // slots [used, gpu_slots) were never filled
     // Slots below `used` that hold no brick.  Loads issued while a render
     // pass runs take only these (or never-filled slots), since no kernel can
@@ -366,24 +367,32 @@ struct BrickCache {
     }
 
     // clock over the slots that hold bricks, skipping those pinned in the
-    // current epoch and giving referenced ones a second chance
-    int64_t clock_victim()
+    // current epoch (and, when `recent`, those loaded since the last frame
+    // ended: prefetched or requested for the frame in progress) and giving
+    // referenced ones a second chance
+    int64_t clock_victim(bool recent)
     {
         for (int64_t it = 0; it < 3 * gpu_slots; ++it) {
             int64_t c = hand;
             hand = (hand + 1) % gpu_slots;
             if (c >= used || slot_brick[(size_t)c] < 0) continue;
-            if (pin[(size_t)c] == epoch) continue;
+            if (pin[(size_t)c] == epoch || (recent && pin[(size_t)c] >= protect_from)) continue;
             if (ref[(size_t)c]) { ref[(size_t)c] = 0; continue; }
             return c;
         }
         return -1;
     }
 
+    uint64_t recent_exhausted = ~0ull;   // epoch in which every unprotected slot was taken
+
     int64_t pick_victim()
     {
         int64_t s = take_free();
-        return s >= 0 ? s : clock_victim();
+        if (s < 0 && recent_exhausted != epoch) {
+            s = clock_victim(true);
+            if (s < 0) recent_exhausted = epoch;   // do not rescan the ring for every victim
+        }
+        return s >= 0 ? s : clock_victim(false);
     }
 };
 

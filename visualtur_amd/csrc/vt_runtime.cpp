@@ -290,7 +290,7 @@ int replenish_free_slots(vt_context* ctx)
     c.epoch++;
     int64_t nt = 0;
     for (; nt < need; ++nt) {
-        int64_t v = c.clock_victim();
+        int64_t v = c.clock_victim(true);
         if (v < 0) break;
         int64_t old = c.slot_brick[(size_t)v];
         c.table[(size_t)old] = -1;
@@ -892,6 +892,8 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
         if (int rc = replenish_free_slots(ctx)) return rc;
         ms_service += now_ms() - t0;
     }
+    c.epoch++;
+    c.protect_from = c.epoch;
     st.err = (int32_t)ctx->h_ctr->err;
     st.samples = ctx->h_ctr->samples;
     st.nodes = ctx->h_ctr->nodes;
