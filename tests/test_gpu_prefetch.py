@@ -19,7 +19,7 @@ def test_prefetch_cuts_misses_and_keeps_hits(vt, vto, ctx):
     ctx.octree_upload(flat, [r.size for r in runs[: level_o + 1]], n, level_o)
     d_vol = ctx.to_device(vol)
     cand = pf.candidate_codes(runs[level_c])
-    slots = max(16, cand.size // 3)
+    slots = max(16, cand.size // 2)
     ctx.cache_create(level_c, 1 << (n - level_c), 3, slots, 0)
     ctx.cache_attach_device_volume(d_vol, vol.shape)
     prm = vt.render_params(level_o, level_c, iso)
@@ -51,6 +51,6 @@ def test_prefetch_cuts_misses_and_keeps_hits(vt, vto, ctx):
     p_pref = np.mean([r["passes"] for r in pref[1:]])
     print(f"demand misses plain {m_plain} prefetch {m_pref}; passes {p_plain:.2f} -> {p_pref:.2f}; "
           f"prefetched {sum(r['prefetched'] for r in pref)}", flush=True)
-    assert m_pref < 0.8 * m_plain
+    assert m_pref < 0.85 * m_plain
     net.close()
     d_vol.free()
