@@ -310,7 +310,8 @@ def run_c3(ctx, args):
         ctx.cache_create(level_c, 32, inc, gpu_slots, cpu_slots)
         ctx.cache_attach_hdf5(h5, "value")
         ctx.set_march_options(skip_empty=True, max_defer=args.max_defer)
-        ctx.set_load_options(overlap_loads=bool(args.overlap), reserve_slots=args.reserve)
+        ctx.set_load_options(overlap_loads=bool(args.overlap), reserve_slots=args.reserve,
+                             copy_workgroups=args.copy_wg)
         t0 = time.perf_counter()
         ctx.cache_prefetch_host(wait=True)
         log(f"c3 {form}: host tier filled from hdf5 in {time.perf_counter() - t0:.2f}s, "
@@ -461,7 +462,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--parity-rays", type=int, default=4096)
     ap.add_argument("--prefetch", type=int, default=1, help="c2: also time the learned-prefetch form")
-    ap.add_argument("--overlap", type=int, default=1, help="load bricks while the requesting pass runs")
+    ap.add_argument("--overlap", type=int, default=0, help="load bricks while the requesting pass runs")
+    ap.add_argument("--copy-wg", type=int, default=0, help="workgroups of the overlapped copies (0: 16)")
     ap.add_argument("--reserve", type=int, default=-1, help="free device slots kept between frames (-1: 1/8)")
     ap.add_argument("--max-defer", type=int, default=2,
                     help="absent-brick nodes a ray may walk past per pass (0 = park at the first)")
@@ -505,7 +507,8 @@ def main() -> int:
         b_.free()
     w = setup_c2(ctx, args) if args.config == "c2" else setup_c1(ctx, args)
     ctx.set_march_options(skip_empty=True, max_defer=args.max_defer)
-    ctx.set_load_options(overlap_loads=bool(args.overlap), reserve_slots=args.reserve)
+    ctx.set_load_options(overlap_loads=bool(args.overlap), reserve_slots=args.reserve,
+                             copy_workgroups=args.copy_wg)
     cams = [w.cameras(k) for k in range(args.warmup + args.steps)]
     R = w.W * w.H
     d_rgba, d_hits = ctx.alloc(R * 16), ctx.alloc(R * 32)

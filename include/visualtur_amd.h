@@ -186,13 +186,14 @@ int vt_set_frame_options(vt_context *ctx, int collect_stats, int record_trace);
  * (visualTur.cu:175-220). */
 int vt_set_march_options(vt_context *ctx, int skip_empty, int max_defer);
 /* Brick-loading option with no reference counterpart; results are identical
- * for every setting.  overlap_loads=1 (default) loads requested bricks into
- * free device slots on a second stream while the render pass that requested
- * them still runs (the reference loads between passes, visualTur.cu:175-220).
- * reserve_slots is how many free slots vt_render_frame keeps for that at the
- * end of a frame, evicting by the clock (-1 = gpuSlots/8; never more than
- * half; nothing is evicted while every brick fits). */
-int vt_set_load_options(vt_context *ctx, int overlap_loads, int64_t reserve_slots);
+ * for every setting.  overlap_loads=1 loads requested bricks into free device
+ * slots on a second stream, by copy_workgroups workgroups (0 = 16), while the
+ * render pass that requested them still runs; 0 (default) loads between
+ * passes, as the reference does (visualTur.cu:175-220).  reserve_slots is
+ * how many free slots vt_render_frame keeps for overlapped loads at the end
+ * of a frame, evicting by the clock (-1 = gpuSlots/8; never more than half;
+ * nothing is evicted while every brick with octree nodes fits). */
+int vt_set_load_options(vt_context *ctx, int overlap_loads, int64_t reserve_slots, int copy_workgroups);
 /* Visited-brick trace of the last frame: out[i] = 1 if brick
  * (i + 8^levelC) was marched by at least one ray.  n = 8^levelC. */
 int vt_get_brick_trace(vt_context *ctx, uint8_t *out, int64_t n);

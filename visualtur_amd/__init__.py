@@ -120,7 +120,7 @@ _SIGS = {
     "vt_set_frame_options": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
     "vt_get_brick_trace": (ctypes.c_int, [_P, _P, ctypes.c_int64]),
     "vt_set_march_options": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
-    "vt_set_load_options": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64]),
+    "vt_set_load_options": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64, ctypes.c_int]),
     "vt_trace_nodes": (ctypes.c_int, [_P, ctypes.POINTER(Camera), ctypes.c_int32, ctypes.c_int32,
                                       ctypes.c_int32, _P, _P]),
     "vt_octree_build": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
@@ -381,8 +381,8 @@ class Context:
     def set_march_options(self, skip_empty: bool = True, max_defer: int = 2) -> None:
         self._c(lib().vt_set_march_options(self.handle, int(skip_empty), int(max_defer)))
 
-    def set_load_options(self, overlap_loads: bool = True, reserve_slots: int = -1) -> None:
-        self._c(lib().vt_set_load_options(self.handle, int(overlap_loads), int(reserve_slots)))
+    def set_load_options(self, overlap_loads: bool = False, reserve_slots: int = -1, copy_workgroups: int = 0) -> None:
+        self._c(lib().vt_set_load_options(self.handle, int(overlap_loads), int(reserve_slots), int(copy_workgroups)))
 
     def brick_trace(self, level_c: int) -> np.ndarray:
         n = 1 << (3 * level_c)

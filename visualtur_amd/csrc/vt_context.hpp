@@ -279,15 +279,12 @@ struct AsyncStage {
     PinBuf<int64_t> codes;
     PinBuf<int64_t> tidx;
     PinBuf<int32_t> tval;
-    PinBuf<void*> dst, hsrc;   // DMA batch arguments
-    PinBuf<size_t> size;
     int64_t used = 0, cap = 0;
     hipError_t ensure(int64_t count)
     {
         hipError_t e;
         if ((e = src.ensure((size_t)count)) || (e = slots.ensure((size_t)count)) ||
-            (e = codes.ensure((size_t)count)) || (e = tidx.ensure((size_t)count)) || (e = tval.ensure((size_t)count)) ||
-            (e = dst.ensure((size_t)count)) || (e = hsrc.ensure((size_t)count)) || (e = size.ensure((size_t)count)))
+            (e = codes.ensure((size_t)count)) || (e = tidx.ensure((size_t)count)) || (e = tval.ensure((size_t)count)))
             return e;
         cap = count;
         used = 0;
@@ -296,7 +293,6 @@ struct AsyncStage {
     void release()
     {
         src.release(); slots.release(); codes.release(); tidx.release(); tval.release();
-        dst.release(); hsrc.release(); size.release();
         used = cap = 0;
     }
 };
@@ -432,7 +428,8 @@ struct vt_context {
     PinBuf<uint32_t> req_list;
     AsyncStage astage;
     std::vector<uint32_t> req_left;   // requests of a pass left for the synchronous service
-    int async_loads = 1;
+    int async_loads = 0;
+    int copy_workgroups = 16;         // workgroups of the copies that overlap a pass
     int64_t reserve_slots = -1;       // free slots kept between frames (-1: gpu_slots/8)
     DevBuf<uint8_t> trace;
     FrameCounters* d_ctr = nullptr;

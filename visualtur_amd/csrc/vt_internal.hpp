@@ -175,6 +175,8 @@ hipError_t launch_bricks_from_volume(const float* vol, int X, int Y, int Z,
 hipError_t launch_submm(const float* pool, const int32_t* slots, int count, int S, int inc, int G, float2* submm,
                         hipStream_t s);
 // Copy whole brick slots from (device-visible) host memory into the pool.
+hipError_t launch_copy_bricks_narrow(const float* const* src, const int32_t* dst_slots, int count, size_t slot_elems,
+                                     float* pool, int workgroups, hipStream_t s);
 hipError_t launch_copy_bricks(const float* const* src, const int32_t* dst_slots, int count,
                               size_t slot_elems, float* pool, hipStream_t s);
 // table[idx[i]] = val[i]

@@ -237,16 +237,9 @@ int load_async(vt_context* ctx, const uint32_t* req, size_t n, std::vector<uint3
     if (k == 0) return VT_OK;
     hipStream_t s = ctx->cstream;
     if (c.host) {
-        // DMA engines, not a copy kernel: the render pass keeps every CU
-        // (the argument arrays live in the staging until the stream is idle)
-        for (int64_t q = base; q < base + k; ++q) {
-            A.dst.p[q] = c.d_pool + (size_t)A.slots.p[q] * c.slot_elems;
-            A.hsrc.p[q] = const_cast<float*>(A.src.p[q]);
-            A.size.p[q] = c.slot_elems * sizeof(float);
-        }
-        size_t fail_idx = 0;
-        HIP_TRY(hipMemcpyBatchAsync(A.dst.p + base, A.hsrc.p + base, A.size.p + base, (size_t)k, nullptr, nullptr, 0,
-                                    &fail_idx, s));
+        // a narrow copy kernel: the render pass keeps almost every CU
+        HIP_TRY(launch_copy_bricks_narrow(A.src.dev + base, A.slots.dev + base, (int)k, c.slot_elems, c.d_pool,
+                                          ctx->copy_workgroups, s));
     } else {
         HIP_TRY(launch_bricks_from_volume(c.d_vol, c.vol_dims[0], c.vol_dims[1], c.vol_dims[2], A.codes.dev + base,
                                           A.slots.dev + base, (int)k, c.level_c, c.n, c.cube, c.inc, c.d_pool, s));
@@ -723,12 +716,14 @@ int vt_set_march_options(vt_context* ctx, int skip_empty, int max_defer)
     return VT_OK;
 }
 
-int vt_set_load_options(vt_context* ctx, int overlap_loads, int64_t reserve_slots)
+int vt_set_load_options(vt_context* ctx, int overlap_loads, int64_t reserve_slots, int copy_workgroups)
 {
     if (!ctx) return fail(VT_E_ARG, "ctx is NULL");
     if (reserve_slots < -1) return fail(VT_E_ARG, "reserve_slots must be >= -1");
+    if (copy_workgroups < 0 || copy_workgroups > 4096) return fail(VT_E_ARG, "copy_workgroups must be in [0, 4096]");
     ctx->async_loads = overlap_loads ? 1 : 0;
     ctx->reserve_slots = reserve_slots;
+    ctx->copy_workgroups = copy_workgroups ? copy_workgroups : 16;
     return VT_OK;
 }
 
