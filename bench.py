@@ -155,14 +155,14 @@ def prefetch_leg(ctx, w, cams, args, d_rgba, d_hits, plain_ms, plain_uploads, U,
     R = w.W * w.H
     ctx.cache_clear()
     for k in range(args.warmup):
-        net.prefetch(cams[k], w.slots, 0.5)
+        net.prefetch(cams[k], w.slots, args.prefetch_min_prob)
         ctx.render_frame(cams[k], w.prm, d_rgba, d_hits)
     ctx.synchronize()
     ms, pre_ms, uploads, pre_loaded, passes = [], [], [], [], []
     t0 = time.perf_counter()
     for k in range(args.warmup, args.warmup + args.steps):
         f0 = time.perf_counter()
-        pre_loaded.append(net.prefetch(cams[k], w.slots, 0.5))
+        pre_loaded.append(net.prefetch(cams[k], w.slots, args.prefetch_min_prob))
         f1 = time.perf_counter()
         st = ctx.render_frame(cams[k], w.prm, d_rgba, d_hits)
         f2 = time.perf_counter()
@@ -320,7 +320,7 @@ def run_c3(ctx, args):
         ms, pre_ms, uploads, passes, pre_n = [], [], [], [], []
         for k in range(args.warmup + args.steps):
             f0 = time.perf_counter()
-            pre_n.append(use.prefetch(cams[k], gpu_slots, 0.5) if use is not None else 0)
+            pre_n.append(use.prefetch(cams[k], gpu_slots, args.prefetch_min_prob) if use is not None else 0)
             f1 = time.perf_counter()
             st = ctx.render_frame(cams[k], prm, d_rgba, d_hits)
             f2 = time.perf_counter()
@@ -467,6 +467,7 @@ def main() -> int:
     ap.add_argument("--reserve", type=int, default=-1, help="free device slots kept between frames (-1: 1/8)")
     ap.add_argument("--max-defer", type=int, default=2,
                     help="absent-brick nodes a ray may walk past per pass (0 = park at the first)")
+    ap.add_argument("--prefetch-min-prob", type=float, default=0.8)
     ap.add_argument("--train-paths", type=int, default=24)
     ap.add_argument("--train-frames", type=int, default=6)
     ap.add_argument("--train-steps", type=int, default=600)

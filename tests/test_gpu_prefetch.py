@@ -19,8 +19,8 @@ def test_prefetch_cuts_misses_and_keeps_hits(vt, vto, ctx):
     ctx.octree_upload(flat, [r.size for r in runs[: level_o + 1]], n, level_o)
     d_vol = ctx.to_device(vol)
     cand = pf.candidate_codes(runs[level_c])
-    slots = max(16, cand.size // 2)
-    ctx.cache_create(level_c, 1 << (n - level_c), 3, slots, 0)
+    # traces are taken with every brick resident
+    ctx.cache_create(level_c, 1 << (n - level_c), 3, int(cand.size), 0)
     ctx.cache_attach_device_volume(d_vol, vol.shape)
     prm = vt.render_params(level_o, level_c, iso)
     W = H = 128
@@ -29,17 +29,20 @@ def test_prefetch_cuts_misses_and_keeps_hits(vt, vto, ctx):
     net = vt.MLP(ctx, seed=7, max_batch=8192)
     X, Y = pf.build_dataset(ctx, net, train_cams, traces, cand, n, level_c, neg_per_pos=3, seed=1)
     losses, pos_w = pf.train(ctx, net, X, Y, steps=400, batch=4096, lr=1e-3)
-    print(f"candidates {cand.size} slots {slots} samples {X.shape[0]} loss {losses[:10].mean():.4f} -> "
-          f"{losses[-10:].mean():.4f}", flush=True)
-    assert losses[-20:].mean() < 0.7 * losses[:20].mean()
-
     test_cams = pf.random_walk(W, H, 128, 32, seed=7)
-    ctx.cache_clear()
+    work = max(tr.size for tr in pf.collect_traces(ctx, test_cams, prm, level_c))
+    # a device cache that holds a frame's working set with some room, not the volume
+    slots = min(int(cand.size) - 1, int(1.3 * work) + 8)
+    print(f"candidates {cand.size} working set {work} slots {slots} samples {X.shape[0]} "
+          f"loss {losses[:10].mean():.4f} -> {losses[-10:].mean():.4f}", flush=True)
+    assert losses[-20:].mean() < 0.7 * losses[:20].mean()
+    ctx.cache_create(level_c, 1 << (n - level_c), 3, slots, 0)
+    ctx.cache_attach_device_volume(d_vol, vol.shape)
     plain, (rgba_p, hits_p) = pf.run_path(ctx, test_cams, prm)
     ctx.cache_clear()
     pref, (rgba_q, hits_q) = pf.run_path(ctx, test_cams, prm, net, budget=slots, min_prob=0.5)
     # identical pictures and hit records
-    assert np.array_equal(hits_p, hits_q)
+    assert hits_p.tobytes() == hits_q.tobytes()
     assert np.array_equal(rgba_p.view(np.uint32), rgba_q.view(np.uint32))
     # and the same as the oracle
     rc, hc, _ = vto.render(vto.Volume(vol), vto.Octree(runs, n, level_o), test_cams[-1],
@@ -51,6 +54,6 @@ def test_prefetch_cuts_misses_and_keeps_hits(vt, vto, ctx):
     p_pref = np.mean([r["passes"] for r in pref[1:]])
     print(f"demand misses plain {m_plain} prefetch {m_pref}; passes {p_plain:.2f} -> {p_pref:.2f}; "
           f"prefetched {sum(r['prefetched'] for r in pref)}", flush=True)
-    assert m_pref < 0.85 * m_plain
+    assert m_pref < 0.8 * m_plain
     net.close()
     d_vol.free()
