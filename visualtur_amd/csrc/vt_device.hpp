@@ -234,14 +234,34 @@ __device__ __forceinline__ int order_at(uint32_t order, int p) { return (int)((o
 // Octree.cu:620-720 without the stack: the parent of a node is code>>3 and
 // its position among its siblings is found by recomputing the parent's
 // sorted child list, which is a pure function of (parent, ray).
+// Sorted child lists of the last two parents expanded by this ray: the
+// next node is usually a sibling, whose parent's list is then reused.
+struct OrderCache {
+    uint64_t p[2] = {0, 0};   // parent codes (0 = empty; codes are >= 1)
+    uint32_t ord[2] = {0, 0};
+};
+
+__device__ __forceinline__ uint32_t sorted_children_cached(const Ray& r, const OctreeView& o, uint64_t parent,
+                                                           int plevel, OrderCache& c)
+{
+    if (parent == c.p[0]) return c.ord[0];
+    if (parent == c.p[1]) return c.ord[1];
+    uint32_t ord = sorted_children(r, o, parent, plevel);
+    c.p[1] = c.p[0];
+    c.ord[1] = c.ord[0];
+    c.p[0] = parent;
+    c.ord[0] = ord;
+    return ord;
+}
+
 __device__ __forceinline__ uint64_t next_node(const Ray& r, const OctreeView& o,
-                                              uint64_t node, bool descend)
+                                              uint64_t node, bool descend, OrderCache& cache)
 {
     int level = code_level(node);
     for (;;) {
         if (descend) {
             if (level == o.final_level) return node;
-            uint32_t ord = sorted_children(r, o, node, level);
+            uint32_t ord = sorted_children_cached(r, o, node, level, cache);
             if (order_count(ord) > 0) {
                 node = (node << 3) | (uint64_t)order_at(ord, 0);
                 level += 1;
@@ -252,7 +272,7 @@ __device__ __forceinline__ uint64_t next_node(const Ray& r, const OctreeView& o,
         if (level == 0) return 0;
         uint64_t parent = node >> 3;
         int k = (int)(node & 7u);
-        uint32_t ord = sorted_children(r, o, parent, level - 1);
+        uint32_t ord = sorted_children_cached(r, o, parent, level - 1, cache);
         int cnt = order_count(ord);
         int p = 0;
         while (p < cnt && order_at(ord, p) != k) ++p;
@@ -264,6 +284,12 @@ __device__ __forceinline__ uint64_t next_node(const Ray& r, const OctreeView& o,
             level -= 1;
         }
     }
+}
+
+__device__ __forceinline__ uint64_t next_node(const Ray& r, const OctreeView& o, uint64_t node, bool descend)
+{
+    OrderCache cache;
+    return next_node(r, o, node, descend, cache);
 }
 
 // Root acceptance of Octree.cu:729: level 0 holds node 1 and the ray meets
@@ -380,9 +406,18 @@ struct SubGrid {
     int cx0, cy0, cz0;  // global voxel of the brick's first core cell
 };
 
-__device__ __forceinline__ int proven_side(const SubGrid& g, float iso, float px, float py, float pz, int& last_sb,
-                                           float2& last_mm)
+// The sub-block last looked up: its voxel box [x0,x1) x [y0,y1) x [z0,z1)
+// (integer-valued floats) and its proven side.  For p >= 0, (int)p is
+// floor(p), so p lies in the box exactly when its cell lies in the
+// sub-block; NaN fails every comparison.
+struct SideCache {
+    float x0 = 1.0f, y0 = 1.0f, z0 = 1.0f, x1 = 0.0f, y1 = 0.0f, z1 = 0.0f;   // empty
+    int side = SIDE_NONE;
+};
+
+__device__ __forceinline__ int proven_side(const SubGrid& g, float iso, float px, float py, float pz, SideCache& sc)
 {
+    if (px >= sc.x0 && px < sc.x1 && py >= sc.y0 && py < sc.y1 && pz >= sc.z0 && pz < sc.z1) return sc.side;
     if (!g.mm || !(px >= 0.0f) || !(py >= 0.0f) || !(pz >= 0.0f) || !coord_ok(px) || !coord_ok(py) || !coord_ok(pz))
         return SIDE_NONE;
     int cx = (int)px - g.cx0, cy = (int)py - g.cy0, cz = (int)pz - g.cz0;
@@ -390,16 +425,21 @@ __device__ __forceinline__ int proven_side(const SubGrid& g, float iso, float px
     if ((unsigned)cx >= (unsigned)lim || (unsigned)cy >= (unsigned)lim || (unsigned)cz >= (unsigned)lim)
         return SIDE_NONE;
     int sb = ((cx >> 2) * g.G + (cy >> 2)) * g.G + (cz >> 2);
-    if (sb != last_sb) {
-        last_mm = g.mm[sb];
-        last_sb = sb;
-    }
-    float lo = last_mm.x, hi = last_mm.y;
+    float2 mm = g.mm[sb];
+    float lo = mm.x, hi = mm.y;
     float M = fmaxf(fmaxf(fabsf(lo), fabsf(hi)), fabsf(iso));
     float m = M * 1.52587890625e-05f + 1e-30f;   // 2^-16 * M
-    if (lo > iso && (lo - iso) > m) return SIDE_ABOVE;
-    if (hi < iso && (iso - hi) > m) return SIDE_BELOW;
-    return SIDE_NONE;
+    int s = SIDE_NONE;
+    if (lo > iso && (lo - iso) > m) s = SIDE_ABOVE;
+    else if (hi < iso && (iso - hi) > m) s = SIDE_BELOW;
+    sc.x0 = (float)(g.cx0 + (cx & ~3));
+    sc.y0 = (float)(g.cy0 + (cy & ~3));
+    sc.z0 = (float)(g.cz0 + (cz & ~3));
+    sc.x1 = sc.x0 + 4.0f;
+    sc.y1 = sc.y0 + 4.0f;
+    sc.z1 = sc.z0 + 4.0f;
+    sc.side = s;
+    return s;
 }
 
 __device__ __forceinline__ bool march_node_skip(const Ray& r, int nx, int ny, int nz, int side,
@@ -414,14 +454,13 @@ __device__ __forceinline__ bool march_node_skip(const Ray& r, int nx, int ny, in
     int maxStep = cvt_rzi(ceilf((tfar - tnear) * 2.0f));
     if (maxStep > 4 * side + 8) maxStep = 4 * side + 8;
     if (maxStep < 0) return false;
-    int last_sb = -1;
-    float2 last_mm = make_float2(0.0f, 0.0f);
+    SideCache sc;
     // the previous sample: its position F*, its value ant (valid only when
     // ant_known) and its side
     float F0 = X0, F1 = X1, F2 = X2;
     float ant = 0.0f;
     bool ant_known;
-    int ant_side = proven_side(g, iso, X0, X1, X2, last_sb, last_mm);
+    int ant_side = proven_side(g, iso, X0, X1, X2, sc);
     if (ant_side == SIDE_NONE) {
         ant = sample(slot, S, bx, by, bz, X0, X1, X2, err);
         nsamples += 1;
@@ -432,7 +471,7 @@ __device__ __forceinline__ bool march_node_skip(const Ray& r, int nx, int ny, in
     }
     for (int steps = 1; steps <= maxStep; ++steps) {
         X0 = X0 + s0; X1 = X1 + s1; X2 = X2 + s2;
-        int sd = proven_side(g, iso, X0, X1, X2, last_sb, last_mm);
+        int sd = proven_side(g, iso, X0, X1, X2, sc);
         if (sd != SIDE_NONE && sd == ant_side) {   // same strict side: the reference's "ant = sig"
             F0 = X0; F1 = X1; F2 = X2;
             ant_known = false;

@@ -87,7 +87,7 @@ __device__ __forceinline__ bool visit_node(const Ray& r, const OctreeView& oct, 
 __device__ __forceinline__ int walk_ray(const Ray& r, const OctreeView& oct, const BrickView& bv, const Shading& sh,
                                         uint64_t& node, uint8_t* slot_ref, uint8_t* brick_trace, float4& out,
                                         HitRecord& h, uint64_t& bidx, uint32_t& err, uint32_t& nsamples,
-                                        uint32_t& nnodes)
+                                        uint32_t& nnodes, OrderCache& oc)
 {
     for (;;) {
         if (node == 0) {
@@ -100,7 +100,7 @@ __device__ __forceinline__ int walk_ray(const Ray& r, const OctreeView& oct, con
         if (slot < 0) return WALK_ABSENT;
         if (visit_node(r, oct, bv, sh, node, brick, slot, slot_ref, brick_trace, out, h, err, nsamples, nnodes))
             return WALK_HIT;
-        node = next_node(r, oct, node, false);
+        node = next_node(r, oct, node, false, oc);
     }
 }
 
@@ -127,7 +127,7 @@ __device__ __forceinline__ void request_brick(const RenderArgs& a, uint64_t bidx
 __device__ __forceinline__ int walk_defer(const Ray& r, const RenderArgs& a, const BrickView& bv, const Shading& sh,
                                           uint64_t& node, uint32_t id, uint32_t R, int& nd, float4& out,
                                           HitRecord& h, uint64_t& bidx, uint32_t& err, uint32_t& nsamples,
-                                          uint32_t& nnodes)
+                                          uint32_t& nnodes, OrderCache& oc)
 {
     for (;;) {
         if (node == 0) {
@@ -146,7 +146,7 @@ __device__ __forceinline__ int walk_defer(const Ray& r, const RenderArgs& a, con
                               nnodes)) {
             return WALK_HIT;
         }
-        node = next_node(r, a.oct, node, false);
+        node = next_node(r, a.oct, node, false, oc);
     }
 }
 
@@ -189,14 +189,15 @@ __global__ __launch_bounds__(kBlock) void k_render_pass(RenderArgs a)
         HitRecord h;
         uint64_t bidx = 0, node = 0;
         int nd = 0;
+        OrderCache oc;
         // what the pass leaves: WALK_HIT/WALK_MISS = c/h to write, WALK_ABSENT =
         // park on `node`, kKeep = the output already holds the ray's result
         // (final when nd == 0), kBlocked = a deferred node is still absent
         constexpr int kKeep = 3, kBlocked = 4;
         int st;
         if (a.in_list == nullptr) {
-            node = root_ok(r, a.oct) ? next_node(r, a.oct, 1ull, true) : 0ull;
-            st = walk_defer(r, a, bv, sh, node, id, R, nd, c, h, bidx, err, nsamples, nnodes);
+            node = root_ok(r, a.oct) ? next_node(r, a.oct, 1ull, true, oc) : 0ull;
+            st = walk_defer(r, a, bv, sh, node, id, R, nd, c, h, bidx, err, nsamples, nnodes, oc);
         } else {
             const uint8_t ph = a.ray_phase[id];
             const int nprev = ph & PHASE_NMASK;
@@ -227,7 +228,7 @@ __global__ __launch_bounds__(kBlock) void k_render_pass(RenderArgs a)
                 st = kKeep;   // every deferred node missed: the tentative result stands
             } else {
                 node = a.ray_node[id];   // parked on this node, not yet marched
-                st = walk_defer(r, a, bv, sh, node, id, R, nd, c, h, bidx, err, nsamples, nnodes);
+                st = walk_defer(r, a, bv, sh, node, id, R, nd, c, h, bidx, err, nsamples, nnodes, oc);
             }
         }
         if (st == WALK_ABSENT) {
@@ -296,12 +297,13 @@ __global__ __launch_bounds__(kBlock) void k_serve_pass(const ServeJob* __restric
     uint32_t id = (uint32_t)(i * W + j);
     Ray r;
     make_ray(J.cam, i, j, r);
-    uint64_t node = root_ok(r, J.oct) ? next_node(r, J.oct, 1ull, true) : 0ull;
+    OrderCache oc;
+    uint64_t node = root_ok(r, J.oct) ? next_node(r, J.oct, 1ull, true, oc) : 0ull;
     float4 c;
     HitRecord h;
     uint64_t bidx;
     uint32_t err = 0, ns = 0, nn = 0;
-    int st = walk_ray(r, J.oct, J.bv, J.sh, node, nullptr, nullptr, c, h, bidx, err, ns, nn);
+    int st = walk_ray(r, J.oct, J.bv, J.sh, node, nullptr, nullptr, c, h, bidx, err, ns, nn, oc);
     if (st == WALK_ABSENT) {   // cannot happen with a fully resident scene
         err |= 2u;
         c = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
@@ -332,11 +334,12 @@ __global__ __launch_bounds__(kBlock) void k_trace_nodes(TraceArgs a)
     make_ray(a.cam, i, j, r);
     int count = 0;
     if (root_ok(r, a.oct)) {
-        uint64_t node = next_node(r, a.oct, 1ull, true);
+        OrderCache oc;
+        uint64_t node = next_node(r, a.oct, 1ull, true, oc);
         while (node != 0 && count < a.cap_total) {
             if (count < a.max_nodes) a.nodes[(size_t)id * a.max_nodes + count] = node;
             count++;
-            node = next_node(r, a.oct, node, false);
+            node = next_node(r, a.oct, node, false, oc);
         }
     }
     a.counts[id] = count;
