@@ -44,6 +44,22 @@ METRIC = "rays/sec + p50/p99 frame ms on 1 MI355X, prefetch vs plain, identical 
 HBM_PEAK_GBS = 8000.0
 
 
+# rocprofv3 summaries of k_render_pass for the workloads: FETCH_SIZE + WRITE_SIZE per
+# dispatch, each scaled by what the same counter reads for a 256 MiB copy
+PROFILES = {"c1": ROOT / "profiles" / "k_render_pass_256cubed_blobs.md",
+            "c2": ROOT / "profiles" / "k_render_pass_1024cubed_fbm_orbit.md"}
+
+
+def measured_traffic(name: str):
+    """HBM bytes per k_render_pass launch from the committed rocprof summary, or None."""
+    import re
+    path = PROFILES.get(name)
+    if path is None or not path.exists():
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", path.read_text(), re.M)
+    return int(m.group(1)) if m else None
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -548,7 +564,7 @@ def main() -> int:
     launches = sum(passes)
     achieved_gbs = sum(B) / (sum(kernel_ms) * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved_gbs / HBM_PEAK_GBS, 5), "traffic": None,
+                "frac": round(achieved_gbs / HBM_PEAK_GBS, 5), "traffic": measured_traffic(w.name),
                 "kernel": "k_render_pass", "avg_launch_ms": round(sum(kernel_ms) / launches, 4),
                 "bytes_per_launch": int(sum(B) / launches),
                 "bricks_marched_per_frame": int(np.mean(U)), "brick_bytes": Sb,
