@@ -325,7 +325,7 @@ def run_c3(ctx, args):
     for form in ("plain", "prefetch"):
         ctx.cache_create(level_c, 32, inc, gpu_slots, cpu_slots)
         ctx.cache_attach_hdf5(h5, "value")
-        ctx.set_march_options(skip_empty=True, max_defer=args.max_defer)
+        ctx.set_march_options(skip_empty=True, max_defer=args.max_defer, persistent=bool(args.persistent))
         ctx.set_load_options(overlap_loads=bool(args.overlap), reserve_slots=args.reserve,
                              copy_workgroups=args.copy_wg)
         t0 = time.perf_counter()
@@ -481,6 +481,7 @@ def main() -> int:
     ap.add_argument("--overlap", type=int, default=0, help="load bricks while the requesting pass runs")
     ap.add_argument("--copy-wg", type=int, default=0, help="workgroups of the overlapped copies (0: 16)")
     ap.add_argument("--reserve", type=int, default=-1, help="free device slots kept between frames (-1: 1/8)")
+    ap.add_argument("--persistent", type=int, default=1, help="persistent render pass with lane refill")
     ap.add_argument("--max-defer", type=int, default=2,
                     help="absent-brick nodes a ray may walk past per pass (0 = park at the first)")
     ap.add_argument("--prefetch-min-prob", type=float, default=0.8)
@@ -523,7 +524,7 @@ def main() -> int:
         a_.free()
         b_.free()
     w = setup_c2(ctx, args) if args.config == "c2" else setup_c1(ctx, args)
-    ctx.set_march_options(skip_empty=True, max_defer=args.max_defer)
+    ctx.set_march_options(skip_empty=True, max_defer=args.max_defer, persistent=bool(args.persistent))
     ctx.set_load_options(overlap_loads=bool(args.overlap), reserve_slots=args.reserve,
                              copy_workgroups=args.copy_wg)
     cams = [w.cameras(k) for k in range(args.warmup + args.steps)]
@@ -581,7 +582,7 @@ def main() -> int:
         "kernel_ms_per_frame": round(float(np.mean(kernel_ms)), 4),
         "passes_per_frame": float(np.mean(passes)), "bricks_uploaded_per_frame": float(np.mean(uploads)),
         "rays_parked_per_frame": float(np.mean(parked)), "max_defer": args.max_defer,
-        "overlap_loads": args.overlap, "reserve_slots": args.reserve,
+        "overlap_loads": args.overlap, "reserve_slots": args.reserve, "persistent": args.persistent,
         "bricks_overlapped_per_frame": float(np.mean(overl)),
         "bricks_marched_per_frame": float(np.mean(U)),
         "device_miss_rate": round(float(sum(uploads)) / max(1, sum(U)), 5),

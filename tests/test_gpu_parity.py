@@ -303,3 +303,31 @@ def test_overlapped_loads_are_exact(vt, vto, ctx, slots, reserve, defer):
     rg = np.frombuffer(outs[1][-1][1], np.float32).reshape(-1, 4)
     hg = np.frombuffer(outs[1][-1][0], vt.HIT_DTYPE)
     assert_same(hg, rg, hc, rc)
+
+
+@pytest.mark.parametrize("slots,defer", [(512, 2), (5, 0), (17, 2), (40, 4)])
+def test_persistent_pass_matches_tiled_pass(vt, vto, ctx, slots, defer):
+    """The persistent pass (lanes refilled with new rays) and the one-lane-
+    per-ray tiled pass give the same bytes, pass count and requests."""
+    vol = blobs_volume(64, 16, seed=71)
+    sc = Scene(vto, vol, 0.3, 4, 3)
+    sc.setup_gpu(vt, ctx, slots=slots)
+    cams = [orbit_camera(vt, 131, 77, (32, 32, 32), 105.0, 30.0 + 50.0 * k, 15.0) for k in range(3)]
+    outs = {}
+    try:
+        for persistent in (False, True):
+            log(f"persistent={persistent} slots={slots} defer={defer}")
+            ctx.set_march_options(skip_empty=True, max_defer=defer, persistent=persistent)
+            ctx.cache_clear()
+            rows = []
+            for cam in cams:
+                rg, hg, st = sc.render_gpu(vt, ctx, cam)
+                assert st.err == 0
+                rows.append((hg.tobytes(), rg.tobytes(), st.passes, st.bricks_requested))
+            outs[persistent] = rows
+    finally:
+        ctx.set_march_options()
+    assert outs[False] == outs[True]
+    rc, hc = sc.render_cpu(vto, cams[-1])
+    assert_same(np.frombuffer(outs[True][-1][0], vt.HIT_DTYPE), np.frombuffer(outs[True][-1][1], np.float32).reshape(-1, 4),
+                hc, rc)

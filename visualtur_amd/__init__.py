@@ -119,7 +119,7 @@ _SIGS = {
                                        ctypes.POINTER(FrameStats)]),
     "vt_set_frame_options": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
     "vt_get_brick_trace": (ctypes.c_int, [_P, _P, ctypes.c_int64]),
-    "vt_set_march_options": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
+    "vt_set_march_options": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "vt_set_load_options": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64, ctypes.c_int]),
     "vt_trace_nodes": (ctypes.c_int, [_P, ctypes.POINTER(Camera), ctypes.c_int32, ctypes.c_int32,
                                       ctypes.c_int32, _P, _P]),
@@ -378,8 +378,8 @@ class Context:
                                       d_hits.ptr if d_hits is not None else None, ctypes.byref(st)))
         return st
 
-    def set_march_options(self, skip_empty: bool = True, max_defer: int = 2) -> None:
-        self._c(lib().vt_set_march_options(self.handle, int(skip_empty), int(max_defer)))
+    def set_march_options(self, skip_empty: bool = True, max_defer: int = 2, persistent: bool = True) -> None:
+        self._c(lib().vt_set_march_options(self.handle, int(skip_empty), int(max_defer), int(persistent)))
 
     def set_load_options(self, overlap_loads: bool = False, reserve_slots: int = -1, copy_workgroups: int = 0) -> None:
         self._c(lib().vt_set_load_options(self.handle, int(overlap_loads), int(reserve_slots), int(copy_workgroups)))

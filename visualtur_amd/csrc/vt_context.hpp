@@ -437,6 +437,9 @@ struct vt_context {
     int collect_stats = 0, record_trace = 0;
     int skip_empty = 1;   // sub-block sample skipping in the march
     int max_defer = 2;    // absent-brick nodes a ray walks past per pass
+    int persistent = 1;   // persistent render pass with lane refill
+    int persistent_wgs = 0;
+    DevBuf<uint32_t> d_next;   // its 8 work counters
     int64_t trace_n = 0;
     DevBuf<uint8_t> stage_trav;
     int64_t stage_rays = 0;
@@ -456,7 +459,7 @@ struct vt_context {
     {
         cache.reset();
         if (d_child) (void)hipFree(d_child);
-        ray_node.release(); ray_phase.release(); ray_defer.release(); list[0].release(); list[1].release();
+        ray_node.release(); ray_phase.release(); ray_defer.release(); d_next.release(); list[0].release(); list[1].release();
         if (cstream) (void)hipStreamSynchronize(cstream);
         hits_scratch.release(); req_bits.release(); req_list.release(); trace.release(); stage_trav.release();
         astage.release();

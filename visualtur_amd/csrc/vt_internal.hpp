@@ -68,6 +68,7 @@ struct RenderArgs {
     uint8_t* ray_phase;
     uint64_t* ray_defer;         // [max_defer][R] deferred nodes, or null
     int max_defer;               // 0: park at the first absent brick
+    uint32_t* next;              // persistent form: 8 work counters (one per XCD part), zero at launch
     const uint32_t* in_list;     // null: first pass over the whole frame
     uint32_t in_count;
     uint32_t* out_list;
@@ -164,6 +165,8 @@ hipError_t launch_node_minmax(const float* vol, int X, int Y, int Z, int n, int 
 hipError_t launch_render_cube(const CubeArgs& a, hipStream_t s);
 hipError_t launch_stage_traverse(const StageTraverseArgs& a, hipStream_t s);
 hipError_t launch_stage_raycast(const StageRaycastArgs& a, hipStream_t s);
+hipError_t launch_render_persistent(const RenderArgs& a, int max_workgroups, hipStream_t s);
+int render_persistent_workgroups(int device);   // resident workgroups of the persistent pass on `device`
 hipError_t launch_render_pass(const RenderArgs& a, hipStream_t s);
 hipError_t launch_trace_nodes(const TraceArgs& a, hipStream_t s);
 // Assemble bricks with halo from a device volume [X][Y][Z] (zero outside).

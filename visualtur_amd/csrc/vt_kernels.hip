@@ -263,6 +263,238 @@ __global__ __launch_bounds__(kBlock) void k_render_pass(RenderArgs a)
     }
 }
 
+// ------------------------------------------------- persistent render pass
+// The same per-ray work as k_render_pass, but a fixed grid of workgroups
+// loops over the pass's rays: a lane whose ray finishes (or parks) takes a
+// new ray when a quarter of its wave is idle, so waves stay full and every
+// memory latency serves ~64 rays instead of the few still running in a
+// fixed 8x8 tile.  Each iteration marches at most one node per lane.
+//
+// Work items of a pass: the first pass enumerates 16x16 tiles (256 items
+// each, in k_render_pass's pixel order, items outside the image skipped);
+// later passes enumerate in_list.  The items are cut into 8 contiguous parts,
+// one per XCD (workgroups are dealt to XCDs round-robin, so blockIdx % 8 is
+// the home part), each with its own counter; a wave whose part is used up
+// takes items from the next part.  Every item is taken exactly once, and a
+// wave leaves when all parts are used up and its lanes are idle.
+namespace {
+
+enum LaneMode : int { LANE_IDLE = 0, LANE_WALK = 1, LANE_DEFER = 2 };
+
+__device__ __forceinline__ void part_range(uint32_t total, uint32_t unit, int p, uint32_t& beg, uint32_t& end)
+{
+    uint32_t per = ((total + unit - 1) / unit + 7) / 8 * unit;
+    beg = per * (uint32_t)p;
+    end = beg + per;
+    if (beg > total) beg = total;
+    if (end > total) end = total;
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(kBlock) void k_render_persistent(RenderArgs a)
+{
+    const int W = a.cam.W, H = a.cam.H;
+    const uint32_t R = (uint32_t)W * (uint32_t)H;
+    const bool first = a.in_list == nullptr;
+    const uint32_t total = first ? (uint32_t)a.tiles_total * 256u : a.in_count;
+    const uint32_t unit = first ? 256u : 1u;
+    const int lane = (int)__lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    BrickView bv{a.brick_slot, a.pool, a.S, a.inc, a.slot_elems, a.level_c, a.shift, a.brick_base, a.submm, a.G};
+    Shading sh{a.iso, a.lx, a.ly, a.lz, a.color_scale};
+
+    int part = (int)(blockIdx.x & 7u);   // wave-uniform
+    int parts_left = 8;                  // wave-uniform: parts not yet found used up
+    uint32_t err = 0, nsamples = 0, nnodes = 0;
+
+    int mode = LANE_IDLE;
+    uint32_t id = 0;
+    Ray r;
+    uint64_t node = 0;
+    int nd = 0, k = 0, nprev = 0;
+    uint8_t tent = 0;
+    OrderCache oc;
+
+    for (;;) {
+        // ---- refill idle lanes
+        const unsigned long long idle = __ballot(mode == LANE_IDLE);
+        const int n_idle = __popcll(idle);
+        if (parts_left > 0 && n_idle >= 16) {
+            uint32_t base = 0, got = 0, beg = 0;
+            // lane 0 claims items; the others take them by rank
+            if (lane == 0) {
+                while (parts_left > 0) {
+                    uint32_t pb, pe;
+                    part_range(total, unit, part, pb, pe);
+                    uint32_t b0 = pe > pb ? atomicAdd(&a.next[part], (uint32_t)n_idle) : 0xffffffffu;
+                    if (pe > pb && b0 < pe - pb) {
+                        base = b0;
+                        got = min((uint32_t)n_idle, pe - pb - b0);
+                        beg = pb;
+                        break;
+                    }
+                    part = (part + 1) & 7;
+                    parts_left--;
+                }
+            }
+            base = __shfl(base, 0);
+            got = __shfl(got, 0);
+            beg = __shfl(beg, 0);
+            part = __shfl(part, 0);
+            parts_left = __shfl(parts_left, 0);
+            const uint32_t rank = (uint32_t)__popcll(idle & below);
+            if (mode == LANE_IDLE && rank < got) {
+                const uint32_t item = beg + base + rank;
+                bool valid = true;
+                if (first) {
+                    const int t = (int)(item >> 8), q = (int)(item & 255u);
+                    const int ty = t / a.tiles_x, tx = t % a.tiles_x;
+                    const int w = q >> 6, ln = q & 63;
+                    const int j = tx * 16 + (w & 1) * 8 + (ln & 7);
+                    const int i = ty * 16 + (w >> 1) * 8 + (ln >> 3);
+                    valid = i < H && j < W;
+                    id = valid ? (uint32_t)(i * W + j) : 0u;
+                } else {
+                    id = a.in_list[item];
+                }
+                if (valid) {
+                    make_ray(a.cam, (int)(id / (uint32_t)W), (int)(id % (uint32_t)W), r);
+                    oc = OrderCache();
+                    nd = 0;
+                    k = 0;
+                    if (first) {
+                        node = root_ok(r, a.oct) ? next_node(r, a.oct, 1ull, true, oc) : 0ull;
+                        nprev = 0;
+                        tent = 0;
+                        mode = LANE_WALK;
+                    } else {
+                        const uint8_t ph = a.ray_phase[id];
+                        nprev = ph & PHASE_NMASK;
+                        tent = ph & PHASE_TENT;
+                        if (nprev > 0) {
+                            mode = LANE_DEFER;
+                        } else {
+                            node = a.ray_node[id];
+                            mode = LANE_WALK;
+                        }
+                    }
+                }
+            }
+        }
+        if (__ballot(mode != LANE_IDLE) == 0) {
+            if (parts_left == 0) break;
+            continue;
+        }
+
+        // ---- one node per active lane
+        bool park = false;
+        if (mode != LANE_IDLE) {
+            const bool walk = mode == LANE_WALK;
+            const uint64_t target = walk ? node : a.ray_defer[(size_t)k * R + id];
+            const bool ended = walk && target == 0;
+            uint64_t brick = 0, bidx = 0;
+            int slot = -1;
+            if (!ended) {
+                brick = target >> bv.shift;
+                bidx = brick - bv.brick_base;
+                slot = bv.brick_slot[bidx];
+            }
+            float4 c;
+            HitRecord h;
+            bool hit = false;
+            if (slot >= 0)
+                hit = visit_node(r, a.oct, bv, sh, target, brick, slot, a.slot_ref, a.brick_trace, c, h, err, nsamples,
+                                 nnodes);
+            bool advance = false;
+            if (walk) {
+                if (ended || hit) {
+                    if (ended) set_miss(c, h);
+                    a.rgba[id] = c;
+                    a.hits[id] = h;
+                    if (nd > 0) {   // tentative: deferred nodes come first
+                        a.ray_phase[id] = (uint8_t)(PHASE_TENT | nd);
+                        park = true;
+                    }
+                    mode = LANE_IDLE;
+                } else if (slot < 0) {
+                    request_brick(a, bidx);
+                    if (nd < a.max_defer) {
+                        a.ray_defer[(size_t)nd * R + id] = target;
+                        nd += 1;
+                        advance = true;
+                    } else {
+                        a.ray_node[id] = target;
+                        a.ray_phase[id] = (uint8_t)nd;
+                        park = true;
+                        mode = LANE_IDLE;
+                    }
+                } else {
+                    advance = true;
+                }
+            } else {
+                if (slot < 0) {   // still absent: keep the rest of the list, park again
+                    for (int q = k; q < nprev; ++q)
+                        a.ray_defer[(size_t)(q - k) * R + id] = a.ray_defer[(size_t)q * R + id];
+                    request_brick(a, bidx);
+                    a.ray_phase[id] = (uint8_t)(tent | (nprev - k));
+                    park = true;
+                    mode = LANE_IDLE;
+                } else if (hit) {   // every earlier node missed: final
+                    a.rgba[id] = c;
+                    a.hits[id] = h;
+                    mode = LANE_IDLE;
+                } else if (++k == nprev) {
+                    if (tent) {
+                        mode = LANE_IDLE;   // the tentative result stands
+                    } else {
+                        node = a.ray_node[id];   // resume the walk
+                        nd = 0;
+                        mode = LANE_WALK;
+                    }
+                }
+            }
+            if (advance) node = next_node(r, a.oct, target, false, oc);
+        }
+        const uint32_t pslot = wave_atomic_add(&a.ctr->parked_count, park);
+        if (park) a.out_list[pslot] = id;
+    }
+
+    if (err) atomicOr(&a.ctr->err, err);
+    if (a.collect_stats) {
+        unsigned long long s = nsamples, nn = nnodes;
+        for (int off = 32; off > 0; off >>= 1) {
+            s += __shfl_xor(s, off);
+            nn += __shfl_xor(nn, off);
+        }
+        if (lane == 0) {
+            atomicAdd(&a.ctr->samples, s);
+            atomicAdd(&a.ctr->nodes, nn);
+        }
+    }
+}
+
+int render_persistent_workgroups(int device)
+{
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
+        return 1024;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_render_persistent, kBlock, 0) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 4;
+    return cus * per_cu;
+}
+
+hipError_t launch_render_persistent(const RenderArgs& a, int max_workgroups, hipStream_t s)
+{
+    uint32_t total = a.in_list ? a.in_count : (uint32_t)a.tiles_total * 256u;
+    if (total == 0) return hipSuccess;
+    uint32_t grid = (total + kBlock - 1) / kBlock;
+    if (grid > (uint32_t)max_workgroups) grid = (uint32_t)max_workgroups;
+    hipLaunchKernelGGL(k_render_persistent, dim3(grid), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_render_pass(const RenderArgs& a, hipStream_t s)
 {
     unsigned grid;

@@ -342,6 +342,8 @@ int vt_context_create(int device, vt_context** out)
     HIP_TRY(hipEventCreate(&ctx->ev0));
     HIP_TRY(hipEventCreate(&ctx->ev1));
     HIP_TRY(hipMalloc((void**)&ctx->d_ctr, sizeof(FrameCounters)));
+    if (ctx->d_next.ensure(8) != hipSuccess) return fail(VT_E_NOMEM, "work counters");
+    ctx->persistent_wgs = render_persistent_workgroups(device);
     HIP_TRY(hipHostMalloc((void**)&ctx->h_ctr, sizeof(FrameCounters), hipHostMallocDefault));
     *out = ctx.release();
     return VT_OK;
@@ -707,12 +709,13 @@ int vt_set_frame_options(vt_context* ctx, int collect_stats, int record_trace)
     return VT_OK;
 }
 
-int vt_set_march_options(vt_context* ctx, int skip_empty, int max_defer)
+int vt_set_march_options(vt_context* ctx, int skip_empty, int max_defer, int persistent)
 {
     if (!ctx) return fail(VT_E_ARG, "ctx is NULL");
     if (max_defer < 0 || max_defer > kMaxDefer) return fail(VT_E_ARG, "max_defer must be in [0, 16]");
     ctx->skip_empty = skip_empty ? 1 : 0;
     ctx->max_defer = max_defer;
+    ctx->persistent = persistent ? 1 : 0;
     return VT_OK;
 }
 
@@ -813,9 +816,14 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
     a.out_list = ctx->list[cur].p;
     const bool overlap = ctx->async_loads != 0;
     uint32_t* rq = ctx->req_list.p;
+    a.next = ctx->d_next.p;
     for (int pass = 0;; ++pass) {
+        if (ctx->persistent) HIP_TRY(hipMemsetAsync(ctx->d_next.p, 0, 8 * sizeof(uint32_t), ctx->stream));
         HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-        HIP_TRY(launch_render_pass(a, ctx->stream));
+        if (ctx->persistent)
+            HIP_TRY(launch_render_persistent(a, ctx->persistent_wgs, ctx->stream));
+        else
+            HIP_TRY(launch_render_pass(a, ctx->stream));
         HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
         HIP_TRY(hipMemcpyAsync(ctx->h_ctr, ctx->d_ctr, sizeof(FrameCounters), hipMemcpyDeviceToHost, ctx->stream));
         // while the pass runs: load requested bricks into free slots
