@@ -481,7 +481,7 @@ def main() -> int:
     ap.add_argument("--overlap", type=int, default=0, help="load bricks while the requesting pass runs")
     ap.add_argument("--copy-wg", type=int, default=0, help="workgroups of the overlapped copies (0: 16)")
     ap.add_argument("--reserve", type=int, default=-1, help="free device slots kept between frames (-1: 1/8)")
-    ap.add_argument("--persistent", type=int, default=1, help="persistent render pass with lane refill")
+    ap.add_argument("--persistent", type=int, default=0, help="persistent render pass with lane refill")
     ap.add_argument("--max-defer", type=int, default=2,
                     help="absent-brick nodes a ray may walk past per pass (0 = park at the first)")
     ap.add_argument("--prefetch-min-prob", type=float, default=0.8)

@@ -183,8 +183,8 @@ int vt_set_frame_options(vt_context *ctx, int collect_stats, int record_trace);
  * past in one pass (their bricks are requested, and they are marched in a
  * later pass before the ray's tentative result is accepted); 0 parks a ray
  * at its first absent brick, as the reference's staged loop does
- * (visualTur.cu:175-220).  persistent=1 (default) runs each pass as a fixed
- * grid whose lanes take a new ray when theirs finishes; 0 runs one lane per
+ * (visualTur.cu:175-220).  persistent=1 runs each pass as a fixed grid whose
+ * lanes take a new ray when theirs finishes; 0 (default) runs one lane per
  * ray in 16x16-pixel workgroups. */
 int vt_set_march_options(vt_context *ctx, int skip_empty, int max_defer, int persistent);
 /* Brick-loading option with no reference counterpart; results are identical

@@ -331,3 +331,21 @@ def test_persistent_pass_matches_tiled_pass(vt, vto, ctx, slots, defer):
     rc, hc = sc.render_cpu(vto, cams[-1])
     assert_same(np.frombuffer(outs[True][-1][0], vt.HIT_DTYPE), np.frombuffer(outs[True][-1][1], np.float32).reshape(-1, 4),
                 hc, rc)
+
+
+def test_iso_change_keeps_the_cache_exact(vt, vto, ctx):
+    """The sub-block side maps follow the frame's isovalue: a cache filled
+    at one iso renders a second iso (with its own octree) exactly."""
+    vol = blobs_volume(64, 14, seed=81)
+    a, b = Scene(vto, vol, 0.25, 4, 2), Scene(vto, vol, 0.45, 4, 2)
+    a.setup_gpu(vt, ctx, slots=40)
+    b.prm_g = vt.render_params(b.level_o, b.level_c, b.iso)
+    cam = orbit_camera(vt, 96, 72, (32, 32, 32), 100.0, 65.0)
+    for sc in (a, b, a):
+        flat, sizes = flat_runs(sc.runs, sc.level_o)
+        ctx.octree_upload(flat, sizes, sc.n, sc.level_o)
+        log(f"iso {sc.iso}")
+        rg, hg, st = sc.render_gpu(vt, ctx, cam)   # the cache keeps the bricks of the previous iso
+        rc, hc = sc.render_cpu(vto, cam)
+        assert st.err == 0
+        assert_same(hg, rg, hc, rc)

@@ -378,7 +378,7 @@ class Context:
                                       d_hits.ptr if d_hits is not None else None, ctypes.byref(st)))
         return st
 
-    def set_march_options(self, skip_empty: bool = True, max_defer: int = 2, persistent: bool = True) -> None:
+    def set_march_options(self, skip_empty: bool = True, max_defer: int = 2, persistent: bool = False) -> None:
         self._c(lib().vt_set_march_options(self.handle, int(skip_empty), int(max_defer), int(persistent)))
 
     def set_load_options(self, overlap_loads: bool = False, reserve_slots: int = -1, copy_workgroups: int = 0) -> None:

@@ -308,6 +308,12 @@ struct BrickCache {
     uint8_t* d_ref = nullptr;
     float2* d_submm = nullptr;   // G^3 sub-block (min, max) per slot, null when bricks are not 4k wide
     int G = 0;
+    // per slot: 2-bit proven side of each sub-block for side_iso (Morton
+    // order, 64 per uint4); null unless G is a power of two
+    uint4* d_side = nullptr;
+    int side_regions = 0;
+    float side_iso = 0.0f;
+    bool side_valid = false;
     std::vector<int32_t> table;
     std::vector<int64_t> slot_brick;
     std::vector<uint8_t> ref, ref_tmp;
@@ -344,6 +350,7 @@ struct BrickCache {
         if (d_table) (void)hipFree(d_table);
         if (d_ref) (void)hipFree(d_ref);
         if (d_submm) (void)hipFree(d_submm);
+        if (d_side) (void)hipFree(d_side);
         h_src.release(); h_slots.release(); h_codes.release(); h_tidx.release(); h_tval.release();
         d_src.release(); d_slots.release(); d_codes.release(); d_tidx.release(); d_tval.release();
     }
@@ -437,7 +444,7 @@ struct vt_context {
     int collect_stats = 0, record_trace = 0;
     int skip_empty = 1;   // sub-block sample skipping in the march
     int max_defer = 2;    // absent-brick nodes a ray walks past per pass
-    int persistent = 1;   // persistent render pass with lane refill
+    int persistent = 0;   // persistent render pass with lane refill (measured slower; option)
     int persistent_wgs = 0;
     DevBuf<uint32_t> d_next;   // its 8 work counters
     int64_t trace_n = 0;
