@@ -301,6 +301,10 @@ struct AsyncStage {
 struct BrickCache {
     int level_c = 0, cube = 0, inc = 0, S = 0;
     int n = 0;   // octree depth the brick codes refer to (set on first use)
+    size_t brick_elems = 0;   // S^3 floats of a brick
+    // floats between device slots: S^3 rounded up to 256 bytes, so no cache
+    // line holds two slots (a copy into a free slot while a pass runs must
+    // not share a line that the pass reads from its neighbour)
     size_t slot_elems = 0;
     int64_t nbricks = 0, gpu_slots = 0, cpu_slots = 0;
     float* d_pool = nullptr;

@@ -176,18 +176,19 @@ hipError_t launch_trace_nodes(const TraceArgs& a, hipStream_t s);
 // Assemble bricks with halo from a device volume [X][Y][Z] (zero outside).
 hipError_t launch_bricks_from_volume(const float* vol, int X, int Y, int Z,
                                      const int64_t* brick_codes, const int32_t* slots, int count,
-                                     int level_c, int n, int cube_dim, int inc,
-                                     float* pool, hipStream_t s);
+                                     int level_c, int n, int cube_dim, int inc, size_t stride,
+                                     float* pool, hipStream_t s);   // slot b at pool + b * stride
 // (min, max) of every 4x4x4-cell sub-block (5^3 samples) of the given slots
 hipError_t launch_sidemap(const float2* submm, const int32_t* slots, int count, int G, float iso, uint4* side,
                           hipStream_t s);   // slots == null: slots 0..count-1
-hipError_t launch_submm(const float* pool, const int32_t* slots, int count, int S, int inc, int G, float2* submm,
-                        hipStream_t s);
-// Copy whole brick slots from (device-visible) host memory into the pool.
+hipError_t launch_submm(const float* pool, const int32_t* slots, int count, int S, int inc, int G, size_t stride,
+                        float2* submm, hipStream_t s);
+// Copy bricks of slot_elems floats from (device-visible) host memory into
+// the pool, slot b at pool + b * stride.
 hipError_t launch_copy_bricks_narrow(const float* const* src, const int32_t* dst_slots, int count, size_t slot_elems,
-                                     float* pool, int workgroups, hipStream_t s);
+                                     size_t stride, float* pool, int workgroups, hipStream_t s);
 hipError_t launch_copy_bricks(const float* const* src, const int32_t* dst_slots, int count,
-                              size_t slot_elems, float* pool, hipStream_t s);
+                              size_t slot_elems, size_t stride, float* pool, hipStream_t s);
 // table[idx[i]] = val[i]
 hipError_t launch_scatter_i32(int32_t* table, const int64_t* idx, const int32_t* val, int count,
                               hipStream_t s);

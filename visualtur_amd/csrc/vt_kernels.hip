@@ -758,14 +758,14 @@ __global__ __launch_bounds__(256) void k_bricks_from_volume(const float* __restr
                                                             int X, int Y, int Z,
                                                             const int64_t* __restrict__ codes,
                                                             const int32_t* __restrict__ slots,
-                                                            int level_c, int n, int S, int inc,
+                                                            int level_c, int n, int S, int inc, size_t stride,
                                                             float* __restrict__ pool)
 {
     int b = blockIdx.x;
     int bx, by, bz;
     node_min((uint64_t)codes[b], level_c, n, bx, by, bz);
     bx -= inc; by -= inc; bz -= inc;
-    float* dst = pool + (size_t)slots[b] * ((size_t)S * S * S);
+    float* dst = pool + (size_t)slots[b] * stride;
     int total = S * S * S;
     for (int e = threadIdx.x; e < total; e += blockDim.x) {
         int z = e % S, y = (e / S) % S, x = e / (S * S);
@@ -779,13 +779,13 @@ __global__ __launch_bounds__(256) void k_bricks_from_volume(const float* __restr
 
 hipError_t launch_bricks_from_volume(const float* vol, int X, int Y, int Z,
                                      const int64_t* brick_codes, const int32_t* slots, int count,
-                                     int level_c, int n, int cube_dim, int inc,
+                                     int level_c, int n, int cube_dim, int inc, size_t stride,
                                      float* pool, hipStream_t s)
 {
     if (count <= 0) return hipSuccess;
     int S = cube_dim + 2 * inc;
     hipLaunchKernelGGL(k_bricks_from_volume, dim3(count), dim3(256), 0, s, vol, X, Y, Z,
-                       brick_codes, slots, level_c, n, S, inc, pool);
+                       brick_codes, slots, level_c, n, S, inc, stride, pool);
     return hipGetLastError();
 }
 
@@ -798,11 +798,11 @@ constexpr int kCopySplit = 8;
 
 __global__ __launch_bounds__(256) void k_copy_bricks(const float* const* __restrict__ src,
                                                      const int32_t* __restrict__ dst_slots,
-                                                     size_t slot_elems, float* __restrict__ pool)
+                                                     size_t slot_elems, size_t stride, float* __restrict__ pool)
 {
     const int b = blockIdx.y;
     const float* s = src[b];
-    float* d = pool + (size_t)dst_slots[b] * slot_elems;
+    float* d = pool + (size_t)dst_slots[b] * stride;
     const size_t n4 = slot_elems / 4;
     const size_t per = (n4 + kCopySplit - 1) / kCopySplit;
     const size_t beg = per * blockIdx.x, end = beg + per < n4 ? beg + per : n4;
@@ -826,14 +826,15 @@ __global__ __launch_bounds__(256) void k_copy_bricks(const float* const* __restr
 // CUs' worth of wave slots, with enough 16-byte reads in flight for PCIe.
 __global__ __launch_bounds__(256) void k_copy_bricks_narrow(const float* const* __restrict__ src,
                                                             const int32_t* __restrict__ dst_slots, int count,
-                                                            size_t slot_elems, float* __restrict__ pool)
+                                                            size_t slot_elems, size_t stride,
+                                                            float* __restrict__ pool)
 {
     const size_t n4 = slot_elems / 4;
     const size_t per = (n4 + kCopySplit - 1) / kCopySplit;
     for (int item = blockIdx.x; item < count * kCopySplit; item += gridDim.x) {
         const int b = item / kCopySplit, part = item % kCopySplit;
         const float* s = src[b];
-        float* d = pool + (size_t)dst_slots[b] * slot_elems;
+        float* d = pool + (size_t)dst_slots[b] * stride;
         const size_t beg = per * part, end = beg + per < n4 ? beg + per : n4;
         const float4* __restrict__ s4 = reinterpret_cast<const float4*>(s);
         float4* __restrict__ d4 = reinterpret_cast<float4*>(d);
@@ -852,22 +853,22 @@ __global__ __launch_bounds__(256) void k_copy_bricks_narrow(const float* const* 
 }
 
 hipError_t launch_copy_bricks_narrow(const float* const* src, const int32_t* dst_slots, int count, size_t slot_elems,
-                                     float* pool, int workgroups, hipStream_t s)
+                                     size_t stride, float* pool, int workgroups, hipStream_t s)
 {
     if (count <= 0) return hipSuccess;
     if ((int64_t)count * kCopySplit > INT32_MAX) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_copy_bricks_narrow, dim3(workgroups), dim3(256), 0, s, src, dst_slots, count, slot_elems,
-                       pool);
+                       stride, pool);
     return hipGetLastError();
 }
 
 hipError_t launch_copy_bricks(const float* const* src, const int32_t* dst_slots, int count,
-                              size_t slot_elems, float* pool, hipStream_t s)
+                              size_t slot_elems, size_t stride, float* pool, hipStream_t s)
 {
     for (int o = 0; o < count; o += 65535) {   // grid y is at most 65535
         const int m = count - o < 65535 ? count - o : 65535;
         hipLaunchKernelGGL(k_copy_bricks, dim3(kCopySplit, m), dim3(256), 0, s, src + o, dst_slots + o, slot_elems,
-                           pool);
+                           stride, pool);
         if (hipError_t e = hipGetLastError()) return e;
     }
     return hipSuccess;
@@ -892,11 +893,11 @@ hipError_t launch_copy_probe(const float* src, float* dst, size_t bytes, hipStre
 // one workgroup per (slot, sub-block row): min and max of the 5^3 samples
 // of each 4x4x4-cell sub-block of a brick's core
 __global__ __launch_bounds__(64) void k_submm(const float* __restrict__ pool, const int32_t* __restrict__ slots,
-                                              int S, int inc, int G, float2* __restrict__ submm)
+                                              int S, int inc, int G, size_t stride, float2* __restrict__ submm)
 {
     const int b = blockIdx.x;   // brick in the list
     const int slot = slots[b];
-    const float* p = pool + (size_t)slot * S * S * S;
+    const float* p = pool + (size_t)slot * stride;
     float2* out = submm + (size_t)slot * G * G * G;
     for (int sb = threadIdx.x; sb < G * G * G; sb += blockDim.x) {
         int gx = sb / (G * G), gy = (sb / G) % G, gz = sb % G;
@@ -949,11 +950,11 @@ hipError_t launch_sidemap(const float2* submm, const int32_t* slots, int count, 
     return hipGetLastError();
 }
 
-hipError_t launch_submm(const float* pool, const int32_t* slots, int count, int S, int inc, int G, float2* submm,
-                        hipStream_t s)
+hipError_t launch_submm(const float* pool, const int32_t* slots, int count, int S, int inc, int G, size_t stride,
+                        float2* submm, hipStream_t s)
 {
     if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_submm, dim3(count), dim3(64), 0, s, pool, slots, S, inc, G, submm);
+    hipLaunchKernelGGL(k_submm, dim3(count), dim3(64), 0, s, pool, slots, S, inc, G, stride, submm);
     return hipGetLastError();
 }
 

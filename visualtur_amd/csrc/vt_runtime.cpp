@@ -171,16 +171,18 @@ int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t
         if (c.d_src.ensure((size_t)n_load) != hipSuccess) return fail(VT_E_NOMEM, "staging");
         HIP_TRY(hipMemcpyAsync(c.d_src.p, c.h_src.p, (size_t)n_load * sizeof(float*), hipMemcpyHostToDevice,
                                ctx->stream));
-        HIP_TRY(launch_copy_bricks(c.d_src.p, c.d_slots.p, (int)n_load, c.slot_elems, c.d_pool, ctx->stream));
+        HIP_TRY(launch_copy_bricks(c.d_src.p, c.d_slots.p, (int)n_load, c.brick_elems, c.slot_elems, c.d_pool,
+                                   ctx->stream));
     } else {
         if (c.d_codes.ensure((size_t)n_load) != hipSuccess) return fail(VT_E_NOMEM, "staging");
         HIP_TRY(hipMemcpyAsync(c.d_codes.p, c.h_codes.p, (size_t)n_load * 8, hipMemcpyHostToDevice, ctx->stream));
         HIP_TRY(launch_bricks_from_volume(c.d_vol, c.vol_dims[0], c.vol_dims[1], c.vol_dims[2], c.d_codes.p,
-                                          c.d_slots.p, (int)n_load, c.level_c, c.n, c.cube, c.inc, c.d_pool,
+                                          c.d_slots.p, (int)n_load, c.level_c, c.n, c.cube, c.inc, c.slot_elems, c.d_pool,
                                           ctx->stream));
     }
     if (c.d_submm)
-        HIP_TRY(launch_submm(c.d_pool, c.d_slots.p, (int)n_load, c.S, c.inc, c.G, c.d_submm, ctx->stream));
+        HIP_TRY(launch_submm(c.d_pool, c.d_slots.p, (int)n_load, c.S, c.inc, c.G, c.slot_elems, c.d_submm,
+                             ctx->stream));
     if (c.d_side && c.side_valid)
         HIP_TRY(launch_sidemap(c.d_submm, c.d_slots.p, (int)n_load, c.G, c.side_iso, c.d_side, ctx->stream));
     HIP_TRY(launch_scatter_i32(c.d_table, c.d_tidx.p, c.d_tval.p, (int)nt, ctx->stream));
@@ -240,13 +242,15 @@ int load_async(vt_context* ctx, const uint32_t* req, size_t n, std::vector<uint3
     hipStream_t s = ctx->cstream;
     if (c.host) {
         // a narrow copy kernel: the render pass keeps almost every CU
-        HIP_TRY(launch_copy_bricks_narrow(A.src.dev + base, A.slots.dev + base, (int)k, c.slot_elems, c.d_pool,
+        HIP_TRY(launch_copy_bricks_narrow(A.src.dev + base, A.slots.dev + base, (int)k, c.brick_elems, c.slot_elems,
+                                          c.d_pool,
                                           ctx->copy_workgroups, s));
     } else {
         HIP_TRY(launch_bricks_from_volume(c.d_vol, c.vol_dims[0], c.vol_dims[1], c.vol_dims[2], A.codes.dev + base,
-                                          A.slots.dev + base, (int)k, c.level_c, c.n, c.cube, c.inc, c.d_pool, s));
+                                          A.slots.dev + base, (int)k, c.level_c, c.n, c.cube, c.inc, c.slot_elems, c.d_pool, s));
     }
-    if (c.d_submm) HIP_TRY(launch_submm(c.d_pool, A.slots.dev + base, (int)k, c.S, c.inc, c.G, c.d_submm, s));
+    if (c.d_submm)
+        HIP_TRY(launch_submm(c.d_pool, A.slots.dev + base, (int)k, c.S, c.inc, c.G, c.slot_elems, c.d_submm, s));
     if (c.d_side && c.side_valid)
         HIP_TRY(launch_sidemap(c.d_submm, A.slots.dev + base, (int)k, c.G, c.side_iso, c.d_side, s));
     HIP_TRY(launch_scatter_i32(c.d_table, A.tidx.dev + base, A.tval.dev + base, (int)k, s));
@@ -543,7 +547,8 @@ int vt_cache_create(vt_context* ctx, int32_t levelC, int32_t cubeDim, int32_t cu
     c->cube = cubeDim;
     c->inc = cubeInc;
     c->S = (int)S;
-    c->slot_elems = (size_t)(S * S * S);
+    c->brick_elems = (size_t)(S * S * S);
+    c->slot_elems = (c->brick_elems + 63) / 64 * 64;
     c->nbricks = 1ll << (3 * levelC);
     c->gpu_slots = std::min<int64_t>(gpuSlots, c->nbricks);
     c->cpu_slots = cpuSlots;
@@ -617,12 +622,12 @@ int vt_cache_attach_hdf5(vt_context* ctx, const char* file, const char* dataset)
     h->cube = c.cube;
     h->inc = c.inc;
     h->S = c.S;
-    h->slot_elems = c.slot_elems;
+    h->slot_elems = c.brick_elems;   // the host tier stores bricks back to back
     h->nbricks = c.nbricks;
     h->capacity = std::min<int64_t>(c.cpu_slots, c.nbricks);
     h->file = file;
     h->dataset = dataset;
-    size_t bytes = (size_t)h->capacity * c.slot_elems * sizeof(float);
+    size_t bytes = (size_t)h->capacity * c.brick_elems * sizeof(float);
     if (hipHostMalloc((void**)&h->pool, bytes, hipHostMallocMapped) != hipSuccess)
         return fail(VT_E_NOMEM, "pinned host tier allocation failed (" + std::to_string(bytes) + " bytes)");
     HIP_TRY(hipHostGetDevicePointer((void**)&h->pool_dev, h->pool, 0));

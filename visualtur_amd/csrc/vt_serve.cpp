@@ -259,9 +259,10 @@ int vt_server_add_scene(vt_server* sv, const float* d_vol, int32_t X, int32_t Y,
     for (int64_t o = 0; o < nb; o += 65536) {
         int cnt = (int)std::min<int64_t>(65536, nb - o);
         HIP_TRY(launch_bricks_from_volume(d_vol, X, Y, Z, dcodes.p + o, sc->d_slot + o, cnt, levelC, n, cube, cubeInc,
+                                          sc->slot_elems,
                                           sc->d_pool, s));
         if (sc->d_submm)
-            HIP_TRY(launch_submm(sc->d_pool, sc->d_slot + o, cnt, sc->S, cubeInc, sc->G, sc->d_submm, s));
+            HIP_TRY(launch_submm(sc->d_pool, sc->d_slot + o, cnt, sc->S, cubeInc, sc->G, sc->slot_elems, sc->d_submm, s));
     }
     HIP_TRY(hipStreamSynchronize(s));
     *scene_out = (int32_t)sv->scenes.size();
