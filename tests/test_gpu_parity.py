@@ -276,6 +276,7 @@ def test_overlapped_loads_are_exact(vt, vto, ctx, slots, reserve, defer):
     sc = Scene(vto, vol, 0.3, 4, 3)
     sc.setup_gpu(vt, ctx, slots=slots)
     cams = [orbit_camera(vt, 96, 80, (32, 32, 32), 110.0, 40.0 + 9.0 * k, 25.0) for k in range(5)]
+    refs = [sc.render_cpu(vto, cam) for cam in cams]
     outs = {}
     try:
         ctx.set_march_options(skip_empty=True, max_defer=defer)
@@ -284,9 +285,11 @@ def test_overlapped_loads_are_exact(vt, vto, ctx, slots, reserve, defer):
             ctx.set_load_options(overlap_loads=bool(overlap), reserve_slots=reserve)
             ctx.cache_clear()
             frames, moved = [], 0
-            for cam in cams:
+            for f, cam in enumerate(cams):
                 rg, hg, st = sc.render_gpu(vt, ctx, cam)
                 assert st.err == 0
+                log(f"frame {f}: passes {st.passes} uploads {st.bricks_uploaded} overlapped {st.bricks_overlapped}")
+                assert_same(hg, rg, refs[f][1], refs[f][0])
                 frames.append((hg.tobytes(), rg.tobytes()))
                 moved += st.bricks_overlapped
                 assert ctx.cache_stats()["gpu_resident"] <= slots
@@ -298,11 +301,14 @@ def test_overlapped_loads_are_exact(vt, vto, ctx, slots, reserve, defer):
     finally:
         ctx.set_march_options()
         ctx.set_load_options()
-    assert outs[0] == outs[1]
-    rc, hc = sc.render_cpu(vto, cams[-1])
-    rg = np.frombuffer(outs[1][-1][1], np.float32).reshape(-1, 4)
-    hg = np.frombuffer(outs[1][-1][0], vt.HIT_DTYPE)
-    assert_same(hg, rg, hc, rc)
+    for f in range(len(cams)):
+        h0, h1 = (np.frombuffer(outs[m][f][0], vt.HIT_DTYPE) for m in (0, 1))
+        r0, r1 = (np.frombuffer(outs[m][f][1], np.uint32).reshape(-1, 4) for m in (0, 1))
+        bad = np.nonzero(np.any(h0.view(np.uint8).reshape(h0.size, -1) != h1.view(np.uint8).reshape(h1.size, -1), 1) |
+                         np.any(r0 != r1, 1))[0]
+        for i in bad[:4]:
+            log(f"frame {f} ray {i}: between passes {h0[i]} {r0[i]}; overlapped {h1[i]} {r1[i]}")
+        assert bad.size == 0, (f, bad.size)
 
 
 @pytest.mark.parametrize("slots,defer", [(512, 2), (5, 0), (17, 2), (40, 4)])

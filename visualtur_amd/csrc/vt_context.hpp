@@ -312,12 +312,6 @@ struct BrickCache {
     uint8_t* d_ref = nullptr;
     float2* d_submm = nullptr;   // G^3 sub-block (min, max) per slot, null when bricks are not 4k wide
     int G = 0;
-    // per slot: 2-bit proven side of each sub-block for side_iso (Morton
-    // order, 64 per uint4); null unless G is a power of two
-    uint4* d_side = nullptr;
-    int side_regions = 0;
-    float side_iso = 0.0f;
-    bool side_valid = false;
     std::vector<int32_t> table;
     std::vector<int64_t> slot_brick;
     std::vector<uint8_t> ref, ref_tmp;
@@ -354,7 +348,6 @@ struct BrickCache {
         if (d_table) (void)hipFree(d_table);
         if (d_ref) (void)hipFree(d_ref);
         if (d_submm) (void)hipFree(d_submm);
-        if (d_side) (void)hipFree(d_side);
         h_src.release(); h_slots.release(); h_codes.release(); h_tidx.release(); h_tval.release();
         d_src.release(); d_slots.release(); d_codes.release(); d_tidx.release(); d_tval.release();
     }

@@ -404,7 +404,6 @@ struct SubGrid {
     const float2* mm;   // G^3 (min, max) of this brick, x-major, or null
     int G;              // sub-blocks per axis (cube / 4)
     int cx0, cy0, cz0;  // global voxel of the brick's first core cell
-    const uint4* side = nullptr;   // this brick's side map (replaces mm when set)
 };
 
 // Side of iso that a sub-block's (min, max) proves for every trilinear value
@@ -425,38 +424,20 @@ __device__ __forceinline__ int side_from_minmax(float lo, float hi, float iso)
 struct SideCache {
     float x0 = 1.0f, y0 = 1.0f, z0 = 1.0f, x1 = 0.0f, y1 = 0.0f, z1 = 0.0f;   // empty
     int side = SIDE_NONE;
-    int region = -1;   // side-map region held in `bits`
-    uint4 bits;
 };
 
 __device__ __forceinline__ int proven_side(const SubGrid& g, float iso, float px, float py, float pz, SideCache& sc)
 {
     if (px >= sc.x0 && px < sc.x1 && py >= sc.y0 && py < sc.y1 && pz >= sc.z0 && pz < sc.z1) return sc.side;
-    if ((!g.mm && !g.side) || !(px >= 0.0f) || !(py >= 0.0f) || !(pz >= 0.0f) || !coord_ok(px) || !coord_ok(py) ||
-        !coord_ok(pz))
+    if (!g.mm || !(px >= 0.0f) || !(py >= 0.0f) || !(pz >= 0.0f) || !coord_ok(px) || !coord_ok(py) || !coord_ok(pz))
         return SIDE_NONE;
     int cx = (int)px - g.cx0, cy = (int)py - g.cy0, cz = (int)pz - g.cz0;
     int lim = 4 * g.G;
     if ((unsigned)cx >= (unsigned)lim || (unsigned)cy >= (unsigned)lim || (unsigned)cz >= (unsigned)lim)
         return SIDE_NONE;
-    int s;
-    if (g.side) {
-        // Morton index of the sub-block; 64 of them (2 bits each) per uint4
-        uint32_t mi = (dilate3((uint32_t)(cx >> 2)) << 2) | (dilate3((uint32_t)(cy >> 2)) << 1) |
-                      dilate3((uint32_t)(cz >> 2));
-        int region = (int)(mi >> 6);
-        if (region != sc.region) {
-            sc.bits = g.side[region];
-            sc.region = region;
-        }
-        uint32_t e = mi & 63u, q = e >> 4;
-        uint32_t word = q == 0 ? sc.bits.x : (q == 1 ? sc.bits.y : (q == 2 ? sc.bits.z : sc.bits.w));
-        s = (int)((word >> ((e & 15u) * 2u)) & 3u);
-    } else {
-        int sb = ((cx >> 2) * g.G + (cy >> 2)) * g.G + (cz >> 2);
-        float2 mm = g.mm[sb];
-        s = side_from_minmax(mm.x, mm.y, iso);
-    }
+    int sb = ((cx >> 2) * g.G + (cy >> 2)) * g.G + (cz >> 2);
+    float2 mm = g.mm[sb];
+    int s = side_from_minmax(mm.x, mm.y, iso);
     sc.x0 = (float)(g.cx0 + (cx & ~3));
     sc.y0 = (float)(g.cy0 + (cy & ~3));
     sc.z0 = (float)(g.cz0 + (cz & ~3));

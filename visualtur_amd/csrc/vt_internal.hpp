@@ -26,8 +26,6 @@ struct BrickView {
     uint64_t brick_base;         // 8^levelC
     const float2* submm;         // per slot G^3 sub-block (min, max), or null
     int G;                       // sub-blocks per axis (cubeDim / 4)
-    const uint4* side = nullptr; // per slot: 2-bit proven sides for the frame's iso, or null (use submm)
-    int side_regions = 0;        // uint4 regions per slot (64 sub-blocks each)
 };
 
 struct Shading {
@@ -80,8 +78,6 @@ struct RenderArgs {
     uint8_t* brick_trace;        // optional, 1 byte per brick
     const float2* submm;         // sub-block min/max per slot, or null (no skipping)
     int G;
-    const uint4* side;           // per-slot side maps for this frame's iso, or null
-    int side_regions;
     FrameCounters* ctr;
     float4* rgba;
     HitRecord* hits;
@@ -179,8 +175,6 @@ hipError_t launch_bricks_from_volume(const float* vol, int X, int Y, int Z,
                                      int level_c, int n, int cube_dim, int inc, size_t stride,
                                      float* pool, hipStream_t s);   // slot b at pool + b * stride
 // (min, max) of every 4x4x4-cell sub-block (5^3 samples) of the given slots
-hipError_t launch_sidemap(const float2* submm, const int32_t* slots, int count, int G, float iso, uint4* side,
-                          hipStream_t s);   // slots == null: slots 0..count-1
 hipError_t launch_submm(const float* pool, const int32_t* slots, int count, int S, int inc, int G, size_t stride,
                         float2* submm, hipStream_t s);
 // Copy bricks of slot_elems floats from (device-visible) host memory into

@@ -61,8 +61,7 @@ __device__ __forceinline__ bool visit_node(const Ray& r, const OctreeView& oct, 
     nnodes += 1;
     bool hit;
     if (bv.submm) {
-        SubGrid g{bv.submm + (size_t)slot * bv.G * bv.G * bv.G, bv.G, bx + bv.inc, by + bv.inc, bz + bv.inc,
-                  bv.side ? bv.side + (size_t)slot * bv.side_regions : nullptr};
+        SubGrid g{bv.submm + (size_t)slot * bv.G * bv.G * bv.G, bv.G, bx + bv.inc, by + bv.inc, bz + bv.inc};
         hit = march_node_skip(r, nx, ny, nz, side, sp, bv.S, bx, by, bz, g, sh.iso, Xx, Xy, Xz, err, nsamples);
     } else {
         hit = march_node(r, nx, ny, nz, side, sp, bv.S, bx, by, bz, sh.iso, Xx, Xy, Xz, err, nsamples);
@@ -184,8 +183,7 @@ __global__ __launch_bounds__(kBlock) void k_render_pass(RenderArgs a)
         int i = (int)(id / (uint32_t)W), j = (int)(id % (uint32_t)W);
         Ray r;
         make_ray(a.cam, i, j, r);
-        BrickView bv{a.brick_slot, a.pool, a.S, a.inc, a.slot_elems, a.level_c, a.shift, a.brick_base, a.submm, a.G,
-                 a.side, a.side_regions};
+        BrickView bv{a.brick_slot, a.pool, a.S, a.inc, a.slot_elems, a.level_c, a.shift, a.brick_base, a.submm, a.G};
         Shading sh{a.iso, a.lx, a.ly, a.lz, a.color_scale};
         float4 c;
         HitRecord h;
@@ -303,8 +301,7 @@ __global__ __launch_bounds__(kBlock) void k_render_persistent(RenderArgs a)
     const uint32_t unit = first ? 256u : 1u;
     const int lane = (int)__lane_id();
     const unsigned long long below = (1ull << lane) - 1ull;
-    BrickView bv{a.brick_slot, a.pool, a.S, a.inc, a.slot_elems, a.level_c, a.shift, a.brick_base, a.submm, a.G,
-                 a.side, a.side_regions};
+    BrickView bv{a.brick_slot, a.pool, a.S, a.inc, a.slot_elems, a.level_c, a.shift, a.brick_base, a.submm, a.G};
     Shading sh{a.iso, a.lx, a.ly, a.lz, a.color_scale};
 
     int part = (int)(blockIdx.x & 7u);   // wave-uniform
@@ -914,40 +911,6 @@ __global__ __launch_bounds__(64) void k_submm(const float* __restrict__ pool, co
         // a NaN sample makes every comparison false, so nothing is skipped
         out[sb] = nan ? make_float2(__builtin_nanf(""), __builtin_nanf("")) : make_float2(lo, hi);
     }
-}
-
-// One thread per (slot, region of 64 Morton-ordered sub-blocks): the 2-bit
-// proven side (0 none, 1 above, 2 below) of each sub-block for `iso`.
-__global__ void k_sidemap(const float2* __restrict__ submm, const int32_t* __restrict__ slots, int count, int G,
-                          int regions, float iso, uint4* __restrict__ side)
-{
-    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t >= (int64_t)count * regions) return;
-    const int b = (int)(t / regions), region = (int)(t % regions);
-    const int slot = slots ? slots[b] : b;
-    const float2* mm = submm + (size_t)slot * G * G * G;
-    const uint32_t n3 = (uint32_t)(G * G * G);
-    uint32_t w[4] = {0u, 0u, 0u, 0u};
-#pragma unroll 4
-    for (uint32_t e = 0; e < 64; ++e) {
-        uint32_t mi = (uint32_t)region * 64u + e;
-        if (mi >= n3) break;
-        int gx = (int)compact3(mi >> 2), gy = (int)compact3(mi >> 1), gz = (int)compact3(mi);
-        float2 v = mm[(gx * G + gy) * G + gz];
-        w[e >> 4] |= (uint32_t)side_from_minmax(v.x, v.y, iso) << ((e & 15u) * 2u);
-    }
-    side[(size_t)slot * regions + region] = make_uint4(w[0], w[1], w[2], w[3]);
-}
-
-hipError_t launch_sidemap(const float2* submm, const int32_t* slots, int count, int G, float iso, uint4* side,
-                          hipStream_t s)
-{
-    if (count <= 0) return hipSuccess;
-    const int regions = (G * G * G + 63) / 64;
-    const int64_t n = (int64_t)count * regions;
-    hipLaunchKernelGGL(k_sidemap, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, submm, slots, count, G, regions,
-                       iso, side);
-    return hipGetLastError();
 }
 
 hipError_t launch_submm(const float* pool, const int32_t* slots, int count, int S, int inc, int G, size_t stride,

@@ -183,8 +183,6 @@ int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t
     if (c.d_submm)
         HIP_TRY(launch_submm(c.d_pool, c.d_slots.p, (int)n_load, c.S, c.inc, c.G, c.slot_elems, c.d_submm,
                              ctx->stream));
-    if (c.d_side && c.side_valid)
-        HIP_TRY(launch_sidemap(c.d_submm, c.d_slots.p, (int)n_load, c.G, c.side_iso, c.d_side, ctx->stream));
     HIP_TRY(launch_scatter_i32(c.d_table, c.d_tidx.p, c.d_tval.p, (int)nt, ctx->stream));
     // the staging buffers are reused by the next call: finish the copies
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -251,8 +249,6 @@ int load_async(vt_context* ctx, const uint32_t* req, size_t n, std::vector<uint3
     }
     if (c.d_submm)
         HIP_TRY(launch_submm(c.d_pool, A.slots.dev + base, (int)k, c.S, c.inc, c.G, c.slot_elems, c.d_submm, s));
-    if (c.d_side && c.side_valid)
-        HIP_TRY(launch_sidemap(c.d_submm, A.slots.dev + base, (int)k, c.G, c.side_iso, c.d_side, s));
     HIP_TRY(launch_scatter_i32(c.d_table, A.tidx.dev + base, A.tval.dev + base, (int)k, s));
     A.used += k;
     c.stat_uploads += k;
@@ -563,11 +559,6 @@ int vt_cache_create(vt_context* ctx, int32_t levelC, int32_t cubeDim, int32_t cu
         c->G = cubeDim / 4;
         size_t mm = (size_t)c->gpu_slots * c->G * c->G * c->G * sizeof(float2);
         if (hipMalloc((void**)&c->d_submm, mm) != hipSuccess) return fail(VT_E_NOMEM, "sub-block min/max");
-        if ((c->G & (c->G - 1)) == 0) {   // Morton-ordered side maps need a power-of-two grid
-            c->side_regions = (c->G * c->G * c->G + 63) / 64;
-            size_t sb = (size_t)c->gpu_slots * c->side_regions * sizeof(uint4);
-            if (hipMalloc((void**)&c->d_side, sb) != hipSuccess) return fail(VT_E_NOMEM, "sub-block side maps");
-        }
     }
     c->table.assign((size_t)c->nbricks, -1);
     c->slot_brick.assign((size_t)c->gpu_slots, -1);
@@ -814,23 +805,6 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
     a.brick_trace = ctx->record_trace ? ctx->trace.p : nullptr;
     a.submm = ctx->skip_empty ? c.d_submm : nullptr;
     a.G = c.G;
-    if (ctx->skip_empty && c.d_side) {
-        uint32_t want, have;
-        memcpy(&want, &p->iso, 4);
-        memcpy(&have, &c.side_iso, 4);
-        if (!c.side_valid || want != have) {
-            // loads still in flight compute maps for the old iso: let them land first
-            if (int rc = copy_stream_idle(ctx)) return rc;
-            c.side_iso = p->iso;
-            c.side_valid = true;
-            HIP_TRY(launch_sidemap(c.d_submm, nullptr, (int)c.used, c.G, c.side_iso, c.d_side, ctx->stream));
-        }
-        a.side = c.d_side;
-        a.side_regions = c.side_regions;
-    } else {
-        a.side = nullptr;
-        a.side_regions = 0;
-    }
     a.ctr = ctx->d_ctr;
     a.rgba = reinterpret_cast<float4*>(d_rgba);
     a.hits = hits;
