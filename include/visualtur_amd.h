@@ -307,17 +307,21 @@ int vt_mlp_gradients(vt_mlp *m, const float *d_x, const float *d_y, int64_t N, f
 int vt_mlp_train_step(vt_mlp *m, const float *d_x, const float *d_y, int64_t N, float lr, float pos_weight,
                       float *loss_out);
 /* features of (camera, brick) pairs: camera position and brick centre in
- * cube units minus 0.5, view direction, levelC/10, distance / cube side,
- * cosine of the angle between the view direction and the brick. */
+ * cube units minus 0.5, view direction, levelC/10, and the brick centre's
+ * image-plane position (u, v) = 2*distance*(d.right / w, d.up / h) / (d.look)
+ * with d = centre - camera (inside the frame for |u|, |v| <= 1), clamped to
+ * [-4, 4], and 4 for a centre behind the camera. */
 int vt_mlp_features(vt_mlp *m, const vt_camera *cam, int32_t nLevels, int32_t levelC, const uint64_t *d_codes,
                     int64_t N, float *d_x);
 
 /* Score every candidate brick (the bricks holding octree nodes, i.e. the
  * level-levelC run list) for the camera with the MLP, then load the
  * highest-scoring non-resident ones with probability >= min_prob, at most
- * `budget`, into free slots or in place of resident bricks that score
- * lower.  Hits are never affected: the render still decides every brick
- * it marches.  *loaded (may be NULL) receives the number of bricks loaded. */
+ * `budget`, into free slots or in place of resident bricks that score lower
+ * and below probability 1 - min_prob.  The copies are issued on the
+ * context's copy stream without waiting, so they overlap the next render.
+ * Hits are never affected: the render still decides every brick it marches.
+ * *loaded (may be NULL) receives the number of bricks loaded. */
 int vt_prefetch(vt_context *ctx, vt_mlp *m, const vt_camera *cam, int64_t budget, float min_prob, int64_t *loaded);
 
 /* --------------------------------------------------------- serving loop */

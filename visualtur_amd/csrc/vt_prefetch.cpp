@@ -314,8 +314,11 @@ extern "C" int vt_prefetch(vt_context* ctx, vt_mlp* m, const vt_camera* cam, int
             int64_t b = (int64_t)(ctx->cand_codes[(size_t)i] - base);
             if (c.table[(size_t)b] < 0 && score[(size_t)i] >= zmin) want.push_back(i);
         }
-        std::sort(want.begin(), want.end(), [&](int64_t a, int64_t b) { return score[(size_t)a] > score[(size_t)b]; });
+        // only the best `n` are loaded: order just those (ties by candidate order)
         int64_t n = std::min<int64_t>((int64_t)want.size(), budget);
+        std::partial_sort(want.begin(), want.begin() + n, want.end(), [&](int64_t a, int64_t b) {
+            return score[(size_t)a] > score[(size_t)b] || (score[(size_t)a] == score[(size_t)b] && a < b);
+        });
         if (n > c.available()) {
             // make room: evict resident bricks that score below the candidate
             // they make room for (bricks that are not candidates score -inf)
@@ -328,13 +331,19 @@ extern "C" int vt_prefetch(vt_context* ctx, vt_mlp* m, const vt_camera* cam, int
             std::vector<int64_t> res;
             for (int64_t sl = 0; sl < c.used; ++sl)
                 if (c.slot_brick[(size_t)sl] >= 0) res.push_back(sl);
-            std::sort(res.begin(), res.end(),
-                      [&](int64_t x, int64_t y) { return slot_score[(size_t)x] < slot_score[(size_t)y]; });
+            const size_t need = std::min(res.size(), (size_t)(n - c.available()));
+            std::partial_sort(res.begin(), res.begin() + need, res.end(), [&](int64_t x, int64_t y) {
+                return slot_score[(size_t)x] < slot_score[(size_t)y] ||
+                       (slot_score[(size_t)x] == slot_score[(size_t)y] && x < y);
+            });
             std::vector<int64_t> idx;
             std::vector<int32_t> val;
             for (size_t j = 0; j < res.size() && c.available() < n; ++j) {
                 const int64_t v = res[j];
-                if (!(score[(size_t)want[(size_t)c.available()]] > slot_score[(size_t)v])) break;
+                // a resident goes only for a better candidate, and only while
+                // its own probability is below 1 - min_prob
+                if (!(score[(size_t)want[(size_t)c.available()]] > slot_score[(size_t)v]) || !(slot_score[(size_t)v] < -zmin))
+                    break;
                 const int64_t old = c.slot_brick[(size_t)v];
                 c.table[(size_t)old] = -1;
                 c.slot_brick[(size_t)v] = -1;
