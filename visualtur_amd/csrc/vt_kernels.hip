@@ -161,12 +161,12 @@ __global__ __launch_bounds__(kBlock) void k_render_pass(RenderArgs a)
         active = idx < a.in_count;
         id = active ? a.in_list[idx] : 0;
     } else {
-        // XCD-aware tile order: consecutive workgroups are dealt round-robin
-        // to the 8 XCDs, so give each XCD a contiguous run of tiles (shared
-        // bricks stay in that XCD's L2).
-        int b = blockIdx.x;
-        int per_xcd = (a.tiles_total + 7) / 8;
-        int t = (b % 8) * per_xcd + (b / 8);
+        // Workgroup b renders tile b.  Workgroups are dealt round-robin to the
+        // 8 XCDs, so neighbouring tiles land on different XCDs: the expensive
+        // rays, which cluster in the image, spread over all CUs.  (Giving each
+        // XCD a contiguous band of tiles, for L2 reuse, measured 8% slower on
+        // the 1024^3 workload: the bands holding the surface became the tail.)
+        const int t = (int)blockIdx.x;
         active = t < a.tiles_total;
         int ty = active ? t / a.tiles_x : 0, tx = active ? t % a.tiles_x : 0;
         int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -502,8 +502,7 @@ hipError_t launch_render_pass(const RenderArgs& a, hipStream_t s)
         if (a.in_count == 0) return hipSuccess;
         grid = (a.in_count + kBlock - 1) / kBlock;
     } else {
-        int per_xcd = (a.tiles_total + 7) / 8;
-        grid = (unsigned)(per_xcd * 8);
+        grid = (unsigned)a.tiles_total;
     }
     hipLaunchKernelGGL(k_render_pass, dim3(grid), dim3(kBlock), 0, s, a);
     return hipGetLastError();
