@@ -150,28 +150,33 @@ __device__ __forceinline__ int walk_defer(const Ray& r, const RenderArgs& a, con
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_render_pass(RenderArgs a)
+// Render-pass workgroup: kPassBlock/64 waves, each rendering one 8x8 pixel
+// tile (the tiles of a workgroup are consecutive in row-major tile order).
+constexpr int kPassBlock = 64;
+
+__global__ __launch_bounds__(kPassBlock) void k_render_pass(RenderArgs a)
 {
     const int W = a.cam.W, H = a.cam.H;
     const uint32_t R = (uint32_t)W * (uint32_t)H;
     uint32_t id;
     bool active = true;
     if (a.in_list) {
-        uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
+        uint32_t idx = blockIdx.x * kPassBlock + threadIdx.x;
         active = idx < a.in_count;
         id = active ? a.in_list[idx] : 0;
     } else {
-        // Workgroup b renders tile b.  Workgroups are dealt round-robin to the
-        // 8 XCDs, so neighbouring tiles land on different XCDs: the expensive
-        // rays, which cluster in the image, spread over all CUs.  (Giving each
-        // XCD a contiguous band of tiles, for L2 reuse, measured 8% slower on
-        // the 1024^3 workload: the bands holding the surface became the tail.)
-        const int t = (int)blockIdx.x;
-        active = t < a.tiles_total;
-        int ty = active ? t / a.tiles_x : 0, tx = active ? t % a.tiles_x : 0;
-        int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        int j = tx * 16 + (w & 1) * 8 + (lane & 7);
-        int i = ty * 16 + (w >> 1) * 8 + (lane >> 3);
+        // Workgroups take tiles in order and are dealt round-robin to the 8
+        // XCDs, so neighbouring tiles land on different XCDs and the
+        // expensive rays, which cluster in the image, spread over all CUs.
+        // (Giving each XCD a contiguous band of tiles, for L2 reuse, measured
+        // 8% slower on the 1024^3 workload: the bands holding the surface
+        // became the tail.)
+        const int tx8 = (W + 7) >> 3, ty8 = (H + 7) >> 3;
+        const int t = (int)blockIdx.x * (kPassBlock / 64) + (int)(threadIdx.x >> 6);
+        const int lane = threadIdx.x & 63;
+        active = t < tx8 * ty8;
+        const int i = (active ? t / tx8 : 0) * 8 + (lane >> 3);
+        const int j = (active ? t % tx8 : 0) * 8 + (lane & 7);
         active = active && i < H && j < W;
         id = active ? (uint32_t)(i * W + j) : 0;
     }
@@ -500,11 +505,13 @@ hipError_t launch_render_pass(const RenderArgs& a, hipStream_t s)
     unsigned grid;
     if (a.in_list) {
         if (a.in_count == 0) return hipSuccess;
-        grid = (a.in_count + kBlock - 1) / kBlock;
+        grid = (a.in_count + kPassBlock - 1) / kPassBlock;
     } else {
-        grid = (unsigned)a.tiles_total;
+        const unsigned tiles = (unsigned)((a.cam.W + 7) / 8) * (unsigned)((a.cam.H + 7) / 8);
+        const unsigned per = kPassBlock / 64;
+        grid = (tiles + per - 1) / per;
     }
-    hipLaunchKernelGGL(k_render_pass, dim3(grid), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_render_pass, dim3(grid), dim3(kPassBlock), 0, s, a);
     return hipGetLastError();
 }
 
