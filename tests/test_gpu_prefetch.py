@@ -24,15 +24,17 @@ def test_prefetch_cuts_misses_and_keeps_hits(vt, vto, ctx):
     ctx.cache_attach_device_volume(d_vol, vol.shape)
     prm = vt.render_params(level_o, level_c, iso)
     W = H = 128
-    train_cams = [c for s in range(24) for c in pf.random_walk(W, H, 128, 8, seed=1000 + s)]
+    radius = (0.6, 1.0)   # close views: a frame sees under half of the bricks
+    train_cams = [c for s in range(24) for c in pf.random_walk(W, H, 128, 8, seed=1000 + s, radius=radius)]
     traces = pf.collect_traces(ctx, train_cams, prm, level_c)
     net = vt.MLP(ctx, seed=7, max_batch=8192)
     X, Y = pf.build_dataset(ctx, net, train_cams, traces, cand, n, level_c, neg_per_pos=3, seed=1)
     losses, pos_w = pf.train(ctx, net, X, Y, steps=400, batch=4096, lr=1e-3)
-    test_cams = pf.random_walk(W, H, 128, 48, seed=7)
+    test_cams = pf.random_walk(W, H, 128, 48, seed=7, radius=radius)
     work = int(np.median([tr.size for tr in pf.collect_traces(ctx, test_cams, prm, level_c)]))
-    # a device cache that holds a typical frame's working set, not the volume
-    slots = min(int(cand.size) - 1, work + 8)
+    # a device cache with room for a typical frame's working set and the
+    # bricks about to enter the view, but not for the volume
+    slots = min(int(cand.size) - 1, int(1.6 * work) + 8)
     print(f"candidates {cand.size} working set {work} slots {slots} samples {X.shape[0]} "
           f"loss {losses[:10].mean():.4f} -> {losses[-10:].mean():.4f}", flush=True)
     assert losses[-20:].mean() < 0.7 * losses[:20].mean()
@@ -40,7 +42,7 @@ def test_prefetch_cuts_misses_and_keeps_hits(vt, vto, ctx):
     ctx.cache_attach_device_volume(d_vol, vol.shape)
     plain, (rgba_p, hits_p) = pf.run_path(ctx, test_cams, prm)
     ctx.cache_clear()
-    pref, (rgba_q, hits_q) = pf.run_path(ctx, test_cams, prm, net, budget=slots, min_prob=0.5)
+    pref, (rgba_q, hits_q) = pf.run_path(ctx, test_cams, prm, net, budget=slots, min_prob=0.8)
     # identical pictures and hit records
     assert hits_p.tobytes() == hits_q.tobytes()
     assert np.array_equal(rgba_p.view(np.uint32), rgba_q.view(np.uint32))
@@ -54,6 +56,6 @@ def test_prefetch_cuts_misses_and_keeps_hits(vt, vto, ctx):
     p_pref = np.mean([r["passes"] for r in pref[1:]])
     print(f"demand misses plain {m_plain} prefetch {m_pref}; passes {p_plain:.2f} -> {p_pref:.2f}; "
           f"prefetched {sum(r['prefetched'] for r in pref)}", flush=True)
-    assert m_pref < 0.85 * m_plain
+    assert m_pref < 0.7 * m_plain
     net.close()
     d_vol.free()
