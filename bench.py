@@ -162,7 +162,7 @@ def prefetch_leg(ctx, w, cams, args, d_rgba, d_hits, plain_ms, plain_uploads, U,
              for c in pf.random_walk(w.W // 4, w.H // 4, 1 << w.n, args.train_frames, seed=9000 + s,
                                      radius=(1.9, 2.6))]
     traces = pf.collect_traces(ctx, tcams, w.prm, w.level_c)
-    net = vt.MLP(ctx, seed=11, max_batch=16384)
+    net = vt.MLP(ctx, seed=11, max_batch=65536)
     X, Y = pf.build_dataset(ctx, net, tcams, traces, cand, w.n, w.level_c, neg_per_pos=3, seed=2)
     losses, pos_w = pf.train(ctx, net, X, Y, steps=args.train_steps, batch=8192, lr=1e-3)
     train_s = time.perf_counter() - t0
@@ -301,7 +301,7 @@ def run_c3(ctx, args):
     tcams = [c for s in range(args.c3_train_paths)
              for c in pf.random_walk(480, 270, D, args.c3_train_frames, seed=30000 + s, radius=radius)]
     traces = pf.collect_traces(ctx, tcams, prm, level_c)
-    net = vt.MLP(ctx, seed=13, max_batch=16384)
+    net = vt.MLP(ctx, seed=13, max_batch=65536)
     X, Y = pf.build_dataset(ctx, net, tcams, traces, cand, n, level_c, neg_per_pos=3, seed=3, max_pos=1024)
     losses, _ = pf.train(ctx, net, X, Y, steps=args.c3_train_steps, batch=8192, lr=1e-3)
     train_s = time.perf_counter() - t0

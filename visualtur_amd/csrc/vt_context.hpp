@@ -112,6 +112,7 @@ struct HostTier {
     std::condition_variable cv;
     int slabs_done = 0;           // x-slabs of bricks fully loaded
     int64_t filled = 0;           // bricks the reader has placed
+    std::atomic<bool> complete{false};   // every slab loaded: lookups need no lock
     bool reader_failed = false;
     std::string reader_err;
     int64_t file_reads = 0;
@@ -176,6 +177,7 @@ struct HostTier {
                 slabs_done = (int)bx + 1;
                 file_reads += placed;
                 filled += placed;
+                if (bx + 1 == B) complete.store(true, std::memory_order_release);
             }
             cv.notify_all();
         }
@@ -212,6 +214,11 @@ struct HostTier {
     // device-visible pointer of brick `bidx`, reading it if needed.
     int get(int64_t bidx, uint64_t epoch, const float** out, int64_t* misses)
     {
+        if (full_mode && complete.load(std::memory_order_acquire)) {
+            if (slot_of[bidx] < 0) return fail(VT_E_ARG, "requested brick holds no octree node");
+            *out = pool_dev + (size_t)slot_of[bidx] * slot_elems;
+            return VT_OK;
+        }
         if (full_mode) {
             int32_t code_bx;
             {

@@ -30,11 +30,12 @@ struct vt_mlp {
     DevBuf<float> z, dz, y, loss;
     DevBuf<float> feats;
     DevBuf<uint64_t> codes;
+    PinBuf<float> h_score;        // prefetch scores of the candidates
     ~vt_mlp()
     {
         P.release(); G.release(); Mo.release(); Vo.release(); Pb.release();
         X.release(); H1.release(); H2.release(); H3.release(); dH1.release(); dH2.release(); dH3.release();
-        z.release(); dz.release(); y.release(); loss.release(); feats.release(); codes.release();
+        z.release(); dz.release(); y.release(); loss.release(); feats.release(); codes.release(); h_score.release();
     }
 };
 
@@ -299,12 +300,13 @@ extern "C" int vt_prefetch(vt_context* ctx, vt_mlp* m, const vt_camera* cam, int
         // score every candidate (chunks of the MLP batch size)
         Camera k;
         memcpy(&k, cam, sizeof k);
-        std::vector<float> score((size_t)NC);
+        if (m->h_score.ensure((size_t)NC) != hipSuccess) return fail(VT_E_NOMEM, "score staging");
+        float* score = m->h_score.p;   // pinned: the copies below are true DMA
         for (int64_t o = 0; o < NC; o += m->max_batch) {
             int nb = (int)std::min<int64_t>(m->max_batch, NC - o);
             HIP_TRY(launch_features(k, ctx->n, c.level_c, ctx->d_cand.p + o, nb, m->feats.p, s));
             if (int rc = forward(m, m->feats.p, nb)) return rc;
-            HIP_TRY(hipMemcpyAsync(score.data() + o, m->z.p, (size_t)nb * 4, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(score + o, m->z.p, (size_t)nb * 4, hipMemcpyDeviceToHost, s));
         }
         HIP_TRY(hipStreamSynchronize(s));
         const uint64_t base = 1ull << (3 * c.level_c);
