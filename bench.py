@@ -326,8 +326,6 @@ def run_c3(ctx, args):
         ctx.cache_create(level_c, 32, inc, gpu_slots, cpu_slots)
         ctx.cache_attach_hdf5(h5, "value")
         ctx.set_march_options(skip_empty=True, max_defer=args.max_defer, persistent=bool(args.persistent))
-        ctx.set_load_options(overlap_loads=bool(args.overlap), reserve_slots=args.reserve,
-                             copy_workgroups=args.copy_wg)
         t0 = time.perf_counter()
         ctx.cache_prefetch_host(wait=True)
         log(f"c3 {form}: host tier filled from hdf5 in {time.perf_counter() - t0:.2f}s, "
@@ -478,9 +476,6 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--parity-rays", type=int, default=4096)
     ap.add_argument("--prefetch", type=int, default=1, help="c2: also time the learned-prefetch form")
-    ap.add_argument("--overlap", type=int, default=0, help="load bricks while the requesting pass runs")
-    ap.add_argument("--copy-wg", type=int, default=0, help="workgroups of the overlapped copies (0: 16)")
-    ap.add_argument("--reserve", type=int, default=-1, help="free device slots kept between frames (-1: 1/8)")
     ap.add_argument("--persistent", type=int, default=0, help="persistent render pass with lane refill")
     ap.add_argument("--max-defer", type=int, default=2,
                     help="absent-brick nodes a ray may walk past per pass (0 = park at the first)")
@@ -525,8 +520,6 @@ def main() -> int:
         b_.free()
     w = setup_c2(ctx, args) if args.config == "c2" else setup_c1(ctx, args)
     ctx.set_march_options(skip_empty=True, max_defer=args.max_defer, persistent=bool(args.persistent))
-    ctx.set_load_options(overlap_loads=bool(args.overlap), reserve_slots=args.reserve,
-                             copy_workgroups=args.copy_wg)
     cams = [w.cameras(k) for k in range(args.warmup + args.steps)]
     R = w.W * w.H
     d_rgba, d_hits = ctx.alloc(R * 16), ctx.alloc(R * 32)
@@ -535,7 +528,7 @@ def main() -> int:
         ctx.render_frame(cams[k], w.prm, d_rgba, d_hits)
     ctx.synchronize()
 
-    frame_ms, kernel_ms, passes, uploads, parked, overl = [], [], [], [], [], []
+    frame_ms, kernel_ms, passes, uploads, parked = [], [], [], [], []
     t0 = time.perf_counter()
     for k in range(args.warmup, args.warmup + args.steps):
         f0 = time.perf_counter()
@@ -545,7 +538,6 @@ def main() -> int:
         passes.append(st.passes)
         uploads.append(st.bricks_uploaded)
         parked.append(st.rays_parked)
-        overl.append(st.bricks_overlapped)
     ctx.synchronize()
     total_s = time.perf_counter() - t0
     rays_per_s = R * args.steps / total_s
@@ -582,8 +574,7 @@ def main() -> int:
         "kernel_ms_per_frame": round(float(np.mean(kernel_ms)), 4),
         "passes_per_frame": float(np.mean(passes)), "bricks_uploaded_per_frame": float(np.mean(uploads)),
         "rays_parked_per_frame": float(np.mean(parked)), "max_defer": args.max_defer,
-        "overlap_loads": args.overlap, "reserve_slots": args.reserve, "persistent": args.persistent,
-        "bricks_overlapped_per_frame": float(np.mean(overl)),
+        "persistent": args.persistent,
         "bricks_marched_per_frame": float(np.mean(U)),
         "device_miss_rate": round(float(sum(uploads)) / max(1, sum(U)), 5),
         "roofline": roofline,

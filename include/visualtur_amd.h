@@ -85,7 +85,6 @@ typedef struct {
     double ms_total;            /* frame wall time */
     double ms_kernel;           /* device time of the render passes */
     double ms_service;          /* host time spent servicing misses */
-    int64_t bricks_overlapped;  /* of bricks_uploaded: loaded while a pass was running */
 } vt_frame_stats;
 
 typedef struct {
@@ -187,15 +186,6 @@ int vt_set_frame_options(vt_context *ctx, int collect_stats, int record_trace);
  * lanes take a new ray when theirs finishes; 0 (default) runs one lane per
  * ray in 16x16-pixel workgroups. */
 int vt_set_march_options(vt_context *ctx, int skip_empty, int max_defer, int persistent);
-/* Brick-loading option with no reference counterpart; results are identical
- * for every setting.  overlap_loads=1 loads requested bricks into free device
- * slots on a second stream, by copy_workgroups workgroups (0 = 16), while the
- * render pass that requested them still runs; 0 (default) loads between
- * passes, as the reference does (visualTur.cu:175-220).  reserve_slots is
- * how many free slots vt_render_frame keeps for overlapped loads at the end
- * of a frame, evicting by the clock (-1 = gpuSlots/8; never more than half;
- * nothing is evicted while every brick with octree nodes fits). */
-int vt_set_load_options(vt_context *ctx, int overlap_loads, int64_t reserve_slots, int copy_workgroups);
 /* Visited-brick trace of the last frame: out[i] = 1 if brick
  * (i + 8^levelC) was marched by at least one ray.  n = 8^levelC. */
 int vt_get_brick_trace(vt_context *ctx, uint8_t *out, int64_t n);
@@ -319,7 +309,8 @@ int vt_mlp_features(vt_mlp *m, const vt_camera *cam, int32_t nLevels, int32_t le
  * highest-scoring non-resident ones with probability >= min_prob, at most
  * `budget`, into free slots or in place of resident bricks that score lower
  * and below probability 1 - min_prob.  The copies are issued on the
- * context's copy stream without waiting, so they overlap the next render.
+ * context's copy stream without waiting; the next vt_render_frame waits for
+ * them before its first pass.
  * Hits are never affected: the render still decides every brick it marches.
  * *loaded (may be NULL) receives the number of bricks loaded. */
 int vt_prefetch(vt_context *ctx, vt_mlp *m, const vt_camera *cam, int64_t budget, float min_prob, int64_t *loaded);

@@ -120,9 +120,8 @@ def test_hdf5_host_tier_paths(vt, vto, ctx, tmp_path):
     vto.octree_write(octf, sc.iso, 64, vol.shape, sc.n, sc.runs)
     cam = orbit_camera(vt, 80, 64, (32, 30, 25), 140.0, 120.0)
     rc, hc = sc.render_cpu(vto, cam)
-    for cpu_slots, gpu_slots, overlap in [(64, 64, 1), (64, 6, 1), (9, 5, 1), (64, 6, 0), (9, 5, 0)]:
-        log(f"hdf5 cpu_slots={cpu_slots} gpu_slots={gpu_slots} overlap={overlap}")
-        ctx.set_load_options(overlap_loads=bool(overlap))
+    for cpu_slots, gpu_slots in [(64, 64), (64, 6), (9, 5)]:
+        log(f"hdf5 cpu_slots={cpu_slots} gpu_slots={gpu_slots}")
         iso, n, rd = ctx.octree_load_file(octf, 3)
         assert (n, rd) == (6, (64, 60, 50)) and iso == np.float32(0.5)
         ctx.cache_create(2, 16, 3, gpu_slots, cpu_slots)
@@ -136,7 +135,6 @@ def test_hdf5_host_tier_paths(vt, vto, ctx, tmp_path):
         rg, hg = d_rgba.download(np.float32, (R, 4)), d_hits.download(vt.HIT_DTYPE, (R,))
         d_rgba.free(); d_hits.free()
         assert_same(hg, rg, hc, rc)
-    ctx.set_load_options()
 
 
 def test_file_read_box_matches_oracle(vt, vto, tmp_path):
@@ -265,50 +263,6 @@ def test_deferred_nodes_are_exact(vt, vto, ctx, level_o, slots):
     finally:
         ctx.set_march_options()
         ctx.set_frame_options(False, False)
-
-
-@pytest.mark.parametrize("slots,reserve,defer", [(40, -1, 2), (40, 20, 0), (96, 8, 4), (200, 100, 2)])
-def test_overlapped_loads_are_exact(vt, vto, ctx, slots, reserve, defer):
-    """Loading requested bricks while the requesting pass still runs (into
-    free slots, on the copy stream) gives the same bytes as loading between
-    passes, frame after frame, and the same hits as the oracle."""
-    vol = blobs_volume(64, 16, seed=61)
-    sc = Scene(vto, vol, 0.3, 4, 3)
-    sc.setup_gpu(vt, ctx, slots=slots)
-    cams = [orbit_camera(vt, 96, 80, (32, 32, 32), 110.0, 40.0 + 9.0 * k, 25.0) for k in range(5)]
-    refs = [sc.render_cpu(vto, cam) for cam in cams]
-    outs = {}
-    try:
-        ctx.set_march_options(skip_empty=True, max_defer=defer)
-        for overlap in (0, 1):
-            log(f"overlap={overlap} slots={slots} reserve={reserve} defer={defer}")
-            ctx.set_load_options(overlap_loads=bool(overlap), reserve_slots=reserve)
-            ctx.cache_clear()
-            frames, moved = [], 0
-            for f, cam in enumerate(cams):
-                rg, hg, st = sc.render_gpu(vt, ctx, cam)
-                assert st.err == 0
-                log(f"frame {f}: passes {st.passes} uploads {st.bricks_uploaded} overlapped {st.bricks_overlapped}")
-                assert_same(hg, rg, refs[f][1], refs[f][0])
-                frames.append((hg.tobytes(), rg.tobytes()))
-                moved += st.bricks_overlapped
-                assert ctx.cache_stats()["gpu_resident"] <= slots
-            outs[overlap] = frames
-            if overlap:
-                assert moved > 0   # some loads did overlap a pass
-            else:
-                assert moved == 0
-    finally:
-        ctx.set_march_options()
-        ctx.set_load_options()
-    for f in range(len(cams)):
-        h0, h1 = (np.frombuffer(outs[m][f][0], vt.HIT_DTYPE) for m in (0, 1))
-        r0, r1 = (np.frombuffer(outs[m][f][1], np.uint32).reshape(-1, 4) for m in (0, 1))
-        bad = np.nonzero(np.any(h0.view(np.uint8).reshape(h0.size, -1) != h1.view(np.uint8).reshape(h1.size, -1), 1) |
-                         np.any(r0 != r1, 1))[0]
-        for i in bad[:4]:
-            log(f"frame {f} ray {i}: between passes {h0[i]} {r0[i]}; overlapped {h1[i]} {r1[i]}")
-        assert bad.size == 0, (f, bad.size)
 
 
 @pytest.mark.parametrize("slots,defer", [(512, 2), (5, 0), (17, 2), (40, 4)])

@@ -43,8 +43,7 @@ class FrameStats(ctypes.Structure):
     _fields_ = [("passes", ctypes.c_int32), ("err", ctypes.c_int32), ("bricks_requested", ctypes.c_int64),
                 ("bricks_uploaded", ctypes.c_int64), ("host_tier_misses", ctypes.c_int64),
                 ("rays_parked", ctypes.c_int64), ("samples", ctypes.c_uint64), ("nodes", ctypes.c_uint64),
-                ("ms_total", ctypes.c_double), ("ms_kernel", ctypes.c_double), ("ms_service", ctypes.c_double),
-                ("bricks_overlapped", ctypes.c_int64)]
+                ("ms_total", ctypes.c_double), ("ms_kernel", ctypes.c_double), ("ms_service", ctypes.c_double)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -120,7 +119,6 @@ _SIGS = {
     "vt_set_frame_options": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
     "vt_get_brick_trace": (ctypes.c_int, [_P, _P, ctypes.c_int64]),
     "vt_set_march_options": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
-    "vt_set_load_options": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64, ctypes.c_int]),
     "vt_trace_nodes": (ctypes.c_int, [_P, ctypes.POINTER(Camera), ctypes.c_int32, ctypes.c_int32,
                                       ctypes.c_int32, _P, _P]),
     "vt_octree_build": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
@@ -380,9 +378,6 @@ class Context:
 
     def set_march_options(self, skip_empty: bool = True, max_defer: int = 2, persistent: bool = False) -> None:
         self._c(lib().vt_set_march_options(self.handle, int(skip_empty), int(max_defer), int(persistent)))
-
-    def set_load_options(self, overlap_loads: bool = False, reserve_slots: int = -1, copy_workgroups: int = 0) -> None:
-        self._c(lib().vt_set_load_options(self.handle, int(overlap_loads), int(reserve_slots), int(copy_workgroups)))
 
     def brick_trace(self, level_c: int) -> np.ndarray:
         n = 1 << (3 * level_c)

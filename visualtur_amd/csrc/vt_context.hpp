@@ -415,7 +415,7 @@ struct vt_file {
 struct vt_context {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipStream_t cstream = nullptr;   // brick loads that overlap a render pass
+    hipStream_t cstream = nullptr;   // prefetch loads (vt_render_frame waits for them)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // octree
     bool octree_ready = false;
@@ -432,16 +432,12 @@ struct vt_context {
     DevBuf<uint32_t> list[2];
     DevBuf<HitRecord> hits_scratch;
     DevBuf<uint32_t> req_bits;
-    // request list of the running pass: brick index + 1 per entry, written by
-    // the render kernel with system-scope stores into coherent host memory
-    // so that the host can start loads while the pass still runs; 0 = not
-    // yet written
+    // request list of a pass: brick index + 1 per entry, written by the
+    // render kernel with system-scope stores into coherent host memory and
+    // read by the host after the pass (no copy back); 0 = not written
     PinBuf<uint32_t> req_list;
     AsyncStage astage;
     std::vector<uint32_t> req_left;   // requests of a pass left for the synchronous service
-    int async_loads = 0;
-    int copy_workgroups = 16;         // workgroups of the copies that overlap a pass
-    int64_t reserve_slots = -1;       // free slots kept between frames (-1: gpu_slots/8)
     DevBuf<uint8_t> trace;
     FrameCounters* d_ctr = nullptr;
     FrameCounters* h_ctr = nullptr;

@@ -179,8 +179,6 @@ hipError_t launch_submm(const float* pool, const int32_t* slots, int count, int 
                         float2* submm, hipStream_t s);
 // Copy bricks of slot_elems floats from (device-visible) host memory into
 // the pool, slot b at pool + b * stride.
-hipError_t launch_copy_bricks_narrow(const float* const* src, const int32_t* dst_slots, int count, size_t slot_elems,
-                                     size_t stride, float* pool, int workgroups, hipStream_t s);
 hipError_t launch_copy_bricks(const float* const* src, const int32_t* dst_slots, int count,
                               size_t slot_elems, size_t stride, float* pool, hipStream_t s);
 // table[idx[i]] = val[i]

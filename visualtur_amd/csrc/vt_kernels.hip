@@ -824,47 +824,6 @@ __global__ __launch_bounds__(256) void k_copy_bricks(const float* const* __restr
         for (size_t q = n4 * 4 + threadIdx.x; q < slot_elems; q += 256) d[q] = s[q];
 }
 
-// The same copy by a fixed, small number of workgroups that loop over the
-// (brick, part) items: it runs beside a render pass while taking only a few
-// CUs' worth of wave slots, with enough 16-byte reads in flight for PCIe.
-__global__ __launch_bounds__(256) void k_copy_bricks_narrow(const float* const* __restrict__ src,
-                                                            const int32_t* __restrict__ dst_slots, int count,
-                                                            size_t slot_elems, size_t stride,
-                                                            float* __restrict__ pool)
-{
-    const size_t n4 = slot_elems / 4;
-    const size_t per = (n4 + kCopySplit - 1) / kCopySplit;
-    for (int item = blockIdx.x; item < count * kCopySplit; item += gridDim.x) {
-        const int b = item / kCopySplit, part = item % kCopySplit;
-        const float* s = src[b];
-        float* d = pool + (size_t)dst_slots[b] * stride;
-        const size_t beg = per * part, end = beg + per < n4 ? beg + per : n4;
-        const float4* __restrict__ s4 = reinterpret_cast<const float4*>(s);
-        float4* __restrict__ d4 = reinterpret_cast<float4*>(d);
-        size_t e = beg + threadIdx.x;
-        for (; e + 768 < end; e += 1024) {
-            float4 v0 = s4[e], v1 = s4[e + 256], v2 = s4[e + 512], v3 = s4[e + 768];
-            d4[e] = v0;
-            d4[e + 256] = v1;
-            d4[e + 512] = v2;
-            d4[e + 768] = v3;
-        }
-        for (; e < end; e += 256) d4[e] = s4[e];
-        if (part == 0)
-            for (size_t q = n4 * 4 + threadIdx.x; q < slot_elems; q += 256) d[q] = s[q];
-    }
-}
-
-hipError_t launch_copy_bricks_narrow(const float* const* src, const int32_t* dst_slots, int count, size_t slot_elems,
-                                     size_t stride, float* pool, int workgroups, hipStream_t s)
-{
-    if (count <= 0) return hipSuccess;
-    if ((int64_t)count * kCopySplit > INT32_MAX) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_copy_bricks_narrow, dim3(workgroups), dim3(256), 0, s, src, dst_slots, count, slot_elems,
-                       stride, pool);
-    return hipGetLastError();
-}
-
 hipError_t launch_copy_bricks(const float* const* src, const int32_t* dst_slots, int count,
                               size_t slot_elems, size_t stride, float* pool, hipStream_t s)
 {

@@ -191,12 +191,11 @@ int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t
 }
 
 // Loads the absent bricks of `req` that fit in free slots on the copy
-// stream, without waiting; the others are appended to `left`.  No kernel
-// reads a free slot, and the table entry of a brick is written after its
-// copy, so a render pass may run meanwhile: its rays see either the old
-// entry (and defer or park) or the complete brick.  Host-tier slots read
-// here stay pinned in the current epoch, which ends only after the copy
-// stream has been synchronized.
+// stream, without waiting; the others are appended to `left`.  The page-table
+// entry of a brick is written after its copy.  vt_render_frame waits for the
+// copy stream before its first pass, so no pass runs beside these copies.
+// Host-tier slots read here stay pinned in the current epoch, which ends only
+// after the copy stream has been synchronized.
 int load_async(vt_context* ctx, const uint32_t* req, size_t n, std::vector<uint32_t>& left, int64_t* loaded,
                int64_t* misses)
 {
@@ -239,10 +238,8 @@ int load_async(vt_context* ctx, const uint32_t* req, size_t n, std::vector<uint3
     if (k == 0) return VT_OK;
     hipStream_t s = ctx->cstream;
     if (c.host) {
-        // a narrow copy kernel: the render pass keeps almost every CU
-        HIP_TRY(launch_copy_bricks_narrow(A.src.dev + base, A.slots.dev + base, (int)k, c.brick_elems, c.slot_elems,
-                                          c.d_pool,
-                                          ctx->copy_workgroups, s));
+        HIP_TRY(launch_copy_bricks(A.src.dev + base, A.slots.dev + base, (int)k, c.brick_elems, c.slot_elems,
+                                   c.d_pool, s));
     } else {
         HIP_TRY(launch_bricks_from_volume(c.d_vol, c.vol_dims[0], c.vol_dims[1], c.vol_dims[2], A.codes.dev + base,
                                           A.slots.dev + base, (int)k, c.level_c, c.n, c.cube, c.inc, c.slot_elems, c.d_pool, s));
@@ -261,46 +258,6 @@ int copy_stream_idle(vt_context* ctx)
 {
     HIP_TRY(hipStreamSynchronize(ctx->cstream));
     ctx->astage.used = 0;
-    return VT_OK;
-}
-
-// Between frames (no kernel running, copy stream idle): evict by the clock
-// until `reserve_slots` slots are free for the loads of the next pass.  A
-// cache that can hold every brick with octree nodes never evicts.
-int replenish_free_slots(vt_context* ctx)
-{
-    BrickCache& c = *ctx->cache;
-    int64_t needed = c.nbricks;   // bricks a ray can request: those holding octree nodes
-    if (ctx->octree_ready && ctx->max_level >= c.level_c && (int)ctx->up_runs.size() > c.level_c) {
-        const std::vector<uint64_t>& r = ctx->up_runs[(size_t)c.level_c];
-        needed = 0;
-        for (size_t j = 0; j + 1 < r.size(); j += 2) needed += (int64_t)(r[j + 1] - r[j] + 1);
-    }
-    if (c.gpu_slots >= needed) return VT_OK;
-    int64_t target = ctx->reserve_slots >= 0 ? ctx->reserve_slots : c.gpu_slots / 8;
-    target = std::min(target, c.gpu_slots / 2);
-    int64_t need = target - c.available();
-    if (need <= 0) return VT_OK;
-    if (int rc = fold_ref_bits(ctx)) return rc;
-    if (c.h_tidx.ensure((size_t)need) != hipSuccess || c.h_tval.ensure((size_t)need) != hipSuccess)
-        return fail(VT_E_NOMEM, "pinned staging allocation failed");
-    c.epoch++;
-    int64_t nt = 0;
-    for (; nt < need; ++nt) {
-        int64_t v = c.clock_victim(true);
-        if (v < 0) break;
-        int64_t old = c.slot_brick[(size_t)v];
-        c.table[(size_t)old] = -1;
-        c.slot_brick[(size_t)v] = -1;
-        c.ref[(size_t)v] = 0;
-        c.free_list.push_back(v);
-        c.h_tidx.p[nt] = old;
-        c.h_tval.p[nt] = -1;
-        c.stat_evictions++;
-    }
-    if (nt == 0) return VT_OK;
-    HIP_TRY(launch_scatter_i32(c.d_table, c.h_tidx.dev, c.h_tval.dev, (int)nt, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
     return VT_OK;
 }
 
@@ -724,17 +681,6 @@ int vt_set_march_options(vt_context* ctx, int skip_empty, int max_defer, int per
     return VT_OK;
 }
 
-int vt_set_load_options(vt_context* ctx, int overlap_loads, int64_t reserve_slots, int copy_workgroups)
-{
-    if (!ctx) return fail(VT_E_ARG, "ctx is NULL");
-    if (reserve_slots < -1) return fail(VT_E_ARG, "reserve_slots must be >= -1");
-    if (copy_workgroups < 0 || copy_workgroups > 4096) return fail(VT_E_ARG, "copy_workgroups must be in [0, 4096]");
-    ctx->async_loads = overlap_loads ? 1 : 0;
-    ctx->reserve_slots = reserve_slots;
-    ctx->copy_workgroups = copy_workgroups ? copy_workgroups : 16;
-    return VT_OK;
-}
-
 int vt_get_brick_trace(vt_context* ctx, uint8_t* out, int64_t n)
 {
     if (!ctx || !out) return fail(VT_E_ARG, "NULL argument");
@@ -819,7 +765,8 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
     a.in_list = nullptr;
     a.in_count = 0;
     a.out_list = ctx->list[cur].p;
-    const bool overlap = ctx->async_loads != 0;
+    // bricks still arriving from a prefetch land before the first pass
+    if (int rc = copy_stream_idle(ctx)) return rc;
     uint32_t* rq = ctx->req_list.p;
     a.next = ctx->d_next.p;
     for (int pass = 0;; ++pass) {
@@ -831,56 +778,27 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
             HIP_TRY(launch_render_pass(a, ctx->stream));
         HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
         HIP_TRY(hipMemcpyAsync(ctx->h_ctr, ctx->d_ctr, sizeof(FrameCounters), hipMemcpyDeviceToHost, ctx->stream));
-        // while the pass runs: load requested bricks into free slots
-        std::vector<uint32_t>& left = ctx->req_left;
-        left.clear();
-        uint32_t consumed = 0;
-        int64_t overlapped = 0, misses = 0;
-        if (overlap) {
-            std::vector<uint32_t> batch;
-            double last = now_ms();
-            for (;;) {
-                hipError_t q = hipEventQuery(ctx->ev1);
-                if (q != hipSuccess && q != hipErrorNotReady) HIP_TRY(q);
-                while (consumed < (uint32_t)c.nbricks) {
-                    uint32_t v = __atomic_load_n(&rq[consumed], __ATOMIC_ACQUIRE);
-                    if (v == 0) break;
-                    batch.push_back(v - 1);
-                    consumed++;
-                }
-                const double t = now_ms();
-                if (!batch.empty() && (q == hipSuccess || batch.size() >= 64 || t - last > 0.02)) {
-                    if (int rc = load_async(ctx, batch.data(), batch.size(), left, &overlapped, &misses)) return rc;
-                    batch.clear();
-                    last = t;
-                }
-                if (q == hipSuccess) break;
-            }
-        }
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         float ms = 0.0f;
         HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         ms_kernel += ms;
         st.passes = pass + 1;
-        st.bricks_overlapped += overlapped;
-        st.bricks_uploaded += overlapped;
-        st.host_tier_misses += misses;
         const uint32_t parked = ctx->h_ctr->parked_count;
         const uint32_t nreq = ctx->h_ctr->req_count;
         if (nreq > (uint32_t)c.nbricks) return fail(VT_E_STATE, "request list overflow");
         st.bricks_requested += nreq;
         // every entry below nreq was stored before the kernel ended
-        for (uint32_t i = consumed; i < nreq; ++i) {
+        std::vector<uint32_t>& left = ctx->req_left;
+        left.clear();
+        for (uint32_t i = 0; i < nreq; ++i) {
             uint32_t v = __atomic_load_n(&rq[i], __ATOMIC_ACQUIRE);
             if (v == 0) return fail(VT_E_STATE, "request entry not visible after the pass");
             left.push_back(v - 1);
         }
         memset(rq, 0, (size_t)nreq * 4);
-        c.stat_missed += (int64_t)nreq - (int64_t)left.size();   // the rest is counted by service_requests
         if (parked == 0) break;
         st.rays_parked += parked;
         double t0 = now_ms();
-        if (int rc = copy_stream_idle(ctx)) return rc;
         if (!left.empty()) {
             int rc = service_requests(ctx, left.data(), (int64_t)left.size(), &st);
             if (rc != VT_OK) return rc;
@@ -893,12 +811,6 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
         cur ^= 1;
         a.out_list = ctx->list[cur].p;
         if (pass > 1000000) return fail(VT_E_STATE, "frame did not converge");
-    }
-    if (int rc = copy_stream_idle(ctx)) return rc;
-    if (overlap) {
-        double t0 = now_ms();
-        if (int rc = replenish_free_slots(ctx)) return rc;
-        ms_service += now_ms() - t0;
     }
     c.epoch++;
     c.protect_from = c.epoch;
