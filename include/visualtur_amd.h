@@ -360,6 +360,13 @@ int vt_server_run(vt_server *sv, const vt_request *reqs, int64_t n, int32_t mode
  * traffic, read bytes + written bytes, that calibrates HBM byte counters. */
 int vt_copy_probe(vt_context *ctx, const void *d_src, void *d_dst, size_t bytes);
 
+/* Write a frame (host copy of the W*H float4 pixel buffer) as a binary PPM,
+ * laid out as testVisualTur.cu:67-107 lays out its FreeImage bitmap: the
+ * image is H pixels wide and W tall, pixel (x = i, y = j) holds ray
+ * i*W + j, y counts from the bottom row, and each channel is the value
+ * times 255 truncated to a byte (clamped to [0, 255]; NaN gives 0). */
+int vt_image_write_ppm(const float *h_rgba, int32_t W, int32_t H, const char *path);
+
 /* ---------------------------------------------------------------- files */
 
 /* FileManager::FileManager (FileManager.cu:11-37), read-only. */

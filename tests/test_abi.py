@@ -63,3 +63,27 @@ def test_file_manager_host_paths(vt, vto, tmp_path):
     assert e.value.code == -3
     with pytest.raises(vt.VTError):
         vt.File(tmp_path / "f0.h5", "no_such_dataset")
+
+
+def test_ppm_writer_layout(vt, tmp_path):
+    """vt_image_write_ppm lays the frame out as testVisualTur.cu:67-107 lays
+    out its bitmap: H wide, W tall, pixel (x=i, y=j from the bottom) = ray
+    i*W + j, channels = value*255 truncated."""
+    W, H = 3, 2
+    rgba = np.zeros((W * H, 4), np.float32)
+    rgba[:, 0] = np.arange(W * H) / 10.0        # ray id in red: id/10 * 255 truncated
+    rgba[4, 1] = 0.5                             # 127.5 -> 127
+    rgba[5, 2] = 7.0                             # clamped to 255
+    rgba[1, 1] = np.nan                          # NaN -> 0
+    path = tmp_path / "f.ppm"
+    vt.write_ppm(path, rgba, W, H)
+    data = path.read_bytes()
+    assert data.startswith(b"P6\n2 3\n255\n")
+    px = np.frombuffer(data[len(b"P6\n2 3\n255\n"):], np.uint8).reshape(3, 2, 3)   # rows top-down, x, rgb
+    for r in range(3):
+        for x in range(2):
+            ray = x * W + (W - 1 - r)
+            assert px[r, x, 0] == int(np.float32(ray / 10.0) * np.float32(255.0)), (r, x)
+    assert px[3 - 1 - 1, 1, 1] == 127      # ray 4 = (x=1, y=1) -> row 1
+    assert px[3 - 1 - 2, 1, 2] == 255      # ray 5 = (x=1, y=2) -> row 0
+    assert px[3 - 1 - 1, 0, 1] == 0        # ray 1 = (x=0, y=1) -> row 1, NaN

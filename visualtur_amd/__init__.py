@@ -129,6 +129,7 @@ _SIGS = {
                                           ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _P]),
     "vt_volume_write_hdf5": (ctypes.c_int, [_P, _P, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_char_p,
                                             ctypes.c_char_p, ctypes.c_int32]),
+    "vt_image_write_ppm": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_char_p]),
     "vt_copy_probe": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
     "vt_server_create": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(_P)]),
     "vt_server_destroy": (ctypes.c_int, [_P]),
@@ -197,6 +198,15 @@ def look_at(eye, target, up_hint=(0.0, 1.0, 0.0)):
     right /= np.linalg.norm(right)
     up = np.cross(right, look)
     return look.astype(np.float32), up.astype(np.float32), right.astype(np.float32)
+
+
+def write_ppm(path, rgba: np.ndarray, W: int, H: int) -> None:
+    """Write a host RGBA frame (W*H x 4 float32) as a PPM laid out like the
+    reference's FreeImage bitmap (vt_image_write_ppm)."""
+    a = np.ascontiguousarray(rgba, np.float32)
+    if a.size != W * H * 4:
+        raise ValueError("rgba must hold W*H float4 pixels")
+    _check(lib().vt_image_write_ppm(a.ctypes.data, W, H, str(path).encode()))
 
 
 def make_camera(W: int, H: int, pos, look=(0, 0, -1), up=(0, 1, 0), right=None, fov_v_deg: float = 45.0,

@@ -1254,6 +1254,37 @@ extern "C" int vt_render_cube(vt_context* ctx, const vt_camera* cam, float iso, 
     return VT_OK;
 }
 
+extern "C" int vt_image_write_ppm(const float* h_rgba, int32_t W, int32_t H, const char* path)
+{
+    if (!h_rgba || !path) return fail(VT_E_ARG, "NULL argument");
+    if (W <= 0 || H <= 0) return fail(VT_E_ARG, "image size must be positive");
+    auto byte = [](float v) -> unsigned char {
+        float s = v * 255.0f;
+        if (!(s > 0.0f)) return 0;   // also NaN
+        if (s >= 255.0f) return 255;
+        return (unsigned char)s;     // truncation, as the reference's BYTE conversion
+    };
+    const int width = H, height = W;   // FreeImage_Allocate(H, W, 24)
+    std::vector<unsigned char> img((size_t)width * height * 3);
+    for (int r = 0; r < height; ++r) {   // PPM rows run top-down, FreeImage's bottom-up
+        const int y = height - 1 - r;
+        for (int x = 0; x < width; ++x) {
+            const float* p = h_rgba + ((size_t)x * W + y) * 4;   // ray i*W + j with i = x, j = y
+            unsigned char* q = img.data() + ((size_t)r * width + x) * 3;
+            q[0] = byte(p[0]);
+            q[1] = byte(p[1]);
+            q[2] = byte(p[2]);
+        }
+    }
+    FILE* f = fopen(path, "wb");
+    if (!f) return fail(VT_E_IO, std::string("cannot create ") + path);
+    fprintf(f, "P6\n%d %d\n255\n", width, height);
+    size_t wrote = fwrite(img.data(), 1, img.size(), f);
+    int closed = fclose(f);
+    if (wrote != img.size() || closed != 0) return fail(VT_E_IO, std::string("writing ") + path);
+    return VT_OK;
+}
+
 extern "C" int vt_copy_probe(vt_context* ctx, const void* d_src, void* d_dst, size_t bytes)
 {
     if (!ctx || !d_src || !d_dst || bytes % 16) return fail(VT_E_ARG, "bad arguments");
