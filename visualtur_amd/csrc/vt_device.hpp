@@ -183,6 +183,17 @@ __device__ __forceinline__ uint32_t sorted_children(const Ray& r, const OctreeVi
     int px, py, pz;
     node_min(parent, plevel, o.n, px, py, pz);
     int side = 1 << (o.n - plevel - 1);
+    // The children's faces lie on three planes per axis (parent min, middle,
+    // max).  slab_box computes (plane - origin) * inverse per face, so these
+    // nine products are exactly its per-face values; each child's slab is
+    // then slab_box's comparisons on the two planes that bound it per axis.
+    float tx[3], ty[3], tz[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        tx[q] = ((float)(px + q * side) - r.ox) * r.ix;
+        ty[q] = ((float)(py + q * side) - r.oy) * r.iy;
+        tz[q] = ((float)(pz + q * side) - r.oz) * r.iz;
+    }
     float tn[8], tf[8];
     uint32_t valid = 0;
     bool has_nan = false;
@@ -190,9 +201,18 @@ __device__ __forceinline__ uint32_t sorted_children(const Ray& r, const OctreeVi
     for (int k = 0; k < 8; ++k) {
         tn[k] = 0.0f; tf[k] = 0.0f;
         if ((members >> k) & 1) {
-            float a, b;
-            bool hit = slab(r, px + ((k >> 2) & 1) * side, py + ((k >> 1) & 1) * side,
-                            pz + (k & 1) * side, side, a, b);
+            const int bx = (k >> 2) & 1, by = (k >> 1) & 1, bz = k & 1;
+            float tmin = r.ix >= 0 ? tx[bx] : tx[bx + 1], tmax = r.ix >= 0 ? tx[bx + 1] : tx[bx];
+            float tymin = r.iy >= 0 ? ty[by] : ty[by + 1], tymax = r.iy >= 0 ? ty[by + 1] : ty[by];
+            float tzmin = r.iz >= 0 ? tz[bz] : tz[bz + 1], tzmax = r.iz >= 0 ? tz[bz + 1] : tz[bz];
+            bool hit = true;
+            if ((tmin > tymax) || (tymin > tmax)) hit = false;
+            if (tymin > tmin) tmin = tymin;
+            if (tymax < tmax) tmax = tymax;
+            if ((tmin > tzmax) || (tzmin > tmax)) hit = false;
+            if (tzmin > tmin) tmin = tzmin;
+            if (tzmax < tmax) tmax = tzmax;
+            float a = (tmin < 0.0f) ? 0.0f : tmin, b = tmax;
             if (hit && b >= 0.0f) {
                 valid |= 1u << k;
                 tn[k] = a; tf[k] = b;
