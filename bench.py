@@ -325,7 +325,8 @@ def run_c3(ctx, args):
     for form in ("plain", "prefetch"):
         ctx.cache_create(level_c, 32, inc, gpu_slots, cpu_slots)
         ctx.cache_attach_hdf5(h5, "value")
-        ctx.set_march_options(skip_empty=True, max_defer=args.max_defer, persistent=bool(args.persistent))
+        ctx.set_march_options(skip_empty=True, max_defer=args.max_defer, persistent=bool(args.persistent),
+                          max_defer_late=None if args.max_defer_late < 0 else args.max_defer_late)
         t0 = time.perf_counter()
         ctx.cache_prefetch_host(wait=True)
         log(f"c3 {form}: host tier filled from hdf5 in {time.perf_counter() - t0:.2f}s, "
@@ -477,6 +478,7 @@ def main() -> int:
     ap.add_argument("--parity-rays", type=int, default=4096)
     ap.add_argument("--prefetch", type=int, default=1, help="c2: also time the learned-prefetch form")
     ap.add_argument("--persistent", type=int, default=0, help="persistent render pass with lane refill")
+    ap.add_argument("--max-defer-late", type=int, default=-1, help="max-defer of the passes after the first (-1: same)")
     ap.add_argument("--max-defer", type=int, default=2,
                     help="absent-brick nodes a ray may walk past per pass (0 = park at the first)")
     ap.add_argument("--prefetch-min-prob", type=float, default=0.8)
@@ -519,7 +521,8 @@ def main() -> int:
         a_.free()
         b_.free()
     w = setup_c2(ctx, args) if args.config == "c2" else setup_c1(ctx, args)
-    ctx.set_march_options(skip_empty=True, max_defer=args.max_defer, persistent=bool(args.persistent))
+    ctx.set_march_options(skip_empty=True, max_defer=args.max_defer, persistent=bool(args.persistent),
+                          max_defer_late=None if args.max_defer_late < 0 else args.max_defer_late)
     cams = [w.cameras(k) for k in range(args.warmup + args.steps)]
     R = w.W * w.H
     d_rgba, d_hits = ctx.alloc(R * 16), ctx.alloc(R * 32)
@@ -574,7 +577,7 @@ def main() -> int:
         "kernel_ms_per_frame": round(float(np.mean(kernel_ms)), 4),
         "passes_per_frame": float(np.mean(passes)), "bricks_uploaded_per_frame": float(np.mean(uploads)),
         "rays_parked_per_frame": float(np.mean(parked)), "max_defer": args.max_defer,
-        "persistent": args.persistent,
+        "persistent": args.persistent, "max_defer_late": args.max_defer_late,
         "bricks_marched_per_frame": float(np.mean(U)),
         "device_miss_rate": round(float(sum(uploads)) / max(1, sum(U)), 5),
         "roofline": roofline,

@@ -179,13 +179,14 @@ int vt_set_frame_options(vt_context *ctx, int collect_stats, int record_trace);
  * every setting.  skip_empty=1 (default) skips the evaluation of samples
  * whose 4x4x4-cell sub-block lies strictly on one side of iso.  max_defer
  * (0..16, default 2) is how many nodes with an absent brick a ray may walk
- * past in one pass (their bricks are requested, and they are marched in a
- * later pass before the ray's tentative result is accepted); 0 parks a ray
- * at its first absent brick, as the reference's staged loop does
- * (visualTur.cu:175-220).  persistent=1 runs each pass as a fixed grid whose
+ * past in the first pass (their bricks are requested, and they are marched
+ * in a later pass before the ray's tentative result is accepted);
+ * max_defer_late (0..16) is the same for the later passes, which carry only
+ * the parked rays; 0 parks a ray at its first absent brick, as the
+ * reference's staged loop does (visualTur.cu:175-220).  persistent=1 runs each pass as a fixed grid whose
  * lanes take a new ray when theirs finishes; 0 (default) runs one lane per
  * ray in 16x16-pixel workgroups. */
-int vt_set_march_options(vt_context *ctx, int skip_empty, int max_defer, int persistent);
+int vt_set_march_options(vt_context *ctx, int skip_empty, int max_defer, int max_defer_late, int persistent);
 /* Visited-brick trace of the last frame: out[i] = 1 if brick
  * (i + 8^levelC) was marched by at least one ray.  n = 8^levelC. */
 int vt_get_brick_trace(vt_context *ctx, uint8_t *out, int64_t n);

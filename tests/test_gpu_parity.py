@@ -74,8 +74,9 @@ def test_render_all_resident(vt, vto, ctx, level_o, level_c, inc):
         assert (hc["node"] != 0).mean() > 0.1   # the view actually hits the surface
 
 
-@pytest.mark.parametrize("slots,defer", [(1, 0), (2, 0), (5, 0), (17, 0), (1, 4), (2, 16), (5, 1), (17, 4)])
-def test_capacity_invariance(vt, vto, ctx, slots, defer):
+@pytest.mark.parametrize("slots,defer,late", [(1, 0, 0), (2, 0, 0), (5, 0, 0), (17, 0, 0), (1, 4, 4), (2, 16, 16),
+                                              (5, 1, 8), (17, 4, 16), (17, 2, 8), (5, 0, 3)])
+def test_capacity_invariance(vt, vto, ctx, slots, defer, late):
     """Hits do not depend on the device cache capacity (the reference's GPU
     tier could overwrite a brick another ray still used, lruCache.cu:254-266)
     nor on how many absent-brick nodes a ray walks past per pass."""
@@ -83,7 +84,7 @@ def test_capacity_invariance(vt, vto, ctx, slots, defer):
     sc = Scene(vto, vol, 0.25, 3, 2)
     sc.setup_gpu(vt, ctx, slots=slots)
     cam = orbit_camera(vt, 64, 64, (32, 32, 32), 100.0, 30.0)
-    ctx.set_march_options(skip_empty=True, max_defer=defer)
+    ctx.set_march_options(skip_empty=True, max_defer=defer, max_defer_late=late)
     try:
         rg, hg, st = sc.render_gpu(vt, ctx, cam)
     finally:

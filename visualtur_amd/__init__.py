@@ -118,7 +118,7 @@ _SIGS = {
                                        ctypes.POINTER(FrameStats)]),
     "vt_set_frame_options": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
     "vt_get_brick_trace": (ctypes.c_int, [_P, _P, ctypes.c_int64]),
-    "vt_set_march_options": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "vt_set_march_options": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "vt_trace_nodes": (ctypes.c_int, [_P, ctypes.POINTER(Camera), ctypes.c_int32, ctypes.c_int32,
                                       ctypes.c_int32, _P, _P]),
     "vt_octree_build": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
@@ -386,8 +386,10 @@ class Context:
                                       d_hits.ptr if d_hits is not None else None, ctypes.byref(st)))
         return st
 
-    def set_march_options(self, skip_empty: bool = True, max_defer: int = 2, persistent: bool = False) -> None:
-        self._c(lib().vt_set_march_options(self.handle, int(skip_empty), int(max_defer), int(persistent)))
+    def set_march_options(self, skip_empty: bool = True, max_defer: int = 2, persistent: bool = False,
+                          max_defer_late: int | None = None) -> None:
+        late = max_defer if max_defer_late is None else max_defer_late
+        self._c(lib().vt_set_march_options(self.handle, int(skip_empty), int(max_defer), int(late), int(persistent)))
 
     def brick_trace(self, level_c: int) -> np.ndarray:
         n = 1 << (3 * level_c)

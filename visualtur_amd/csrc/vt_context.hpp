@@ -443,7 +443,8 @@ struct vt_context {
     FrameCounters* h_ctr = nullptr;
     int collect_stats = 0, record_trace = 0;
     int skip_empty = 1;   // sub-block sample skipping in the march
-    int max_defer = 2;    // absent-brick nodes a ray walks past per pass
+    int max_defer = 2;    // absent-brick nodes a ray walks past in the first pass
+    int max_defer_late = 2;   // ... and in later passes
     int persistent = 0;   // persistent render pass with lane refill (measured slower; option)
     int persistent_wgs = 0;
     DevBuf<uint32_t> d_next;   // its 8 work counters

@@ -671,12 +671,14 @@ int vt_set_frame_options(vt_context* ctx, int collect_stats, int record_trace)
     return VT_OK;
 }
 
-int vt_set_march_options(vt_context* ctx, int skip_empty, int max_defer, int persistent)
+int vt_set_march_options(vt_context* ctx, int skip_empty, int max_defer, int max_defer_late, int persistent)
 {
     if (!ctx) return fail(VT_E_ARG, "ctx is NULL");
-    if (max_defer < 0 || max_defer > kMaxDefer) return fail(VT_E_ARG, "max_defer must be in [0, 16]");
+    if (max_defer < 0 || max_defer > kMaxDefer || max_defer_late < 0 || max_defer_late > kMaxDefer)
+        return fail(VT_E_ARG, "max_defer and max_defer_late must be in [0, 16]");
     ctx->skip_empty = skip_empty ? 1 : 0;
     ctx->max_defer = max_defer;
+    ctx->max_defer_late = max_defer_late;
     ctx->persistent = persistent ? 1 : 0;
     return VT_OK;
 }
@@ -711,7 +713,8 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
     const uint32_t R = (uint32_t)cam->W * (uint32_t)cam->H;
     if (ctx->ray_node.ensure(R) != hipSuccess || ctx->ray_phase.ensure(R) != hipSuccess ||
         ctx->list[0].ensure(R) != hipSuccess || ctx->list[1].ensure(R) != hipSuccess ||
-        (ctx->max_defer > 0 && ctx->ray_defer.ensure((size_t)R * (size_t)ctx->max_defer) != hipSuccess))
+        (std::max(ctx->max_defer, ctx->max_defer_late) > 0 &&
+         ctx->ray_defer.ensure((size_t)R * (size_t)std::max(ctx->max_defer, ctx->max_defer_late)) != hipSuccess))
         return fail(VT_E_NOMEM, "ray state allocation failed");
     HitRecord* hits = reinterpret_cast<HitRecord*>(d_hits);
     if (!hits) {
@@ -743,7 +746,7 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
     a.color_scale = p->color_scale;
     a.ray_node = ctx->ray_node.p;
     a.ray_phase = ctx->ray_phase.p;
-    a.ray_defer = ctx->max_defer > 0 ? ctx->ray_defer.p : nullptr;
+    a.ray_defer = std::max(ctx->max_defer, ctx->max_defer_late) > 0 ? ctx->ray_defer.p : nullptr;
     a.max_defer = ctx->max_defer;
     a.req_bits = ctx->req_bits.p;
     a.req_list = ctx->req_list.p;
@@ -808,6 +811,7 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
         HIP_TRY(hipMemsetAsync(ctx->d_ctr, 0, 8, ctx->stream));   // req_count, parked_count
         a.in_list = ctx->list[cur].p;
         a.in_count = parked;
+        a.max_defer = ctx->max_defer_late;   // few rays remain: let them walk further per pass
         cur ^= 1;
         a.out_list = ctx->list[cur].p;
         if (pass > 1000000) return fail(VT_E_STATE, "frame did not converge");
