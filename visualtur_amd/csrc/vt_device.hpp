@@ -279,21 +279,23 @@ __device__ __forceinline__ uint64_t next_node(const Ray& r, const OctreeView& o,
 {
     int level = code_level(node);
     for (;;) {
+        // one sorted-children site per iteration: the node's own list when
+        // descending, its parent's list when moving to the next sibling
+        if (descend && level == o.final_level) return node;
+        if (!descend && level == 0) return 0;
+        const uint64_t parent = descend ? node : node >> 3;
+        const uint32_t ord = sorted_children_cached(r, o, parent, descend ? level : level - 1, cache);
+        const int cnt = order_count(ord);
         if (descend) {
-            if (level == o.final_level) return node;
-            uint32_t ord = sorted_children_cached(r, o, node, level, cache);
-            if (order_count(ord) > 0) {
+            if (cnt > 0) {
                 node = (node << 3) | (uint64_t)order_at(ord, 0);
                 level += 1;
-                continue;
+            } else {
+                descend = false;
             }
-            descend = false;
+            continue;
         }
-        if (level == 0) return 0;
-        uint64_t parent = node >> 3;
         int k = (int)(node & 7u);
-        uint32_t ord = sorted_children_cached(r, o, parent, level - 1, cache);
-        int cnt = order_count(ord);
         int p = 0;
         while (p < cnt && order_at(ord, p) != k) ++p;
         if (p + 1 < cnt) {
@@ -531,12 +533,27 @@ __device__ __forceinline__ void shade(const float* __restrict__ slot, int S, int
                                       float lx, float ly, float lz, float color_scale,
                                       float4& out, uint32_t& err)
 {
-    float n0 = (sample(slot, S, bx, by, bz, Xx - 1.0f, Xy, Xz, err) -
-                sample(slot, S, bx, by, bz, Xx + 1.0f, Xy, Xz, err)) / 2.0f;
-    float n1 = (sample(slot, S, bx, by, bz, Xx, Xy - 1.0f, Xz, err) -
-                sample(slot, S, bx, by, bz, Xx, Xy + 1.0f, Xz, err)) / 2.0f;
-    float n2 = (sample(slot, S, bx, by, bz, Xx, Xy, Xz - 1.0f, err) -
-                sample(slot, S, bx, by, bz, Xx, Xy, Xz + 1.0f, err)) / 2.0f;
+    // the six central-difference samples in one loop (one inlined sample
+    // site): step q takes axis q>>1, at -1 for even q and +1 for odd q;
+    // n_axis = (s(-1) - s(+1)) / 2 exactly as written out per axis
+    float n0 = 0.0f, n1 = 0.0f, n2 = 0.0f, lo = 0.0f;
+#pragma unroll 1
+    for (int q = 0; q < 6; ++q) {
+        const int ax = q >> 1;
+        const bool plus = (q & 1) != 0;
+        const float px = ax == 0 ? (plus ? Xx + 1.0f : Xx - 1.0f) : Xx;
+        const float py = ax == 1 ? (plus ? Xy + 1.0f : Xy - 1.0f) : Xy;
+        const float pz = ax == 2 ? (plus ? Xz + 1.0f : Xz - 1.0f) : Xz;
+        const float s = sample(slot, S, bx, by, bz, px, py, pz, err);
+        if (!plus) {
+            lo = s;
+        } else {
+            const float v = (lo - s) / 2.0f;
+            n0 = ax == 0 ? v : n0;
+            n1 = ax == 1 ? v : n1;
+            n2 = ax == 2 ? v : n2;
+        }
+    }
     float ninv = 1.0f / sqrtf(n0 * n0 + n1 * n1 + n2 * n2);
     n0 = n0 * ninv; n1 = n1 * ninv; n2 = n2 * ninv;
     float l0 = Xx - lx, l1 = Xy - ly, l2 = Xz - lz;

@@ -59,14 +59,12 @@ __device__ __forceinline__ bool visit_node(const Ray& r, const OctreeView& oct, 
     const float* sp = bv.pool + (size_t)slot * bv.slot_elems;
     float Xx, Xy, Xz;
     nnodes += 1;
-    bool hit;
-    if (bv.submm) {
-        SubGrid g{bv.submm + (size_t)slot * bv.G * bv.G * bv.G, bv.G, bx + bv.inc, by + bv.inc, bz + bv.inc};
-        hit = march_node_skip(r, nx, ny, nz, side, sp, bv.S, bx, by, bz, g, sh.iso, Xx, Xy, Xz, err, nsamples);
-    } else {
-        hit = march_node(r, nx, ny, nz, side, sp, bv.S, bx, by, bz, sh.iso, Xx, Xy, Xz, err, nsamples);
-    }
-    if (!hit) return false;
+    // without sub-block min/max (mm == null) no side is ever proven and
+    // march_node_skip evaluates every sample: march_node's sequence exactly
+    SubGrid g{bv.submm ? bv.submm + (size_t)slot * bv.G * bv.G * bv.G : nullptr, bv.G, bx + bv.inc,
+              by + bv.inc, bz + bv.inc};
+    if (!march_node_skip(r, nx, ny, nz, side, sp, bv.S, bx, by, bz, g, sh.iso, Xx, Xy, Xz, err, nsamples))
+        return false;
     shade(sp, bv.S, bx, by, bz, Xx, Xy, Xz, sh.lx, sh.ly, sh.lz, sh.color_scale, out, err);
     nsamples += 6;
     h.node = node;
@@ -200,40 +198,77 @@ __global__ __launch_bounds__(kPassBlock) void k_render_pass(RenderArgs a)
         // (final when nd == 0), kBlocked = a deferred node is still absent
         constexpr int kKeep = 3, kBlocked = 4;
         int st;
+        // One loop over the nodes this pass marches, so that the march, the
+        // shading and the traversal step are each inlined once: first the
+        // deferred nodes of earlier passes (k < nprev, in walk order), then
+        // the walk itself (from the root in the first pass, from the parked
+        // node in later ones; `adv` = advance past `node` before using it).
+        int nprev = 0, k = 0;
+        uint8_t tent = 0;
+        bool adv, descend;
         if (a.in_list == nullptr) {
-            node = root_ok(r, a.oct) ? next_node(r, a.oct, 1ull, true, oc) : 0ull;
-            st = walk_defer(r, a, bv, sh, node, id, R, nd, c, h, bidx, err, nsamples, nnodes, oc);
+            adv = descend = root_ok(r, a.oct);
+            node = adv ? 1ull : 0ull;
         } else {
             const uint8_t ph = a.ray_phase[id];
-            const int nprev = ph & PHASE_NMASK;
-            int k = 0;
-            bool hit = false;
-            for (; k < nprev; ++k) {   // deferred nodes, in walk order
-                uint64_t s = a.ray_defer[(size_t)k * R + id];
-                uint64_t brick = s >> bv.shift;
-                bidx = brick - bv.brick_base;
-                int slot = bv.brick_slot[bidx];
-                if (slot < 0) break;
-                if (visit_node(r, a.oct, bv, sh, s, brick, slot, a.slot_ref, a.brick_trace, c, h, err, nsamples,
-                               nnodes)) {
-                    hit = true;
+            nprev = ph & PHASE_NMASK;
+            tent = ph & PHASE_TENT;
+            adv = descend = false;
+            // parked on this node, not yet marched (read only when the walk
+            // resumes: a ray with a tentative result has no resume node)
+            if (!tent) node = a.ray_node[id];
+        }
+        for (;;) {
+            const bool deferred = k < nprev;
+            uint64_t cur;
+            if (deferred) {
+                cur = a.ray_defer[(size_t)k * R + id];
+            } else {
+                if (tent) {   // every deferred node missed: the tentative result stands
+                    st = kKeep;
                     break;
                 }
+                if (adv) {
+                    node = next_node(r, a.oct, node, descend, oc);
+                    adv = descend = false;
+                }
+                if (node == 0) {
+                    set_miss(c, h);
+                    st = WALK_MISS;
+                    break;
+                }
+                cur = node;
             }
-            if (hit) {
-                st = WALK_HIT;
-            } else if (k < nprev) {
-                for (int q = k; q < nprev; ++q) a.ray_defer[(size_t)(q - k) * R + id] = a.ray_defer[(size_t)q * R + id];
-                nd = nprev - k;
+            const uint64_t brick = cur >> bv.shift;
+            bidx = brick - bv.brick_base;
+            const int slot = bv.brick_slot[bidx];
+            if (slot < 0) {
+                if (deferred) {   // still absent: keep it and the ones after it
+                    for (int q = k; q < nprev; ++q)
+                        a.ray_defer[(size_t)(q - k) * R + id] = a.ray_defer[(size_t)q * R + id];
+                    nd = nprev - k;
+                    request_brick(a, bidx);
+                    a.ray_phase[id] = (uint8_t)(tent | nd);   // resume node and tentative result kept
+                    park = true;
+                    st = kBlocked;
+                    break;
+                }
+                if (nd >= a.max_defer) {
+                    st = WALK_ABSENT;
+                    break;
+                }
+                a.ray_defer[(size_t)nd * R + id] = cur;
+                nd += 1;
                 request_brick(a, bidx);
-                a.ray_phase[id] = (uint8_t)((ph & PHASE_TENT) | nd);   // resume node and tentative result kept
-                park = true;
-                st = kBlocked;
-            } else if (ph & PHASE_TENT) {
-                st = kKeep;   // every deferred node missed: the tentative result stands
+            } else if (visit_node(r, a.oct, bv, sh, cur, brick, slot, a.slot_ref, a.brick_trace, c, h, err,
+                                  nsamples, nnodes)) {
+                st = WALK_HIT;   // final when found in a deferred node (nd == 0 there)
+                break;
+            }
+            if (deferred) {
+                k += 1;
             } else {
-                node = a.ray_node[id];   // parked on this node, not yet marched
-                st = walk_defer(r, a, bv, sh, node, id, R, nd, c, h, bidx, err, nsamples, nnodes, oc);
+                adv = true;
             }
         }
         if (st == WALK_ABSENT) {
