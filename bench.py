@@ -327,6 +327,8 @@ def run_c3(ctx, args):
         ctx.cache_attach_hdf5(h5, "value")
         ctx.set_march_options(skip_empty=True, max_defer=args.max_defer, persistent=bool(args.persistent),
                           max_defer_late=None if args.max_defer_late < 0 else args.max_defer_late)
+        if args.wide_lanes is not None:
+            ctx.set_wide_march(args.wide_lanes)
         t0 = time.perf_counter()
         ctx.cache_prefetch_host(wait=True)
         log(f"c3 {form}: host tier filled from hdf5 in {time.perf_counter() - t0:.2f}s, "
@@ -479,6 +481,8 @@ def main() -> int:
     ap.add_argument("--prefetch", type=int, default=1, help="c2: also time the learned-prefetch form")
     ap.add_argument("--persistent", type=int, default=0, help="persistent render pass with lane refill")
     ap.add_argument("--max-defer-late", type=int, default=-1, help="max-defer of the passes after the first (-1: same)")
+    ap.add_argument("--wide-lanes", type=int, default=None,
+                    help="lane budget of the wide march in later passes (0: off; default: the library's)")
     ap.add_argument("--max-defer", type=int, default=2,
                     help="absent-brick nodes a ray may walk past per pass (0 = park at the first)")
     ap.add_argument("--prefetch-min-prob", type=float, default=0.8)
@@ -523,6 +527,8 @@ def main() -> int:
     w = setup_c2(ctx, args) if args.config == "c2" else setup_c1(ctx, args)
     ctx.set_march_options(skip_empty=True, max_defer=args.max_defer, persistent=bool(args.persistent),
                           max_defer_late=None if args.max_defer_late < 0 else args.max_defer_late)
+    if args.wide_lanes is not None:
+        ctx.set_wide_march(args.wide_lanes)
     cams = [w.cameras(k) for k in range(args.warmup + args.steps)]
     R = w.W * w.H
     d_rgba, d_hits = ctx.alloc(R * 16), ctx.alloc(R * 32)
@@ -577,7 +583,7 @@ def main() -> int:
         "kernel_ms_per_frame": round(float(np.mean(kernel_ms)), 4),
         "passes_per_frame": float(np.mean(passes)), "bricks_uploaded_per_frame": float(np.mean(uploads)),
         "rays_parked_per_frame": float(np.mean(parked)), "max_defer": args.max_defer,
-        "persistent": args.persistent, "max_defer_late": args.max_defer_late,
+        "persistent": args.persistent, "max_defer_late": args.max_defer_late, "wide_lanes": args.wide_lanes,
         "bricks_marched_per_frame": float(np.mean(U)),
         "device_miss_rate": round(float(sum(uploads)) / max(1, sum(U)), 5),
         "roofline": roofline,

@@ -185,8 +185,16 @@ int vt_set_frame_options(vt_context *ctx, int collect_stats, int record_trace);
  * the parked rays; 0 parks a ray at its first absent brick, as the
  * reference's staged loop does (visualTur.cu:175-220).  persistent=1 runs each pass as a fixed grid whose
  * lanes take a new ray when theirs finishes; 0 (default) runs one lane per
- * ray in 16x16-pixel workgroups. */
+ * ray in one-wave workgroups of 8x8 pixels. */
 int vt_set_march_options(vt_context *ctx, int skip_empty, int max_defer, int max_defer_late, int persistent);
+/* Wide march in the passes after the first, which carry only the parked
+ * rays (no reference counterpart; results are identical for every
+ * setting): each ray gets 2^k lanes (k <= 6) that march its nodes together,
+ * the largest 2^k with rays x 2^k <= lane_budget (default 262144, the lanes
+ * of about one wave per SIMD); 0 keeps one lane per ray in
+ * every pass; -k (k in 1..6) gives every later pass 2^k lanes per ray.  Not
+ * used with persistent=1. */
+int vt_set_wide_march(vt_context *ctx, int64_t lane_budget);
 /* Visited-brick trace of the last frame: out[i] = 1 if brick
  * (i + 8^levelC) was marched by at least one ray.  n = 8^levelC. */
 int vt_get_brick_trace(vt_context *ctx, uint8_t *out, int64_t n);

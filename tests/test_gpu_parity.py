@@ -294,6 +294,45 @@ def test_persistent_pass_matches_tiled_pass(vt, vto, ctx, slots, defer):
                 hc, rc)
 
 
+@pytest.mark.parametrize("slots,defer,skip", [(5, 0, True), (17, 2, True), (40, 4, True), (17, 2, False),
+                                              (512, 2, True)])
+def test_wide_march_matches_one_lane(vt, vto, ctx, slots, defer, skip):
+    """The later passes with 2..64 lanes per ray (every fixed width, and the
+    lane budget's choice) give the bytes, pass count, requests, sample and
+    node counts of the one-lane pass."""
+    vol = blobs_volume(64, 16, seed=91)
+    sc = Scene(vto, vol, 0.3, 4, 3)
+    sc.setup_gpu(vt, ctx, slots=slots)
+    cams = [orbit_camera(vt, 131, 77, (32, 32, 32), 105.0, 20.0 + 70.0 * k, 15.0) for k in range(3)]
+    outs = {}
+    ctx.set_frame_options(collect_stats=True)
+    try:
+        for wide in (0, -1, -2, -3, -4, -5, -6, 4096):
+            log(f"wide={wide} slots={slots} defer={defer} skip={skip}")
+            ctx.set_march_options(skip_empty=skip, max_defer=defer)
+            ctx.set_wide_march(wide)
+            ctx.cache_clear()
+            rows = []
+            for cam in cams:
+                rg, hg, st = sc.render_gpu(vt, ctx, cam)
+                assert st.err == 0
+                rows.append((hg.tobytes(), rg.tobytes(), st.passes, st.bricks_requested, st.samples, st.nodes))
+            outs[wide] = rows
+    finally:
+        ctx.set_wide_march()
+        ctx.set_march_options()
+        ctx.set_frame_options(False, False)
+    for wide, rows in outs.items():
+        for k, (row, ref) in enumerate(zip(rows, outs[0])):
+            assert row[:2] == ref[:2], (wide, k, "bytes")
+            assert row[2:] == ref[2:], (wide, k, row[2:], ref[2:])
+    if slots < 512:
+        assert max(r[2] for r in outs[0]) >= 2   # later passes ran
+    rc, hc = sc.render_cpu(vto, cams[-1])
+    assert_same(np.frombuffer(outs[-6][-1][0], vt.HIT_DTYPE), np.frombuffer(outs[-6][-1][1], np.float32).reshape(-1, 4),
+                hc, rc)
+
+
 def test_iso_change_keeps_the_cache_exact(vt, vto, ctx):
     """The sub-block side maps follow the frame's isovalue: a cache filled
     at one iso renders a second iso (with its own octree) exactly."""

@@ -305,6 +305,15 @@ struct AsyncStage {
 };
 
 // ----------------------------------------------------------- brick cache
+// Byte offsets of the staging arrays of a load of n bricks (n table entries
+// for the loads and up to n for the evictions): source pointers, brick codes,
+// table indices, slots, table values.
+struct StageLayout {
+    size_t src, codes, tidx, slots, tval, bytes;
+    explicit StageLayout(size_t n)
+        : src(0), codes(8 * n), tidx(16 * n), slots(32 * n), tval(36 * n), bytes(44 * n) {}
+};
+
 struct BrickCache {
     int level_c = 0, cube = 0, inc = 0, S = 0;
     int n = 0;   // octree depth the brick codes refer to (set on first use)
@@ -321,7 +330,8 @@ struct BrickCache {
     int G = 0;
     std::vector<int32_t> table;
     std::vector<int64_t> slot_brick;
-    std::vector<uint8_t> ref, ref_tmp;
+    std::vector<uint8_t> ref;
+    PinBuf<uint8_t> h_ref;   // the device reference bytes, read back after each render pass
     std::vector<uint64_t> pin;
     uint64_t epoch = 0;
     uint64_t protect_from = 0;    // bricks loaded in epochs >= this (since the last frame ended) go last
@@ -334,17 +344,10 @@ struct BrickCache {
     const float* d_vol = nullptr;
     int vol_dims[3] = {0, 0, 0};
     std::unique_ptr<HostTier> host;
-    // staging
-    PinBuf<const float*> h_src;
-    PinBuf<int32_t> h_slots;
-    PinBuf<int64_t> h_codes;
-    PinBuf<int64_t> h_tidx;
-    PinBuf<int32_t> h_tval;
-    DevBuf<const float*> d_src;
-    DevBuf<int32_t> d_slots;
-    DevBuf<int64_t> d_codes;
-    DevBuf<int64_t> d_tidx;
-    DevBuf<int32_t> d_tval;
+    // staging of one load: the arrays of StageLayout in one pinned buffer,
+    // moved to the device with one copy
+    PinBuf<uint8_t> h_stage;
+    DevBuf<uint8_t> d_stage;
     // stats
     int64_t stat_missed = 0, stat_uploads = 0, stat_evictions = 0;
 
@@ -355,8 +358,9 @@ struct BrickCache {
         if (d_table) (void)hipFree(d_table);
         if (d_ref) (void)hipFree(d_ref);
         if (d_submm) (void)hipFree(d_submm);
-        h_src.release(); h_slots.release(); h_codes.release(); h_tidx.release(); h_tval.release();
-        d_src.release(); d_slots.release(); d_codes.release(); d_tidx.release(); d_tval.release();
+        h_stage.release();
+        d_stage.release();
+        h_ref.release();
     }
 
     int64_t resident() const { return used - (int64_t)free_list.size(); }
@@ -445,6 +449,7 @@ struct vt_context {
     int skip_empty = 1;   // sub-block sample skipping in the march
     int max_defer = 2;    // absent-brick nodes a ray walks past in the first pass
     int max_defer_late = 2;   // ... and in later passes
+    int64_t wide_lanes = 1 << 18;   // lane budget of the wide march in later passes (0: off, -k: 2^k lanes)
     int persistent = 0;   // persistent render pass with lane refill (measured slower; option)
     int persistent_wgs = 0;
     DevBuf<uint32_t> d_next;   // its 8 work counters

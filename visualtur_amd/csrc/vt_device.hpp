@@ -527,6 +527,23 @@ __device__ __forceinline__ bool march_node_skip(const Ray& r, int nx, int ny, in
     return false;
 }
 
+// rayCaster.cu:238-251 from the central-difference gradient (n0, n1, n2)
+__device__ __forceinline__ void shade_normal(float n0, float n1, float n2, float Xx, float Xy, float Xz, float lx,
+                                             float ly, float lz, float color_scale, float4& out)
+{
+    float ninv = 1.0f / sqrtf(n0 * n0 + n1 * n1 + n2 * n2);
+    n0 = n0 * ninv; n1 = n1 * ninv; n2 = n2 * ninv;
+    float l0 = Xx - lx, l1 = Xy - ly, l2 = Xz - lz;
+    float linv = 1.0f / sqrtf(l0 * l0 + l1 * l1 + l2 * l2);
+    l0 = l0 * linv; l1 = l1 * linv; l2 = l2 * linv;
+    float dif = fabsf(n0 * l0 + n1 * l1 + n2 * l2);
+    float a = Xy / color_scale;
+    out.x = (1 - a) * dif;
+    out.y = (a) * dif;
+    out.z = 0.0f * dif;
+    out.w = 1.0f;
+}
+
 // rayCaster.cu:131-137 and 238-251
 __device__ __forceinline__ void shade(const float* __restrict__ slot, int S, int bx, int by, int bz,
                                       float Xx, float Xy, float Xz,
@@ -554,17 +571,7 @@ __device__ __forceinline__ void shade(const float* __restrict__ slot, int S, int
             n2 = ax == 2 ? v : n2;
         }
     }
-    float ninv = 1.0f / sqrtf(n0 * n0 + n1 * n1 + n2 * n2);
-    n0 = n0 * ninv; n1 = n1 * ninv; n2 = n2 * ninv;
-    float l0 = Xx - lx, l1 = Xy - ly, l2 = Xz - lz;
-    float linv = 1.0f / sqrtf(l0 * l0 + l1 * l1 + l2 * l2);
-    l0 = l0 * linv; l1 = l1 * linv; l2 = l2 * linv;
-    float dif = fabsf(n0 * l0 + n1 * l1 + n2 * l2);
-    float a = Xy / color_scale;
-    out.x = (1 - a) * dif;
-    out.y = (a) * dif;
-    out.z = 0.0f * dif;
-    out.w = 1.0f;
+    shade_normal(n0, n1, n2, Xx, Xy, Xz, lx, ly, lz, color_scale, out);
 }
 
 }  // namespace vt

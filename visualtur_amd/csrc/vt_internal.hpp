@@ -168,6 +168,8 @@ hipError_t launch_stage_raycast(const StageRaycastArgs& a, hipStream_t s);
 hipError_t launch_render_persistent(const RenderArgs& a, int max_workgroups, hipStream_t s);
 int render_persistent_workgroups(int device);   // resident workgroups of the persistent pass on `device`
 hipError_t launch_render_pass(const RenderArgs& a, hipStream_t s);
+// a later pass (in_list set) with 2^lg lanes per ray, lg in 0..6
+hipError_t launch_render_wide(const RenderArgs& a, int lg, hipStream_t s);
 hipError_t launch_trace_nodes(const TraceArgs& a, hipStream_t s);
 // Assemble bricks with halo from a device volume [X][Y][Z] (zero outside).
 hipError_t launch_bricks_from_volume(const float* vol, int X, int Y, int Z,
@@ -184,6 +186,21 @@ hipError_t launch_copy_bricks(const float* const* src, const int32_t* dst_slots,
 // table[idx[i]] = val[i]
 hipError_t launch_scatter_i32(int32_t* table, const int64_t* idx, const int32_t* val, int count,
                               hipStream_t s);
+// Per-pass state that the service between two render passes resets: the
+// request bitmap, the first `nctr` counter words (request and parked counts)
+// and the per-slot reference bytes (already read back by the host).
+struct PassReset {
+    uint32_t* req_bits;
+    int64_t nwords;
+    uint32_t* ctr;
+    int nctr;
+    uint8_t* ref;
+    int64_t nref;
+};
+// table[idx[i]] = val[i] for i < count, then the resets of `r`: one launch
+// ends the service between two passes
+hipError_t launch_finish_service(int32_t* table, const int64_t* idx, const int32_t* val, int count,
+                                 const PassReset& r, hipStream_t s);
 // streaming copy of `bytes` (multiple of 16) used to calibrate HBM counters
 hipError_t launch_copy_probe(const float* src, float* dst, size_t bytes, hipStream_t s);
 // camera rays (Camera.cu:39-74)

@@ -7,6 +7,8 @@
 // resident, shades the first crossing and stops.  A ray whose next brick is
 // not resident records the brick in the request set and parks; the host
 // loads the requested bricks and relaunches over the parked rays only.
+#include <algorithm>
+
 #include "vt_internal.hpp"
 
 #pragma clang fp contract(off)
@@ -148,37 +150,206 @@ __device__ __forceinline__ int walk_defer(const Ray& r, const RenderArgs& a, con
     }
 }
 
-// Render-pass workgroup: kPassBlock/64 waves, each rendering one 8x8 pixel
-// tile (the tiles of a workgroup are consecutive in row-major tile order).
-constexpr int kPassBlock = 64;
+// ------------------------------------------------------ wide march
+// In the passes after the first only the parked rays run, a few thousand or
+// fewer, and each walks alone through a chain of dependent loads while the
+// GPU idles.  The wide pass gives each ray a group of L lanes (L = 2^lg,
+// groups aligned inside the wave) that march every node together: lane q of
+// the group takes samples q, q+L, q+2L, ... of the node, so the sub-block
+// lookups and trilinear gathers of L samples are in flight at once, and a
+// group ballot finds the first crossing.  Everything else of the ray (the
+// traversal, the brick lookups, the deferral list) is computed identically
+// by every lane of the group; only the group's first lane writes or issues
+// atomics.
+struct Lanes {
+    int L;         // lanes per ray (1 = the one-lane march)
+    int sub;       // this lane's index in its group
+    int base;      // wave lane of the group's first lane
+    bool leader;   // sub == 0
+};
 
-__global__ __launch_bounds__(kPassBlock) void k_render_pass(RenderArgs a)
+__device__ __forceinline__ uint64_t group_ballot(bool p, const Lanes& g)
+{
+    const unsigned long long m = __ballot(p);
+    return g.L == 64 ? (uint64_t)m : ((uint64_t)m >> g.base) & ((1ull << g.L) - 1ull);
+}
+
+// march_node_skip's sample sequence, crossing test and secant, marched by
+// the L lanes of a group.  Positions are accumulated serially by every lane
+// (X_k = X_(k-1) + step, as the reference does); lane q keeps X_k for its k.
+// A sample is evaluated when its sub-block proves no side; the side of
+// sample k is then the proven side or the side of the evaluated value, and
+// the crossing is the first k >= 1 whose side is not the strict side of
+// sample k-1.  `err` gets exactly the flags of the samples the one-lane march
+// evaluates (every unproven sample up to the crossing, the crossing sample,
+// and the sample before it when that was skipped); `nsamples` gets their
+// count (group-uniform).  Results are group-uniform.
+__device__ __forceinline__ bool march_node_wide(const Ray& r, int nx, int ny, int nz, int side,
+                                                const float* __restrict__ slot, int S, int bx, int by, int bz,
+                                                const SubGrid& sg, float iso, float& Xx, float& Xy, float& Xz,
+                                                uint32_t& err, uint32_t& nsamples, const Lanes& g)
+{
+    float tnear, tfar;
+    if (!slab(r, nx, ny, nz, side, tnear, tfar)) return false;
+    float R0 = r.ox + tnear * r.dx, R1 = r.oy + tnear * r.dy, R2 = r.oz + tnear * r.dz;
+    const float s0 = 0.5f * r.dx, s1 = 0.5f * r.dy, s2 = 0.5f * r.dz;
+    int maxStep = cvt_rzi(ceilf((tfar - tnear) * 2.0f));
+    if (maxStep > 4 * side + 8) maxStep = 4 * side + 8;
+    if (maxStep < 0) return false;
+    SideCache sc;
+    // the previous chunk's last sample: side, value (if evaluated), position
+    int c_side = SIDE_NONE;
+    float c_val = 0.0f;
+    bool c_known = false;
+    float C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
+    uint32_t cnt = 0;
+    const int last = g.base + g.L - 1;
+    for (int k0 = 0; k0 <= maxStep; k0 += g.L) {
+        float P0 = R0, P1 = R1, P2 = R2;
+        for (int j = 0; j < g.L; ++j) {
+            if (j == g.sub) { P0 = R0; P1 = R1; P2 = R2; }
+            R0 = R0 + s0; R1 = R1 + s1; R2 = R2 + s2;
+        }
+        const int k = k0 + g.sub;
+        const bool valid = k <= maxStep;
+        const int ps = valid ? proven_side(sg, iso, P0, P1, P2, sc) : (int)SIDE_NONE;
+        const bool ev = valid && ps == SIDE_NONE;
+        uint32_t e = 0;
+        const float v = ev ? sample(slot, S, bx, by, bz, P0, P1, P2, e) : 0.0f;
+        const int sd = ps != SIDE_NONE ? ps : (ev ? side_of(iso, v) : (int)SIDE_NONE);
+        const int up = __shfl(sd, (g.base + g.sub + 63) & 63);
+        const int prev = g.sub == 0 ? c_side : up;
+        const bool cross = valid && k >= 1 && !(sd != SIDE_NONE && sd == prev);
+        const uint64_t cm = group_ballot(cross, g);
+        const uint64_t evm = group_ballot(ev, g);
+        if (cm == 0) {
+            cnt += (uint32_t)__popcll(evm);
+            err |= e;
+            c_side = __shfl(sd, last);
+            c_val = __shfl(v, last);
+            c_known = __shfl((int)ev, last) != 0;
+            C0 = __shfl(P0, last);
+            C1 = __shfl(P1, last);
+            C2 = __shfl(P2, last);
+            continue;
+        }
+        const int js = __ffsll((unsigned long long)cm) - 1;   // the crossing is sample k0 + js
+        if (g.sub <= js) err |= e;
+        cnt += (uint32_t)__popcll(evm & ((1ull << js) - 1ull)) + 1u;   // unproven before it, and the crossing sample
+        // the crossing sample's value
+        float sv = v;
+        if (g.sub == js && !ev) {
+            uint32_t e2 = 0;
+            sv = sample(slot, S, bx, by, bz, P0, P1, P2, e2);
+            err |= e2;
+        }
+        const float sig = __shfl(sv, g.base + js);
+        const float X0 = __shfl(P0, g.base + js), X1 = __shfl(P1, g.base + js), X2 = __shfl(P2, g.base + js);
+        // the previous sample's value and position
+        float ant, F0, F1, F2;
+        if (js > 0) {
+            const bool known = (evm >> (js - 1)) & 1ull;
+            float av = v;
+            if (g.sub == js - 1 && !known) {
+                uint32_t e3 = 0;
+                av = sample(slot, S, bx, by, bz, P0, P1, P2, e3);
+                err |= e3;
+            }
+            ant = __shfl(av, g.base + js - 1);
+            F0 = __shfl(P0, g.base + js - 1);
+            F1 = __shfl(P1, g.base + js - 1);
+            F2 = __shfl(P2, g.base + js - 1);
+            cnt += known ? 0u : 1u;
+        } else {
+            F0 = C0; F1 = C1; F2 = C2;
+            if (c_known) {
+                ant = c_val;
+            } else {
+                uint32_t e4 = 0;
+                ant = sample(slot, S, bx, by, bz, F0, F1, F2, e4);
+                err |= e4;
+                cnt += 1u;
+            }
+        }
+        const float a = (iso - ant) / (sig - ant);
+        Xx = F0 * (1.0f - a) + X0 * a;
+        Xy = F1 * (1.0f - a) + X1 * a;
+        Xz = F2 * (1.0f - a) + X2 * a;
+        nsamples += cnt;
+        return true;
+    }
+    nsamples += cnt;
+    return false;
+}
+
+// shade() with the six central-difference samples taken by six lanes of the
+// group (L >= 8); the arithmetic after the samples is shade()'s.
+__device__ __forceinline__ void shade_wide(const float* __restrict__ slot, int S, int bx, int by, int bz, float Xx,
+                                           float Xy, float Xz, float lx, float ly, float lz, float color_scale,
+                                           float4& out, uint32_t& err, const Lanes& g)
+{
+    const int q = g.sub < 6 ? g.sub : 0;
+    const int ax = q >> 1;
+    const bool plus = (q & 1) != 0;
+    const float px = ax == 0 ? (plus ? Xx + 1.0f : Xx - 1.0f) : Xx;
+    const float py = ax == 1 ? (plus ? Xy + 1.0f : Xy - 1.0f) : Xy;
+    const float pz = ax == 2 ? (plus ? Xz + 1.0f : Xz - 1.0f) : Xz;
+    uint32_t e = 0;
+    const float sv = g.sub < 6 ? sample(slot, S, bx, by, bz, px, py, pz, e) : 0.0f;
+    err |= e;
+    const float v0 = __shfl(sv, g.base + 0), v1 = __shfl(sv, g.base + 1);
+    const float v2 = __shfl(sv, g.base + 2), v3 = __shfl(sv, g.base + 3);
+    const float v4 = __shfl(sv, g.base + 4), v5 = __shfl(sv, g.base + 5);
+    shade_normal((v0 - v1) / 2.0f, (v2 - v3) / 2.0f, (v4 - v5) / 2.0f, Xx, Xy, Xz, lx, ly, lz, color_scale, out);
+}
+
+// visit_node for a group of lanes (kWide) or one lane.
+template <bool kWide>
+__device__ __forceinline__ bool visit_node_g(const Ray& r, const OctreeView& oct, const BrickView& bv,
+                                             const Shading& sh, uint64_t node, uint64_t brick, int slot,
+                                             uint8_t* slot_ref, uint8_t* brick_trace, float4& out, HitRecord& h,
+                                             uint32_t& err, uint32_t& nsamples, uint32_t& nnodes, const Lanes& g)
+{
+    if (!kWide)
+        return visit_node(r, oct, bv, sh, node, brick, slot, slot_ref, brick_trace, out, h, err, nsamples, nnodes);
+    const int n = oct.n, levelO = oct.final_level;
+    const int side = 1 << (n - levelO);
+    if (g.leader) {
+        if (slot_ref) slot_ref[slot] = 1;
+        if (brick_trace) brick_trace[brick - bv.brick_base] = 1;
+    }
+    int nx, ny, nz, bx, by, bz;
+    node_min(node, levelO, n, nx, ny, nz);
+    node_min(brick, bv.level_c, n, bx, by, bz);
+    bx -= bv.inc; by -= bv.inc; bz -= bv.inc;
+    const float* sp = bv.pool + (size_t)slot * bv.slot_elems;
+    float Xx, Xy, Xz;
+    nnodes += 1;
+    SubGrid sg{bv.submm ? bv.submm + (size_t)slot * bv.G * bv.G * bv.G : nullptr, bv.G, bx + bv.inc,
+               by + bv.inc, bz + bv.inc};
+    if (!march_node_wide(r, nx, ny, nz, side, sp, bv.S, bx, by, bz, sg, sh.iso, Xx, Xy, Xz, err, nsamples, g))
+        return false;
+    if (g.L >= 8)
+        shade_wide(sp, bv.S, bx, by, bz, Xx, Xy, Xz, sh.lx, sh.ly, sh.lz, sh.color_scale, out, err, g);
+    else
+        shade(sp, bv.S, bx, by, bz, Xx, Xy, Xz, sh.lx, sh.ly, sh.lz, sh.color_scale, out, err);
+    nsamples += 6;
+    h.node = node;
+    h.brick = brick;
+    h.voxel[0] = floor_to_int(Xx);
+    h.voxel[1] = floor_to_int(Xy);
+    h.voxel[2] = floor_to_int(Xz);
+    float dx = Xx - r.ox, dy = Xy - r.oy, dz = Xz - r.oz;
+    h.depth = sqrtf(dx * dx + dy * dy + dz * dz);
+    return true;
+}
+
+// One ray of a render pass (ray `id`, when `active`; group-uniform).
+template <bool kWide>
+__device__ __forceinline__ void pass_ray(const RenderArgs& a, uint32_t id, bool active, const Lanes& g)
 {
     const int W = a.cam.W, H = a.cam.H;
     const uint32_t R = (uint32_t)W * (uint32_t)H;
-    uint32_t id;
-    bool active = true;
-    if (a.in_list) {
-        uint32_t idx = blockIdx.x * kPassBlock + threadIdx.x;
-        active = idx < a.in_count;
-        id = active ? a.in_list[idx] : 0;
-    } else {
-        // Workgroups take tiles in order and are dealt round-robin to the 8
-        // XCDs, so neighbouring tiles land on different XCDs and the
-        // expensive rays, which cluster in the image, spread over all CUs.
-        // (Giving each XCD a contiguous band of tiles, for L2 reuse, measured
-        // 8% slower on the 1024^3 workload: the bands holding the surface
-        // became the tail.)
-        const int tx8 = (W + 7) >> 3, ty8 = (H + 7) >> 3;
-        const int t = (int)blockIdx.x * (kPassBlock / 64) + (int)(threadIdx.x >> 6);
-        const int lane = threadIdx.x & 63;
-        active = t < tx8 * ty8;
-        const int i = (active ? t / tx8 : 0) * 8 + (lane >> 3);
-        const int j = (active ? t % tx8 : 0) * 8 + (lane & 7);
-        active = active && i < H && j < W;
-        id = active ? (uint32_t)(i * W + j) : 0;
-    }
-
     uint32_t err = 0, nsamples = 0, nnodes = 0;
     bool park = false;
 
@@ -244,11 +415,13 @@ __global__ __launch_bounds__(kPassBlock) void k_render_pass(RenderArgs a)
             const int slot = bv.brick_slot[bidx];
             if (slot < 0) {
                 if (deferred) {   // still absent: keep it and the ones after it
-                    for (int q = k; q < nprev; ++q)
-                        a.ray_defer[(size_t)(q - k) * R + id] = a.ray_defer[(size_t)q * R + id];
+                    if (g.leader) {
+                        for (int q = k; q < nprev; ++q)
+                            a.ray_defer[(size_t)(q - k) * R + id] = a.ray_defer[(size_t)q * R + id];
+                        request_brick(a, bidx);
+                    }
                     nd = nprev - k;
-                    request_brick(a, bidx);
-                    a.ray_phase[id] = (uint8_t)(tent | nd);   // resume node and tentative result kept
+                    if (g.leader) a.ray_phase[id] = (uint8_t)(tent | nd);   // resume node and tentative result kept
                     park = true;
                     st = kBlocked;
                     break;
@@ -257,11 +430,13 @@ __global__ __launch_bounds__(kPassBlock) void k_render_pass(RenderArgs a)
                     st = WALK_ABSENT;
                     break;
                 }
-                a.ray_defer[(size_t)nd * R + id] = cur;
+                if (g.leader) {
+                    a.ray_defer[(size_t)nd * R + id] = cur;
+                    request_brick(a, bidx);
+                }
                 nd += 1;
-                request_brick(a, bidx);
-            } else if (visit_node(r, a.oct, bv, sh, cur, brick, slot, a.slot_ref, a.brick_trace, c, h, err,
-                                  nsamples, nnodes)) {
+            } else if (visit_node_g<kWide>(r, a.oct, bv, sh, cur, brick, slot, a.slot_ref, a.brick_trace, c, h,
+                                           err, nsamples, nnodes, g)) {
                 st = WALK_HIT;   // final when found in a deferred node (nd == 0 there)
                 break;
             }
@@ -271,27 +446,30 @@ __global__ __launch_bounds__(kPassBlock) void k_render_pass(RenderArgs a)
                 adv = true;
             }
         }
-        if (st == WALK_ABSENT) {
-            request_brick(a, bidx);
-            a.ray_node[id] = node;
-            a.ray_phase[id] = (uint8_t)nd;
-            park = true;
-        } else if (st == WALK_HIT || st == WALK_MISS) {
-            a.rgba[id] = c;
-            a.hits[id] = h;
-            if (nd > 0) {
-                a.ray_phase[id] = (uint8_t)(PHASE_TENT | nd);
+        if (g.leader) {
+            if (st == WALK_ABSENT) {
+                request_brick(a, bidx);
+                a.ray_node[id] = node;
+                a.ray_phase[id] = (uint8_t)nd;
                 park = true;
+            } else if (st == WALK_HIT || st == WALK_MISS) {
+                a.rgba[id] = c;
+                a.hits[id] = h;
+                if (nd > 0) {
+                    a.ray_phase[id] = (uint8_t)(PHASE_TENT | nd);
+                    park = true;
+                }
             }
         }
     }
 
     // parked rays: one atomic per wave
+    park = park && g.leader;
     uint32_t pslot = wave_atomic_add(&a.ctr->parked_count, park);
     if (park) a.out_list[pslot] = id;
     if (err) atomicOr(&a.ctr->err, err);
     if (a.collect_stats) {
-        unsigned long long s = nsamples, nn = nnodes;
+        unsigned long long s = g.leader ? nsamples : 0u, nn = g.leader ? nnodes : 0u;
         for (int off = 32; off > 0; off >>= 1) {
             s += __shfl_xor(s, off);
             nn += __shfl_xor(nn, off);
@@ -301,6 +479,52 @@ __global__ __launch_bounds__(kPassBlock) void k_render_pass(RenderArgs a)
             atomicAdd(&a.ctr->nodes, nn);
         }
     }
+}
+
+// Render-pass workgroup: kPassBlock/64 waves, each rendering one 8x8 pixel
+// tile (the tiles of a workgroup are consecutive in row-major tile order).
+constexpr int kPassBlock = 64;
+
+__global__ __launch_bounds__(kPassBlock) void k_render_pass(RenderArgs a)
+{
+    const int W = a.cam.W, H = a.cam.H;
+    uint32_t id;
+    bool active = true;
+    if (a.in_list) {
+        uint32_t idx = blockIdx.x * kPassBlock + threadIdx.x;
+        active = idx < a.in_count;
+        id = active ? a.in_list[idx] : 0;
+    } else {
+        // Workgroups take tiles in order and are dealt round-robin to the 8
+        // XCDs, so neighbouring tiles land on different XCDs and the
+        // expensive rays, which cluster in the image, spread over all CUs.
+        // (Giving each XCD a contiguous band of tiles, for L2 reuse, measured
+        // 8% slower on the 1024^3 workload: the bands holding the surface
+        // became the tail.)
+        const int tx8 = (W + 7) >> 3, ty8 = (H + 7) >> 3;
+        const int t = (int)blockIdx.x * (kPassBlock / 64) + (int)(threadIdx.x >> 6);
+        const int lane = threadIdx.x & 63;
+        active = t < tx8 * ty8;
+        const int i = (active ? t / tx8 : 0) * 8 + (lane >> 3);
+        const int j = (active ? t % tx8 : 0) * 8 + (lane & 7);
+        active = active && i < H && j < W;
+        id = active ? (uint32_t)(i * W + j) : 0;
+    }
+    const Lanes g{1, 0, (int)(threadIdx.x & 63), true};
+    pass_ray<false>(a, id, active, g);
+}
+
+// A later pass with 2^lg lanes per ray: ray in_list[t >> lg] for thread t.
+__global__ __launch_bounds__(kPassBlock) void k_render_wide(RenderArgs a, int lg)
+{
+    const uint32_t t = blockIdx.x * kPassBlock + threadIdx.x;
+    const uint32_t idx = t >> lg;
+    const bool active = idx < a.in_count;
+    const uint32_t id = active ? a.in_list[idx] : 0;
+    const int L = 1 << lg;
+    const int lane = (int)(threadIdx.x & 63);
+    const Lanes g{L, lane & (L - 1), lane & ~(L - 1), (lane & (L - 1)) == 0};
+    pass_ray<true>(a, id, active, g);
 }
 
 // ------------------------------------------------- persistent render pass
@@ -547,6 +771,16 @@ hipError_t launch_render_pass(const RenderArgs& a, hipStream_t s)
         grid = (tiles + per - 1) / per;
     }
     hipLaunchKernelGGL(k_render_pass, dim3(grid), dim3(kPassBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_render_wide(const RenderArgs& a, int lg, hipStream_t s)
+{
+    if (!a.in_list || lg < 0 || lg > 6) return hipErrorInvalidValue;
+    if (a.in_count == 0) return hipSuccess;
+    const uint64_t threads = (uint64_t)a.in_count << lg;
+    const unsigned grid = (unsigned)((threads + kPassBlock - 1) / kPassBlock);
+    hipLaunchKernelGGL(k_render_wide, dim3(grid), dim3(kPassBlock), 0, s, a, lg);
     return hipGetLastError();
 }
 
@@ -932,6 +1166,26 @@ hipError_t launch_scatter_i32(int32_t* table, const int64_t* idx, const int32_t*
 {
     if (count <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_scatter_i32, dim3((count + 255) / 256), dim3(256), 0, s, table, idx, val, count);
+    return hipGetLastError();
+}
+
+__global__ void k_finish_service(int32_t* table, const int64_t* idx, const int32_t* val, int count, PassReset r)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t q = i; q < count; q += stride) table[idx[q]] = val[q];
+    for (int64_t q = i; q < r.nwords; q += stride) r.req_bits[q] = 0u;
+    for (int64_t q = i; q < r.nref; q += stride) r.ref[q] = 0;
+    if (i < r.nctr) r.ctr[i] = 0u;
+}
+
+hipError_t launch_finish_service(int32_t* table, const int64_t* idx, const int32_t* val, int count,
+                                 const PassReset& r, hipStream_t s)
+{
+    int64_t work = std::max<int64_t>(std::max<int64_t>(count, r.nwords), std::max<int64_t>(r.nref, r.nctr));
+    if (work <= 0) return hipSuccess;
+    int blocks = (int)std::min<int64_t>((work + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_finish_service, dim3(blocks), dim3(256), 0, s, table, idx, val, count, r);
     return hipGetLastError();
 }
 

@@ -70,20 +70,24 @@ int check_camera(const vt_camera* cam)
 }  // namespace
 
 // Fold the device reference bits (set by the render kernel on every brick
-// it marches) into the host clock.
-int fold_ref_bits(vt_context* ctx)
+// it marches) into the host clock.  With `copied`, the bytes were already
+// read back into h_ref after the last pass, and the service clears them on
+// the device; otherwise they are read and cleared here.
+int fold_ref_bits(vt_context* ctx, bool copied = false)
 {
     BrickCache& c = *ctx->cache;
     if (c.used == 0) return VT_OK;
-    HIP_TRY(hipMemcpyAsync(c.ref_tmp.data(), c.d_ref, (size_t)c.gpu_slots, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    for (int64_t s = 0; s < c.gpu_slots; ++s) c.ref[(size_t)s] |= c.ref_tmp[(size_t)s];
-    HIP_TRY(hipMemsetAsync(c.d_ref, 0, (size_t)c.gpu_slots, ctx->stream));
+    if (!copied) {
+        HIP_TRY(hipMemcpyAsync(c.h_ref.p, c.d_ref, (size_t)c.gpu_slots, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    for (int64_t s = 0; s < c.gpu_slots; ++s) c.ref[(size_t)s] |= c.h_ref.p[(size_t)s];
+    if (!copied) HIP_TRY(hipMemsetAsync(c.d_ref, 0, (size_t)c.gpu_slots, ctx->stream));
     return VT_OK;
 }
 
 int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t* misses, int32_t* slots_out,
-                const int64_t* victims = nullptr);
+                const int64_t* victims = nullptr, const PassReset* reset = nullptr);
 int copy_stream_idle(vt_context* ctx);
 
 // Starts the host tier once the brick geometry is known; with the octree
@@ -99,11 +103,15 @@ void host_tier_start(vt_context* ctx)
 
 // Load `count` requested bricks (indices relative to 8^levelC) into the
 // device pool as far as capacity allows.  Bricks loaded here are pinned
-// against eviction until the next call.
-int service_requests(vt_context* ctx, const uint32_t* reqs, int64_t count, vt_frame_stats* st)
+// against eviction until the next call.  Between two render passes
+// (`reset` given) the reference bytes were read back with the pass counters,
+// the work is only enqueued (the next pass follows on the same stream), and
+// the last kernel of the service also resets the per-pass state.
+int service_requests(vt_context* ctx, const uint32_t* reqs, int64_t count, vt_frame_stats* st,
+                     const PassReset* reset = nullptr)
 {
     BrickCache& c = *ctx->cache;
-    if (int rc = fold_ref_bits(ctx)) return rc;
+    if (int rc = fold_ref_bits(ctx, reset != nullptr)) return rc;
     std::vector<uint32_t> sorted;   // bricks still absent (a load on the copy stream may have placed some)
     sorted.reserve((size_t)count);
     for (int64_t i = 0; i < count; ++i)
@@ -114,7 +122,8 @@ int service_requests(vt_context* ctx, const uint32_t* reqs, int64_t count, vt_fr
     if (c.host && !c.host->full_mode) n_load = std::min<int64_t>(n_load, c.host->capacity);
     c.epoch++;
     int64_t misses = 0;
-    if (int rc = load_bricks(ctx, sorted.data(), n_load, &misses, nullptr)) return rc;
+    if (int rc = load_bricks(ctx, sorted.data(), n_load, &misses, nullptr, nullptr, reset)) return rc;
+    if (n_load <= 0 && reset) HIP_TRY(launch_finish_service(c.d_table, nullptr, nullptr, 0, *reset, ctx->stream));
     c.stat_missed += count;
     if (st) {
         st->bricks_uploaded += n_load;
@@ -124,17 +133,25 @@ int service_requests(vt_context* ctx, const uint32_t* reqs, int64_t count, vt_fr
 }
 
 // Place `n_load` non-resident bricks into victim slots (skipping slots
-// pinned in the current epoch), then fill them from the brick source.
+// pinned in the current epoch), then fill them from the brick source.  The
+// staging arrays go to the device in one copy.  Without `reset` the call
+// waits for the loads; with it (between two render passes) the loads are
+// only enqueued, their last kernel also resets the per-pass state, and the
+// staging is next rewritten after the following pass has been waited for.
 int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t* misses, int32_t* slots_out,
-                const int64_t* victims)
+                const int64_t* victims, const PassReset* reset)
 {
     BrickCache& c = *ctx->cache;
     if (n_load <= 0) return VT_OK;
     if (int rc = copy_stream_idle(ctx)) return rc;   // evictions below must not hit a slot still being filled
-    if (c.h_src.ensure((size_t)n_load) != hipSuccess || c.h_slots.ensure((size_t)n_load) != hipSuccess ||
-        c.h_codes.ensure((size_t)n_load) != hipSuccess || c.h_tidx.ensure((size_t)(2 * n_load)) != hipSuccess ||
-        c.h_tval.ensure((size_t)(2 * n_load)) != hipSuccess)
-        return fail(VT_E_NOMEM, "pinned staging allocation failed");
+    const StageLayout L((size_t)n_load);
+    if (c.h_stage.ensure(L.bytes) != hipSuccess || c.d_stage.ensure(L.bytes) != hipSuccess)
+        return fail(VT_E_NOMEM, "staging allocation failed");
+    const float** h_src = reinterpret_cast<const float**>(c.h_stage.p + L.src);
+    int64_t* h_codes = reinterpret_cast<int64_t*>(c.h_stage.p + L.codes);
+    int64_t* h_tidx = reinterpret_cast<int64_t*>(c.h_stage.p + L.tidx);
+    int32_t* h_slots = reinterpret_cast<int32_t*>(c.h_stage.p + L.slots);
+    int32_t* h_tval = reinterpret_cast<int32_t*>(c.h_stage.p + L.tval);
     int64_t nt = 0;
     for (int64_t q = 0; q < n_load; ++q) {
         int64_t b = bricks[q];
@@ -143,49 +160,48 @@ int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t
         int64_t old = c.slot_brick[(size_t)slot];
         if (old >= 0) {
             c.table[(size_t)old] = -1;
-            c.h_tidx.p[nt] = old; c.h_tval.p[nt] = -1; nt++;
+            h_tidx[nt] = old; h_tval[nt] = -1; nt++;
             c.stat_evictions++;
         }
         c.slot_brick[(size_t)slot] = b;
         c.table[(size_t)b] = (int32_t)slot;
-        c.h_tidx.p[nt] = b; c.h_tval.p[nt] = (int32_t)slot; nt++;
+        h_tidx[nt] = b; h_tval[nt] = (int32_t)slot; nt++;
         c.pin[(size_t)slot] = c.epoch;
         c.ref[(size_t)slot] = 1;
-        c.h_slots.p[q] = (int32_t)slot;
-        c.h_codes.p[q] = b + (int64_t)(1ull << (3 * c.level_c));
+        h_slots[q] = (int32_t)slot;
+        h_codes[q] = b + (int64_t)(1ull << (3 * c.level_c));
         if (slots_out) slots_out[q] = (int32_t)slot;
+        h_src[q] = nullptr;
         if (c.host) {
             const float* src = nullptr;
             int rc = c.host->get(b, c.epoch, &src, misses);
             if (rc != VT_OK) return rc;
-            c.h_src.p[q] = src;
+            h_src[q] = src;
         }
     }
-    if (c.d_slots.ensure((size_t)n_load) != hipSuccess || c.d_tidx.ensure((size_t)nt) != hipSuccess ||
-        c.d_tval.ensure((size_t)nt) != hipSuccess)
-        return fail(VT_E_NOMEM, "device staging allocation failed");
-    HIP_TRY(hipMemcpyAsync(c.d_slots.p, c.h_slots.p, (size_t)n_load * 4, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(c.d_tidx.p, c.h_tidx.p, (size_t)nt * 8, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(c.d_tval.p, c.h_tval.p, (size_t)nt * 4, hipMemcpyHostToDevice, ctx->stream));
+    // everything up to the last used table value, in one copy
+    HIP_TRY(hipMemcpyAsync(c.d_stage.p, c.h_stage.p, L.tval + (size_t)nt * 4, hipMemcpyHostToDevice, ctx->stream));
+    const float* const* d_src = reinterpret_cast<const float* const*>(c.d_stage.p + L.src);
+    const int64_t* d_codes = reinterpret_cast<const int64_t*>(c.d_stage.p + L.codes);
+    const int64_t* d_tidx = reinterpret_cast<const int64_t*>(c.d_stage.p + L.tidx);
+    const int32_t* d_slots = reinterpret_cast<const int32_t*>(c.d_stage.p + L.slots);
+    const int32_t* d_tval = reinterpret_cast<const int32_t*>(c.d_stage.p + L.tval);
     if (c.host) {
-        if (c.d_src.ensure((size_t)n_load) != hipSuccess) return fail(VT_E_NOMEM, "staging");
-        HIP_TRY(hipMemcpyAsync(c.d_src.p, c.h_src.p, (size_t)n_load * sizeof(float*), hipMemcpyHostToDevice,
-                               ctx->stream));
-        HIP_TRY(launch_copy_bricks(c.d_src.p, c.d_slots.p, (int)n_load, c.brick_elems, c.slot_elems, c.d_pool,
-                                   ctx->stream));
+        HIP_TRY(launch_copy_bricks(d_src, d_slots, (int)n_load, c.brick_elems, c.slot_elems, c.d_pool, ctx->stream));
     } else {
-        if (c.d_codes.ensure((size_t)n_load) != hipSuccess) return fail(VT_E_NOMEM, "staging");
-        HIP_TRY(hipMemcpyAsync(c.d_codes.p, c.h_codes.p, (size_t)n_load * 8, hipMemcpyHostToDevice, ctx->stream));
-        HIP_TRY(launch_bricks_from_volume(c.d_vol, c.vol_dims[0], c.vol_dims[1], c.vol_dims[2], c.d_codes.p,
-                                          c.d_slots.p, (int)n_load, c.level_c, c.n, c.cube, c.inc, c.slot_elems, c.d_pool,
+        HIP_TRY(launch_bricks_from_volume(c.d_vol, c.vol_dims[0], c.vol_dims[1], c.vol_dims[2], d_codes, d_slots,
+                                          (int)n_load, c.level_c, c.n, c.cube, c.inc, c.slot_elems, c.d_pool,
                                           ctx->stream));
     }
     if (c.d_submm)
-        HIP_TRY(launch_submm(c.d_pool, c.d_slots.p, (int)n_load, c.S, c.inc, c.G, c.slot_elems, c.d_submm,
-                             ctx->stream));
-    HIP_TRY(launch_scatter_i32(c.d_table, c.d_tidx.p, c.d_tval.p, (int)nt, ctx->stream));
-    // the staging buffers are reused by the next call: finish the copies
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(launch_submm(c.d_pool, d_slots, (int)n_load, c.S, c.inc, c.G, c.slot_elems, c.d_submm, ctx->stream));
+    if (reset) {
+        HIP_TRY(launch_finish_service(c.d_table, d_tidx, d_tval, (int)nt, *reset, ctx->stream));
+    } else {
+        HIP_TRY(launch_scatter_i32(c.d_table, d_tidx, d_tval, (int)nt, ctx->stream));
+        // the staging buffers are reused by the next call: finish the copies
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
     c.stat_uploads += n_load;
     return VT_OK;
 }
@@ -520,7 +536,7 @@ int vt_cache_create(vt_context* ctx, int32_t levelC, int32_t cubeDim, int32_t cu
     c->table.assign((size_t)c->nbricks, -1);
     c->slot_brick.assign((size_t)c->gpu_slots, -1);
     c->ref.assign((size_t)c->gpu_slots, 0);
-    c->ref_tmp.assign((size_t)c->gpu_slots, 0);
+    if (c->h_ref.ensure((size_t)c->gpu_slots) != hipSuccess) return fail(VT_E_NOMEM, "pinned reference bytes");
     c->pin.assign((size_t)c->gpu_slots, 0);
     ctx->cache = std::move(c);
     // request structures sized to the brick count
@@ -683,6 +699,14 @@ int vt_set_march_options(vt_context* ctx, int skip_empty, int max_defer, int max
     return VT_OK;
 }
 
+int vt_set_wide_march(vt_context* ctx, int64_t lane_budget)
+{
+    if (!ctx) return fail(VT_E_ARG, "ctx is NULL");
+    if (lane_budget < -6) return fail(VT_E_ARG, "lane_budget must be >= -6");
+    ctx->wide_lanes = lane_budget;
+    return VT_OK;
+}
+
 int vt_get_brick_trace(vt_context* ctx, uint8_t* out, int64_t n)
 {
     if (!ctx || !out) return fail(VT_E_ARG, "NULL argument");
@@ -775,12 +799,22 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
     for (int pass = 0;; ++pass) {
         if (ctx->persistent) HIP_TRY(hipMemsetAsync(ctx->d_next.p, 0, 8 * sizeof(uint32_t), ctx->stream));
         HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+        // later passes: 2^lg lanes per parked ray within the lane budget
+        int lg = 0;
+        if (a.in_list && ctx->wide_lanes > 0)
+            while (lg < 6 && ((int64_t)a.in_count << (lg + 1)) <= ctx->wide_lanes) ++lg;
+        else if (a.in_list && ctx->wide_lanes < 0)
+            lg = (int)-ctx->wide_lanes;
         if (ctx->persistent)
             HIP_TRY(launch_render_persistent(a, ctx->persistent_wgs, ctx->stream));
+        else if (lg > 0)
+            HIP_TRY(launch_render_wide(a, lg, ctx->stream));
         else
             HIP_TRY(launch_render_pass(a, ctx->stream));
         HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
         HIP_TRY(hipMemcpyAsync(ctx->h_ctr, ctx->d_ctr, sizeof(FrameCounters), hipMemcpyDeviceToHost, ctx->stream));
+        // the reference bytes come back under the same wait, for the service
+        HIP_TRY(hipMemcpyAsync(c.h_ref.p, c.d_ref, (size_t)c.gpu_slots, hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         float ms = 0.0f;
         HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -802,13 +836,17 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
         if (parked == 0) break;
         st.rays_parked += parked;
         double t0 = now_ms();
+        // the service's last kernel clears the request bitmap, the request
+        // and parked counts and the reference bytes for the next pass
+        const PassReset reset{ctx->req_bits.p, (int64_t)ctx->req_bits.n, &ctx->d_ctr->req_count, 2, c.d_ref,
+                              c.gpu_slots};
         if (!left.empty()) {
-            int rc = service_requests(ctx, left.data(), (int64_t)left.size(), &st);
+            int rc = service_requests(ctx, left.data(), (int64_t)left.size(), &st, &reset);
             if (rc != VT_OK) return rc;
+        } else {
+            HIP_TRY(launch_finish_service(c.d_table, nullptr, nullptr, 0, reset, ctx->stream));
         }
         ms_service += now_ms() - t0;
-        HIP_TRY(hipMemsetAsync(ctx->req_bits.p, 0, ctx->req_bits.n * 4, ctx->stream));
-        HIP_TRY(hipMemsetAsync(ctx->d_ctr, 0, 8, ctx->stream));   // req_count, parked_count
         a.in_list = ctx->list[cur].p;
         a.in_count = parked;
         a.max_defer = ctx->max_defer_late;   // few rays remain: let them walk further per pass
