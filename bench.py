@@ -44,14 +44,16 @@ METRIC = "rays/sec + p50/p99 frame ms on 1 MI355X, prefetch vs plain, identical 
 HBM_PEAK_GBS = 8000.0
 
 
-# rocprofv3 summaries of k_render_pass for the workloads: FETCH_SIZE + WRITE_SIZE per
+# rocprofv3 summaries of the render passes (k_render_pass, one lane per ray, and
+# k_render_wide, 2^k lanes per ray) for the workloads: FETCH_SIZE + WRITE_SIZE per
 # dispatch, each scaled by what the same counter reads for a 256 MiB copy
-PROFILES = {"c1": ROOT / "profiles" / "k_render_pass_256cubed_blobs.md",
-            "c2": ROOT / "profiles" / "k_render_pass_1024cubed_fbm_orbit.md"}
+PROFILES = {"c1": ROOT / "profiles" / "render_passes_256cubed_blobs.md",
+            "c2": ROOT / "profiles" / "render_passes_1024cubed_fbm_orbit.md"}
+RENDER_KERNELS = "k_render_pass+k_render_wide"
 
 
 def measured_traffic(name: str):
-    """HBM bytes per k_render_pass launch from the committed rocprof summary, or None."""
+    """HBM bytes per render-pass launch from the committed rocprof summary, or None."""
     import re
     path = PROFILES.get(name)
     if path is None or not path.exists():
@@ -327,8 +329,8 @@ def run_c3(ctx, args):
         ctx.cache_attach_hdf5(h5, "value")
         ctx.set_march_options(skip_empty=True, max_defer=args.max_defer, persistent=bool(args.persistent),
                           max_defer_late=None if args.max_defer_late < 0 else args.max_defer_late)
-        if args.wide_lanes is not None:
-            ctx.set_wide_march(args.wide_lanes)
+        if args.wide_lanes is not None or args.wide_first is not None:
+            ctx.set_wide_march(*wide_budgets(args))
         t0 = time.perf_counter()
         ctx.cache_prefetch_host(wait=True)
         log(f"c3 {form}: host tier filled from hdf5 in {time.perf_counter() - t0:.2f}s, "
@@ -464,6 +466,12 @@ def parity_sample(w, ctx, cam, oracle_objs, nrays: int, seed: int):
             "hit_fraction": float((hc["node"] != 0).mean())}
 
 
+def wide_budgets(args) -> tuple[int, int]:
+    """Lane budgets (later passes, first pass); unset ones keep the library defaults."""
+    return (1 << 18 if args.wide_lanes is None else args.wide_lanes,
+            1 << 21 if args.wide_first is None else args.wide_first)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -483,6 +491,8 @@ def main() -> int:
     ap.add_argument("--max-defer-late", type=int, default=-1, help="max-defer of the passes after the first (-1: same)")
     ap.add_argument("--wide-lanes", type=int, default=None,
                     help="lane budget of the wide march in later passes (0: off; default: the library's)")
+    ap.add_argument("--wide-first", type=int, default=None,
+                    help="lane budget of the wide march in the first pass (0: off; default: the library's)")
     ap.add_argument("--max-defer", type=int, default=2,
                     help="absent-brick nodes a ray may walk past per pass (0 = park at the first)")
     ap.add_argument("--prefetch-min-prob", type=float, default=0.8)
@@ -527,8 +537,8 @@ def main() -> int:
     w = setup_c2(ctx, args) if args.config == "c2" else setup_c1(ctx, args)
     ctx.set_march_options(skip_empty=True, max_defer=args.max_defer, persistent=bool(args.persistent),
                           max_defer_late=None if args.max_defer_late < 0 else args.max_defer_late)
-    if args.wide_lanes is not None:
-        ctx.set_wide_march(args.wide_lanes)
+    if args.wide_lanes is not None or args.wide_first is not None:
+        ctx.set_wide_march(*wide_budgets(args))
     cams = [w.cameras(k) for k in range(args.warmup + args.steps)]
     R = w.W * w.H
     d_rgba, d_hits = ctx.alloc(R * 16), ctx.alloc(R * 32)
@@ -567,7 +577,7 @@ def main() -> int:
     achieved_gbs = sum(B) / (sum(kernel_ms) * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved_gbs / HBM_PEAK_GBS, 5), "traffic": measured_traffic(w.name),
-                "kernel": "k_render_pass", "avg_launch_ms": round(sum(kernel_ms) / launches, 4),
+                "kernel": RENDER_KERNELS, "avg_launch_ms": round(sum(kernel_ms) / launches, 4),
                 "bytes_per_launch": int(sum(B) / launches),
                 "bricks_marched_per_frame": int(np.mean(U)), "brick_bytes": Sb,
                 "samples_per_frame": int(np.mean(samples))}
@@ -583,7 +593,7 @@ def main() -> int:
         "kernel_ms_per_frame": round(float(np.mean(kernel_ms)), 4),
         "passes_per_frame": float(np.mean(passes)), "bricks_uploaded_per_frame": float(np.mean(uploads)),
         "rays_parked_per_frame": float(np.mean(parked)), "max_defer": args.max_defer,
-        "persistent": args.persistent, "max_defer_late": args.max_defer_late, "wide_lanes": args.wide_lanes,
+        "persistent": args.persistent, "max_defer_late": args.max_defer_late, "wide_lanes": args.wide_lanes, "wide_first": args.wide_first,
         "bricks_marched_per_frame": float(np.mean(U)),
         "device_miss_rate": round(float(sum(uploads)) / max(1, sum(U)), 5),
         "roofline": roofline,

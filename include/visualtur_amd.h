@@ -192,9 +192,11 @@ int vt_set_march_options(vt_context *ctx, int skip_empty, int max_defer, int max
  * setting): each ray gets 2^k lanes (k <= 6) that march its nodes together,
  * the largest 2^k with rays x 2^k <= lane_budget (default 262144, the lanes
  * of about one wave per SIMD); 0 keeps one lane per ray in
- * every pass; -k (k in 1..6) gives every later pass 2^k lanes per ray.  Not
- * used with persistent=1. */
-int vt_set_wide_march(vt_context *ctx, int64_t lane_budget);
+ * every pass; -k (k in 1..6) gives every later pass 2^k lanes per ray.
+ * first_pass_budget does the same for the first pass, which carries every
+ * ray of the frame (default 2097152: 8 lanes per ray at 512x512, one lane
+ * per ray from 1920x1080 on).  Not used with persistent=1. */
+int vt_set_wide_march(vt_context *ctx, int64_t lane_budget, int64_t first_pass_budget);
 /* Visited-brick trace of the last frame: out[i] = 1 if brick
  * (i + 8^levelC) was marched by at least one ray.  n = 8^levelC. */
 int vt_get_brick_trace(vt_context *ctx, uint8_t *out, int64_t n);

@@ -297,9 +297,9 @@ def test_persistent_pass_matches_tiled_pass(vt, vto, ctx, slots, defer):
 @pytest.mark.parametrize("slots,defer,skip", [(5, 0, True), (17, 2, True), (40, 4, True), (17, 2, False),
                                               (512, 2, True)])
 def test_wide_march_matches_one_lane(vt, vto, ctx, slots, defer, skip):
-    """The later passes with 2..64 lanes per ray (every fixed width, and the
-    lane budget's choice) give the bytes, pass count, requests, sample and
-    node counts of the one-lane pass."""
+    """Passes with 2..64 lanes per ray (the later passes at every fixed
+    width and by the lane budget, and the first pass too) give the bytes,
+    pass count, requests, sample and node counts of the one-lane passes."""
     vol = blobs_volume(64, 16, seed=91)
     sc = Scene(vto, vol, 0.3, 4, 3)
     sc.setup_gpu(vt, ctx, slots=slots)
@@ -307,10 +307,17 @@ def test_wide_march_matches_one_lane(vt, vto, ctx, slots, defer, skip):
     outs = {}
     ctx.set_frame_options(collect_stats=True)
     try:
-        for wide in (0, -1, -2, -3, -4, -5, -6, 4096):
+        for wide in (0, -1, -2, -3, -4, -5, -6, 4096, "first-2", "first-6", "first"):
             log(f"wide={wide} slots={slots} defer={defer} skip={skip}")
             ctx.set_march_options(skip_empty=skip, max_defer=defer)
-            ctx.set_wide_march(wide)
+            if wide == "first-2":     # 4 lanes per ray in every pass
+                ctx.set_wide_march(-2, -2)
+            elif wide == "first-6":   # one wave per ray in every pass
+                ctx.set_wide_march(-6, -6)
+            elif wide == "first":     # budgets: 2 lanes per ray in the first pass of 131x77
+                ctx.set_wide_march(1 << 18, 2 * 131 * 77)
+            else:
+                ctx.set_wide_march(wide, 0)
             ctx.cache_clear()
             rows = []
             for cam in cams:

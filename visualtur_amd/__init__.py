@@ -119,7 +119,7 @@ _SIGS = {
     "vt_set_frame_options": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
     "vt_get_brick_trace": (ctypes.c_int, [_P, _P, ctypes.c_int64]),
     "vt_set_march_options": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
-    "vt_set_wide_march": (ctypes.c_int, [_P, ctypes.c_int64]),
+    "vt_set_wide_march": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int64]),
     "vt_trace_nodes": (ctypes.c_int, [_P, ctypes.POINTER(Camera), ctypes.c_int32, ctypes.c_int32,
                                       ctypes.c_int32, _P, _P]),
     "vt_octree_build": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
@@ -392,11 +392,12 @@ class Context:
         late = max_defer if max_defer_late is None else max_defer_late
         self._c(lib().vt_set_march_options(self.handle, int(skip_empty), int(max_defer), int(late), int(persistent)))
 
-    def set_wide_march(self, lane_budget: int = 1 << 18) -> None:
+    def set_wide_march(self, lane_budget: int = 1 << 18, first_pass_budget: int = 1 << 21) -> None:
         """Later passes give each parked ray up to 64 lanes that march its
         nodes together, within rays x lanes <= lane_budget (0: off; -k:
-        2^k lanes in every later pass)."""
-        self._c(lib().vt_set_wide_march(self.handle, int(lane_budget)))
+        2^k lanes in every later pass); first_pass_budget likewise for the
+        first pass."""
+        self._c(lib().vt_set_wide_march(self.handle, int(lane_budget), int(first_pass_budget)))
 
     def brick_trace(self, level_c: int) -> np.ndarray:
         n = 1 << (3 * level_c)

@@ -450,6 +450,7 @@ struct vt_context {
     int max_defer = 2;    // absent-brick nodes a ray walks past in the first pass
     int max_defer_late = 2;   // ... and in later passes
     int64_t wide_lanes = 1 << 18;   // lane budget of the wide march in later passes (0: off, -k: 2^k lanes)
+    int64_t wide_first = 1 << 21;   // ... and in the first pass
     int persistent = 0;   // persistent render pass with lane refill (measured slower; option)
     int persistent_wgs = 0;
     DevBuf<uint32_t> d_next;   // its 8 work counters

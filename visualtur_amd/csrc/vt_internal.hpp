@@ -168,7 +168,7 @@ hipError_t launch_stage_raycast(const StageRaycastArgs& a, hipStream_t s);
 hipError_t launch_render_persistent(const RenderArgs& a, int max_workgroups, hipStream_t s);
 int render_persistent_workgroups(int device);   // resident workgroups of the persistent pass on `device`
 hipError_t launch_render_pass(const RenderArgs& a, hipStream_t s);
-// a later pass (in_list set) with 2^lg lanes per ray, lg in 0..6
+// a pass with 2^lg lanes per ray, lg in 0..6 (first pass: in_list null)
 hipError_t launch_render_wide(const RenderArgs& a, int lg, hipStream_t s);
 hipError_t launch_trace_nodes(const TraceArgs& a, hipStream_t s);
 // Assemble bricks with halo from a device volume [X][Y][Z] (zero outside).

@@ -514,15 +514,36 @@ __global__ __launch_bounds__(kPassBlock) void k_render_pass(RenderArgs a)
     pass_ray<false>(a, id, active, g);
 }
 
-// A later pass with 2^lg lanes per ray: ray in_list[t >> lg] for thread t.
+// A pass with 2^lg lanes per ray over groups of 64 rays: the first pass's
+// 8x8 tiles, or 64 consecutive entries of in_list.  A group takes 2^lg
+// one-wave workgroups, and they all go to one XCD: workgroup b is wave
+// (b/8) mod 2^lg of group ((b/8) >> lg)*8 + b mod 8 (workgroups are dealt
+// round-robin to the 8 XCDs), so the group's bricks are fetched into one
+// L2, and neighbouring groups still spread over all XCDs.
 __global__ __launch_bounds__(kPassBlock) void k_render_wide(RenderArgs a, int lg)
 {
-    const uint32_t t = blockIdx.x * kPassBlock + threadIdx.x;
-    const uint32_t idx = t >> lg;
-    const bool active = idx < a.in_count;
-    const uint32_t id = active ? a.in_list[idx] : 0;
-    const int L = 1 << lg;
     const int lane = (int)(threadIdx.x & 63);
+    const uint32_t b = blockIdx.x;
+    const uint32_t grp = ((b >> 3) >> lg) * 8u + (b & 7u);
+    const uint32_t w = (b >> 3) & ((1u << lg) - 1u);
+    const uint32_t p = (w << (6 - lg)) + ((uint32_t)lane >> lg);   // ray of the group, 0..63
+    const uint32_t idx = grp * 64u + p;
+    bool active;
+    uint32_t id;
+    if (a.in_list) {
+        active = idx < a.in_count;
+        id = active ? a.in_list[idx] : 0;
+    } else {
+        const int W = a.cam.W, H = a.cam.H;
+        const int tx8 = (W + 7) >> 3, ty8 = (H + 7) >> 3;
+        const int tile = (int)grp;
+        active = tile < tx8 * ty8;
+        const int i = (active ? tile / tx8 : 0) * 8 + (int)(p >> 3);
+        const int j = (active ? tile % tx8 : 0) * 8 + (int)(p & 7);
+        active = active && i < H && j < W;
+        id = active ? (uint32_t)(i * W + j) : 0;
+    }
+    const int L = 1 << lg;
     const Lanes g{L, lane & (L - 1), lane & ~(L - 1), (lane & (L - 1)) == 0};
     pass_ray<true>(a, id, active, g);
 }
@@ -776,10 +797,17 @@ hipError_t launch_render_pass(const RenderArgs& a, hipStream_t s)
 
 hipError_t launch_render_wide(const RenderArgs& a, int lg, hipStream_t s)
 {
-    if (!a.in_list || lg < 0 || lg > 6) return hipErrorInvalidValue;
-    if (a.in_count == 0) return hipSuccess;
-    const uint64_t threads = (uint64_t)a.in_count << lg;
-    const unsigned grid = (unsigned)((threads + kPassBlock - 1) / kPassBlock);
+    if (lg < 0 || lg > 6) return hipErrorInvalidValue;
+    uint64_t rays;
+    if (a.in_list) {
+        rays = a.in_count;
+    } else {
+        rays = 64ull * (uint64_t)((a.cam.W + 7) / 8) * (uint64_t)((a.cam.H + 7) / 8);
+    }
+    if (rays == 0) return hipSuccess;
+    // groups of 64 rays, padded to a multiple of 8 (one per XCD), 2^lg workgroups each
+    const uint64_t groups = (rays + 63) / 64;
+    const unsigned grid = (unsigned)(((groups + 7) / 8 * 8) << lg);
     hipLaunchKernelGGL(k_render_wide, dim3(grid), dim3(kPassBlock), 0, s, a, lg);
     return hipGetLastError();
 }

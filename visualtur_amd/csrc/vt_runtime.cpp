@@ -699,11 +699,12 @@ int vt_set_march_options(vt_context* ctx, int skip_empty, int max_defer, int max
     return VT_OK;
 }
 
-int vt_set_wide_march(vt_context* ctx, int64_t lane_budget)
+int vt_set_wide_march(vt_context* ctx, int64_t lane_budget, int64_t first_pass_budget)
 {
     if (!ctx) return fail(VT_E_ARG, "ctx is NULL");
-    if (lane_budget < -6) return fail(VT_E_ARG, "lane_budget must be >= -6");
+    if (lane_budget < -6 || first_pass_budget < -6) return fail(VT_E_ARG, "lane budgets must be >= -6");
     ctx->wide_lanes = lane_budget;
+    ctx->wide_first = first_pass_budget;
     return VT_OK;
 }
 
@@ -799,12 +800,14 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
     for (int pass = 0;; ++pass) {
         if (ctx->persistent) HIP_TRY(hipMemsetAsync(ctx->d_next.p, 0, 8 * sizeof(uint32_t), ctx->stream));
         HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-        // later passes: 2^lg lanes per parked ray within the lane budget
+        // 2^lg lanes per ray within the pass's lane budget
         int lg = 0;
-        if (a.in_list && ctx->wide_lanes > 0)
-            while (lg < 6 && ((int64_t)a.in_count << (lg + 1)) <= ctx->wide_lanes) ++lg;
-        else if (a.in_list && ctx->wide_lanes < 0)
-            lg = (int)-ctx->wide_lanes;
+        const int64_t budget = a.in_list ? ctx->wide_lanes : ctx->wide_first;
+        const int64_t rays = a.in_list ? (int64_t)a.in_count : (int64_t)R;
+        if (budget > 0)
+            while (lg < 6 && (rays << (lg + 1)) <= budget) ++lg;
+        else if (budget < 0)
+            lg = (int)-budget;
         if (ctx->persistent)
             HIP_TRY(launch_render_persistent(a, ctx->persistent_wgs, ctx->stream));
         else if (lg > 0)
