@@ -331,6 +331,8 @@ def run_c3(ctx, args):
                           max_defer_late=None if args.max_defer_late < 0 else args.max_defer_late)
         if args.wide_lanes is not None or args.wide_first is not None:
             ctx.set_wide_march(*wide_budgets(args))
+        if args.early_reserve is not None:
+            ctx.set_early_loads(args.early_reserve)
         t0 = time.perf_counter()
         ctx.cache_prefetch_host(wait=True)
         log(f"c3 {form}: host tier filled from hdf5 in {time.perf_counter() - t0:.2f}s, "
@@ -491,6 +493,8 @@ def main() -> int:
     ap.add_argument("--max-defer-late", type=int, default=-1, help="max-defer of the passes after the first (-1: same)")
     ap.add_argument("--wide-lanes", type=int, default=None,
                     help="lane budget of the wide march in later passes (0: off; default: the library's)")
+    ap.add_argument("--early-reserve", type=int, default=None,
+                    help="free slots for loads beside the first pass (0: off; default: the library's)")
     ap.add_argument("--wide-first", type=int, default=None,
                     help="lane budget of the wide march in the first pass (0: off; default: the library's)")
     ap.add_argument("--max-defer", type=int, default=2,
@@ -539,6 +543,8 @@ def main() -> int:
                           max_defer_late=None if args.max_defer_late < 0 else args.max_defer_late)
     if args.wide_lanes is not None or args.wide_first is not None:
         ctx.set_wide_march(*wide_budgets(args))
+    if args.early_reserve is not None:
+        ctx.set_early_loads(args.early_reserve)
     cams = [w.cameras(k) for k in range(args.warmup + args.steps)]
     R = w.W * w.H
     d_rgba, d_hits = ctx.alloc(R * 16), ctx.alloc(R * 32)
@@ -547,7 +553,7 @@ def main() -> int:
         ctx.render_frame(cams[k], w.prm, d_rgba, d_hits)
     ctx.synchronize()
 
-    frame_ms, kernel_ms, passes, uploads, parked = [], [], [], [], []
+    frame_ms, kernel_ms, passes, uploads, parked, early = [], [], [], [], [], []
     t0 = time.perf_counter()
     for k in range(args.warmup, args.warmup + args.steps):
         f0 = time.perf_counter()
@@ -557,6 +563,7 @@ def main() -> int:
         passes.append(st.passes)
         uploads.append(st.bricks_uploaded)
         parked.append(st.rays_parked)
+        early.append(st.bricks_early)
     ctx.synchronize()
     total_s = time.perf_counter() - t0
     rays_per_s = R * args.steps / total_s
@@ -592,8 +599,9 @@ def main() -> int:
         "p50_ms": round(float(np.percentile(ms, 50)), 4), "p99_ms": round(float(np.percentile(ms, 99)), 4),
         "kernel_ms_per_frame": round(float(np.mean(kernel_ms)), 4),
         "passes_per_frame": float(np.mean(passes)), "bricks_uploaded_per_frame": float(np.mean(uploads)),
+        "bricks_early_per_frame": float(np.mean(early)),
         "rays_parked_per_frame": float(np.mean(parked)), "max_defer": args.max_defer,
-        "persistent": args.persistent, "max_defer_late": args.max_defer_late, "wide_lanes": args.wide_lanes, "wide_first": args.wide_first,
+        "persistent": args.persistent, "max_defer_late": args.max_defer_late, "wide_lanes": args.wide_lanes, "wide_first": args.wide_first, "early_reserve": args.early_reserve,
         "bricks_marched_per_frame": float(np.mean(U)),
         "device_miss_rate": round(float(sum(uploads)) / max(1, sum(U)), 5),
         "roofline": roofline,

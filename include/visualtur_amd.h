@@ -85,6 +85,7 @@ typedef struct {
     double ms_total;            /* frame wall time */
     double ms_kernel;           /* device time of the render passes */
     double ms_service;          /* host time spent servicing misses */
+    int64_t bricks_early;       /* of bricks_uploaded: copied while the first pass ran */
 } vt_frame_stats;
 
 typedef struct {
@@ -197,6 +198,15 @@ int vt_set_march_options(vt_context *ctx, int skip_empty, int max_defer, int max
  * ray of the frame (default 2097152: 8 lanes per ray at 512x512, one lane
  * per ray from 1920x1080 on).  Not used with persistent=1. */
 int vt_set_wide_march(vt_context *ctx, int64_t lane_budget, int64_t first_pass_budget);
+/* Loads beside the first pass (no reference counterpart; results are
+ * identical for every setting): before the first pass of a frame, clock
+ * victims are evicted until min(reserve, bricks the previous frame's first
+ * pass requested) slots are free; while the pass runs, the bricks it
+ * requests are copied into free slots, and they become visible from the
+ * second pass on (page-table entries are written between passes only).
+ * Default 256 (about what the host link moves during a 1080p first pass);
+ * 0 loads only between passes. */
+int vt_set_early_loads(vt_context *ctx, int64_t reserve);
 /* Visited-brick trace of the last frame: out[i] = 1 if brick
  * (i + 8^levelC) was marched by at least one ray.  n = 8^levelC. */
 int vt_get_brick_trace(vt_context *ctx, uint8_t *out, int64_t n);

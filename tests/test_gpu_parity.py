@@ -241,6 +241,7 @@ def test_deferred_nodes_are_exact(vt, vto, ctx, level_o, slots):
     sc = Scene(vto, vol, 0.3, level_o, 3)
     sc.setup_gpu(vt, ctx, slots=slots)
     ctx.set_frame_options(collect_stats=True)
+    ctx.set_early_loads(0)   # pass counts are compared: no timing-dependent loads
     try:
         for theta in (10.0, 140.0):
             cam = orbit_camera(vt, 96, 80, (32, 32, 32), 110.0, theta, 25.0)
@@ -264,6 +265,7 @@ def test_deferred_nodes_are_exact(vt, vto, ctx, level_o, slots):
     finally:
         ctx.set_march_options()
         ctx.set_frame_options(False, False)
+        ctx.set_early_loads()
 
 
 @pytest.mark.parametrize("slots,defer", [(512, 2), (5, 0), (17, 2), (40, 4)])
@@ -275,6 +277,7 @@ def test_persistent_pass_matches_tiled_pass(vt, vto, ctx, slots, defer):
     sc.setup_gpu(vt, ctx, slots=slots)
     cams = [orbit_camera(vt, 131, 77, (32, 32, 32), 105.0, 30.0 + 50.0 * k, 15.0) for k in range(3)]
     outs = {}
+    ctx.set_early_loads(0)   # the persistent pass loads between passes only
     try:
         for persistent in (False, True):
             log(f"persistent={persistent} slots={slots} defer={defer}")
@@ -288,6 +291,7 @@ def test_persistent_pass_matches_tiled_pass(vt, vto, ctx, slots, defer):
             outs[persistent] = rows
     finally:
         ctx.set_march_options()
+        ctx.set_early_loads()
     assert outs[False] == outs[True]
     rc, hc = sc.render_cpu(vto, cams[-1])
     assert_same(np.frombuffer(outs[True][-1][0], vt.HIT_DTYPE), np.frombuffer(outs[True][-1][1], np.float32).reshape(-1, 4),
@@ -306,6 +310,7 @@ def test_wide_march_matches_one_lane(vt, vto, ctx, slots, defer, skip):
     cams = [orbit_camera(vt, 131, 77, (32, 32, 32), 105.0, 20.0 + 70.0 * k, 15.0) for k in range(3)]
     outs = {}
     ctx.set_frame_options(collect_stats=True)
+    ctx.set_early_loads(0)   # loads beside the first pass depend on timing, and so would the counts
     try:
         for wide in (0, -1, -2, -3, -4, -5, -6, 4096, "first-2", "first-6", "first"):
             log(f"wide={wide} slots={slots} defer={defer} skip={skip}")
@@ -329,6 +334,7 @@ def test_wide_march_matches_one_lane(vt, vto, ctx, slots, defer, skip):
         ctx.set_wide_march()
         ctx.set_march_options()
         ctx.set_frame_options(False, False)
+        ctx.set_early_loads()
     for wide, rows in outs.items():
         for k, (row, ref) in enumerate(zip(rows, outs[0])):
             assert row[:2] == ref[:2], (wide, k, "bytes")
@@ -338,6 +344,77 @@ def test_wide_march_matches_one_lane(vt, vto, ctx, slots, defer, skip):
     rc, hc = sc.render_cpu(vto, cams[-1])
     assert_same(np.frombuffer(outs[-6][-1][0], vt.HIT_DTYPE), np.frombuffer(outs[-6][-1][1], np.float32).reshape(-1, 4),
                 hc, rc)
+
+
+@pytest.mark.parametrize("slots,reserve", [(17, 4), (40, 16), (200, 64), (200, 512)])
+def test_early_loads_are_exact(vt, vto, ctx, slots, reserve):
+    """Copying requested bricks beside the first pass (into a reserve of
+    free slots, published only between passes) never changes a byte of the
+    frame."""
+    vol = blobs_volume(64, 16, seed=101)
+    sc = Scene(vto, vol, 0.3, 4, 3)
+    sc.setup_gpu(vt, ctx, slots=slots)
+    cams = [orbit_camera(vt, 320, 240, (32, 32, 32), 95.0, 15.0 + 40.0 * k, 20.0) for k in range(4)]
+    outs, early = {}, 0
+    try:
+        for res in (0, reserve):
+            log(f"early reserve={res} slots={slots}")
+            ctx.set_early_loads(res)
+            ctx.cache_clear()
+            rows = []
+            for cam in cams:
+                rg, hg, st = sc.render_gpu(vt, ctx, cam)
+                assert st.err == 0
+                rows.append((hg.tobytes(), rg.tobytes()))
+                if res:
+                    early += st.bricks_early
+            outs[res] = rows
+    finally:
+        ctx.set_early_loads()
+    assert outs[reserve] == outs[0]
+    print(f"bricks loaded beside the first pass: {early}", flush=True)
+    rc, hc = sc.render_cpu(vto, cams[-1])
+    assert_same(np.frombuffer(outs[reserve][-1][0], vt.HIT_DTYPE),
+                np.frombuffer(outs[reserve][-1][1], np.float32).reshape(-1, 4), hc, rc)
+
+
+def test_early_loads_from_the_host_tier(vt, vto, ctx, tmp_path):
+    """Early loads from the pinned host tier, filled ahead and on demand."""
+    vol = blobs_volume(64, 10, seed=33, dims=(64, 60, 50))
+    sc = Scene(vto, vol, 0.5, 3, 2)
+    h5 = tmp_path / "vol.h5"
+    vt.file_create(h5, "value", vol, chunk=16)
+    cams = [orbit_camera(vt, 256, 200, (32, 30, 25), 120.0, 60.0 * k) for k in range(3)]
+    prm = vt.render_params(3, 2, 0.5)
+    flat, sizes = flat_runs(sc.runs, 3)
+    for cpu_slots, gpu_slots, reserve in [(64, 12, 6), (9, 6, 3)]:
+        outs = {}
+        for res in (0, reserve):
+            log(f"host tier cpu={cpu_slots} gpu={gpu_slots} reserve={res}")
+            ctx.octree_upload(flat, sizes, sc.n, 3)
+            ctx.cache_create(2, 16, 3, gpu_slots, cpu_slots)
+            ctx.cache_attach_hdf5(h5, "value")
+            if cpu_slots >= 64:
+                ctx.cache_prefetch_host(wait=True)
+            ctx.set_early_loads(res)
+            try:
+                rows = []
+                for cam in cams:
+                    R = cam.W * cam.H
+                    d_rgba, d_hits = ctx.alloc(R * 16), ctx.alloc(R * 32)
+                    st = ctx.render_frame(cam, prm, d_rgba, d_hits)
+                    assert st.err == 0
+                    rows.append((d_hits.download(vt.HIT_DTYPE, (R,)).tobytes(),
+                                 d_rgba.download(np.float32, (R, 4)).tobytes()))
+                    d_rgba.free(); d_hits.free()
+                outs[res] = rows
+            finally:
+                ctx.set_early_loads()
+        assert outs[reserve] == outs[0]
+    rc, hc = sc.render_cpu(vto, cams[-1])
+    R = cams[-1].W * cams[-1].H
+    assert_same(np.frombuffer(outs[reserve][-1][0], vt.HIT_DTYPE),
+                np.frombuffer(outs[reserve][-1][1], np.float32).reshape(R, 4), hc, rc)
 
 
 def test_iso_change_keeps_the_cache_exact(vt, vto, ctx):

@@ -40,9 +40,13 @@ def test_prefetch_cuts_misses_and_keeps_hits(vt, vto, ctx):
     assert losses[-20:].mean() < 0.7 * losses[:20].mean()
     ctx.cache_create(level_c, 1 << (n - level_c), 3, slots, 0)
     ctx.cache_attach_device_volume(d_vol, vol.shape)
-    plain, (rgba_p, hits_p) = pf.run_path(ctx, test_cams, prm)
-    ctx.cache_clear()
-    pref, (rgba_q, hits_q) = pf.run_path(ctx, test_cams, prm, net, budget=slots, min_prob=0.8)
+    ctx.set_early_loads(0)   # demand misses are compared: no loads that depend on timing
+    try:
+        plain, (rgba_p, hits_p) = pf.run_path(ctx, test_cams, prm)
+        ctx.cache_clear()
+        pref, (rgba_q, hits_q) = pf.run_path(ctx, test_cams, prm, net, budget=slots, min_prob=0.8)
+    finally:
+        ctx.set_early_loads()
     # identical pictures and hit records
     assert hits_p.tobytes() == hits_q.tobytes()
     assert np.array_equal(rgba_p.view(np.uint32), rgba_q.view(np.uint32))

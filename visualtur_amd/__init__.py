@@ -43,7 +43,8 @@ class FrameStats(ctypes.Structure):
     _fields_ = [("passes", ctypes.c_int32), ("err", ctypes.c_int32), ("bricks_requested", ctypes.c_int64),
                 ("bricks_uploaded", ctypes.c_int64), ("host_tier_misses", ctypes.c_int64),
                 ("rays_parked", ctypes.c_int64), ("samples", ctypes.c_uint64), ("nodes", ctypes.c_uint64),
-                ("ms_total", ctypes.c_double), ("ms_kernel", ctypes.c_double), ("ms_service", ctypes.c_double)]
+                ("ms_total", ctypes.c_double), ("ms_kernel", ctypes.c_double), ("ms_service", ctypes.c_double),
+                ("bricks_early", ctypes.c_int64)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -120,6 +121,7 @@ _SIGS = {
     "vt_get_brick_trace": (ctypes.c_int, [_P, _P, ctypes.c_int64]),
     "vt_set_march_options": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "vt_set_wide_march": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int64]),
+    "vt_set_early_loads": (ctypes.c_int, [_P, ctypes.c_int64]),
     "vt_trace_nodes": (ctypes.c_int, [_P, ctypes.POINTER(Camera), ctypes.c_int32, ctypes.c_int32,
                                       ctypes.c_int32, _P, _P]),
     "vt_octree_build": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
@@ -398,6 +400,11 @@ class Context:
         2^k lanes in every later pass); first_pass_budget likewise for the
         first pass."""
         self._c(lib().vt_set_wide_march(self.handle, int(lane_budget), int(first_pass_budget)))
+
+    def set_early_loads(self, reserve: int = 256) -> None:
+        """Keep `reserve` free slots before each first pass and copy the
+        bricks it requests into them while it runs (0: off)."""
+        self._c(lib().vt_set_early_loads(self.handle, int(reserve)))
 
     def brick_trace(self, level_c: int) -> np.ndarray:
         n = 1 << (3 * level_c)

@@ -287,6 +287,7 @@ struct AsyncStage {
     PinBuf<int64_t> tidx;
     PinBuf<int32_t> tval;
     int64_t used = 0, cap = 0;
+    int64_t pub = 0;   // entries below pub have their sub-block min/max and page-table entries written
     hipError_t ensure(int64_t count)
     {
         hipError_t e;
@@ -294,13 +295,13 @@ struct AsyncStage {
             (e = codes.ensure((size_t)count)) || (e = tidx.ensure((size_t)count)) || (e = tval.ensure((size_t)count)))
             return e;
         cap = count;
-        used = 0;
+        used = pub = 0;
         return hipSuccess;
     }
     void release()
     {
         src.release(); slots.release(); codes.release(); tidx.release(); tval.release();
-        used = cap = 0;
+        used = cap = pub = 0;
     }
 };
 
@@ -421,6 +422,7 @@ struct vt_context {
     hipStream_t stream = nullptr;
     hipStream_t cstream = nullptr;   // prefetch loads (vt_render_frame waits for them)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_copy = nullptr;   // the copy stream's work so far, for publishing its loads
     // octree
     bool octree_ready = false;
     int n = 0, max_level = 0;
@@ -451,6 +453,8 @@ struct vt_context {
     int max_defer_late = 2;   // ... and in later passes
     int64_t wide_lanes = 1 << 18;   // lane budget of the wide march in later passes (0: off, -k: 2^k lanes)
     int64_t wide_first = 1 << 21;   // ... and in the first pass
+    int64_t early_reserve = 256;    // free slots kept for loads issued while the first pass runs (0: off)
+    int64_t early_last = INT64_MAX;  // bricks the previous frame's first pass requested
     int persistent = 0;   // persistent render pass with lane refill (measured slower; option)
     int persistent_wgs = 0;
     DevBuf<uint32_t> d_next;   // its 8 work counters
@@ -482,6 +486,7 @@ struct vt_context {
         if (h_ctr) (void)hipHostFree(h_ctr);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        if (ev_copy) (void)hipEventDestroy(ev_copy);
         if (stream) (void)hipStreamDestroy(stream);
         if (cstream) (void)hipStreamDestroy(cstream);
     }

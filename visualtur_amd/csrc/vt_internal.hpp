@@ -180,9 +180,10 @@ hipError_t launch_bricks_from_volume(const float* vol, int X, int Y, int Z,
 hipError_t launch_submm(const float* pool, const int32_t* slots, int count, int S, int inc, int G, size_t stride,
                         float2* submm, hipStream_t s);
 // Copy bricks of slot_elems floats from (device-visible) host memory into
-// the pool, slot b at pool + b * stride.
+// the pool, slot b at pool + b * stride, with `split` workgroups per brick
+// (0: the default of 8).
 hipError_t launch_copy_bricks(const float* const* src, const int32_t* dst_slots, int count,
-                              size_t slot_elems, size_t stride, float* pool, hipStream_t s);
+                              size_t slot_elems, size_t stride, float* pool, hipStream_t s, int split = 0);
 // table[idx[i]] = val[i]
 hipError_t launch_scatter_i32(int32_t* table, const int64_t* idx, const int32_t* val, int count,
                               hipStream_t s);

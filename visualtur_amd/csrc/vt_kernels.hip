@@ -1090,7 +1090,8 @@ hipError_t launch_bricks_from_volume(const float* vol, int X, int Y, int Z,
 }
 
 // Whole-slot copies from device-visible pinned host memory.  Each brick is
-// split over kCopySplit workgroups (blockIdx.x = part, blockIdx.y = brick)
+// split over gridDim.x workgroups (blockIdx.x = part, blockIdx.y = brick;
+// kCopySplit by default, fewer for copies that run beside a render pass)
 // and every lane keeps four 16-byte reads in flight, so that enough PCIe
 // reads are outstanding to cover their latency.  The slot size is a
 // multiple of 4 floats when S is even; otherwise part 0 copies the tail.
@@ -1104,7 +1105,7 @@ __global__ __launch_bounds__(256) void k_copy_bricks(const float* const* __restr
     const float* s = src[b];
     float* d = pool + (size_t)dst_slots[b] * stride;
     const size_t n4 = slot_elems / 4;
-    const size_t per = (n4 + kCopySplit - 1) / kCopySplit;
+    const size_t per = (n4 + gridDim.x - 1) / gridDim.x;
     const size_t beg = per * blockIdx.x, end = beg + per < n4 ? beg + per : n4;
     const float4* __restrict__ s4 = reinterpret_cast<const float4*>(s);
     float4* __restrict__ d4 = reinterpret_cast<float4*>(d);
@@ -1122,11 +1123,12 @@ __global__ __launch_bounds__(256) void k_copy_bricks(const float* const* __restr
 }
 
 hipError_t launch_copy_bricks(const float* const* src, const int32_t* dst_slots, int count,
-                              size_t slot_elems, size_t stride, float* pool, hipStream_t s)
+                              size_t slot_elems, size_t stride, float* pool, hipStream_t s, int split)
 {
+    if (split <= 0) split = kCopySplit;
     for (int o = 0; o < count; o += 65535) {   // grid y is at most 65535
         const int m = count - o < 65535 ? count - o : 65535;
-        hipLaunchKernelGGL(k_copy_bricks, dim3(kCopySplit, m), dim3(256), 0, s, src + o, dst_slots + o, slot_elems,
+        hipLaunchKernelGGL(k_copy_bricks, dim3(split, m), dim3(256), 0, s, src + o, dst_slots + o, slot_elems,
                            stride, pool);
         if (hipError_t e = hipGetLastError()) return e;
     }

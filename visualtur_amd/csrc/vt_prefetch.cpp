@@ -263,7 +263,7 @@ int vt_mlp_features(vt_mlp* m, const vt_camera* cam, int32_t nLevels, int32_t le
 // ====================================================== learned prefetch
 
 int load_async(vt_context* ctx, const uint32_t* req, size_t n, std::vector<uint32_t>& left, int64_t* loaded,
-               int64_t* misses);
+               int64_t* misses, bool publish);
 int copy_stream_idle(vt_context* ctx);
 void host_tier_start(vt_context* ctx);
 
@@ -372,7 +372,7 @@ extern "C" int vt_prefetch(vt_context* ctx, vt_mlp* m, const vt_camera* cam, int
         std::vector<uint32_t> load((size_t)n), left;
         for (int64_t q = 0; q < n; ++q) load[(size_t)q] = (uint32_t)(ctx->cand_codes[(size_t)want[(size_t)q]] - base);
         int64_t misses = 0;
-        if (int rc = load_async(ctx, load.data(), load.size(), left, &nloaded, &misses)) return rc;
+        if (int rc = load_async(ctx, load.data(), load.size(), left, &nloaded, &misses, true)) return rc;
     }
     if (loaded) *loaded = nloaded;
     return VT_OK;

@@ -89,6 +89,7 @@ int fold_ref_bits(vt_context* ctx, bool copied = false)
 int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t* misses, int32_t* slots_out,
                 const int64_t* victims = nullptr, const PassReset* reset = nullptr);
 int copy_stream_idle(vt_context* ctx);
+int publish_loads(vt_context* ctx, hipStream_t s);
 
 // Starts the host tier once the brick geometry is known; with the octree
 // uploaded down to levelC, a tier that holds every brick with octree nodes
@@ -207,13 +208,15 @@ int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t
 }
 
 // Loads the absent bricks of `req` that fit in free slots on the copy
-// stream, without waiting; the others are appended to `left`.  The page-table
-// entry of a brick is written after its copy.  vt_render_frame waits for the
-// copy stream before its first pass, so no pass runs beside these copies.
-// Host-tier slots read here stay pinned in the current epoch, which ends only
-// after the copy stream has been synchronized.
+// stream, without waiting; the others are appended to `left`.  With
+// `publish`, the sub-block min/max and page-table entries follow on the copy
+// stream (no render pass may run then: vt_render_frame waits for the copy
+// stream before its first pass); without, the copies go alone (a render pass
+// may run beside them, it reads no free slot) and publish_loads writes the
+// rest later, between passes.  Host-tier slots read here stay pinned in the
+// current epoch, which ends only after the copy stream has been synchronized.
 int load_async(vt_context* ctx, const uint32_t* req, size_t n, std::vector<uint32_t>& left, int64_t* loaded,
-               int64_t* misses)
+               int64_t* misses, bool publish)
 {
     BrickCache& c = *ctx->cache;
     AsyncStage& A = ctx->astage;
@@ -254,27 +257,126 @@ int load_async(vt_context* ctx, const uint32_t* req, size_t n, std::vector<uint3
     if (k == 0) return VT_OK;
     hipStream_t s = ctx->cstream;
     if (c.host) {
+        // beside a running render pass, two workgroups per brick keep enough
+        // PCIe reads in flight while taking few of the pass's wave slots
         HIP_TRY(launch_copy_bricks(A.src.dev + base, A.slots.dev + base, (int)k, c.brick_elems, c.slot_elems,
-                                   c.d_pool, s));
+                                   c.d_pool, s, publish ? 0 : 2));
     } else {
         HIP_TRY(launch_bricks_from_volume(c.d_vol, c.vol_dims[0], c.vol_dims[1], c.vol_dims[2], A.codes.dev + base,
                                           A.slots.dev + base, (int)k, c.level_c, c.n, c.cube, c.inc, c.slot_elems, c.d_pool, s));
     }
-    if (c.d_submm)
-        HIP_TRY(launch_submm(c.d_pool, A.slots.dev + base, (int)k, c.S, c.inc, c.G, c.slot_elems, c.d_submm, s));
-    HIP_TRY(launch_scatter_i32(c.d_table, A.tidx.dev + base, A.tval.dev + base, (int)k, s));
     A.used += k;
+    if (publish) {
+        if (int rc = publish_loads(ctx, s)) return rc;
+    }
     c.stat_uploads += k;
     *loaded += k;
     return VT_OK;
 }
 
-// Waits for the copy stream; its staging can then be reused.
+// Writes the sub-block min/max and the page-table entries of the copies not
+// yet published, on stream `s` after the copy stream's work so far.  Only
+// called where no render pass runs: a running pass never sees a page-table
+// entry change (an entry written beside it could expose a slot whose lines
+// another XCD's L2 still holds from before the copy).
+int publish_loads(vt_context* ctx, hipStream_t s)
+{
+    BrickCache& c = *ctx->cache;
+    AsyncStage& A = ctx->astage;
+    const int64_t k = A.used - A.pub;
+    if (k <= 0) return VT_OK;
+    if (s != ctx->cstream) {
+        HIP_TRY(hipEventRecord(ctx->ev_copy, ctx->cstream));
+        HIP_TRY(hipStreamWaitEvent(s, ctx->ev_copy, 0));
+    }
+    if (c.d_submm)
+        HIP_TRY(launch_submm(c.d_pool, A.slots.dev + A.pub, (int)k, c.S, c.inc, c.G, c.slot_elems, c.d_submm, s));
+    HIP_TRY(launch_scatter_i32(c.d_table, A.tidx.dev + A.pub, A.tval.dev + A.pub, (int)k, s));
+    A.pub = A.used;
+    return VT_OK;
+}
+
+// Waits for the copy stream (publishing what it loaded) and for the
+// publication; its staging can then be reused.
 int copy_stream_idle(vt_context* ctx)
 {
+    if (ctx->astage.pub < ctx->astage.used) {
+        if (int rc = publish_loads(ctx, ctx->stream)) return rc;
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
     HIP_TRY(hipStreamSynchronize(ctx->cstream));
-    ctx->astage.used = 0;
+    ctx->astage.used = ctx->astage.pub = 0;
     return VT_OK;
+}
+
+// Frees slots (clock victims, page-table entries cleared on the context
+// stream) until `reserve` slots are free, for the loads issued while the
+// first pass runs.
+int make_reserve(vt_context* ctx, int64_t reserve)
+{
+    BrickCache& c = *ctx->cache;
+    int64_t need = std::min<int64_t>(reserve, c.gpu_slots) - c.available();
+    if (need <= 0) return VT_OK;
+    if (int rc = fold_ref_bits(ctx)) return rc;
+    const StageLayout L((size_t)need);
+    if (c.h_stage.ensure(L.bytes) != hipSuccess || c.d_stage.ensure(L.bytes) != hipSuccess)
+        return fail(VT_E_NOMEM, "staging allocation failed");
+    int64_t* h_tidx = reinterpret_cast<int64_t*>(c.h_stage.p + L.tidx);
+    int32_t* h_tval = reinterpret_cast<int32_t*>(c.h_stage.p + L.tval);
+    int64_t nt = 0;
+    for (; nt < need; ++nt) {
+        int64_t slot = c.clock_victim(true);
+        if (slot < 0) slot = c.clock_victim(false);
+        if (slot < 0) break;
+        const int64_t old = c.slot_brick[(size_t)slot];
+        c.table[(size_t)old] = -1;
+        c.slot_brick[(size_t)slot] = -1;
+        c.free_list.push_back(slot);
+        c.stat_evictions++;
+        h_tidx[nt] = old;
+        h_tval[nt] = -1;
+    }
+    if (nt == 0) return VT_OK;
+    HIP_TRY(hipMemcpyAsync(c.d_stage.p + L.tidx, c.h_stage.p + L.tidx, (size_t)nt * 8, hipMemcpyHostToDevice,
+                           ctx->stream));
+    HIP_TRY(hipMemcpyAsync(c.d_stage.p + L.tval, c.h_stage.p + L.tval, (size_t)nt * 4, hipMemcpyHostToDevice,
+                           ctx->stream));
+    HIP_TRY(launch_scatter_i32(c.d_table, reinterpret_cast<const int64_t*>(c.d_stage.p + L.tidx),
+                               reinterpret_cast<const int32_t*>(c.d_stage.p + L.tval), (int)nt, ctx->stream));
+    return VT_OK;
+}
+
+// While the first pass runs (until `done` completes), loads the bricks it
+// requests into the reserved free slots on the copy stream, without
+// publishing them; requests that find no free slot are left to the service
+// after the pass.
+int load_during_pass(vt_context* ctx, hipEvent_t done, int64_t* loaded, int64_t* misses)
+{
+    BrickCache& c = *ctx->cache;
+    const uint32_t* rq = ctx->req_list.p;
+    const size_t cap = (size_t)c.nbricks;
+    std::vector<uint32_t> batch, left;
+    size_t seen = 0;
+    double last = now_ms();
+    for (;;) {
+        const hipError_t q = hipEventQuery(done);
+        if (q != hipSuccess && q != hipErrorNotReady) HIP_TRY(q);
+        if (q == hipSuccess) return VT_OK;   // the service after the pass loads the rest
+        while (seen < cap) {
+            const uint32_t v = __atomic_load_n(&rq[seen], __ATOMIC_ACQUIRE);
+            if (v == 0) break;
+            batch.push_back(v - 1);
+            ++seen;
+        }
+        const double t = now_ms();
+        if (!batch.empty() && (batch.size() >= 16 || t - last > 0.05)) {
+            left.clear();
+            if (int rc = load_async(ctx, batch.data(), batch.size(), left, loaded, misses, false)) return rc;
+            batch.clear();
+            last = t;
+            if (!left.empty()) return VT_OK;   // the reserve is used up
+        }
+    }
 }
 
 namespace {
@@ -318,6 +420,7 @@ int vt_context_create(int device, vt_context** out)
     HIP_TRY(hipStreamCreateWithFlags(&ctx->cstream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreate(&ctx->ev0));
     HIP_TRY(hipEventCreate(&ctx->ev1));
+    HIP_TRY(hipEventCreateWithFlags(&ctx->ev_copy, hipEventDisableTiming));
     HIP_TRY(hipMalloc((void**)&ctx->d_ctr, sizeof(FrameCounters)));
     if (ctx->d_next.ensure(8) != hipSuccess) return fail(VT_E_NOMEM, "work counters");
     ctx->persistent_wgs = render_persistent_workgroups(device);
@@ -510,6 +613,7 @@ int vt_cache_create(vt_context* ctx, int32_t levelC, int32_t cubeDim, int32_t cu
         return fail(VT_E_ARG, "bad cache dimensions");
     int64_t S = cubeDim + 2 * (int64_t)cubeInc;
     if (S > 1024) return fail(VT_E_ARG, "brick too large");
+    ctx->early_last = INT64_MAX;
     if (int rc = set_device(ctx)) return rc;
     std::unique_ptr<BrickCache> c(new BrickCache);
     c->level_c = levelC;
@@ -647,6 +751,7 @@ int vt_cache_clear(vt_context* ctx)
     BrickCache& c = *ctx->cache;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (int rc = copy_stream_idle(ctx)) return rc;
+    ctx->early_last = INT64_MAX;
     HIP_TRY(hipMemset(c.d_table, 0xff, (size_t)c.nbricks * 4));
     HIP_TRY(hipMemset(c.d_ref, 0, (size_t)c.gpu_slots));
     std::fill(c.table.begin(), c.table.end(), -1);
@@ -705,6 +810,15 @@ int vt_set_wide_march(vt_context* ctx, int64_t lane_budget, int64_t first_pass_b
     if (lane_budget < -6 || first_pass_budget < -6) return fail(VT_E_ARG, "lane budgets must be >= -6");
     ctx->wide_lanes = lane_budget;
     ctx->wide_first = first_pass_budget;
+    return VT_OK;
+}
+
+int vt_set_early_loads(vt_context* ctx, int64_t reserve)
+{
+    if (!ctx) return fail(VT_E_ARG, "ctx is NULL");
+    if (reserve < 0) return fail(VT_E_ARG, "reserve must be >= 0");
+    ctx->early_reserve = reserve;
+    ctx->early_last = INT64_MAX;
     return VT_OK;
 }
 
@@ -795,6 +909,13 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
     a.out_list = ctx->list[cur].p;
     // bricks still arriving from a prefetch land before the first pass
     if (int rc = copy_stream_idle(ctx)) return rc;
+    // free slots for the loads issued while the first pass runs: at most as
+    // many as the previous frame's first pass requested, so that a frame
+    // whose bricks are all resident evicts nothing
+    const bool early = ctx->early_reserve > 0 && !ctx->persistent;
+    if (early) {
+        if (int rc = make_reserve(ctx, std::min(ctx->early_reserve, ctx->early_last))) return rc;
+    }
     uint32_t* rq = ctx->req_list.p;
     a.next = ctx->d_next.p;
     for (int pass = 0;; ++pass) {
@@ -815,6 +936,15 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
         else
             HIP_TRY(launch_render_pass(a, ctx->stream));
         HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+        if (early && pass == 0) {
+            int64_t loaded = 0, misses = 0;
+            if (int rc = load_during_pass(ctx, ctx->ev1, &loaded, &misses)) return rc;
+            st.bricks_uploaded += loaded;
+            st.bricks_early += loaded;
+            st.host_tier_misses += misses;
+            // the bricks copied beside the pass become visible to the next one
+            if (int rc = publish_loads(ctx, ctx->stream)) return rc;
+        }
         HIP_TRY(hipMemcpyAsync(ctx->h_ctr, ctx->d_ctr, sizeof(FrameCounters), hipMemcpyDeviceToHost, ctx->stream));
         // the reference bytes come back under the same wait, for the service
         HIP_TRY(hipMemcpyAsync(c.h_ref.p, c.d_ref, (size_t)c.gpu_slots, hipMemcpyDeviceToHost, ctx->stream));
@@ -826,6 +956,7 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
         const uint32_t parked = ctx->h_ctr->parked_count;
         const uint32_t nreq = ctx->h_ctr->req_count;
         if (nreq > (uint32_t)c.nbricks) return fail(VT_E_STATE, "request list overflow");
+        if (pass == 0) ctx->early_last = nreq;
         st.bricks_requested += nreq;
         // every entry below nreq was stored before the kernel ended
         std::vector<uint32_t>& left = ctx->req_left;
