@@ -255,6 +255,60 @@ void vto_screen_size(float distance, float fovH, float fovW, float *h, float *w)
     *w = 2 * distance * tanf((float)(fovW * (M_PI / 180.0)));
 }
 
+/* float3 arithmetic of cutil_math, component by component. */
+static void v3_axpby(const float a[3], float s, const float b[3], float t, float sign, float out[3])
+{
+    /* out = a*s + sign*(b*t), rounded as the reference's a*s + b*t / a*s - b*t */
+    for (int k = 0; k < 3; ++k) {
+        float x = a[k] * s, y = b[k] * t;
+        out[k] = sign > 0 ? x + y : x - y;
+    }
+}
+
+static void v3_cross(const float a[3], const float b[3], float out[3])
+{
+    float r0 = a[1] * b[2] - a[2] * b[1];
+    float r1 = a[2] * b[0] - a[0] * b[2];
+    float r2 = a[0] * b[1] - a[1] * b[0];
+    out[0] = r0; out[1] = r1; out[2] = r2;
+}
+
+/* Camera.cu:83-156.  The angle's sine and cosine are taken in double and
+ * stored as float (:91-92).  RotateX/Y/Z call normalize() without using its
+ * result (:97, :113, :129), so the rotated vector is not renormalised.
+ * MoveUpward moves along `right` and StrafeRight along `up` (:146-156). */
+void vto_camera_move(vto_camera *c, int op, const float *arg)
+{
+    float t[3];
+    if (op == 0) {                 /* Move: position += Direction */
+        for (int k = 0; k < 3; ++k) c->pos[k] = c->pos[k] + arg[k];
+        return;
+    }
+    if (op >= 1 && op <= 3) {
+        float s = (float)sin(arg[0] * (M_PI / 180.0));
+        float co = (float)cos(arg[0] * (M_PI / 180.0));
+        if (op == 1) {             /* look = look*c + up*s; up = -cross(look, right) */
+            v3_axpby(c->look, co, c->up, s, 1.0f, t);
+            memcpy(c->look, t, sizeof t);
+            v3_cross(c->look, c->right, t);
+            for (int k = 0; k < 3; ++k) c->up[k] = (-1.0f) * t[k];
+        } else if (op == 2) {      /* look = look*c - right*s; right = cross(look, up) */
+            v3_axpby(c->look, co, c->right, s, -1.0f, t);
+            memcpy(c->look, t, sizeof t);
+            v3_cross(c->look, c->up, c->right);
+        } else {                   /* right = right*c + up*s; up = -cross(look, right) */
+            v3_axpby(c->right, co, c->up, s, 1.0f, t);
+            memcpy(c->right, t, sizeof t);
+            v3_cross(c->look, c->right, t);
+            for (int k = 0; k < 3; ++k) c->up[k] = (-1.0f) * t[k];
+        }
+        return;
+    }
+    const float *dir = op == 4 ? c->look : op == 5 ? c->right : c->up;
+    const float d = op == 4 ? -arg[0] : arg[0];
+    for (int k = 0; k < 3; ++k) c->pos[k] = c->pos[k] + dir[k] * d;
+}
+
 /* Camera.cu:58-73 (the active #else branch): row i counts from the bottom,
  * pixel id = i*W + j.  normalize() as v * (1/sqrt(dot(v,v))). */
 void vto_ray_dir(const vto_camera *c, int i, int j, float out[3])

@@ -69,6 +69,10 @@ typedef struct {
 
 /* Screen::Screen plane size, Screen.cpp:18-19 (fov is a half-angle). */
 void vto_screen_size(float distance, float fovH, float fovW, float *h, float *w);
+/* Camera.cu:83-156: one camera move.  op: 0 Move(arg[0..2]), 1 RotateX,
+ * 2 RotateY, 3 RotateZ (degrees, arg[0]), 4 MoveForward, 5 MoveUpward,
+ * 6 StrafeRight (distance, arg[0]). */
+void vto_camera_move(vto_camera *c, int op, const float *arg);
 void vto_ray_dir(const vto_camera *cam, int i, int j, float out[3]);
 
 /* ---------------------------------------------------------------- volume */

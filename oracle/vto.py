@@ -51,6 +51,7 @@ _SIGS = {
     "vto_member_exact": (ctypes.c_int, [_P, ctypes.c_int, _U64]),
     "vto_slab": (ctypes.c_int, [_P, _P, _P, _P, ctypes.POINTER(_F), ctypes.POINTER(_F)]),
     "vto_screen_size": (None, [_F, _F, _F, ctypes.POINTER(_F), ctypes.POINTER(_F)]),
+    "vto_camera_move": (None, [ctypes.POINTER(OCamera), ctypes.c_int, _P]),
     "vto_ray_dir": (None, [ctypes.POINTER(OCamera), ctypes.c_int, ctypes.c_int, _P]),
     "vto_voxel": (_F, [ctypes.POINTER(OVolume), ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "vto_sample": (_F, [ctypes.POINTER(OVolume), _F, _F, _F]),
@@ -162,6 +163,22 @@ def camera_from(cam) -> OCamera:
     else:
         ctypes.memmove(ctypes.byref(oc), ctypes.byref(cam), ctypes.sizeof(OCamera))
     return oc
+
+
+CAMERA_MOVES = {"Move": 0, "RotateX": 1, "RotateY": 2, "RotateZ": 3, "MoveForward": 4, "MoveUpward": 5,
+                "StrafeRight": 6}
+
+
+def camera_move(cam: OCamera, op: str, arg) -> OCamera:
+    """Camera.cu:83-156 applied to `cam` in place (and returned); `arg` is a
+    3-vector for Move, an angle in degrees or a distance otherwise."""
+    a = np.ascontiguousarray(np.atleast_1d(np.asarray(arg, np.float32)), np.float32)
+    lib().vto_camera_move(ctypes.byref(cam), CAMERA_MOVES[op], a.ctypes.data)
+    return cam
+
+
+def camera_vectors(cam: OCamera) -> dict:
+    return {k: np.array(getattr(cam, k)[:], np.float32) for k in ("pos", "look", "up", "right")}
 
 
 def screen_size(distance, fov_h, fov_w):

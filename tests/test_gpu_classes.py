@@ -41,6 +41,23 @@ def test_class_surface_matches_oracle(vt, vto, tmp_path):
     V = vto.Volume(vol)
     O = vto.Octree(runs, n, level_o)
     prm = vto.params(level_o, level_c, iso)
+    # the driver's camera moves (testVisualTur.cu:48-50, then per frame a
+    # strafe and a 7-degree turn) equal the oracle's Camera.cu:83-156
+    D = 64
+    oc = vto.OCamera()
+    oc.look[:] = [0.0, 0.0, -1.0]
+    oc.up[:] = [0.0, 1.0, 0.0]
+    oc.right[:] = [1.0, -0.0, 0.0]   # cross(look, up) in float: the y term is (-1)*0 - 0*1 = -0
+    vto.camera_move(oc, "Move", (0.5 * D, 0.5 * D, 0.5 * D + 2.5 * D))
+    vto.camera_move(oc, "MoveForward", 1.0)
+    for m in range(3):
+        got = camera_dict(out / f"camera_{m}.bin")
+        want = vto.camera_vectors(oc)
+        for k in ("pos", "look", "up", "right"):
+            assert np.array_equal(np.array(got[k], np.float32).view(np.uint32), want[k].view(np.uint32)), (m, k)
+        vto.camera_move(oc, "Move", (0.0, 0.0, 0.0))
+        vto.camera_move(oc, "StrafeRight", 0.5 * D / 16.0)
+        vto.camera_move(oc, "RotateY", 7.0)
     hits_total = 0
     for m in range(3):
         cam = camera_dict(out / f"camera_{m}.bin")

@@ -188,3 +188,61 @@ def test_render_background_is_white(vto):
                distance=50.0, W=4, H=4)   # looking away from the volume
     rgba, hits, _ = vto.render(vto.Volume(vol), oct, cam, vto.params(2, 1, 0.5))
     assert (rgba == 1.0).all() and (hits["node"] == 0).all()
+
+
+def _initial_camera(vto):
+    """Camera.cu:17-33: look (0,0,-1), up (0,1,0), right = cross(look, up)."""
+    c = vto.OCamera()
+    c.look[:] = [0.0, 0.0, -1.0]
+    c.up[:] = [0.0, 1.0, 0.0]
+    c.right[:] = [1.0, -0.0, 0.0]   # cross(look, up) in float: the y term is (-1)*0 - 0*1 = -0
+    return c
+
+
+@pytest.mark.parametrize("op,look,up,right", [
+    ("RotateX", (0, 1, 0), (0, 0, 1), (1, 0, 0)),      # look turns towards up; up = -cross(look, right)
+    ("RotateY", (-1, 0, 0), (0, 1, 0), (0, 0, -1)),    # look = look*c - right*s; right = cross(look, up)
+    ("RotateZ", (0, 0, -1), (-1, 0, 0), (0, 1, 0)),    # right turns towards up
+])
+def test_camera_rotations_quarter_turn(vto, op, look, up, right):
+    """A 90-degree turn of the initial frame lands on the closed-form axes
+    (within the double cos(pi/2) = 6e-17 left in the sum)."""
+    c = vto.camera_move(_initial_camera(vto), op, 90.0)
+    v = vto.camera_vectors(c)
+    for k, want in (("look", look), ("up", up), ("right", right)):
+        assert np.allclose(v[k], want, atol=1e-6), (op, k, v[k])
+
+
+def test_camera_moves_along_the_reference_axes(vto):
+    """MoveForward goes along -look, MoveUpward along right, StrafeRight
+    along up (Camera.cu:140-156), Move adds the vector."""
+    c = _initial_camera(vto)
+    vto.camera_move(c, "Move", (1.0, 2.0, 3.0))
+    vto.camera_move(c, "MoveForward", 2.0)
+    assert vto.camera_vectors(c)["pos"].tolist() == [1.0, 2.0, 5.0]
+    vto.camera_move(c, "MoveUpward", 3.0)
+    assert vto.camera_vectors(c)["pos"].tolist() == [4.0, 2.0, 5.0]
+    vto.camera_move(c, "StrafeRight", 5.0)
+    assert vto.camera_vectors(c)["pos"].tolist() == [4.0, 7.0, 5.0]
+
+
+def test_camera_rotation_matches_float64(vto):
+    """A sequence of turns equals the same formulas in float64 to float32
+    rounding (pins operand order and signs for angles off the axes)."""
+    c = _initial_camera(vto)
+    L, U, R = np.array([0, 0, -1.0]), np.array([0, 1.0, 0]), np.array([1.0, 0, 0])
+    for op, a in (("RotateY", 7.0), ("RotateX", -13.0), ("RotateZ", 29.0), ("RotateY", 41.0)):
+        vto.camera_move(c, op, a)
+        s, co = math.sin(math.radians(a)), math.cos(math.radians(a))
+        if op == "RotateX":
+            L = L * co + U * s
+            U = -np.cross(L, R)
+        elif op == "RotateY":
+            L = L * co - R * s
+            R = np.cross(L, U)
+        else:
+            R = R * co + U * s
+            U = -np.cross(L, R)
+    v = vto.camera_vectors(c)
+    for k, want in (("look", L), ("up", U), ("right", R)):
+        assert np.allclose(v[k], want, atol=1e-5), (k, v[k], want)
