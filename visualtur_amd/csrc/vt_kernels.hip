@@ -117,39 +117,6 @@ __device__ __forceinline__ void request_brick(const RenderArgs& a, uint64_t bidx
     }
 }
 
-// walk_ray of the frame passes: a node whose brick is absent is deferred
-// (appended to the ray's deferral list, its brick requested) while the list
-// has room, and the walk goes on past it.  The first crossing in a deferred
-// node, marched in a later pass, precedes every crossing found beyond it,
-// so a walk that ends with nd > 0 gives a tentative result.
-//   WALK_HIT / WALK_MISS : the walk ended (tentative when nd > 0)
-//   WALK_ABSENT          : the list is full and `node`'s brick (bidx) is absent
-__device__ __forceinline__ int walk_defer(const Ray& r, const RenderArgs& a, const BrickView& bv, const Shading& sh,
-                                          uint64_t& node, uint32_t id, uint32_t R, int& nd, float4& out,
-                                          HitRecord& h, uint64_t& bidx, uint32_t& err, uint32_t& nsamples,
-                                          uint32_t& nnodes, OrderCache& oc)
-{
-    for (;;) {
-        if (node == 0) {
-            set_miss(out, h);
-            return WALK_MISS;
-        }
-        uint64_t brick = node >> bv.shift;
-        bidx = brick - bv.brick_base;
-        int slot = bv.brick_slot[bidx];
-        if (slot < 0) {
-            if (nd >= a.max_defer) return WALK_ABSENT;
-            a.ray_defer[(size_t)nd * R + id] = node;
-            nd += 1;
-            request_brick(a, bidx);
-        } else if (visit_node(r, a.oct, bv, sh, node, brick, slot, a.slot_ref, a.brick_trace, out, h, err, nsamples,
-                              nnodes)) {
-            return WALK_HIT;
-        }
-        node = next_node(r, a.oct, node, false, oc);
-    }
-}
-
 // ------------------------------------------------------ wide march
 // In the passes after the first only the parked rays run, a few thousand or
 // fewer, and each walks alone through a chain of dependent loads while the
@@ -374,6 +341,11 @@ __device__ __forceinline__ void pass_ray(const RenderArgs& a, uint32_t id, bool 
         // deferred nodes of earlier passes (k < nprev, in walk order), then
         // the walk itself (from the root in the first pass, from the parked
         // node in later ones; `adv` = advance past `node` before using it).
+        // A node whose brick is absent is deferred (its brick requested)
+        // while the list has room, and the walk goes on past it: the first
+        // crossing in a deferred node, marched in a later pass, precedes any
+        // crossing found beyond it, so a walk that ends with nd > 0 leaves a
+        // tentative result.
         int nprev = 0, k = 0;
         uint8_t tent = 0;
         bool adv, descend;
