@@ -614,6 +614,7 @@ int vt_cache_create(vt_context* ctx, int32_t levelC, int32_t cubeDim, int32_t cu
     int64_t S = cubeDim + 2 * (int64_t)cubeInc;
     if (S > 1024) return fail(VT_E_ARG, "brick too large");
     ctx->early_last = INT64_MAX;
+    ctx->pass_state_clean = false;   // the request bitmap may be reallocated
     if (int rc = set_device(ctx)) return rc;
     std::unique_ptr<BrickCache> c(new BrickCache);
     c->level_c = levelC;
@@ -865,8 +866,12 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
         ctx->trace_n = c.nbricks;
         HIP_TRY(hipMemsetAsync(ctx->trace.p, 0, (size_t)c.nbricks, ctx->stream));
     }
-    HIP_TRY(hipMemsetAsync(ctx->d_ctr, 0, sizeof(FrameCounters), ctx->stream));
-    HIP_TRY(hipMemsetAsync(ctx->req_bits.p, 0, ctx->req_bits.n * 4, ctx->stream));
+    // a frame that ended with no request, no range error and no statistics
+    // left the counters and the request bitmap at zero
+    if (!ctx->pass_state_clean || ctx->collect_stats)
+        HIP_TRY(hipMemsetAsync(ctx->d_ctr, 0, sizeof(FrameCounters), ctx->stream));
+    if (!ctx->pass_state_clean) HIP_TRY(hipMemsetAsync(ctx->req_bits.p, 0, ctx->req_bits.n * 4, ctx->stream));
+    ctx->pass_state_clean = false;
 
     RenderArgs a;
     memset(&a, 0, sizeof a);
@@ -936,6 +941,10 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
         else
             HIP_TRY(launch_render_pass(a, ctx->stream));
         HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+        // the counters and the reference bytes (for the service) follow the
+        // pass on its stream, queued before the host polls
+        HIP_TRY(hipMemcpyAsync(ctx->h_ctr, ctx->d_ctr, sizeof(FrameCounters), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(c.h_ref.p, c.d_ref, (size_t)c.gpu_slots, hipMemcpyDeviceToHost, ctx->stream));
         if (early && pass == 0) {
             int64_t loaded = 0, misses = 0;
             if (int rc = load_during_pass(ctx, ctx->ev1, &loaded, &misses)) return rc;
@@ -945,9 +954,6 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
             // the bricks copied beside the pass become visible to the next one
             if (int rc = publish_loads(ctx, ctx->stream)) return rc;
         }
-        HIP_TRY(hipMemcpyAsync(ctx->h_ctr, ctx->d_ctr, sizeof(FrameCounters), hipMemcpyDeviceToHost, ctx->stream));
-        // the reference bytes come back under the same wait, for the service
-        HIP_TRY(hipMemcpyAsync(c.h_ref.p, c.d_ref, (size_t)c.gpu_slots, hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         float ms = 0.0f;
         HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -967,7 +973,10 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
             left.push_back(v - 1);
         }
         memset(rq, 0, (size_t)nreq * 4);
-        if (parked == 0) break;
+        if (parked == 0) {
+            ctx->pass_state_clean = nreq == 0 && ctx->h_ctr->err == 0;
+            break;
+        }
         st.rays_parked += parked;
         double t0 = now_ms();
         // the service's last kernel clears the request bitmap, the request
@@ -1390,6 +1399,7 @@ int vt_stage_raycast(vt_context* ctx, const vt_camera* cam, const vt_render_para
     a.cubes = reinterpret_cast<VisibleCube*>(d_cubes);
     a.rgba = reinterpret_cast<float4*>(d_rgba);
     a.err = &ctx->d_ctr->err;
+    ctx->pass_state_clean = false;
     HIP_TRY(hipMemsetAsync(&ctx->d_ctr->err, 0, 4, ctx->stream));
     HIP_TRY(launch_stage_raycast(a, ctx->stream));
     uint32_t err = 0;
@@ -1422,6 +1432,7 @@ extern "C" int vt_render_cube(vt_context* ctx, const vt_camera* cam, float iso, 
     a.inc = cubeInc;
     a.rgba = reinterpret_cast<float4*>(d_rgba);
     a.err = &ctx->d_ctr->err;
+    ctx->pass_state_clean = false;
     HIP_TRY(hipMemsetAsync(&ctx->d_ctr->err, 0, 4, ctx->stream));
     HIP_TRY(launch_render_cube(a, ctx->stream));
     HIP_TRY(hipMemcpyAsync(&ctx->h_ctr->err, &ctx->d_ctr->err, 4, hipMemcpyDeviceToHost, ctx->stream));
