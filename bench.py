@@ -333,12 +333,14 @@ def run_c3(ctx, args):
             ctx.set_wide_march(*wide_budgets(args))
         if args.early_reserve is not None:
             ctx.set_early_loads(args.early_reserve)
+        if args.first_nodes is not None:
+            ctx.set_first_pass_budget(args.first_nodes)
         t0 = time.perf_counter()
         ctx.cache_prefetch_host(wait=True)
         log(f"c3 {form}: host tier filled from hdf5 in {time.perf_counter() - t0:.2f}s, "
             f"{ctx.cache_stats()['cpu_resident']} bricks")
         use = net if form == "prefetch" else None
-        ms, pre_ms, uploads, passes, pre_n = [], [], [], [], []
+        ms, pre_ms, uploads, passes, pre_n, kms = [], [], [], [], [], []
         for k in range(args.warmup + args.steps):
             f0 = time.perf_counter()
             pre_n.append(use.prefetch(cams[k], gpu_slots, args.prefetch_min_prob) if use is not None else 0)
@@ -350,13 +352,15 @@ def run_c3(ctx, args):
                 pre_ms.append((f1 - f0) * 1e3)
                 uploads.append(st.bricks_uploaded)
                 passes.append(st.passes)
+                kms.append(st.ms_kernel)
         ctx.synchronize()
         m = np.array(ms)
         legs[form] = {"value": round(R * len(ms) / (m.sum() * 1e-3), 1), "p50_ms": round(float(np.percentile(m, 50)), 3),
                       "p99_ms": round(float(np.percentile(m, 99)), 3),
                       "demand_misses_per_frame": float(np.mean(uploads)), "passes_per_frame": float(np.mean(passes)),
                       "prefetched_per_frame": float(np.mean(pre_n[args.warmup:])),
-                      "prefetch_ms_per_frame": round(float(np.mean(pre_ms)), 3)}
+                      "prefetch_ms_per_frame": round(float(np.mean(pre_ms)), 3),
+                      "kernel_ms_per_frame": round(float(np.mean(kms)), 4)}
         log(f"c3 {form}: {legs[form]}")
         last[form] = (d_hits.download(vt.HIT_DTYPE, (R,)), d_rgba.download(np.float32, (R, 4)))
         ctx.cache_destroy()
@@ -493,6 +497,8 @@ def main() -> int:
     ap.add_argument("--max-defer-late", type=int, default=-1, help="max-defer of the passes after the first (-1: same)")
     ap.add_argument("--wide-lanes", type=int, default=None,
                     help="lane budget of the wide march in later passes (0: off; default: the library's)")
+    ap.add_argument("--first-nodes", type=int, default=None,
+                    help="walk nodes per ray in the first pass before it parks (0: no limit; default: the library's)")
     ap.add_argument("--early-reserve", type=int, default=None,
                     help="free slots for loads beside the first pass (0: off; default: the library's)")
     ap.add_argument("--wide-first", type=int, default=None,
@@ -545,6 +551,8 @@ def main() -> int:
         ctx.set_wide_march(*wide_budgets(args))
     if args.early_reserve is not None:
         ctx.set_early_loads(args.early_reserve)
+    if args.first_nodes is not None:
+        ctx.set_first_pass_budget(args.first_nodes)
     cams = [w.cameras(k) for k in range(args.warmup + args.steps)]
     R = w.W * w.H
     d_rgba, d_hits = ctx.alloc(R * 16), ctx.alloc(R * 32)
@@ -601,7 +609,7 @@ def main() -> int:
         "passes_per_frame": float(np.mean(passes)), "bricks_uploaded_per_frame": float(np.mean(uploads)),
         "bricks_early_per_frame": float(np.mean(early)),
         "rays_parked_per_frame": float(np.mean(parked)), "max_defer": args.max_defer,
-        "persistent": args.persistent, "max_defer_late": args.max_defer_late, "wide_lanes": args.wide_lanes, "wide_first": args.wide_first, "early_reserve": args.early_reserve,
+        "persistent": args.persistent, "max_defer_late": args.max_defer_late, "wide_lanes": args.wide_lanes, "wide_first": args.wide_first, "early_reserve": args.early_reserve, "first_nodes": args.first_nodes,
         "bricks_marched_per_frame": float(np.mean(U)),
         "device_miss_rate": round(float(sum(uploads)) / max(1, sum(U)), 5),
         "roofline": roofline,

@@ -207,6 +207,12 @@ int vt_set_wide_march(vt_context *ctx, int64_t lane_budget, int64_t first_pass_b
  * Default 256 (about what the host link moves during a 1080p first pass);
  * 0 loads only between passes. */
 int vt_set_early_loads(vt_context *ctx, int64_t reserve);
+/* Walk budget of the first pass (no reference counterpart; results are
+ * identical for every setting): a ray that has taken `nodes` walk nodes in
+ * the first pass parks on its next node without requesting anything, and
+ * finishes in the later passes (with the wide march).  Default 16; 0: no
+ * limit. */
+int vt_set_first_pass_budget(vt_context *ctx, int32_t nodes);
 /* Visited-brick trace of the last frame: out[i] = 1 if brick
  * (i + 8^levelC) was marched by at least one ray.  n = 8^levelC. */
 int vt_get_brick_trace(vt_context *ctx, uint8_t *out, int64_t n);

@@ -417,6 +417,42 @@ def test_early_loads_from_the_host_tier(vt, vto, ctx, tmp_path):
                 np.frombuffer(outs[reserve][-1][1], np.float32).reshape(R, 4), hc, rc)
 
 
+@pytest.mark.parametrize("slots", [512, 17])
+def test_first_pass_budget_is_exact(vt, vto, ctx, slots):
+    """Rays that stop after a few walk nodes in the first pass and finish in
+    later passes (wide or not) give the same bytes; with every brick
+    resident the budget alone adds the passes."""
+    vol = blobs_volume(64, 16, seed=111)
+    sc = Scene(vto, vol, 0.3, 4, 3)
+    sc.setup_gpu(vt, ctx, slots=slots)
+    cams = [orbit_camera(vt, 160, 120, (32, 32, 32), 100.0, 25.0 + 60.0 * k, 15.0) for k in range(3)]
+    outs, passes = {}, {}
+    try:
+        for budget, wide in ((0, 1 << 18), (1, 1 << 18), (2, 0), (4, 1 << 18), (16, 1 << 18)):
+            log(f"budget={budget} wide={wide} slots={slots}")
+            ctx.set_first_pass_budget(budget)
+            ctx.set_wide_march(wide)
+            ctx.cache_clear()
+            rows, ps = [], []
+            for cam in cams + cams[-1:]:   # the last view twice: the second time every brick it needs is resident
+                rg, hg, st = sc.render_gpu(vt, ctx, cam)
+                assert st.err == 0
+                rows.append((hg.tobytes(), rg.tobytes()))
+                ps.append(st.passes)
+            outs[budget], passes[budget] = rows, ps
+    finally:
+        ctx.set_first_pass_budget()
+        ctx.set_wide_march()
+    for budget, rows in outs.items():
+        assert rows == outs[0], budget
+    print(f"passes by budget: {passes}", flush=True)
+    if slots == 512:   # the repeated view is resident: only the budget splits the walk
+        assert passes[0][-1] == 1 and passes[1][-1] >= 2
+    rc, hc = sc.render_cpu(vto, cams[-1])
+    assert_same(np.frombuffer(outs[1][-1][0], vt.HIT_DTYPE), np.frombuffer(outs[1][-1][1], np.float32).reshape(-1, 4),
+                hc, rc)
+
+
 def test_iso_change_keeps_the_cache_exact(vt, vto, ctx):
     """The sub-block side maps follow the frame's isovalue: a cache filled
     at one iso renders a second iso (with its own octree) exactly."""

@@ -122,6 +122,7 @@ _SIGS = {
     "vt_set_march_options": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "vt_set_wide_march": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int64]),
     "vt_set_early_loads": (ctypes.c_int, [_P, ctypes.c_int64]),
+    "vt_set_first_pass_budget": (ctypes.c_int, [_P, ctypes.c_int32]),
     "vt_trace_nodes": (ctypes.c_int, [_P, ctypes.POINTER(Camera), ctypes.c_int32, ctypes.c_int32,
                                       ctypes.c_int32, _P, _P]),
     "vt_octree_build": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
@@ -405,6 +406,11 @@ class Context:
         """Keep `reserve` free slots before each first pass and copy the
         bricks it requests into them while it runs (0: off)."""
         self._c(lib().vt_set_early_loads(self.handle, int(reserve)))
+
+    def set_first_pass_budget(self, nodes: int = 16) -> None:
+        """Rays park after `nodes` walk nodes in the first pass and finish
+        in the later passes (0: no limit)."""
+        self._c(lib().vt_set_first_pass_budget(self.handle, int(nodes)))
 
     def brick_trace(self, level_c: int) -> np.ndarray:
         n = 1 << (3 * level_c)

@@ -454,6 +454,7 @@ struct vt_context {
     int64_t wide_lanes = 1 << 18;   // lane budget of the wide march in later passes (0: off, -k: 2^k lanes)
     int64_t wide_first = 1 << 21;   // ... and in the first pass
     int64_t early_reserve = 256;    // free slots kept for loads issued while the first pass runs (0: off)
+    int first_nodes = 16;           // walk nodes per ray in the first pass before it parks (0: no limit)
     bool pass_state_clean = false;  // the last frame left counters and request bits at zero
     int64_t early_last = INT64_MAX;  // bricks the previous frame's first pass requested
     int persistent = 0;   // persistent render pass with lane refill (measured slower; option)

@@ -334,7 +334,8 @@ __device__ __forceinline__ void pass_ray(const RenderArgs& a, uint32_t id, bool 
         // what the pass leaves: WALK_HIT/WALK_MISS = c/h to write, WALK_ABSENT =
         // park on `node`, kKeep = the output already holds the ray's result
         // (final when nd == 0), kBlocked = a deferred node is still absent
-        constexpr int kKeep = 3, kBlocked = 4;
+        constexpr int kKeep = 3, kBlocked = 4, kBudget = 5;
+        int walked = 0;   // walk nodes taken in this pass
         int st;
         // One loop over the nodes this pass marches, so that the march, the
         // shading and the traversal step are each inlined once: first the
@@ -380,6 +381,13 @@ __device__ __forceinline__ void pass_ray(const RenderArgs& a, uint32_t id, bool 
                     st = WALK_MISS;
                     break;
                 }
+                // a ray that has walked its budget parks on its next node,
+                // requesting nothing, and goes on in a later pass
+                if (a.node_budget > 0 && walked >= a.node_budget) {
+                    st = kBudget;
+                    break;
+                }
+                walked += 1;
                 cur = node;
             }
             const uint64_t brick = cur >> bv.shift;
@@ -419,9 +427,9 @@ __device__ __forceinline__ void pass_ray(const RenderArgs& a, uint32_t id, bool 
             }
         }
         if (g.leader) {
-            if (st == WALK_ABSENT) {
-                request_brick(a, bidx);
-                a.ray_node[id] = node;
+            if (st == WALK_ABSENT || st == kBudget) {
+                if (st == WALK_ABSENT) request_brick(a, bidx);
+                a.ray_node[id] = node;   // not yet marched
                 a.ray_phase[id] = (uint8_t)nd;
                 park = true;
             } else if (st == WALK_HIT || st == WALK_MISS) {

@@ -814,6 +814,14 @@ int vt_set_wide_march(vt_context* ctx, int64_t lane_budget, int64_t first_pass_b
     return VT_OK;
 }
 
+int vt_set_first_pass_budget(vt_context* ctx, int32_t nodes)
+{
+    if (!ctx) return fail(VT_E_ARG, "ctx is NULL");
+    if (nodes < 0) return fail(VT_E_ARG, "nodes must be >= 0");
+    ctx->first_nodes = nodes;
+    return VT_OK;
+}
+
 int vt_set_early_loads(vt_context* ctx, int64_t reserve)
 {
     if (!ctx) return fail(VT_E_ARG, "ctx is NULL");
@@ -892,6 +900,7 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
     a.ray_phase = ctx->ray_phase.p;
     a.ray_defer = std::max(ctx->max_defer, ctx->max_defer_late) > 0 ? ctx->ray_defer.p : nullptr;
     a.max_defer = ctx->max_defer;
+    a.node_budget = ctx->first_nodes;
     a.req_bits = ctx->req_bits.p;
     a.req_list = ctx->req_list.p;
     a.slot_ref = c.d_ref;
@@ -993,6 +1002,7 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
         a.in_list = ctx->list[cur].p;
         a.in_count = parked;
         a.max_defer = ctx->max_defer_late;   // few rays remain: let them walk further per pass
+        a.node_budget = 0;                   // and to the end of their walk
         cur ^= 1;
         a.out_list = ctx->list[cur].p;
         if (pass > 1000000) return fail(VT_E_STATE, "frame did not converge");
