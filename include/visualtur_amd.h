@@ -376,6 +376,11 @@ typedef struct {
 } vt_serve_stats;
 int vt_server_create(vt_context *ctx, int64_t hbm_budget_bytes, int32_t max_tiles, vt_server **out);
 int vt_server_destroy(vt_server *sv);
+/* Lanes per ray of the serving launch (results are identical for every
+ * setting): a batch gets 2^k lanes per ray (k <= 3), the largest with
+ * rays x 2^k <= lane_budget (default 2097152), so light batches march each
+ * ray with several lanes; 0 keeps one lane per ray; -k fixes 2^k. */
+int vt_server_set_lanes(vt_server *sv, int64_t lane_budget);
 /* A volume in HBM (C order [X][Y][Z]) becomes a scene: octree levels
  * 0..levelO, bricks of level levelC with cubeInc halo, all resident. */
 int vt_server_add_scene(vt_server *sv, const float *d_vol, int32_t X, int32_t Y, int32_t Z, int32_t levelO,

@@ -49,6 +49,36 @@ def test_mixed_requests_same_alone_and_in_the_mix(vt, vto, ctx):
     db.free()
 
 
+def test_lane_widths_give_the_same_bytes(vt, vto, ctx):
+    """The serving launch with 1, 2, 4 and 8 lanes per ray (and the lane
+    budget's choice) renders every request to the same bytes."""
+    from visualtur_amd.prefetch import random_walk
+
+    vol = blobs_volume(64, 16, seed=15)
+    sv = vt.Server(ctx, hbm_budget=64 << 20, max_tiles=64)
+    dv = ctx.to_device(vol)
+    sc = sv.add_scene(dv, vol.shape, 4, 2)
+    sizes = [(64, 64), (96, 80), (37, 23), (17, 200)]
+    reqs = [{"scene": sc, "cam": random_walk(*sizes[k % 4], 64, 1, seed=300 + k)[0], "iso": [0.2, 0.35][k % 2],
+             "arrival_s": 0.0, "keep": True} for k in range(8)]
+    outs = {}
+    try:
+        for lanes in (0, -1, -2, -3, 1 << 21):
+            sv.set_lanes(lanes)
+            _, _, out = sv.run(reqs, mode=0)
+            outs[lanes] = [(out[k][1].tobytes(), out[k][0].view(np.uint32).tobytes()) for k in range(len(reqs))]
+    finally:
+        sv.set_lanes()
+    for lanes, rows in outs.items():
+        assert rows == outs[0], lanes
+    n, runs = vto.build_octree(vol, reqs[0]["iso"], literal=False)
+    rc, hc, _ = vto.render(vto.Volume(vol), vto.Octree(runs, n, 4), reqs[0]["cam"], vto.params(4, 2, reqs[0]["iso"]))
+    hg = np.frombuffer(outs[-3][0][0], vt.HIT_DTYPE)
+    assert np.array_equal(hg["node"], hc["node"]) and np.array_equal(hg["voxel"], hc["voxel"])
+    sv.close()
+    dv.free()
+
+
 def test_admission_respects_the_budget(vt, ctx):
     vol = blobs_volume(32, 6, seed=1)
     sv = vt.Server(ctx, hbm_budget=2 << 20, max_tiles=64)

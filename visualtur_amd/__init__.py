@@ -137,6 +137,7 @@ _SIGS = {
     "vt_copy_probe": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
     "vt_server_create": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(_P)]),
     "vt_server_destroy": (ctypes.c_int, [_P]),
+    "vt_server_set_lanes": (ctypes.c_int, [_P, ctypes.c_int64]),
     "vt_server_add_scene": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                            ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "vt_server_run": (ctypes.c_int, [_P, _P, ctypes.c_int64, ctypes.c_int32, _P, ctypes.POINTER(ServeStats)]),
@@ -500,6 +501,11 @@ class Server:
         h = ctypes.c_void_p()
         _check(lib().vt_server_create(ctx.handle, hbm_budget, max_tiles, ctypes.byref(h)), ctx.handle)
         self.ctx, self.handle = ctx, h.value
+
+    def set_lanes(self, lane_budget: int = 1 << 21) -> None:
+        """Lanes per ray of a batch: the widest 2^k (k <= 3) within the lane
+        budget; 0 one lane, -k a fixed 2^k."""
+        _check(lib().vt_server_set_lanes(self.handle, int(lane_budget)), self.ctx.handle)
 
     def add_scene(self, d_vol: DeviceBuffer, dims, level_o: int, level_c: int, cube_inc: int = 3) -> int:
         sid = ctypes.c_int32()

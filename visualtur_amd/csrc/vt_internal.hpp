@@ -157,7 +157,8 @@ struct ServeCounters {
 };
 
 // launchers (vt_kernels.hip)
-hipError_t launch_serve_pass(const ServeJob* jobs, const ServeTile* work, ServeCounters* ctr, int max_tiles,
+// the serving pass over ctr->nwork tiles (of at most max_tiles), 2^lg lanes per ray
+hipError_t launch_serve_pass(const ServeJob* jobs, const ServeTile* work, ServeCounters* ctr, int max_tiles, int lg,
                              hipStream_t s);
 // vt_build.hip: min and max over the (s+1)^3 zero-padded samples of every
 // node of one level (s = 2^(n-level)); one workgroup per node

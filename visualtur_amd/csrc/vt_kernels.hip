@@ -79,31 +79,6 @@ __device__ __forceinline__ bool visit_node(const Ray& r, const OctreeView& oct, 
     return true;
 }
 
-// Walks one ray from `node` (the next node to march; 0 = exhausted) until it
-// hits, runs out of nodes, or needs a brick that is not resident.
-//   WALK_HIT    : out/h hold the shaded hit
-//   WALK_MISS   : out/h hold the white background and an empty record
-//   WALK_ABSENT : node = the node whose brick (index bidx) is missing
-__device__ __forceinline__ int walk_ray(const Ray& r, const OctreeView& oct, const BrickView& bv, const Shading& sh,
-                                        uint64_t& node, uint8_t* slot_ref, uint8_t* brick_trace, float4& out,
-                                        HitRecord& h, uint64_t& bidx, uint32_t& err, uint32_t& nsamples,
-                                        uint32_t& nnodes, OrderCache& oc)
-{
-    for (;;) {
-        if (node == 0) {
-            set_miss(out, h);
-            return WALK_MISS;
-        }
-        uint64_t brick = node >> bv.shift;
-        bidx = brick - bv.brick_base;
-        int slot = bv.brick_slot[bidx];
-        if (slot < 0) return WALK_ABSENT;
-        if (visit_node(r, oct, bv, sh, node, brick, slot, slot_ref, brick_trace, out, h, err, nsamples, nnodes))
-            return WALK_HIT;
-        node = next_node(r, oct, node, false, oc);
-    }
-}
-
 // Adds brick `bidx` to this pass's request set.  The first request appends
 // bidx + 1 to the list in coherent host memory with a system-scope store,
 // so the host can load the brick while this pass is still running.
@@ -793,46 +768,81 @@ hipError_t launch_render_wide(const RenderArgs& a, int lg, hipStream_t s)
 }
 
 // --------------------------------------------------------- serving pass
-// One workgroup per 16x16 tile of any admitted request: requests of
-// different sizes, scenes and isovalues share one launch.  Every brick of a
-// serving scene is resident, so each ray finishes in the launch.  The grid
-// has a fixed size (the graph replays it); blocks past the tile count exit.
-__global__ __launch_bounds__(kBlock) void k_serve_pass(const ServeJob* __restrict__ jobs,
-                                                       const ServeTile* __restrict__ work, ServeCounters* ctr)
+// The tiles of every admitted request (16x16 pixels, any size, scene or
+// isovalue) in one launch.  A workgroup is one wave that renders an 8x8
+// quarter of a tile (a group of 64 rays) with 2^lg lanes per ray (the wide
+// march for light batches); the 2^lg waves of a group go to one XCD, as in
+// k_render_wide.  Every brick of a serving scene is resident, so each ray
+// finishes in the launch.  The grid has a fixed size per lg (a captured
+// graph replays it); workgroups past the batch's groups exit.
+template <bool kWide>
+__device__ __forceinline__ void serve_ray(const ServeJob& J, int i, int j, const Lanes& g, uint32_t& err)
 {
-    if (blockIdx.x >= ctr->nwork) return;
-    const ServeTile t = work[blockIdx.x];
-    const ServeJob& J = jobs[t.job];
-    const int W = J.cam.W, H = J.cam.H;
-    int ty = (int)t.tile / J.tiles_x, tx = (int)t.tile % J.tiles_x;
-    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int j = tx * 16 + (w & 1) * 8 + (lane & 7);
-    int i = ty * 16 + (w >> 1) * 8 + (lane >> 3);
-    if (i >= H || j >= W) return;
-    uint32_t id = (uint32_t)(i * W + j);
     Ray r;
     make_ray(J.cam, i, j, r);
     OrderCache oc;
     uint64_t node = root_ok(r, J.oct) ? next_node(r, J.oct, 1ull, true, oc) : 0ull;
     float4 c;
     HitRecord h;
-    uint64_t bidx;
-    uint32_t err = 0, ns = 0, nn = 0;
-    int st = walk_ray(r, J.oct, J.bv, J.sh, node, nullptr, nullptr, c, h, bidx, err, ns, nn, oc);
-    if (st == WALK_ABSENT) {   // cannot happen with a fully resident scene
-        err |= 2u;
-        c = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-        h.node = 0; h.brick = 0; h.voxel[0] = h.voxel[1] = h.voxel[2] = 0; h.depth = 0.0f;
+    uint32_t ns = 0, nn = 0;
+    for (;;) {
+        if (node == 0) {
+            set_miss(c, h);
+            break;
+        }
+        const uint64_t brick = node >> J.bv.shift;
+        const int slot = J.bv.brick_slot[brick - J.bv.brick_base];
+        if (slot < 0) {   // cannot happen with a fully resident scene
+            err |= 2u;
+            set_miss(c, h);
+            break;
+        }
+        if (visit_node_g<kWide>(r, J.oct, J.bv, J.sh, node, brick, slot, nullptr, nullptr, c, h, err, ns, nn, g))
+            break;
+        node = next_node(r, J.oct, node, false, oc);
     }
-    J.rgba[id] = c;
-    J.hits[id] = h;
+    if (g.leader) {
+        const uint32_t id = (uint32_t)(i * J.cam.W + j);
+        J.rgba[id] = c;
+        J.hits[id] = h;
+    }
+}
+
+template <bool kWide>
+__global__ __launch_bounds__(kPassBlock) void k_serve_pass(const ServeJob* __restrict__ jobs,
+                                                           const ServeTile* __restrict__ work, ServeCounters* ctr,
+                                                           int lg)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    const uint32_t b = blockIdx.x;
+    const uint32_t grp = ((b >> 3) >> lg) * 8u + (b & 7u);
+    if (grp >= ctr->nwork * 4u) return;
+    const uint32_t w = (b >> 3) & ((1u << lg) - 1u);
+    const uint32_t p = (w << (6 - lg)) + ((uint32_t)lane >> lg);   // ray of the group, 0..63
+    const ServeTile t = work[grp >> 2];
+    const ServeJob& J = jobs[t.job];
+    const int ty = (int)t.tile / J.tiles_x, tx = (int)t.tile % J.tiles_x;
+    const int q = (int)(grp & 3u);
+    const int i = ty * 16 + (q >> 1) * 8 + (int)(p >> 3);
+    const int j = tx * 16 + (q & 1) * 8 + (int)(p & 7);
+    if (i >= J.cam.H || j >= J.cam.W) return;   // the whole group
+    const int L = 1 << lg;
+    const Lanes g{L, lane & (L - 1), lane & ~(L - 1), (lane & (L - 1)) == 0};
+    uint32_t err = 0;
+    serve_ray<kWide>(J, i, j, g, err);
     if (err) atomicOr(&ctr->err, err);
 }
 
-hipError_t launch_serve_pass(const ServeJob* jobs, const ServeTile* work, ServeCounters* ctr, int max_tiles,
+hipError_t launch_serve_pass(const ServeJob* jobs, const ServeTile* work, ServeCounters* ctr, int max_tiles, int lg,
                              hipStream_t s)
 {
-    hipLaunchKernelGGL(k_serve_pass, dim3(max_tiles), dim3(kBlock), 0, s, jobs, work, ctr);
+    if (lg < 0 || lg > 6 || max_tiles <= 0) return hipErrorInvalidValue;
+    const uint64_t groups = 4ull * (uint64_t)max_tiles;
+    const unsigned grid = (unsigned)(((groups + 7) / 8 * 8) << lg);
+    if (lg == 0)
+        hipLaunchKernelGGL(k_serve_pass<false>, dim3(grid), dim3(kPassBlock), 0, s, jobs, work, ctr, lg);
+    else
+        hipLaunchKernelGGL(k_serve_pass<true>, dim3(grid), dim3(kPassBlock), 0, s, jobs, work, ctr, lg);
     return hipGetLastError();
 }
 

@@ -91,12 +91,17 @@ struct vt_server {
     PinBuf<ServeTile> h_work;
     PinBuf<ServeCounters> h_ctr;
     PinBuf<ServeJob> h_jobs;   // pinned staging, one entry per job slot (async copies read it later)
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // one captured launch per lane width 2^lg (lg = 0..kServeWidths-1)
+    static constexpr int kServeWidths = 4;
+    hipGraph_t graph[kServeWidths] = {};
+    hipGraphExec_t exec[kServeWidths] = {};
+    int64_t lane_budget = 1 << 21;   // lanes of a batch (rays x 2^lg); <= 0: fixed 2^-lane_budget lanes
     ~vt_server()
     {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
+        for (int k = 0; k < kServeWidths; ++k) {
+            if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
+            if (graph[k]) (void)hipGraphDestroy(graph[k]);
+        }
         for (auto& s : scenes) {
             if (s->d_pool) (void)hipFree(s->d_pool);
             if (s->d_slot) (void)hipFree(s->d_slot);
@@ -172,15 +177,25 @@ int vt_server_create(vt_context* ctx, int64_t hbm_budget_bytes, int32_t max_tile
         return fail(VT_E_NOMEM, "serving buffers");
     HIP_TRY(hipMalloc((void**)&sv->d_ctr, sizeof(ServeCounters)));
     HIP_TRY(hipMemset(sv->d_ctr, 0, sizeof(ServeCounters)));
-    // the batch launch, captured once and replayed: the tile list and count
-    // change between replays, the shape never does
-    HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t le = launch_serve_pass(sv->d_jobs.p, sv->d_work.p, sv->d_ctr, max_tiles, ctx->stream);
-    hipError_t ce = hipStreamEndCapture(ctx->stream, &sv->graph);
-    if (le != hipSuccess) return fail(VT_E_HIP, std::string("capture: ") + hipGetErrorString(le));
-    if (ce != hipSuccess) return fail(VT_E_HIP, std::string("capture: ") + hipGetErrorString(ce));
-    HIP_TRY(hipGraphInstantiate(&sv->exec, sv->graph, nullptr, nullptr, 0));
+    // the batch launch at each lane width, captured once and replayed: the
+    // tile list and count change between replays, the shape never does
+    for (int lg = 0; lg < vt_server::kServeWidths; ++lg) {
+        HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t le = launch_serve_pass(sv->d_jobs.p, sv->d_work.p, sv->d_ctr, max_tiles, lg, ctx->stream);
+        hipError_t ce = hipStreamEndCapture(ctx->stream, &sv->graph[lg]);
+        if (le != hipSuccess) return fail(VT_E_HIP, std::string("capture: ") + hipGetErrorString(le));
+        if (ce != hipSuccess) return fail(VT_E_HIP, std::string("capture: ") + hipGetErrorString(ce));
+        HIP_TRY(hipGraphInstantiate(&sv->exec[lg], sv->graph[lg], nullptr, nullptr, 0));
+    }
     *out = sv.release();
+    return VT_OK;
+}
+
+int vt_server_set_lanes(vt_server* sv, int64_t lane_budget)
+{
+    if (!sv) return fail(VT_E_ARG, "sv is NULL");
+    if (lane_budget < -(vt_server::kServeWidths - 1)) return fail(VT_E_ARG, "lane_budget below the widest width");
+    sv->lane_budget = lane_budget;
     return VT_OK;
 }
 
@@ -367,7 +382,13 @@ int vt_server_run(vt_server* sv, const vt_request* reqs, int64_t nreq, int32_t m
         double l0 = now_ms();
         HIP_TRY(hipMemcpyAsync(sv->d_work.p, sv->h_work.p, (size_t)nw * sizeof(ServeTile), hipMemcpyHostToDevice, s));
         HIP_TRY(hipMemcpyAsync(sv->d_ctr, sv->h_ctr.p, sizeof(ServeCounters), hipMemcpyHostToDevice, s));
-        HIP_TRY(hipGraphLaunch(sv->exec, s));
+        // lanes per ray: the widest that keeps the batch within the lane budget
+        int lg = 0;
+        if (sv->lane_budget > 0)
+            while (lg + 1 < vt_server::kServeWidths && ((int64_t)nw * 256 << (lg + 1)) <= sv->lane_budget) ++lg;
+        else
+            lg = (int)std::min<int64_t>(-sv->lane_budget, vt_server::kServeWidths - 1);
+        HIP_TRY(hipGraphLaunch(sv->exec[lg], s));
         HIP_TRY(hipMemcpyAsync(sv->h_ctr.p, sv->d_ctr, sizeof(ServeCounters), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         busy_ms += now_ms() - l0;
