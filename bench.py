@@ -494,7 +494,7 @@ def main() -> int:
     ap.add_argument("--parity-rays", type=int, default=4096)
     ap.add_argument("--prefetch", type=int, default=1, help="c2: also time the learned-prefetch form")
     ap.add_argument("--persistent", type=int, default=0, help="persistent render pass with lane refill")
-    ap.add_argument("--max-defer-late", type=int, default=-1, help="max-defer of the passes after the first (-1: same)")
+    ap.add_argument("--max-defer-late", type=int, default=2, help="max-defer of the passes after the first (-1: same)")
     ap.add_argument("--wide-lanes", type=int, default=None,
                     help="lane budget of the wide march in later passes (0: off; default: the library's)")
     ap.add_argument("--first-nodes", type=int, default=None,
@@ -503,7 +503,7 @@ def main() -> int:
                     help="free slots for loads beside the first pass (0: off; default: the library's)")
     ap.add_argument("--wide-first", type=int, default=None,
                     help="lane budget of the wide march in the first pass (0: off; default: the library's)")
-    ap.add_argument("--max-defer", type=int, default=2,
+    ap.add_argument("--max-defer", type=int, default=1,
                     help="absent-brick nodes a ray may walk past per pass (0 = park at the first)")
     ap.add_argument("--prefetch-min-prob", type=float, default=0.8)
     ap.add_argument("--train-paths", type=int, default=24)

@@ -179,10 +179,10 @@ int vt_set_frame_options(vt_context *ctx, int collect_stats, int record_trace);
 /* Frame options with no reference counterpart; results are identical for
  * every setting.  skip_empty=1 (default) skips the evaluation of samples
  * whose 4x4x4-cell sub-block lies strictly on one side of iso.  max_defer
- * (0..16, default 2) is how many nodes with an absent brick a ray may walk
+ * (0..16, default 1) is how many nodes with an absent brick a ray may walk
  * past in the first pass (their bricks are requested, and they are marched
  * in a later pass before the ray's tentative result is accepted);
- * max_defer_late (0..16) is the same for the later passes, which carry only
+ * max_defer_late (0..16, default 2) is the same for the later passes, which carry only
  * the parked rays; 0 parks a ray at its first absent brick, as the
  * reference's staged loop does (visualTur.cu:175-220).  persistent=1 runs each pass as a fixed grid whose
  * lanes take a new ray when theirs finishes; 0 (default) runs one lane per

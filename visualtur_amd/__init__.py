@@ -391,8 +391,9 @@ class Context:
                                       d_hits.ptr if d_hits is not None else None, ctypes.byref(st)))
         return st
 
-    def set_march_options(self, skip_empty: bool = True, max_defer: int = 2, persistent: bool = False,
-                          max_defer_late: int | None = None) -> None:
+    def set_march_options(self, skip_empty: bool = True, max_defer: int = 1, persistent: bool = False,
+                          max_defer_late: int | None = 2) -> None:
+        """Defaults are the library's; max_defer_late=None uses max_defer in every pass."""
         late = max_defer if max_defer_late is None else max_defer_late
         self._c(lib().vt_set_march_options(self.handle, int(skip_empty), int(max_defer), int(late), int(persistent)))
 

@@ -449,7 +449,7 @@ struct vt_context {
     FrameCounters* h_ctr = nullptr;
     int collect_stats = 0, record_trace = 0;
     int skip_empty = 1;   // sub-block sample skipping in the march
-    int max_defer = 2;    // absent-brick nodes a ray walks past in the first pass
+    int max_defer = 1;    // absent-brick nodes a ray walks past in the first pass
     int max_defer_late = 2;   // ... and in later passes
     int64_t wide_lanes = 1 << 18;   // lane budget of the wide march in later passes (0: off, -k: 2^k lanes)
     int64_t wide_first = 1 << 21;   // ... and in the first pass
