@@ -330,6 +330,13 @@ int vt_mlp_train_step(vt_mlp *m, const float *d_x, const float *d_y, int64_t N, 
  * [-4, 4], and 4 for a centre behind the camera. */
 int vt_mlp_features(vt_mlp *m, const vt_camera *cam, int32_t nLevels, int32_t levelC, const uint64_t *d_codes,
                     int64_t N, float *d_x);
+/* logits of the bricks d_codes[N] (device, level-levelC Morton codes) for the
+ * camera, into d_logits[N] (device): the features above and the forward pass
+ * fused into one launch (64 bricks per workgroup, activations in LDS), equal
+ * bit for bit to vt_mlp_features followed by vt_mlp_forward.  N is not
+ * bounded by max_batch.  vt_prefetch scores its candidates this way. */
+int vt_mlp_score(vt_mlp *m, const vt_camera *cam, int32_t nLevels, int32_t levelC, const uint64_t *d_codes,
+                 int64_t N, float *d_logits);
 
 /* Score every candidate brick (the bricks holding octree nodes, i.e. the
  * level-levelC run list) for the camera with the MLP, then load the

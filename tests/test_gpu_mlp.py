@@ -107,3 +107,50 @@ def test_features_match_oracle(vt, vto, ctx, net):
         assert np.all(got[(dvec @ look) <= 0][:, 10:12] == 4.0)
         d_c.free()
         d_x.free()
+
+
+def test_fused_score_equals_layer_path(vt, ctx, net):
+    """vt_mlp_score (features and forward pass in one launch) returns the
+    logits of vt_mlp_features followed by vt_mlp_forward bit for bit: partial
+    workgroups, N above max_batch, and after the parameters change (by a
+    training step, then by vt_mlp_set_params)."""
+    rng = np.random.default_rng(5)
+    n_levels = 8
+    x, y = batch(rng, 4096)
+    d_x, d_y = ctx.to_device(x), ctx.to_device(y)
+    p0 = net.params()
+    cams = [vt.make_camera(64, 48, (300.0, 20.0, -90.0), *vt.look_at((300.0, 20.0, -90.0), (120, 140, 100))),
+            vt.make_camera(8, 8, (128.0, 100.0, 700.0))]
+
+    def check(level_c, count):
+        base = np.uint64(1) << np.uint64(3 * level_c)
+        every = base + np.arange(8 ** level_c, dtype=np.uint64)
+        assert count <= every.size
+        codes = np.sort(rng.choice(every, count, replace=False)) if count < every.size else every
+        d_c, d_z = ctx.to_device(codes), ctx.alloc(count * 4)
+        d_xb, d_fb = ctx.alloc(net.max_batch * 48), ctx.alloc(net.max_batch * 4)
+        for cam in cams:
+            net.score(cam, n_levels, level_c, d_c, count, d_z)
+            got = d_z.download(np.float32, (count,))
+            ref = np.empty(count, np.float32)
+            for o in range(0, count, net.max_batch):
+                k = min(net.max_batch, count - o)
+                d_cb = ctx.to_device(codes[o:o + k])
+                net.features(cam, n_levels, level_c, d_cb, k, d_xb)
+                net.forward(d_xb, k, d_fb)
+                ref[o:o + k] = d_fb.download(np.float32, (k,))
+                d_cb.free()
+            assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (level_c, count)
+        for b in (d_c, d_z, d_xb, d_fb):
+            b.free()
+
+    try:
+        net.train_step(d_x, d_y, 4096, lr=1e-3)   # weights away from their initial values
+        for level_c, count in ((3, 1), (3, 77), (3, 512), (5, 32768)):
+            check(level_c, count)
+        net.set_params(p0 * np.float32(0.5) + np.float32(0.01))
+        check(5, 4096 + 65)
+    finally:
+        net.set_params(p0)
+        d_x.free()
+        d_y.free()

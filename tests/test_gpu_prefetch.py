@@ -63,3 +63,92 @@ def test_prefetch_cuts_misses_and_keeps_hits(vt, vto, ctx):
     assert m_pref < 0.7 * m_plain
     net.close()
     d_vol.free()
+
+
+def _threshold(z, q):
+    """A logit threshold near quantile q with no logit (nor negated logit)
+    within 1e-3 of it or of its negation, and its probability."""
+    zs = np.sort(z.astype(np.float64))
+    for qq in np.linspace(q, q + 0.2, 41):
+        t = float(np.quantile(zs, qq))
+        if np.min(np.abs(zs - t)) > 1e-3 and np.min(np.abs(zs + t)) > 1e-3:
+            return t, float(1.0 / (1.0 + np.exp(-t)))
+    raise AssertionError("no clean threshold")
+
+
+def test_prefetch_selection_follows_the_scores(vt, vto, ctx):
+    """vt_prefetch loads exactly the bricks its rule names, counted against
+    the rule applied on the host to the logits of vt_mlp_score: the absent
+    candidates at or above the threshold, best first, up to the budget; into
+    free slots, then in place of residents (lowest logit first) while the
+    resident scores below both the candidate it makes room for and
+    logit(1 - min_prob)."""
+    from visualtur_amd import prefetch as pf
+
+    vol = blobs_volume(128, 48, seed=91)
+    iso, level_o, level_c = 0.3, 4, 3
+    n, runs = vto.build_octree(vol, iso, literal=False)
+    flat = np.concatenate(runs[: level_o + 1]).astype(np.uint64)
+    ctx.octree_upload(flat, [r.size for r in runs[: level_o + 1]], n, level_o)
+    d_vol = ctx.to_device(vol)
+    cand = pf.candidate_codes(runs[level_c])
+    net = vt.MLP(ctx, seed=21, max_batch=4096)
+    cam = vt.make_camera(96, 64, (150.0, 40.0, -60.0), *vt.look_at((150.0, 40.0, -60.0), (64, 64, 64)))
+    d_c, d_z = ctx.to_device(cand), ctx.alloc(cand.size * 4)
+    net.score(cam, n, level_c, d_c, cand.size, d_z)
+    z = d_z.download(np.float32, (cand.size,)).astype(np.float64)
+    assert np.unique(z).size == z.size
+    order = np.lexsort((np.arange(z.size), -z))   # best first, ties by candidate index
+    try:
+        # free slots only: budget-limited, then the rest, then nothing
+        t, p = _threshold(z, 0.6)
+        above = int((z >= t).sum())
+        assert 8 < above < cand.size
+        ctx.cache_create(level_c, 1 << (n - level_c), 3, int(cand.size), 0)
+        ctx.cache_attach_device_volume(d_vol, vol.shape)
+        budget = above // 3
+        assert net.prefetch(cam, budget, p) == budget
+        assert net.prefetch(cam, cand.size, p) == above - budget
+        assert net.prefetch(cam, cand.size, p) == 0
+        assert ctx.cache_stats()["gpu_resident"] == above
+        # a full cache holding the lowest-scoring candidates: loads replace
+        # residents under the rule (a threshold at which it evicts some)
+        slots = int(cand.size) // 3
+        low = order[::-1][:slots]
+        resident = set(low.tolist())
+
+        def rule(t):
+            want = [i for i in order if z[i] >= t and i not in resident]
+            budget = len(want) - 3
+            nw, avail = min(len(want), budget), 0
+            for v in sorted(resident, key=lambda i: z[i]):
+                if avail >= nw or not (z[want[avail]] > z[v] and z[v] < -t):
+                    break
+                avail += 1
+            return want, budget, min(nw, avail), avail
+
+        zs = np.sort(z)
+        for q in np.linspace(0.2, 0.95, 76):
+            t = float(np.quantile(zs, q))
+            if np.min(np.abs(zs - t)) <= 1e-3 or np.min(np.abs(zs + t)) <= 1e-3:
+                continue
+            want, budget, expect, evicted = rule(t)
+            if budget > 0 and evicted > 0:
+                break
+        else:
+            raise AssertionError("no threshold at which the rule evicts")
+        p = float(1.0 / (1.0 + np.exp(-t)))
+        ctx.cache_create(level_c, 1 << (n - level_c), 3, slots, 0)
+        ctx.cache_attach_device_volume(d_vol, vol.shape)
+        ctx.cache_load_bricks(cand[low])
+        ev0 = ctx.cache_stats()["evictions"]
+        print(f"candidates {cand.size} slots {slots} wanted {len(want)} expect {expect} evictions {evicted}",
+              flush=True)
+        assert net.prefetch(cam, budget, p) == expect
+        assert ctx.cache_stats()["evictions"] - ev0 == evicted
+        assert ctx.cache_stats()["gpu_resident"] == slots
+    finally:
+        net.close()
+        d_c.free()
+        d_z.free()
+        d_vol.free()

@@ -155,6 +155,8 @@ _SIGS = {
                                          ctypes.POINTER(ctypes.c_float)]),
     "vt_mlp_features": (ctypes.c_int, [_P, ctypes.POINTER(Camera), ctypes.c_int32, ctypes.c_int32, _P,
                                        ctypes.c_int64, _P]),
+    "vt_mlp_score": (ctypes.c_int, [_P, ctypes.POINTER(Camera), ctypes.c_int32, ctypes.c_int32, _P,
+                                    ctypes.c_int64, _P]),
     "vt_file_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_P),
                                     ctypes.POINTER(ctypes.c_int64)]),
     "vt_file_close": (ctypes.c_int, [_P]),
@@ -493,6 +495,11 @@ class MLP:
     def features(self, cam: Camera, n_levels: int, level_c: int, d_codes: DeviceBuffer, n: int,
                  d_x: DeviceBuffer) -> None:
         _check(lib().vt_mlp_features(self.handle, ctypes.byref(cam), n_levels, level_c, d_codes.ptr, n, d_x.ptr))
+
+    def score(self, cam: Camera, n_levels: int, level_c: int, d_codes: DeviceBuffer, n: int,
+              d_logits: DeviceBuffer) -> None:
+        """Logits of n bricks for the camera in one fused launch (vt_mlp_score)."""
+        _check(lib().vt_mlp_score(self.handle, ctypes.byref(cam), n_levels, level_c, d_codes.ptr, n, d_logits.ptr))
 
 
 class Server:

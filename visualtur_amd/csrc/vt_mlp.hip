@@ -195,16 +195,13 @@ __global__ void k_pack_x(const float* __restrict__ x, int N, bf16* __restrict__ 
 }
 
 // 12 features of (camera, brick): camera position and brick centre in
-// units of the cube (centred), view direction, brick level, distance, and
-// the cosine between the view direction and the direction to the brick
-__global__ void k_features(Camera cam, int n, int level_c, const uint64_t* __restrict__ codes, int N,
-                           float* __restrict__ x)
+// units of the cube (centred), view direction, brick level, and the brick
+// centre's image-plane position (u, v)
+__device__ __forceinline__ void brick_features(const Camera& cam, int n, int level_c, uint64_t code, float* o)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
     const float D = (float)(1 << n);
     int bx, by, bz;
-    node_min(codes[i], level_c, n, bx, by, bz);
+    node_min(code, level_c, n, bx, by, bz);
     const float half = 0.5f * (float)(1 << (n - level_c));
     float c0 = (float)bx + half, c1 = (float)by + half, c2 = (float)bz + half;
     float d0 = c0 - cam.pos[0], d1 = c1 - cam.pos[1], d2 = c2 - cam.pos[2];
@@ -219,7 +216,6 @@ __global__ void k_features(Camera cam, int n, int level_c, const uint64_t* __res
         u = fminf(fmaxf(dr / dl * (2.0f * cam.distance / cam.w), -4.0f), 4.0f);
         v = fminf(fmaxf(du / dl * (2.0f * cam.distance / cam.h), -4.0f), 4.0f);
     }
-    float* o = x + (size_t)i * 12;
     o[0] = cam.pos[0] / D - 0.5f;
     o[1] = cam.pos[1] / D - 0.5f;
     o[2] = cam.pos[2] / D - 0.5f;
@@ -234,12 +230,165 @@ __global__ void k_features(Camera cam, int n, int level_c, const uint64_t* __res
     o[11] = v;
 }
 
+__global__ void k_features(Camera cam, int n, int level_c, const uint64_t* __restrict__ codes, int N,
+                           float* __restrict__ x)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    brick_features(cam, n, level_c, codes[i], x + (size_t)i * 12);
+}
+
+// WT[n][k] = W[k][n] for the three hidden layers' BF16 weights (W1 32x256,
+// W2 and W3 256x256, row-major [k][n] in Pb), packed back to back
+__global__ void k_pack_wt(const bf16* __restrict__ W1, const bf16* __restrict__ W2, const bf16* __restrict__ W3,
+                          bf16* __restrict__ WT)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int n1 = 256 * 32, n2 = 256 * 256;
+    if (e < n1) {
+        int nn = e / 32, k = e % 32;
+        WT[e] = W1[k * 256 + nn];
+    } else if (e < n1 + 2 * n2) {
+        int f = (e - n1) % n2, nn = f / 256, k = f % 256;
+        WT[e] = (e < n1 + n2 ? W2 : W3)[k * 256 + nn];
+    }
+}
+
+// Scoring: features -> three 256-wide layers -> logit for 64 candidates per
+// 256-thread workgroup.  The activations stay in LDS between layers (one
+// 64x256 BF16 buffer, overwritten in place once every wave holds its layer's
+// accumulators in registers); the transposed weights are read as MFMA B operands
+// straight from L2 (16 bytes per lane).  Every output is accumulated with
+// v_mfma_f32_16x16x32_bf16 over k = 0, 32, 64, ... as in k_gemm, and the bias,
+// ReLU, BF16 rounding and logit reduction are k_gemm's and k_logit's, so the
+// logits equal the layer-by-layer path's bit for bit.
+constexpr int SR = 64, SH = 256, LDH = SH + 8;
+
+// H[r][c] = bf16(relu(sum_k H[r][k] WT[c][k] + bias[c])) for the 64 columns
+// of this wave; the first barrier keeps every wave's reads of the layer's
+// input ahead of the writes of its output
+template <int K>
+__device__ __forceinline__ void score_layer(bf16 (*H)[LDH], const bf16* __restrict__ WT,
+                                            const float* __restrict__ bias, int wave, int lane)
+{
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const int kq = 8 * (lane >> 4), l16 = lane & 15, c0 = 64 * wave;
+#pragma unroll 2
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        bf16x8 b[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            b[j] = *reinterpret_cast<const bf16x8*>(WT + (size_t)(c0 + 16 * j + l16) * K + k0 + kq);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            bf16x8 a = *reinterpret_cast<const bf16x8*>(&H[16 * i + l16][k0 + kq]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int col = c0 + 16 * j + l16;
+            const float bc = bias[col];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                float x = acc[i][j][v] + bc;
+                H[16 * i + 4 * (lane >> 4) + v][col] = (bf16)(x > 0.0f ? x : 0.0f);
+            }
+        }
+}
+
+// Writes z[i] for the N candidates.  With a page table: a resident
+// candidate's logit goes to slot_score[its slot]; an absent one at or above
+// zmin is appended to (want_idx, want_z) through the counter *nwant.
+__global__ __launch_bounds__(256) void k_score(Camera cam, int n, int level_c, const uint64_t* __restrict__ codes,
+                                               int N, const bf16* __restrict__ WT, const float* __restrict__ P,
+                                               ScoreParams sp, float* __restrict__ z, ScoreFilter f)
+{
+    __shared__ __attribute__((aligned(16))) bf16 H[SR][LDH];
+    const int r0 = blockIdx.x * SR;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    if (tid < SR) {
+        float x[12];
+        const int i = r0 + tid;
+        if (i < N) {
+            brick_features(cam, n, level_c, codes[i], x);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 12; ++j) x[j] = 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < 32; ++j) H[tid][j] = (bf16)(j < 12 ? x[j] : 0.0f);
+    }
+    __syncthreads();
+    score_layer<32>(H, WT, P + sp.b1, wave, lane);
+    __syncthreads();
+    score_layer<SH>(H, WT + SH * 32, P + sp.b2, wave, lane);
+    __syncthreads();
+    score_layer<SH>(H, WT + SH * 32 + SH * SH, P + sp.b3, wave, lane);
+    __syncthreads();
+    const float* w4 = P + sp.w4;
+    for (int r = 16 * wave; r < 16 * wave + 16; ++r) {
+        float s = 0.0f;
+        for (int j = lane; j < SH; j += 64) s += (float)H[r][j] * w4[j];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+        const int i = r0 + r;
+        if (lane != 0 || i >= N) continue;
+        const float zi = s + P[sp.b4];
+        z[i] = zi;
+        if (!f.table) continue;
+        const int32_t slot = f.table[codes[i] - f.base];
+        if (slot >= 0) {
+            f.slot_score[slot] = zi;
+        } else if (zi >= f.zmin) {
+            const uint32_t k = atomicAdd(f.nwant, 1u);
+            f.want_idx[k] = i;
+            f.want_z[k] = zi;
+        }
+    }
+}
+
+__global__ void k_fill_f32(float* __restrict__ x, int n, float v)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
 }  // namespace
 
 hipError_t launch_pack_x(const float* x, int N, bf16* X, hipStream_t s)
 {
     size_t total = (size_t)N * 32;
     hipLaunchKernelGGL(k_pack_x, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, x, N, X);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_wt(const bf16* W1, const bf16* W2, const bf16* W3, bf16* WT, hipStream_t s)
+{
+    constexpr int total = 256 * 32 + 2 * 256 * 256;
+    hipLaunchKernelGGL(k_pack_wt, dim3((total + 255) / 256), dim3(256), 0, s, W1, W2, W3, WT);
+    return hipGetLastError();
+}
+
+hipError_t launch_score(const Camera& cam, int n, int level_c, const uint64_t* codes, int N, const bf16* WT,
+                        const float* P, const ScoreParams& sp, float* z, const ScoreFilter& f, hipStream_t s)
+{
+    if (N <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_score, dim3((N + SR - 1) / SR), dim3(256), 0, s, cam, n, level_c, codes, N, WT, P, sp, z, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f32(float* x, int n, float v, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_f32, dim3((n + 255) / 256), dim3(256), 0, s, x, n, v);
     return hipGetLastError();
 }
 

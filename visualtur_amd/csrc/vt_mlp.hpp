@@ -32,4 +32,26 @@ hipError_t launch_pack_x(const float* x, int N, bf16* X, hipStream_t s);
 hipError_t launch_features(const Camera& cam, int n, int level_c, const uint64_t* codes, int N, float* x,
                            hipStream_t s);
 
+// offsets (in floats) of the biases and the logit weights in the fp32 parameters
+struct ScoreParams {
+    int b1, b2, b3, w4, b4;
+};
+// optional filtering of the scored candidates against the page table
+// (table == nullptr: logits only); codes[i] - base indexes the table
+struct ScoreFilter {
+    const int32_t* table = nullptr;
+    uint64_t base = 0;
+    float zmin = 0.0f;
+    float* slot_score = nullptr;   // logit of each resident candidate, by slot
+    uint32_t* nwant = nullptr;     // absent candidates with logit >= zmin: count,
+    int32_t* want_idx = nullptr;   // candidate index
+    float* want_z = nullptr;       // and logit, in no particular order
+};
+// WT = the hidden layers' BF16 weights transposed to [n][k] (256x32, 256x256, 256x256)
+hipError_t launch_pack_wt(const bf16* W1, const bf16* W2, const bf16* W3, bf16* WT, hipStream_t s);
+// fused features + MLP forward of N candidate bricks: logits into z[N]
+hipError_t launch_score(const Camera& cam, int n, int level_c, const uint64_t* codes, int N, const bf16* WT,
+                        const float* P, const ScoreParams& sp, float* z, const ScoreFilter& f, hipStream_t s);
+hipError_t launch_fill_f32(float* x, int n, float v, hipStream_t s);
+
 }  // namespace vt

@@ -30,12 +30,31 @@ struct vt_mlp {
     DevBuf<float> z, dz, y, loss;
     DevBuf<float> feats;
     DevBuf<uint64_t> codes;
-    PinBuf<float> h_score;        // prefetch scores of the candidates
+    // fused scoring: transposed BF16 hidden-layer weights (rebuilt after
+    // every change of the parameters), candidate logits, and the filtered
+    // results (absent candidates at or above the threshold, resident
+    // candidates' logits by slot) with their pinned host copies
+    DevBuf<bf16> WT;
+    bool wt_stale = true;
+    DevBuf<float> zc, want_z, slot_score;
+    DevBuf<int32_t> want_idx;
+    DevBuf<uint32_t> nwant;
+    PinBuf<float> h_want_z, h_slot_score;
+    PinBuf<int32_t> h_want_idx;
+    PinBuf<uint32_t> h_nwant;
+    // eviction staging of vt_prefetch (reused after the next call's first wait)
+    PinBuf<int64_t> h_eidx;
+    PinBuf<int32_t> h_eval;
+    DevBuf<int64_t> d_eidx;
+    DevBuf<int32_t> d_eval;
     ~vt_mlp()
     {
         P.release(); G.release(); Mo.release(); Vo.release(); Pb.release();
         X.release(); H1.release(); H2.release(); H3.release(); dH1.release(); dH2.release(); dH3.release();
-        z.release(); dz.release(); y.release(); loss.release(); feats.release(); codes.release(); h_score.release();
+        z.release(); dz.release(); y.release(); loss.release(); feats.release(); codes.release();
+        WT.release(); zc.release(); want_z.release(); slot_score.release(); want_idx.release(); nwant.release();
+        h_want_z.release(); h_slot_score.release(); h_want_idx.release(); h_nwant.release();
+        h_eidx.release(); h_eval.release(); d_eidx.release(); d_eval.release();
     }
 };
 
@@ -65,7 +84,24 @@ void to_exposed(const std::vector<float>& in, float* out)
 int refresh_bf16(vt_mlp* m)
 {
     HIP_TRY(launch_to_bf16(m->P.p, m->Pb.p, kParams, m->ctx->stream));
+    m->wt_stale = true;
     return VT_OK;
+}
+
+// the scoring kernel's transposed weights, from the current BF16 copies
+int ensure_wt(vt_mlp* m)
+{
+    if (!m->wt_stale) return VT_OK;
+    if (m->WT.ensure((size_t)kHid * kInPad + 2 * (size_t)kHid * kHid) != hipSuccess)
+        return fail(VT_E_NOMEM, "scoring weights");
+    HIP_TRY(launch_pack_wt(m->Pb.p + kW1, m->Pb.p + kW2, m->Pb.p + kW3, m->WT.p, m->ctx->stream));
+    m->wt_stale = false;
+    return VT_OK;
+}
+
+ScoreParams score_params()
+{
+    return ScoreParams{(int)kB1, (int)kB2, (int)kB3, (int)kW4, (int)kB4};
 }
 
 int forward(vt_mlp* m, const float* d_x, int N)
@@ -234,6 +270,7 @@ int vt_mlp_train_step(vt_mlp* m, const float* d_x, const float* d_y, int64_t N, 
     if (int rc = forward(m, d_x, (int)N)) return rc;
     if (int rc = backward(m, d_y, (int)N, pos_weight)) return rc;
     m->step++;
+    m->wt_stale = true;
     HIP_TRY(launch_adam(m->P.p, m->G.p, m->Mo.p, m->Vo.p, m->Pb.p, (int)kParams, lr, 0.9f, 0.999f, 1e-8f, m->step,
                         m->ctx->stream));
     if (loss_out) {
@@ -259,6 +296,21 @@ int vt_mlp_features(vt_mlp* m, const vt_camera* cam, int32_t nLevels, int32_t le
 }
 
 }  // extern "C"
+
+int vt_mlp_score(vt_mlp* m, const vt_camera* cam, int32_t nLevels, int32_t levelC, const uint64_t* d_codes,
+                 int64_t N, float* d_logits)
+{
+    if (!m || !cam || !d_codes || !d_logits) return fail(VT_E_ARG, "NULL argument");
+    if (N <= 0 || N > INT32_MAX) return fail(VT_E_ARG, "N must be in [1, 2^31)");
+    if (levelC < 0 || levelC > nLevels || nLevels > 20) return fail(VT_E_ARG, "bad levels");
+    Camera c;
+    memcpy(&c, cam, sizeof c);
+    if (int rc = ensure_wt(m)) return rc;
+    HIP_TRY(launch_score(c, nLevels, levelC, d_codes, (int)N, m->WT.p, m->P.p, score_params(), d_logits, ScoreFilter{},
+                         m->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    return VT_OK;
+}
 
 // ====================================================== learned prefetch
 
@@ -297,80 +349,128 @@ extern "C" int vt_prefetch(vt_context* ctx, vt_mlp* m, const vt_camera* cam, int
     const int64_t NC = (int64_t)ctx->cand_codes.size();
     int64_t nloaded = 0;
     if (NC > 0 && budget > 0) {
-        // score every candidate (chunks of the MLP batch size)
+        // score every candidate in one fused launch; the kernel filters them
+        // against the page table, which must first take in the page-table
+        // writes of loads still publishing on the copy stream
+        if (NC > INT32_MAX) return fail(VT_E_STATE, "too many candidate bricks");
         Camera k;
         memcpy(&k, cam, sizeof k);
-        if (m->h_score.ensure((size_t)NC) != hipSuccess) return fail(VT_E_NOMEM, "score staging");
-        float* score = m->h_score.p;   // pinned: the copies below are true DMA
-        for (int64_t o = 0; o < NC; o += m->max_batch) {
-            int nb = (int)std::min<int64_t>(m->max_batch, NC - o);
-            HIP_TRY(launch_features(k, ctx->n, c.level_c, ctx->d_cand.p + o, nb, m->feats.p, s));
-            if (int rc = forward(m, m->feats.p, nb)) return rc;
-            HIP_TRY(hipMemcpyAsync(score + o, m->z.p, (size_t)nb * 4, hipMemcpyDeviceToHost, s));
-        }
-        HIP_TRY(hipStreamSynchronize(s));
+        const size_t nc = (size_t)NC, ns = (size_t)c.gpu_slots;
+        if (m->zc.ensure(nc) != hipSuccess || m->want_z.ensure(nc) != hipSuccess ||
+            m->want_idx.ensure(nc) != hipSuccess || m->nwant.ensure(1) != hipSuccess ||
+            m->slot_score.ensure(ns) != hipSuccess || m->h_want_z.ensure(nc) != hipSuccess ||
+            m->h_want_idx.ensure(nc) != hipSuccess || m->h_nwant.ensure(1) != hipSuccess ||
+            m->h_slot_score.ensure(ns) != hipSuccess)
+            return fail(VT_E_NOMEM, "scoring buffers");
+        if (int rc = ensure_wt(m)) return rc;
         const uint64_t base = 1ull << (3 * c.level_c);
         const float zmin = logf(min_prob / (1.0f - min_prob));   // logit threshold (-inf at 0)
-        std::vector<int64_t> want;   // absent candidates above the threshold, best first
-        for (int64_t i = 0; i < NC; ++i) {
-            int64_t b = (int64_t)(ctx->cand_codes[(size_t)i] - base);
-            if (c.table[(size_t)b] < 0 && score[(size_t)i] >= zmin) want.push_back(i);
+        HIP_TRY(hipEventRecord(ctx->ev_copy, ctx->cstream));
+        HIP_TRY(hipStreamWaitEvent(s, ctx->ev_copy, 0));
+        HIP_TRY(hipMemsetAsync(m->nwant.p, 0, sizeof(uint32_t), s));
+        HIP_TRY(launch_fill_f32(m->slot_score.p, (int)ns, -INFINITY, s));   // slots without a candidate
+        ScoreFilter f;
+        f.table = c.d_table;
+        f.base = base;
+        f.zmin = zmin;
+        f.slot_score = m->slot_score.p;
+        f.nwant = m->nwant.p;
+        f.want_idx = m->want_idx.p;
+        f.want_z = m->want_z.p;
+        HIP_TRY(launch_score(k, ctx->n, c.level_c, ctx->d_cand.p, (int)NC, m->WT.p, m->P.p, score_params(), m->zc.p, f,
+                             s));
+        // the count, the head of the wanted list and the slot scores in one wait
+        const size_t head = std::min<size_t>(nc, 16384);
+        HIP_TRY(hipMemcpyAsync(m->h_nwant.p, m->nwant.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(m->h_want_idx.p, m->want_idx.p, head * 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(m->h_want_z.p, m->want_z.p, head * 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(m->h_slot_score.p, m->slot_score.p, ns * 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        const size_t nw = std::min<size_t>(*m->h_nwant.p, nc);
+        if (nw > head) {
+            HIP_TRY(hipMemcpyAsync(m->h_want_idx.p + head, m->want_idx.p + head, (nw - head) * 4,
+                                   hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(m->h_want_z.p + head, m->want_z.p + head, (nw - head) * 4, hipMemcpyDeviceToHost,
+                                   s));
+            HIP_TRY(hipStreamSynchronize(s));
         }
-        // only the best `n` are loaded: order just those (ties by candidate order)
-        int64_t n = std::min<int64_t>((int64_t)want.size(), budget);
-        std::partial_sort(want.begin(), want.begin() + n, want.end(), [&](int64_t a, int64_t b) {
-            return score[(size_t)a] > score[(size_t)b] || (score[(size_t)a] == score[(size_t)b] && a < b);
-        });
-        if (n > c.available()) {
-            // make room: evict resident bricks that score below the candidate
-            // they make room for (bricks that are not candidates score -inf)
-            if (int rc = copy_stream_idle(ctx)) return rc;
-            std::vector<float> slot_score((size_t)c.gpu_slots, -INFINITY);
-            for (int64_t i = 0; i < NC; ++i) {
-                int32_t sl = c.table[(size_t)(ctx->cand_codes[(size_t)i] - base)];
-                if (sl >= 0) slot_score[(size_t)sl] = score[(size_t)i];
+        // absent candidates above the threshold (logit, candidate index), taken
+        // best first (ties by candidate index, so the order does not depend on
+        // the order in which the kernel appended them) from a heap, as far as
+        // the loads reach: a few hundred of possibly thousands
+        std::vector<std::pair<float, int32_t>> want;
+        want.reserve(nw);
+        for (size_t q = 0; q < nw; ++q) {
+            const int32_t i = m->h_want_idx.p[q];
+            if (c.table[(size_t)(ctx->cand_codes[(size_t)i] - base)] < 0) want.emplace_back(m->h_want_z.p[q], i);
+        }
+        const auto below = [](const std::pair<float, int32_t>& a, const std::pair<float, int32_t>& b) {
+            return a.first < b.first || (a.first == b.first && a.second > b.second);
+        };
+        std::make_heap(want.begin(), want.end(), below);
+        std::vector<std::pair<float, int32_t>> best;   // best first
+        auto want_end = want.end();
+        auto take = [&](size_t k) {   // extend `best` to k entries
+            while (best.size() < k && want_end != want.begin()) {
+                std::pop_heap(want.begin(), want_end, below);
+                --want_end;
+                best.push_back(*want_end);
             }
+        };
+        int64_t n = std::min<int64_t>((int64_t)want.size(), budget);
+        if (n > c.available()) {
+            // make room: evict resident bricks, lowest logit first (ties by
+            // slot; bricks that are not candidates score -inf), each only for
+            // a better candidate and only while its own probability is below
+            // 1 - min_prob
+            if (int rc = copy_stream_idle(ctx)) return rc;
+            const float* slot_score = m->h_slot_score.p;
             std::vector<int64_t> res;
+            res.reserve((size_t)c.used);
             for (int64_t sl = 0; sl < c.used; ++sl)
                 if (c.slot_brick[(size_t)sl] >= 0) res.push_back(sl);
-            const size_t need = std::min(res.size(), (size_t)(n - c.available()));
-            std::partial_sort(res.begin(), res.begin() + need, res.end(), [&](int64_t x, int64_t y) {
-                return slot_score[(size_t)x] < slot_score[(size_t)y] ||
-                       (slot_score[(size_t)x] == slot_score[(size_t)y] && x < y);
-            });
-            std::vector<int64_t> idx;
-            std::vector<int32_t> val;
-            for (size_t j = 0; j < res.size() && c.available() < n; ++j) {
-                const int64_t v = res[j];
-                // a resident goes only for a better candidate, and only while
-                // its own probability is below 1 - min_prob
-                if (!(score[(size_t)want[(size_t)c.available()]] > slot_score[(size_t)v]) || !(slot_score[(size_t)v] < -zmin))
-                    break;
+            const auto above = [&](int64_t x, int64_t y) {
+                return slot_score[(size_t)x] > slot_score[(size_t)y] ||
+                       (slot_score[(size_t)x] == slot_score[(size_t)y] && x > y);
+            };
+            std::make_heap(res.begin(), res.end(), above);
+            if (m->h_eidx.ensure(ns) != hipSuccess || m->h_eval.ensure(ns) != hipSuccess ||
+                m->d_eidx.ensure(ns) != hipSuccess || m->d_eval.ensure(ns) != hipSuccess)
+                return fail(VT_E_NOMEM, "eviction staging");
+            size_t ne = 0;
+            auto res_end = res.end();
+            while (c.available() < n && res_end != res.begin()) {
+                const int64_t v = res.front();
+                take((size_t)c.available() + 1);   // the candidate this slot would take
+                const float zc = best[(size_t)c.available()].first;
+                if (!(zc > slot_score[(size_t)v]) || !(slot_score[(size_t)v] < -zmin)) break;
+                std::pop_heap(res.begin(), res_end, above);
+                --res_end;
                 const int64_t old = c.slot_brick[(size_t)v];
                 c.table[(size_t)old] = -1;
                 c.slot_brick[(size_t)v] = -1;
                 c.ref[(size_t)v] = 0;
                 c.free_list.push_back(v);
                 c.stat_evictions++;
-                idx.push_back(old);
-                val.push_back(-1);
+                m->h_eidx.p[ne] = old;
+                m->h_eval.p[ne] = -1;
+                ++ne;
             }
-            if (!idx.empty()) {
-                DevBuf<int64_t> di;
-                DevBuf<int32_t> dv;
-                if (di.ensure(idx.size()) != hipSuccess || dv.ensure(val.size()) != hipSuccess)
-                    return fail(VT_E_NOMEM, "eviction staging");
-                HIP_TRY(hipMemcpyAsync(di.p, idx.data(), idx.size() * 8, hipMemcpyHostToDevice, s));
-                HIP_TRY(hipMemcpyAsync(dv.p, val.data(), val.size() * 4, hipMemcpyHostToDevice, s));
-                HIP_TRY(launch_scatter_i32(c.d_table, di.p, dv.p, (int)idx.size(), s));
-                HIP_TRY(hipStreamSynchronize(s));
+            if (ne > 0) {
+                // the staging is pinned: the host rewrites it only after the
+                // next call's first wait on this stream
+                HIP_TRY(hipMemcpyAsync(m->d_eidx.p, m->h_eidx.p, ne * 8, hipMemcpyHostToDevice, s));
+                HIP_TRY(hipMemcpyAsync(m->d_eval.p, m->h_eval.p, ne * 4, hipMemcpyHostToDevice, s));
+                HIP_TRY(launch_scatter_i32(c.d_table, m->d_eidx.p, m->d_eval.p, (int)ne, s));
             }
             n = std::min<int64_t>(n, c.available());
         }
+        take((size_t)n);
         // into free slots, on the copy stream, without waiting: the next
         // render pass runs while these bricks arrive
         std::vector<uint32_t> load((size_t)n), left;
-        for (int64_t q = 0; q < n; ++q) load[(size_t)q] = (uint32_t)(ctx->cand_codes[(size_t)want[(size_t)q]] - base);
+        for (int64_t q = 0; q < n; ++q)
+            load[(size_t)q] = (uint32_t)(ctx->cand_codes[(size_t)best[(size_t)q].second] - base);
         int64_t misses = 0;
         if (int rc = load_async(ctx, load.data(), load.size(), left, &nloaded, &misses, true)) return rc;
     }
