@@ -416,6 +416,226 @@ def run_c3(ctx, args):
     return out
 
 
+def morton(level: int, bx, by, bz) -> np.ndarray:
+    """Codes of level-`level` nodes (leading 1 bit, then x, y, z bits interleaved, x highest)."""
+    bx, by, bz = (np.asarray(c, np.uint64) for c in (bx, by, bz))
+    code = np.full(np.broadcast(bx, by, bz).shape, np.uint64(1) << np.uint64(3 * level), np.uint64)
+    for i in range(level):
+        b = np.uint64(i)
+        code |= (((bx >> b) & np.uint64(1)) << np.uint64(3 * i + 2)) | \
+                (((by >> b) & np.uint64(1)) << np.uint64(3 * i + 1)) | (((bz >> b) & np.uint64(1)) << np.uint64(3 * i))
+    return code
+
+
+def in_runs(runs: np.ndarray, codes: np.ndarray) -> np.ndarray:
+    """Membership of `codes` in a run list (sorted inclusive [start, end] pairs)."""
+    starts, ends = runs[0::2], runs[1::2]
+    i = np.searchsorted(starts, codes, side="right").astype(np.int64) - 1
+    return (i >= 0) & (codes <= ends[np.maximum(i, 0)])
+
+
+def fly_through(W: int, H: int, frames: int, X: int, Y: int, Z: int):
+    """A fly-through: enters the volume through the x = 0 face at mid
+    height and depth and flies along +x to 85 % of its length with a slow
+    sideways and vertical weave, looking along the path."""
+    def pos(t):
+        return np.array([-0.15 * X + 1.0 * X * t, 0.5 * Y + 0.15 * Y * math.sin(2 * math.pi * t),
+                         0.5 * Z + 0.1 * Z * math.sin(4 * math.pi * t)], np.float64)
+    cams = []
+    for k in range(frames):
+        t = k / max(1, frames - 1)
+        eye = pos(t)
+        look, up, right = vt.look_at(eye, pos(t + 0.02))
+        cams.append(vt.make_camera(W, H, eye, look, up, right, fov_v_deg=45.0))
+    return cams
+
+
+def run_c5(ctx, args):
+    """Config 5 with the volume resident in HBM as the brick source:
+    4096x4096x2048 float32 (128 GiB), a seeded turbulence-like field (64
+    random-phase Fourier modes, |k|^-5/3 amplitudes), iso at the 70th
+    percentile of a seeded 1% sample (41 random x-slabs); logical cube 4096^3
+    with zero padding above z = 2048; 32^3 bricks (levels 0..7, 2^21 bricks)
+    with a 3-voxel halo; device brick cache 48 GB; a 3840x2160 fly-through.
+    The 128 GB HDF5 file and the 96 GB pinned host tier of the configuration
+    are replaced by the volume in HBM (bricks are copied from it on a miss).
+    Size-independent checks at full size: the generator against its float32
+    formula and its own slabs, the builder's level-7 nodes against the cell
+    test on one layer of bricks, hits unchanged with an 8 GB cache, and
+    (unless --no-cpu) sampled rays against the oracle."""
+    from visualtur_amd.volumes import C5_SEED, fourier_field, fourier_modes
+
+    X, Y, Z = 4096, 4096, 2048
+    level_c, inc = 7, 3
+    Sb = brick_bytes(32, inc)
+    modes = fourier_modes(64, C5_SEED, X)
+    slab_n = Y * Z
+
+    def gen(d, x0, nx):
+        ctx.volume_generate(vt.GEN_FOURIER, (X, Y, Z), C5_SEED, d, shapes=modes, x0=x0, nx=nx)
+
+    checks = {}
+    # iso: 70th percentile of a seeded 1% voxel sample (41 of 4096 x-slabs)
+    t0 = time.perf_counter()
+    rng = np.random.default_rng(C5_SEED)
+    xs = np.sort(rng.choice(X, 41, replace=False))
+    slab = ctx.alloc(slab_n * 4)
+    sample = np.empty((xs.size, slab_n), np.float32)
+    for i, x in enumerate(xs):
+        gen(slab, int(x), 1)
+        sample[i] = slab.download(np.float32, (slab_n,))
+    iso = float(np.float32(np.percentile(sample, 70)))
+    log(f"c5 iso {iso:.6f} (70th percentile of {sample.size} voxels) in {time.perf_counter() - t0:.1f}s")
+    # the volume
+    t0 = time.perf_counter()
+    d_vol = ctx.alloc(X * Y * Z * 4)
+    gen(d_vol, 0, X)
+    log(f"c5 volume generated in {time.perf_counter() - t0:.2f}s")
+    # generator: the full buffer holds the slabs generated alone (64-bit
+    # offsets), and they equal the float32 formula
+    same = all(np.array_equal(d_vol.download_at(int(x) * slab_n * 4, np.float32, (slab_n,)).view(np.uint32),
+                              sample[i].view(np.uint32)) for i, x in ((0, xs[0]), (xs.size - 1, xs[-1])))
+    yz = rng.integers(0, [Y, Z], (4096, 2))
+    ref = fourier_field(modes, np.full(4096, xs[-1]), yz[:, 0], yz[:, 1])
+    err = float(np.max(np.abs(sample[-1].reshape(Y, Z)[yz[:, 0], yz[:, 1]] - ref)))
+    checks["generator"] = {"slabs_equal": bool(same), "max_abs_err_vs_formula": err, "points": 4096}
+    del sample
+    # octree on the GPU
+    t0 = time.perf_counter()
+    n, sizes = ctx.octree_build(d_vol, (X, Y, Z), iso, False, level_c)
+    runs = [ctx.octree_runs(l, sizes[l]) for l in range(level_c + 1)]
+    log(f"c5 octree built in {time.perf_counter() - t0:.2f}s (n={n}, level-{level_c} runs {sizes[level_c] // 2}, "
+        f"leaf runs {sizes[-1] // 2})")
+    # builder: every level-7 node of the brick layer x in [2048, 2080)
+    # against the cell test (corners on both sides of iso, zero outside)
+    t0 = time.perf_counter()
+    bx, x0 = 64, 2048
+    layer = ctx.alloc(33 * slab_n * 4)
+    gen(layer, x0, 33)
+    below = np.full((33, Y + 1, Z + 1), np.float32(0.0) < np.float32(iso))
+    below[:, :Y, :Z] = layer.download(np.float32, (33, Y, Z)) < np.float32(iso)
+    layer.free()
+    any_b = np.zeros((32, Y, Z), bool)
+    all_b = np.ones((32, Y, Z), bool)
+    for dx in (0, 1):
+        for dy in (0, 1):
+            for dz in (0, 1):
+                v = below[dx:dx + 32, dy:dy + Y, dz:dz + Z]
+                any_b |= v
+                all_b &= v
+    cell_on = any_b & ~all_b
+    del below, any_b, all_b
+    expect = np.zeros((128, 128), bool)   # (by, bz); bz >= 64 lies in the zero padding
+    expect[:, :64] = cell_on.reshape(32, 128, 32, 64, 32).any(axis=(0, 2, 4))
+    del cell_on
+    by, bz = np.meshgrid(np.arange(128), np.arange(128), indexing="ij")
+    got = in_runs(runs[level_c], morton(level_c, np.full(by.shape, bx), by, bz))
+    checks["builder_layer"] = {"bricks": int(expect.size), "with_nodes": int(expect.sum()),
+                               "mismatches": int((got != expect).sum())}
+    log(f"c5 builder check {checks['builder_layer']} in {time.perf_counter() - t0:.1f}s")
+    prm = vt.render_params(level_c, level_c, iso)
+    W, H = 3840, 2160
+    R = W * H
+    cams = fly_through(W, H, args.warmup + args.steps, X, Y, Z)
+    d_rgba, d_hits = ctx.alloc(R * 16), ctx.alloc(R * 32)
+    gpu_slots = int(48e9 // Sb)
+    ctx.cache_create(level_c, 32, inc, gpu_slots, 0)
+    ctx.cache_attach_device_volume(d_vol, (X, Y, Z))
+    ctx.set_march_options(skip_empty=True, max_defer=args.max_defer, persistent=bool(args.persistent),
+                          max_defer_late=None if args.max_defer_late < 0 else args.max_defer_late)
+    for k in range(args.warmup):
+        ctx.render_frame(cams[k], prm, d_rgba, d_hits)
+    ctx.synchronize()
+    ms, kms, passes, uploads = [], [], [], []
+    t0 = time.perf_counter()
+    for k in range(args.warmup, args.warmup + args.steps):
+        f0 = time.perf_counter()
+        st = ctx.render_frame(cams[k], prm, d_rgba, d_hits)
+        ms.append((time.perf_counter() - f0) * 1e3)
+        kms.append(st.ms_kernel)
+        passes.append(st.passes)
+        uploads.append(st.bricks_uploaded)
+    ctx.synchronize()
+    total_s = time.perf_counter() - t0
+    last = (d_hits.download(vt.HIT_DTYPE, (R,)), d_rgba.download(np.float32, (R, 4)))
+    resident = ctx.cache_stats()["gpu_resident"]
+    # visited bricks of the timed frames (trace replay, untimed)
+    ctx.set_frame_options(collect_stats=False, record_trace=True)
+    U = []
+    for k in range(args.warmup, args.warmup + args.steps):
+        ctx.render_frame(cams[k], prm, d_rgba, d_hits)
+        U.append(int(ctx.brick_trace(level_c).sum()))
+    ctx.set_frame_options(False, False)
+    # the last frame again through an 8 GB cache, from cold: the same bytes
+    ctx.cache_destroy()
+    ctx.cache_create(level_c, 32, inc, int(8e9 // Sb), 0)
+    ctx.cache_attach_device_volume(d_vol, (X, Y, Z))
+    st = ctx.render_frame(cams[-1], prm, d_rgba, d_hits)
+    small = (d_hits.download(vt.HIT_DTYPE, (R,)), d_rgba.download(np.float32, (R, 4)))
+    checks["small_cache_same_frame"] = {"identical": bool(small[0].tobytes() == last[0].tobytes() and
+                                                           small[1].tobytes() == last[1].tobytes()),
+                                        "hit_fraction": round(float((last[0]["node"] != 0).mean()), 4),
+                                        "passes": int(st.passes), "uploads": int(st.bricks_uploaded)}
+    ctx.cache_destroy()
+    m = np.array(ms)
+    octree_bytes = 8 * sum(r.size for r in runs)
+    B = [u * Sb + R * 48 + octree_bytes for u in U]
+    achieved = sum(B) / (sum(kms) * 1e-3) / 1e9
+    out = {"metric": METRIC, "value": round(R * args.steps / total_s, 1), "unit": "rays/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(float(m.mean()), 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "4096x4096x2048 turbulence (64 Fourier modes, k^-5/3), iso = 70th percentile, "
+                                  "resident in HBM as the brick source, 32^3 bricks + 3-voxel halo, device cache "
+                                  "48 GB, 3840x2160 fly-through",
+                      "form": "plain", "iso": iso, "gpu_slots": gpu_slots, "bricks": 1 << (3 * level_c),
+                      "bricks_with_nodes": int(sum(int(runs[level_c][j + 1] - runs[level_c][j]) + 1
+                                                   for j in range(0, runs[level_c].size, 2)))},
+           "p50_ms": round(float(np.percentile(m, 50)), 3), "p99_ms": round(float(np.percentile(m, 99)), 3),
+           "kernel_ms_per_frame": round(float(np.mean(kms)), 4), "passes_per_frame": float(np.mean(passes)),
+           "bricks_uploaded_per_frame": float(np.mean(uploads)), "bricks_marched_per_frame": float(np.mean(U)),
+           "device_miss_rate": round(float(sum(uploads)) / max(1, sum(U)), 5), "resident_after": int(resident),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": RENDER_KERNELS},
+           "checks": checks, "cpu_baseline": None}
+    if not args.no_cpu:
+        from oracle import vto
+
+        # the volume in host memory for the oracle (128 GiB, in 8 GiB pieces)
+        t0 = time.perf_counter()
+        vol_host = np.empty((X, Y, Z), np.float32)
+        step = 256
+        for xa in range(0, X, step):
+            part = vol_host[xa:xa + step]
+            vt._check(vt.lib().vt_memcpy_d2h(ctx.handle, part.ctypes.data, d_vol.ptr + xa * slab_n * 4,
+                                             part.nbytes), ctx.handle)
+        log(f"c5 volume copied to the host in {time.perf_counter() - t0:.1f}s")
+        ctx.cache_create(level_c, 32, inc, gpu_slots, 0)
+        ctx.cache_attach_device_volume(d_vol, (X, Y, Z))
+        w = Workload()
+        w.prm = prm
+        objs = (vto.Volume(vol_host), vto.Octree(runs, n, level_c), vto.params(level_c, level_c, iso))
+        out["parity"] = parity_sample(w, ctx, cams[-1], objs, args.parity_rays, seed=5)
+        log(f"c5 parity {out['parity']}")
+        threads = len(os.sched_getaffinity(0))
+        ids = np.arange(0, R, max(1, R // 65536), dtype=np.int64)
+        t0 = time.perf_counter()
+        done = 0
+        while time.perf_counter() - t0 < args.cpu_seconds:
+            vto.render_list(*objs[:2], cams[-1], objs[2], ids)
+            done += ids.size
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": done / dt, "unit": "rays/s", "cores": threads, "kind": "port",
+                               "sample": f"{done} rays ({ids.size} evenly spaced rays of the last frame, repeated), "
+                                         f"{dt:.1f}s wall, all bricks in memory"}
+        ctx.cache_destroy()
+        del objs, vol_host
+    d_rgba.free()
+    d_hits.free()
+    slab.free()
+    d_vol.free()
+    return out
+
+
 def cpu_baseline(w, cams, seconds: float):
     """Reference algorithm (the C oracle) on this host's cores, over whole
     frames of the same workload until `seconds` of wall time have elapsed."""
@@ -483,7 +703,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--c3-train-paths", type=int, default=512)
     ap.add_argument("--c3-train-frames", type=int, default=4)
     ap.add_argument("--c3-train-steps", type=int, default=1500)
@@ -532,8 +752,8 @@ def main() -> int:
         ctx.close()
         print(json.dumps(out), flush=True)
         return 0
-    if args.config == "c3":
-        out = run_c3(ctx, args)
+    if args.config in ("c3", "c5"):
+        out = run_c3(ctx, args) if args.config == "c3" else run_c5(ctx, args)
         ctx.close()
         print(json.dumps(out), flush=True)
         return 0

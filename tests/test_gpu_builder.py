@@ -70,3 +70,48 @@ def test_volume_write_hdf5_roundtrip(vt, ctx, tmp_path):
             assert f.dims == dims
             assert np.array_equal(f.read_box((0, 0, 0), dims), vol)
     d.free()
+
+
+def test_fourier_generator_matches_formula_and_renders(vt, vto, ctx):
+    """The turbulence generator (64 Fourier modes, |k|^-5/3 amplitudes) on a
+    volume shaped like the 4096x4096x2048 configuration (half as deep as it
+    is wide, so the logical cube is zero-padded in z): the device field equals
+    its float32 formula, the GPU octree equals the oracle's, and a frame
+    flying through it matches the oracle ray for ray."""
+    from visualtur_amd.volumes import fourier_field, fourier_modes
+
+    dims = (128, 128, 64)
+    modes = fourier_modes(64, 77, dims[0], cycles=(1.0, 8.0))
+    d = ctx.alloc(int(np.prod(dims)) * 4)
+    ctx.volume_generate(vt.GEN_FOURIER, dims, 77, d, shapes=modes)
+    got = d.download(np.float32, dims)
+    xx, yy, zz = np.meshgrid(*(np.arange(s) for s in dims), indexing="ij")
+    ref = fourier_field(modes, xx, yy, zz)
+    assert np.max(np.abs(got - ref)) <= 1e-6
+    iso = float(np.float32(np.percentile(got, 70)))
+    level_c = 3
+    n_c, runs_c = vto.build_octree(got, iso, literal=False)
+    n_g, sizes = ctx.octree_build(d, dims, iso, False, level_c)
+    assert n_g == n_c == 7
+    for l in range(level_c + 1):
+        assert np.array_equal(ctx.octree_runs(l, sizes[l]), runs_c[l]), l
+    cube = 1 << (n_c - level_c)
+    ctx.cache_create(level_c, cube, 3, 1 << (3 * level_c), 0)
+    ctx.cache_attach_device_volume(d, dims)
+    prm = vt.render_params(level_c, level_c, iso)
+    eye = (20.0, 64.0, 32.0)
+    cam = vt.make_camera(96, 64, eye, *vt.look_at(eye, (120.0, 70.0, 30.0)))
+    R = cam.W * cam.H
+    d_rgba, d_hits = ctx.alloc(R * 16), ctx.alloc(R * 32)
+    ctx.render_frame(cam, prm, d_rgba, d_hits)
+    hits = d_hits.download(vt.HIT_DTYPE, (R,))
+    rgba = d_rgba.download(np.float32, (R, 4))
+    rc, hc, _ = vto.render(vto.Volume(got), vto.Octree(runs_c, n_c, level_c), cam, vto.params(level_c, level_c, iso))
+    assert (hc["node"] != 0).mean() > 0.2
+    for f in ("node", "brick", "voxel"):
+        assert np.array_equal(hits[f], hc[f]), f
+    assert np.allclose(hits["depth"], hc["depth"], atol=1e-4, equal_nan=True)
+    assert np.allclose(rgba, rc, atol=1.0 / 255, equal_nan=True)
+    ctx.cache_destroy()
+    for b in (d, d_rgba, d_hits):
+        b.free()

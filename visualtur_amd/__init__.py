@@ -261,6 +261,14 @@ class DeviceBuffer:
         _check(lib().vt_memcpy_d2h(self.ctx.handle, out.ctypes.data, self.ptr, out.nbytes), self.ctx.handle)
         return out
 
+    def download_at(self, offset: int, dtype, shape) -> np.ndarray:
+        """Copy `shape` elements starting `offset` bytes into the buffer to the host."""
+        out = np.empty(shape, dtype)
+        assert 0 <= offset and offset + out.nbytes <= self.nbytes
+        _check(lib().vt_memcpy_d2h(self.ctx.handle, out.ctypes.data, self.ptr + int(offset), out.nbytes),
+               self.ctx.handle)
+        return out
+
     def zero(self) -> None:
         _check(lib().vt_memset(self.ctx.handle, self.ptr, 0, self.nbytes), self.ctx.handle)
 

@@ -45,3 +45,38 @@ def sdf_shapes(count: int, seed: int, extent: int, radius=(0.01, 0.08)) -> np.nd
     out[:, 5] = out[:, 4] * rng.uniform(0.15, 0.4, count)
     out[:, 6] = rng.integers(0, 3, count)
     return out
+
+
+C5_SEED = 0x5EED0005
+
+
+def fourier_modes(count: int, seed: int, extent: int, cycles=(1.0, 32.0)) -> np.ndarray:
+    """Parameters of the seeded turbulence-like field (GEN_FOURIER layout:
+    kx, ky, kz in radians per voxel, phase, amplitude per mode).  `count`
+    modes with wavenumbers log-uniform in `cycles` (cycles per extent),
+    directions uniform on the sphere, phases uniform in [0, 2 pi), and
+    amplitudes |k|^(-5/3) normalised to sum to 1, so the field
+    sum_m a_m cos(k_m . x + phi_m) lies in [-1, 1]."""
+    rng = np.random.default_rng(seed)
+    cyc = np.exp(rng.uniform(np.log(cycles[0]), np.log(cycles[1]), count))
+    d = rng.normal(size=(count, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    amp = cyc ** (-5.0 / 3.0)
+    out = np.zeros((count, 5), np.float32)
+    out[:, :3] = d * (2.0 * np.pi * cyc / extent)[:, None]
+    out[:, 3] = rng.uniform(0.0, 2.0 * np.pi, count)
+    out[:, 4] = amp / amp.sum()
+    return out
+
+
+def fourier_field(modes: np.ndarray, x, y, z) -> np.ndarray:
+    """The GEN_FOURIER field at integer voxel coordinates, in float32 with
+    the generator's order of operations: per mode the phase argument
+    ((kx*x + ky*y) + kz*z) + phi, each step rounded, then a*cos summed in
+    mode order."""
+    x, y, z = (np.asarray(c, np.float32) for c in (x, y, z))
+    v = np.zeros(np.broadcast(x, y, z).shape, np.float32)
+    for m in np.asarray(modes, np.float32):
+        arg = ((m[0] * x + m[1] * y) + m[2] * z) + m[3]
+        v = v + m[4] * np.cos(arg)
+    return v
