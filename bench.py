@@ -205,7 +205,7 @@ def prefetch_leg(ctx, w, cams, args, d_rgba, d_hits, plain_ms, plain_uploads, U,
     m = np.array(ms)
     return {"value": round(R * args.steps / total, 1), "unit": "rays/s",
             "p50_ms": round(float(np.percentile(m, 50)), 4), "p99_ms": round(float(np.percentile(m, 99)), 4),
-            "prefetch_ms_per_frame": round(float(np.mean(pre_ms)), 4),
+            "prefetch_ms_per_frame": round(float(np.mean(pre_ms)), 4), "candidates": int(cand.size),
             "demand_misses_per_frame": float(np.mean(uploads)),
             "prefetched_per_frame": float(np.mean(pre_loaded)),
             "device_miss_rate": round(float(sum(uploads)) / max(1, sum(U)), 5),

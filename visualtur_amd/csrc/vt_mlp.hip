@@ -138,16 +138,28 @@ __global__ void k_outer_mask(const float* __restrict__ dz, const float* __restri
     dH[e] = (bf16)(h > 0.0f ? dz[i] * w[j] : 0.0f);
 }
 
-// column sums out[j] (+)= sum_i X[i][j] * (s ? s[i] : 1), one thread per column chunk
+// column sums out[j] (+)= sum_i X[i][j] * (s ? s[i] : 1): one column per
+// thread, kColsumRows rows per workgroup (enough workgroups to cover the
+// chip at a training batch of 8192), four independent partial sums per
+// thread so that four loads are in flight
+constexpr int kColsumRows = 32;
+
 __global__ void k_colsum(const bf16* __restrict__ X, const float* __restrict__ s, int N, int width,
                          float* __restrict__ out)
 {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
-    int r0 = blockIdx.y * 256, r1 = min(N, r0 + 256);
+    int r0 = blockIdx.y * kColsumRows, r1 = min(N, r0 + kColsumRows);
     if (j >= width) return;
-    float acc = 0.0f;
-    for (int i = r0; i < r1; ++i) acc += (float)X[(size_t)i * width + j] * (s ? s[i] : 1.0f);
-    atomicAdd(&out[j], acc);
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+    int i = r0;
+    for (; i + 4 <= r1; i += 4) {
+        a0 += (float)X[(size_t)i * width + j] * (s ? s[i] : 1.0f);
+        a1 += (float)X[(size_t)(i + 1) * width + j] * (s ? s[i + 1] : 1.0f);
+        a2 += (float)X[(size_t)(i + 2) * width + j] * (s ? s[i + 2] : 1.0f);
+        a3 += (float)X[(size_t)(i + 3) * width + j] * (s ? s[i + 3] : 1.0f);
+    }
+    for (; i < r1; ++i) a0 += (float)X[(size_t)i * width + j] * (s ? s[i] : 1.0f);
+    atomicAdd(&out[j], (a0 + a1) + (a2 + a3));
 }
 
 __global__ void k_sum(const float* __restrict__ x, int N, float* __restrict__ out)
@@ -452,7 +464,7 @@ hipError_t launch_outer_mask(const float* dz, const float* w, const bf16* H, int
 
 hipError_t launch_colsum(const bf16* X, const float* sc, int N, int width, float* out, hipStream_t s)
 {
-    dim3 grid((width + 255) / 256, (N + 255) / 256);
+    dim3 grid((width + 255) / 256, (N + kColsumRows - 1) / kColsumRows);
     hipLaunchKernelGGL(k_colsum, grid, dim3(256), 0, s, X, sc, N, width, out);
     return hipGetLastError();
 }
