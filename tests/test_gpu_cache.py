@@ -1,0 +1,96 @@
+"""Miss counts of the device brick cache (page table + clock) on recorded
+visited-brick traces: with room for every brick, a frame loads exactly the
+bricks it is the first to visit; with less room, hits are unchanged and the
+loads never fall below those first visits.  A frame-granular LRU run on the
+same traces gives the miss count the clock is compared with."""
+import numpy as np
+import pytest
+
+from conftest import blobs_volume
+
+pytestmark = pytest.mark.gpu
+
+
+def lru_misses(traces, capacity):
+    """Misses of an LRU cache of `capacity` bricks that sees each frame's
+    visited set at once (touch order within a frame by brick index)."""
+    last = {}
+    misses = []
+    for f, t in enumerate(traces):
+        need = np.nonzero(t)[0]
+        m = 0
+        for b in need:
+            if int(b) not in last:
+                m += 1
+                if len(last) >= capacity:
+                    # the least recently used brick not needed by this frame
+                    victim = min((k for k in last if not t[k]), key=lambda k: (last[k], k))
+                    del last[victim]
+            last[int(b)] = f
+        misses.append(m)
+    return misses
+
+
+def run_path(vt, ctx, cams, prm, level_c, R):
+    d_rgba, d_hits = ctx.alloc(R * 16), ctx.alloc(R * 32)
+    ups, traces, hits = [], [], []
+    for cam in cams:
+        st = ctx.render_frame(cam, prm, d_rgba, d_hits)
+        ups.append(int(st.bricks_uploaded))
+        traces.append(ctx.brick_trace(level_c).astype(bool))
+        hits.append(d_hits.download(vt.HIT_DTYPE, (R,)).tobytes())
+    d_rgba.free()
+    d_hits.free()
+    return ups, traces, hits
+
+
+def test_misses_against_first_visits_and_lru(vt, vto, ctx):
+    from visualtur_amd import prefetch as pf
+
+    vol = blobs_volume(128, 48, seed=91)
+    iso, level_o, level_c = 0.3, 4, 3
+    n, runs = vto.build_octree(vol, iso, literal=False)
+    flat = np.concatenate(runs[: level_o + 1]).astype(np.uint64)
+    ctx.octree_upload(flat, [r.size for r in runs[: level_o + 1]], n, level_o)
+    d_vol = ctx.to_device(vol)
+    cand = pf.candidate_codes(runs[level_c])
+    prm = vt.render_params(level_o, level_c, iso)
+    W = H = 96
+    cams = pf.random_walk(W, H, 128, 32, seed=17, radius=(0.6, 1.0))
+    # every request is a brick the ray marches next: no deferral, no loads
+    # beside the first pass
+    ctx.set_march_options(skip_empty=True, max_defer=0, max_defer_late=0)
+    ctx.set_early_loads(0)
+    ctx.set_frame_options(collect_stats=True, record_trace=True)
+    try:
+        ctx.cache_create(level_c, 1 << (n - level_c), 3, int(cand.size), 0)
+        ctx.cache_attach_device_volume(d_vol, vol.shape)
+        ups, traces, hits = run_path(vt, ctx, cams, prm, level_c, W * H)
+        seen = np.zeros_like(traces[0])
+        first = []
+        for t in traces:
+            first.append(int((t & ~seen).sum()))
+            seen |= t
+        assert ups == first
+        assert ctx.cache_stats()["evictions"] == 0
+        # room for the largest frame, not for the path
+        biggest = max(int(t.sum()) for t in traces)
+        slots = biggest + 4
+        assert slots < int(seen.sum())
+        ctx.cache_create(level_c, 1 << (n - level_c), 3, slots, 0)
+        ctx.cache_attach_device_volume(d_vol, vol.shape)
+        ups2, traces2, hits2 = run_path(vt, ctx, cams, prm, level_c, W * H)
+        assert hits2 == hits and all(np.array_equal(a, b) for a, b in zip(traces2, traces))
+        assert sum(ups2) >= sum(first)
+        lru = lru_misses(traces, slots)
+        print(f"bricks visited {int(seen.sum())}, largest frame {biggest}, slots {slots}: "
+              f"first visits {sum(first)}, clock {sum(ups2)}, frame LRU {sum(lru)}", flush=True)
+        assert sum(lru) >= sum(first)
+        # the clock (reference bits, no timestamps) stays close to LRU: 342
+        # against 306 loads when measured
+        assert sum(ups2) <= 1.25 * sum(lru)
+    finally:
+        ctx.set_frame_options(False, False)
+        ctx.set_march_options()
+        ctx.set_early_loads()
+        d_vol.free()
