@@ -158,6 +158,13 @@ int vt_cache_prefetch_host(vt_context *ctx, int wait);
 /* Load the given bricks (level-levelC codes) into the device pool ahead of
  * use, as far as capacity allows (lruCache::updateCube, lruCache.cu:168-296). */
 int vt_cache_load_bricks(vt_context *ctx, const uint64_t *codes, int64_t count);
+/* Copy the device slot of a resident brick (level-levelC code) to out[S^3],
+ * S = cubeDim + 2*cubeInc, x slowest: the box [min - cubeInc, min + cubeDim +
+ * cubeInc) of the volume, zero outside it, as lruCache's GPU slot holds it
+ * (the box read into a CPU slot, lruCache.cu:226 with FileManager.cu:86-167,
+ * then copied to the GPU slot, lruCache.cu:266).  VT_E_STATE when the brick
+ * is not resident. */
+int vt_cache_read_brick(vt_context *ctx, uint64_t code, float *out);
 /* Drop every device-resident brick. */
 int vt_cache_clear(vt_context *ctx);
 /* lruCache::printStatistics (lruCache.cu:318-322) as numbers. */

@@ -94,3 +94,53 @@ def test_misses_against_first_visits_and_lru(vt, vto, ctx):
         ctx.set_march_options()
         ctx.set_early_loads()
         d_vol.free()
+
+
+@pytest.mark.parametrize("source", ["host_full", "host_on_demand", "device_volume"])
+def test_brick_slots_equal_box_reads(vt, vto, ctx, tmp_path, source):
+    """Every brick in the device pool holds the zero-padded box
+    [min - cubeInc, min + cubeDim + cubeInc) of the volume, byte for byte,
+    as FileManager reads it (FileManager.cu:86-167): from the HDF5 file
+    through a host tier that holds every brick, through an on-demand host
+    tier of 9 slots, and from the volume in HBM."""
+    dims = (64, 60, 50)
+    vol = blobs_volume(64, 10, seed=31, dims=dims)
+    h5 = tmp_path / "vol.h5"
+    vt.file_create(h5, "value", vol, chunk=16)
+    n, runs = vto.build_octree(vol, 0.5, literal=False)
+    level_c, cube, inc = 2, 16, 3
+    S = cube + 2 * inc
+    flat = np.concatenate(runs[: level_c + 1]).astype(np.uint64)
+    ctx.octree_upload(flat, [r.size for r in runs[: level_c + 1]], n, level_c)
+    nb = 8 ** level_c
+    d_vol = None
+    if source == "device_volume":
+        d_vol = ctx.to_device(vol)
+        ctx.cache_create(level_c, cube, inc, nb, 0)
+        ctx.cache_attach_device_volume(d_vol, dims)
+    else:
+        ctx.cache_create(level_c, cube, inc, nb, nb if source == "host_full" else 9)
+        ctx.cache_attach_hdf5(h5, "value")
+        if source == "host_full":
+            ctx.cache_prefetch_host(wait=True)
+    # a host tier that holds every brick holds only the bricks with octree
+    # nodes (the others are refused); the other sources serve any brick
+    from visualtur_amd import prefetch as pf
+
+    codes = pf.candidate_codes(runs[level_c]) if source == "host_full" else np.arange(nb, 2 * nb, dtype=np.uint64)
+    assert codes.size >= 16
+    try:
+        with vt.File(h5, "value") as f:
+            for code in codes:
+                ctx.cache_load_bricks([code])
+                mn = np.array(vto.decode(int(code), level_c, n), np.int64)
+                box = f.read_box(tuple(mn - inc), tuple(mn + cube + inc))
+                got = ctx.cache_read_brick(int(code), S)
+                assert got.view(np.uint32).tobytes() == box.view(np.uint32).tobytes(), (source, int(code))
+        with pytest.raises(vt.VTError):
+            ctx.cache_clear()
+            ctx.cache_read_brick(int(codes[0]), S)
+    finally:
+        ctx.cache_destroy()
+        if d_vol is not None:
+            d_vol.free()

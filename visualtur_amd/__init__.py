@@ -112,6 +112,7 @@ _SIGS = {
     "vt_cache_attach_device_volume": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "vt_cache_prefetch_host": (ctypes.c_int, [_P, ctypes.c_int]),
     "vt_cache_load_bricks": (ctypes.c_int, [_P, _P, ctypes.c_int64]),
+    "vt_cache_read_brick": (ctypes.c_int, [_P, ctypes.c_uint64, _P]),
     "vt_cache_clear": (ctypes.c_int, [_P]),
     "vt_cache_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(CacheStats)]),
     "vt_cache_destroy": (ctypes.c_int, [_P]),
@@ -382,6 +383,12 @@ class Context:
     def cache_load_bricks(self, codes) -> None:
         c = np.ascontiguousarray(codes, np.uint64)
         self._c(lib().vt_cache_load_bricks(self.handle, c.ctypes.data, c.size))
+
+    def cache_read_brick(self, code: int, S: int) -> np.ndarray:
+        """The device slot of a resident brick as an S^3 array (vt_cache_read_brick)."""
+        out = np.empty((S, S, S), np.float32)
+        self._c(lib().vt_cache_read_brick(self.handle, int(code), out.ctypes.data))
+        return out
 
     def cache_clear(self) -> None:
         self._c(lib().vt_cache_clear(self.handle))

@@ -746,6 +746,24 @@ int vt_cache_load_bricks(vt_context* ctx, const uint64_t* codes, int64_t count)
     return service_requests(ctx, req.data(), (int64_t)req.size(), nullptr);
 }
 
+int vt_cache_read_brick(vt_context* ctx, uint64_t code, float* out)
+{
+    if (!ctx || !out) return fail(VT_E_ARG, "NULL argument");
+    if (!ctx->cache) return fail(VT_E_STATE, "no cache");
+    BrickCache& c = *ctx->cache;
+    const uint64_t base = 1ull << (3 * c.level_c);
+    if (code < base || code >= 2 * base) return fail(VT_E_ARG, "brick code outside levelC");
+    if (int rc = set_device(ctx)) return rc;
+    // loads still on the copy stream land (and are published) first
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (int rc = copy_stream_idle(ctx)) return rc;
+    const int32_t slot = c.table[(size_t)(code - base)];
+    if (slot < 0) return fail(VT_E_STATE, "brick not resident");
+    HIP_TRY(hipMemcpy(out, c.d_pool + (size_t)slot * c.slot_elems, c.brick_elems * sizeof(float),
+                      hipMemcpyDeviceToHost));
+    return VT_OK;
+}
+
 int vt_cache_clear(vt_context* ctx)
 {
     if (!ctx || !ctx->cache) return fail(VT_E_STATE, "no cache");
