@@ -144,3 +144,50 @@ def test_brick_slots_equal_box_reads(vt, vto, ctx, tmp_path, source):
         ctx.cache_destroy()
         if d_vol is not None:
             d_vol.free()
+
+
+@pytest.mark.parametrize("early", [0, 256])
+def test_brick_source_failure_is_reported_and_recoverable(vt, vto, ctx, tmp_path, early):
+    """A brick source that fails (the HDF5 file truncated before the
+    on-demand host tier first opens it) fails the frame with VT_E_IO, from
+    the service between passes (early=0) and from the loads beside the first
+    pass (early=256), with nothing left running.  The cache stays
+    consistent: once the file is whole again the same context renders the
+    frame exactly."""
+    dims = (64, 60, 50)
+    vol = blobs_volume(64, 10, seed=31, dims=dims)
+    h5 = tmp_path / "vol.h5"
+    vt.file_create(h5, "value", vol, chunk=16)
+    whole = h5.read_bytes()
+    iso, level_o, level_c = 0.3, 3, 2
+    n, runs = vto.build_octree(vol, iso, literal=False)
+    flat = np.concatenate(runs[: level_o + 1]).astype(np.uint64)
+    ctx.octree_upload(flat, [r.size for r in runs[: level_o + 1]], n, level_o)
+    ctx.cache_create(level_c, 16, 3, 64, 9)   # an on-demand host tier opens the file at its first read
+    ctx.cache_attach_hdf5(h5, "value")
+    h5.write_bytes(whole[: len(whole) // 2])
+    ctx.set_early_loads(early)
+    eye = (75.0, 60.0, 55.0)
+    cam = vt.make_camera(80, 64, eye, *vt.look_at(eye, (32.0, 30.0, 25.0)))
+    prm = vt.render_params(level_o, level_c, iso)
+    R = cam.W * cam.H
+    d_rgba, d_hits = ctx.alloc(R * 16), ctx.alloc(R * 32)
+    try:
+        with pytest.raises(vt.VTError) as e:
+            ctx.render_frame(cam, prm, d_rgba, d_hits)
+        assert e.value.code == -3   # VT_E_IO
+        h5.write_bytes(whole)
+        ctx.render_frame(cam, prm, d_rgba, d_hits)
+        hits = d_hits.download(vt.HIT_DTYPE, (R,))
+        rgba = d_rgba.download(np.float32, (R, 4))
+        rc, hc, _ = vto.render(vto.Volume(vol), vto.Octree(runs, n, level_o), cam, vto.params(level_o, level_c, iso))
+        assert (hc["node"] != 0).mean() > 0.2
+        for f in ("node", "brick", "voxel"):
+            assert np.array_equal(hits[f], hc[f]), f
+        assert np.allclose(hits["depth"], hc["depth"], atol=1e-4, equal_nan=True)
+        assert np.allclose(rgba, rc, atol=1.0 / 255, equal_nan=True)
+    finally:
+        ctx.set_early_loads()
+        ctx.cache_destroy()
+        d_rgba.free()
+        d_hits.free()

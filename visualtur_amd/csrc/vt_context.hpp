@@ -240,7 +240,13 @@ struct HostTier {
             *out = pool_dev + (size_t)s * slot_elems;
             return VT_OK;
         }
-        // choose a host slot: unused first, then clock skipping pinned
+        // the file first: a source that cannot be opened takes no slot
+        if (!sync_reader.is_open()) {
+            std::string err;
+            if (!sync_reader.open(file, dataset, 64ull << 20, err)) return fail(VT_E_IO, err);
+        }
+        // choose a host slot: unused first, then clock skipping pinned (a
+        // slot emptied by a failed read holds no brick and is taken as is)
         int64_t slot = -1;
         if (used < capacity) {
             slot = used++;
@@ -254,11 +260,10 @@ struct HostTier {
                 break;
             }
             if (slot < 0) return fail(VT_E_STATE, "host tier: every slot is pinned");
-            slot_of[brick_of[slot]] = -1;
         }
-        if (!sync_reader.is_open()) {
-            std::string err;
-            if (!sync_reader.open(file, dataset, 64ull << 20, err)) return fail(VT_E_IO, err);
+        if (brick_of[slot] >= 0) {
+            slot_of[(size_t)brick_of[slot]] = -1;
+            brick_of[slot] = -1;
         }
         int x, y, z;
         node_min(bidx + (1ull << (3 * level_c)), level_c, n, x, y, z);

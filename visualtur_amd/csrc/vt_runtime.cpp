@@ -154,6 +154,7 @@ int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t
     int32_t* h_slots = reinterpret_cast<int32_t*>(c.h_stage.p + L.slots);
     int32_t* h_tval = reinterpret_cast<int32_t*>(c.h_stage.p + L.tval);
     int64_t nt = 0;
+    int fail_rc = VT_OK;   // a brick source that fails: the bricks before it still load
     for (int64_t q = 0; q < n_load; ++q) {
         int64_t b = bricks[q];
         int64_t slot = victims ? victims[q] : c.pick_victim();
@@ -176,7 +177,19 @@ int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t
         if (c.host) {
             const float* src = nullptr;
             int rc = c.host->get(b, c.epoch, &src, misses);
-            if (rc != VT_OK) return rc;
+            if (rc != VT_OK) {
+                // b stays absent and its slot free (an evicted brick stays
+                // evicted); the bricks staged before it are loaded below, so
+                // the host and device tables agree when the error returns
+                c.table[(size_t)b] = -1;
+                c.slot_brick[(size_t)slot] = -1;
+                c.ref[(size_t)slot] = 0;
+                c.free_list.push_back(slot);
+                nt--;
+                fail_rc = rc;
+                n_load = q;
+                break;
+            }
             h_src[q] = src;
         }
     }
@@ -204,7 +217,7 @@ int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t
         HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
     c.stat_uploads += n_load;
-    return VT_OK;
+    return fail_rc;
 }
 
 // Loads the absent bricks of `req` that fit in free slots on the copy
@@ -223,6 +236,7 @@ int load_async(vt_context* ctx, const uint32_t* req, size_t n, std::vector<uint3
     const int64_t base = A.used;
     int64_t k = 0;
     bool full = false;
+    int fail_rc = VT_OK;   // a brick source that fails: the bricks before it still load
     for (size_t i = 0; i < n; ++i) {
         const uint32_t b = req[i];
         if (c.table[(size_t)b] >= 0) continue;
@@ -241,7 +255,11 @@ int load_async(vt_context* ctx, const uint32_t* req, size_t n, std::vector<uint3
                 left.push_back(b);
                 continue;
             }
-            if (rc != VT_OK) return rc;
+            if (rc != VT_OK) {
+                c.free_list.push_back(slot);
+                fail_rc = rc;
+                break;
+            }
         }
         c.slot_brick[(size_t)slot] = b;
         c.table[(size_t)b] = (int32_t)slot;
@@ -254,7 +272,7 @@ int load_async(vt_context* ctx, const uint32_t* req, size_t n, std::vector<uint3
         A.tval.p[base + k] = (int32_t)slot;
         k++;
     }
-    if (k == 0) return VT_OK;
+    if (k == 0) return fail_rc;
     hipStream_t s = ctx->cstream;
     if (c.host) {
         // beside a running render pass, two workgroups per brick keep enough
@@ -271,7 +289,7 @@ int load_async(vt_context* ctx, const uint32_t* req, size_t n, std::vector<uint3
     }
     c.stat_uploads += k;
     *loaded += k;
-    return VT_OK;
+    return fail_rc;
 }
 
 // Writes the sub-block min/max and the page-table entries of the copies not
@@ -859,8 +877,8 @@ int vt_get_brick_trace(vt_context* ctx, uint8_t* out, int64_t n)
     return VT_OK;
 }
 
-int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_params* p, float* d_rgba,
-                    vt_hit* d_hits, vt_frame_stats* stats)
+static int render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_params* p, float* d_rgba,
+                        vt_hit* d_hits, vt_frame_stats* stats)
 {
     double t_start = now_ms();
     if (!ctx || !p || !d_rgba) return fail(VT_E_ARG, "NULL argument");
@@ -1036,6 +1054,20 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
     if (stats) *stats = st;
     if (st.err) return fail(VT_E_RANGE, "a sample fell outside its brick slot (halo too small for this geometry)");
     return VT_OK;
+}
+
+int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_params* p, float* d_rgba,
+                    vt_hit* d_hits, vt_frame_stats* stats)
+{
+    const int rc = render_frame(ctx, cam, p, d_rgba, d_hits, stats);
+    if (rc != VT_OK && ctx) {
+        // a frame that fails part-way (a brick source that fails while the
+        // first pass runs, say) leaves no kernel or copy running: the caller
+        // may free its buffers or the cache at once
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamSynchronize(ctx->cstream);
+    }
+    return rc;
 }
 
 int vt_trace_nodes(vt_context* ctx, const vt_camera* cam, int32_t finalLevel, int32_t maxNodes, int32_t capTotal,
