@@ -19,11 +19,23 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int TM = 64, TN = 64, TK = 32, LDSK = TK + 8;   // +8: 80-byte rows, 16-byte aligned
 
+// eight consecutive elements of an operand row, `valid` of them in range
+// (0..8): one 16-byte load when all are in range and `vec` (16-byte aligned
+// rows), else element by element with zeros past the end
+__device__ __forceinline__ bf16x8 load8(const bf16* __restrict__ p, int valid, bool vec)
+{
+    if (vec && valid == 8) return *reinterpret_cast<const bf16x8*>(p);
+    bf16x8 v;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = i < valid ? p[i] : (bf16)0.0f;
+    return v;
+}
+
 template <bool TA, bool TB, int EPI>
 __global__ __launch_bounds__(256) void k_gemm(const bf16* __restrict__ A, int lda, const bf16* __restrict__ B,
                                               int ldb, float* __restrict__ Cf, bf16* __restrict__ Cb, int ldc,
                                               int M, int N, int K, int kchunk, const float* __restrict__ bias,
-                                              const bf16* __restrict__ mask, int ldm)
+                                              const bf16* __restrict__ mask, int ldm, bool vec)
 {
     __shared__ __attribute__((aligned(16))) bf16 As[TM][LDSK];   // As[m][k]
     __shared__ __attribute__((aligned(16))) bf16 Bs[TN][LDSK];   // Bs[n][k] (B transposed)
@@ -32,7 +44,6 @@ __global__ __launch_bounds__(256) void k_gemm(const bf16* __restrict__ A, int ld
     const int kend = min(K, kbeg + kchunk);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    const bf16 zero = (bf16)0.0f;
     f32x4 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -40,23 +51,30 @@ __global__ __launch_bounds__(256) void k_gemm(const bf16* __restrict__ A, int ld
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 
     for (int k0 = kbeg; k0 < kend; k0 += TK) {
-        // stage the A and B tiles; the fastest-varying thread index walks the
-        // contiguous dimension of each operand in global memory
-        for (int e = tid; e < TM * TK; e += 256) {
-            int r, c;
-            if (TA) { r = e % TM; c = e / TM; } else { r = e / TK; c = e % TK; }
-            int gm = m0 + r, gk = k0 + c;
-            bf16 v = zero;
-            if (gm < M && gk < kend) v = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
-            As[r][c] = v;
+        // stage the 64x32 A and B tiles: each thread moves one run of eight
+        // elements along the operand's contiguous dimension (k for a
+        // row-major operand, m or n for a transposed one)
+        if (!TA) {
+            const int r = tid >> 2, c = (tid & 3) * 8, gm = m0 + r, gk = k0 + c;
+            const int valid = gm < M ? max(0, min(8, kend - gk)) : 0;
+            *reinterpret_cast<bf16x8*>(&As[r][c]) = load8(A + (size_t)gm * lda + gk, valid, vec);
+        } else {
+            const int kr = tid >> 3, c = (tid & 7) * 8, gk = k0 + kr, gm = m0 + c;
+            const int valid = gk < kend ? max(0, min(8, M - gm)) : 0;
+            const bf16x8 v = load8(A + (size_t)gk * lda + gm, valid, vec);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) As[c + i][kr] = v[i];
         }
-        for (int e = tid; e < TN * TK; e += 256) {
-            int r, c;
-            if (TB) { r = e / TK; c = e % TK; } else { r = e % TN; c = e / TN; }
-            int gn = n0 + r, gk = k0 + c;
-            bf16 v = zero;
-            if (gn < N && gk < kend) v = TB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn];
-            Bs[r][c] = v;
+        if (TB) {
+            const int r = tid >> 2, c = (tid & 3) * 8, gn = n0 + r, gk = k0 + c;
+            const int valid = gn < N ? max(0, min(8, kend - gk)) : 0;
+            *reinterpret_cast<bf16x8*>(&Bs[r][c]) = load8(B + (size_t)gn * ldb + gk, valid, vec);
+        } else {
+            const int kr = tid >> 3, c = (tid & 7) * 8, gk = k0 + kr, gn = n0 + c;
+            const int valid = gk < kend ? max(0, min(8, N - gn)) : 0;
+            const bf16x8 v = load8(B + (size_t)gk * ldb + gn, valid, vec);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) Bs[c + i][kr] = v[i];
         }
         __syncthreads();
         const int kk = 8 * (lane >> 4);
@@ -419,8 +437,10 @@ static hipError_t gemm_t(const bf16* A, int lda, const bf16* B, int ldb, float* 
     kchunk = (kchunk + TK - 1) / TK * TK;
     int nz = (K + kchunk - 1) / kchunk;
     dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, nz);
+    // 16-byte runs need rows that start on 16 bytes
+    const bool vec = lda % 8 == 0 && ldb % 8 == 0 && ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0;
     hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), grid, dim3(256), 0, s, A, lda, B, ldb, Cf, Cb, ldc, M, N, K, kchunk,
-                       bias, mask, ldm);
+                       bias, mask, ldm, vec);
     return hipGetLastError();
 }
 
