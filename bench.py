@@ -48,7 +48,8 @@ HBM_PEAK_GBS = 8000.0
 # k_render_wide, 2^k lanes per ray) for the workloads: FETCH_SIZE + WRITE_SIZE per
 # dispatch, each scaled by what the same counter reads for a 256 MiB copy
 PROFILES = {"c1": ROOT / "profiles" / "render_passes_256cubed_blobs.md",
-            "c2": ROOT / "profiles" / "render_passes_1024cubed_fbm_orbit.md"}
+            "c2": ROOT / "profiles" / "render_passes_1024cubed_fbm_orbit.md",
+            "c5": ROOT / "profiles" / "render_passes_4096x4096x2048_turbulence_flythrough.md"}
 RENDER_KERNELS = "k_render_pass+k_render_wide"
 
 
@@ -581,6 +582,7 @@ def run_c5(ctx, args):
     octree_bytes = 8 * sum(r.size for r in runs)
     B = [u * Sb + R * 48 + octree_bytes for u in U]
     achieved = sum(B) / (sum(kms) * 1e-3) / 1e9
+    launches = sum(passes)
     out = {"metric": METRIC, "value": round(R * args.steps / total_s, 1), "unit": "rays/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(float(m.mean()), 4),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -595,7 +597,10 @@ def run_c5(ctx, args):
            "bricks_uploaded_per_frame": float(np.mean(uploads)), "bricks_marched_per_frame": float(np.mean(U)),
            "device_miss_rate": round(float(sum(uploads)) / max(1, sum(U)), 5), "resident_after": int(resident),
            "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": RENDER_KERNELS},
+                        "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic("c5"),
+                        "kernel": RENDER_KERNELS, "avg_launch_ms": round(sum(kms) / launches, 4),
+                        "bytes_per_launch": int(sum(B) / launches), "bricks_marched_per_frame": int(np.mean(U)),
+                        "brick_bytes": Sb},
            "checks": checks, "cpu_baseline": None}
     if not args.no_cpu:
         from oracle import vto
@@ -739,6 +744,13 @@ def main() -> int:
         return 2
 
     ctx = vt.Context(0)
+    if args.copy_probe:
+        nb = 256 << 20
+        a_, b_ = ctx.alloc(nb), ctx.alloc(nb)
+        for _ in range(3):
+            ctx.copy_probe(a_, b_, nb)
+        a_.free()
+        b_.free()
     if args.config == "c4":
         sweep = serving_sweep(ctx, args)
         best = max(sweep.values(), key=lambda r: r["batched"]["rays_per_s"])
@@ -757,13 +769,6 @@ def main() -> int:
         ctx.close()
         print(json.dumps(out), flush=True)
         return 0
-    if args.copy_probe:
-        nb = 256 << 20
-        a_, b_ = ctx.alloc(nb), ctx.alloc(nb)
-        for _ in range(3):
-            ctx.copy_probe(a_, b_, nb)
-        a_.free()
-        b_.free()
     w = setup_c2(ctx, args) if args.config == "c2" else setup_c1(ctx, args)
     ctx.set_march_options(skip_empty=True, max_defer=args.max_defer, persistent=bool(args.persistent),
                           max_defer_late=None if args.max_defer_late < 0 else args.max_defer_late)
