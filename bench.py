@@ -336,6 +336,8 @@ def run_c3(ctx, args):
             ctx.set_early_loads(args.early_reserve)
         if args.first_nodes is not None:
             ctx.set_first_pass_budget(args.first_nodes)
+        if args.supertile is not None:
+            ctx.set_first_pass_tiles(args.supertile)
         t0 = time.perf_counter()
         ctx.cache_prefetch_host(wait=True)
         log(f"c3 {form}: host tier filled from hdf5 in {time.perf_counter() - t0:.2f}s, "
@@ -544,6 +546,8 @@ def run_c5(ctx, args):
     ctx.cache_attach_device_volume(d_vol, (X, Y, Z))
     ctx.set_march_options(skip_empty=True, max_defer=args.max_defer, persistent=bool(args.persistent),
                           max_defer_late=None if args.max_defer_late < 0 else args.max_defer_late)
+    if args.supertile is not None:
+        ctx.set_first_pass_tiles(args.supertile)
     for k in range(args.warmup):
         ctx.render_frame(cams[k], prm, d_rgba, d_hits)
     ctx.synchronize()
@@ -722,6 +726,8 @@ def main() -> int:
     ap.add_argument("--max-defer-late", type=int, default=2, help="max-defer of the passes after the first (-1: same)")
     ap.add_argument("--wide-lanes", type=int, default=None,
                     help="lane budget of the wide march in later passes (0: off; default: the library's)")
+    ap.add_argument("--supertile", type=int, default=None,
+                    help="first pass: S x S blocks of 8x8 tiles per XCD (1: single tiles; default: the library's)")
     ap.add_argument("--first-nodes", type=int, default=None,
                     help="walk nodes per ray in the first pass before it parks (0: no limit; default: the library's)")
     ap.add_argument("--early-reserve", type=int, default=None,
@@ -778,6 +784,8 @@ def main() -> int:
         ctx.set_early_loads(args.early_reserve)
     if args.first_nodes is not None:
         ctx.set_first_pass_budget(args.first_nodes)
+    if args.supertile is not None:
+        ctx.set_first_pass_tiles(args.supertile)
     cams = [w.cameras(k) for k in range(args.warmup + args.steps)]
     R = w.W * w.H
     d_rgba, d_hits = ctx.alloc(R * 16), ctx.alloc(R * 32)

@@ -469,3 +469,36 @@ def test_iso_change_keeps_the_cache_exact(vt, vto, ctx):
         rc, hc = sc.render_cpu(vto, cam)
         assert st.err == 0
         assert_same(hg, rg, hc, rc)
+
+
+@pytest.mark.parametrize("slots", [512, 17])
+def test_first_pass_tile_blocks_are_exact(vt, vto, ctx, slots):
+    """The one-lane first pass gives the same bytes whether the XCDs are
+    dealt single 8x8 tiles or S x S blocks of them (image sizes that leave
+    partial blocks at the right and bottom edges)."""
+    vol = blobs_volume(64, 16, seed=111)
+    sc = Scene(vto, vol, 0.3, 4, 3)
+    sc.setup_gpu(vt, ctx, slots=slots)
+    cams = [orbit_camera(vt, 160, 120, (32, 32, 32), 100.0, 25.0 + 60.0 * k, 15.0) for k in range(2)]
+    cams.append(orbit_camera(vt, 93, 71, (32, 32, 32), 90.0, 200.0, -10.0))
+    outs = {}
+    try:
+        ctx.set_wide_march(1 << 18, 0)   # the first pass runs one lane per ray
+        for S in (1, 2, 3, 4, 8):
+            log(f"supertile={S} slots={slots}")
+            ctx.set_first_pass_tiles(S)
+            ctx.cache_clear()
+            rows = []
+            for cam in cams:
+                rg, hg, st = sc.render_gpu(vt, ctx, cam)
+                assert st.err == 0
+                rows.append((hg.tobytes(), rg.tobytes()))
+            outs[S] = rows
+    finally:
+        ctx.set_first_pass_tiles()
+        ctx.set_wide_march()
+    for S, rows in outs.items():
+        assert rows == outs[1], S
+    rc, hc = sc.render_cpu(vto, cams[-1])
+    assert_same(np.frombuffer(outs[4][-1][0], vt.HIT_DTYPE), np.frombuffer(outs[4][-1][1], np.float32).reshape(-1, 4),
+                hc, rc)

@@ -124,6 +124,7 @@ _SIGS = {
     "vt_set_wide_march": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int64]),
     "vt_set_early_loads": (ctypes.c_int, [_P, ctypes.c_int64]),
     "vt_set_first_pass_budget": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "vt_set_first_pass_tiles": (ctypes.c_int, [_P, ctypes.c_int32]),
     "vt_trace_nodes": (ctypes.c_int, [_P, ctypes.POINTER(Camera), ctypes.c_int32, ctypes.c_int32,
                                       ctypes.c_int32, _P, _P]),
     "vt_octree_build": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
@@ -430,6 +431,11 @@ class Context:
         """Rays park after `nodes` walk nodes in the first pass and finish
         in the later passes (0: no limit)."""
         self._c(lib().vt_set_first_pass_budget(self.handle, int(nodes)))
+
+    def set_first_pass_tiles(self, supertile: int = 1) -> None:
+        """Deal S x S blocks of 8x8 tiles to the XCDs in the one-lane first
+        pass (1: single tiles)."""
+        self._c(lib().vt_set_first_pass_tiles(self.handle, int(supertile)))
 
     def brick_trace(self, level_c: int) -> np.ndarray:
         n = 1 << (3 * level_c)

@@ -69,6 +69,7 @@ struct RenderArgs {
     uint64_t* ray_defer;         // [max_defer][R] deferred nodes, or null
     int max_defer;               // 0: park at the first absent brick
     int node_budget;             // walk nodes a ray may take in this pass before it parks (0: no limit)
+    int supertile;               // first pass: S x S blocks of 8x8 tiles dealt to the XCDs (<= 1: single tiles)
     uint32_t* next;              // persistent form: 8 work counters (one per XCD part), zero at launch
     const uint32_t* in_list;     // null: first pass over the whole frame
     uint32_t in_count;

@@ -455,11 +455,25 @@ __global__ __launch_bounds__(kPassBlock) void k_render_pass(RenderArgs a)
         // expensive rays, which cluster in the image, spread over all CUs.
         // (Giving each XCD a contiguous band of tiles, for L2 reuse, measured
         // 8% slower on the 1024^3 workload: the bands holding the surface
-        // became the tail.)
+        // became the tail.)  With supertile S > 1 the XCDs are dealt S x S
+        // blocks of tiles instead (block q to XCD q % 8, the k-th workgroup
+        // of an XCD taking tile k % S^2 of its (k / S^2)-th block), so the
+        // tiles of a block share one L2 while the blocks of every XCD still
+        // spread over the image.
         const int tx8 = (W + 7) >> 3, ty8 = (H + 7) >> 3;
-        const int t = (int)blockIdx.x * (kPassBlock / 64) + (int)(threadIdx.x >> 6);
         const int lane = threadIdx.x & 63;
-        active = t < tx8 * ty8;
+        int t;
+        if (a.supertile > 1) {
+            const int S = a.supertile, sx = (tx8 + S - 1) / S, sy = (ty8 + S - 1) / S;
+            const int k = (int)(blockIdx.x >> 3), r = k % (S * S);
+            const int q = (k / (S * S)) * 8 + (int)(blockIdx.x & 7);
+            const int tx = (q % sx) * S + r % S, ty = (q / sx) * S + r / S;
+            active = q < sx * sy && tx < tx8 && ty < ty8;
+            t = active ? ty * tx8 + tx : 0;
+        } else {
+            t = (int)blockIdx.x * (kPassBlock / 64) + (int)(threadIdx.x >> 6);
+        }
+        active = active && t < tx8 * ty8;
         const int i = (active ? t / tx8 : 0) * 8 + (lane >> 3);
         const int j = (active ? t % tx8 : 0) * 8 + (lane & 7);
         active = active && i < H && j < W;
@@ -742,9 +756,15 @@ hipError_t launch_render_pass(const RenderArgs& a, hipStream_t s)
         if (a.in_count == 0) return hipSuccess;
         grid = (a.in_count + kPassBlock - 1) / kPassBlock;
     } else {
-        const unsigned tiles = (unsigned)((a.cam.W + 7) / 8) * (unsigned)((a.cam.H + 7) / 8);
-        const unsigned per = kPassBlock / 64;
-        grid = (tiles + per - 1) / per;
+        const unsigned tx8 = (unsigned)((a.cam.W + 7) / 8), ty8 = (unsigned)((a.cam.H + 7) / 8);
+        if (a.supertile > 1) {
+            // every XCD gets the same number of S x S blocks (the last ones may be empty)
+            const unsigned S = (unsigned)a.supertile, blocks = ((tx8 + S - 1) / S) * ((ty8 + S - 1) / S);
+            grid = (blocks + 7) / 8 * 8 * S * S;
+        } else {
+            const unsigned per = kPassBlock / 64;
+            grid = (tx8 * ty8 + per - 1) / per;
+        }
     }
     hipLaunchKernelGGL(k_render_pass, dim3(grid), dim3(kPassBlock), 0, s, a);
     return hipGetLastError();

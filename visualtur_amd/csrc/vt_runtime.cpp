@@ -850,6 +850,14 @@ int vt_set_wide_march(vt_context* ctx, int64_t lane_budget, int64_t first_pass_b
     return VT_OK;
 }
 
+int vt_set_first_pass_tiles(vt_context* ctx, int32_t supertile)
+{
+    if (!ctx) return fail(VT_E_ARG, "ctx is NULL");
+    if (supertile < 1 || supertile > 16) return fail(VT_E_ARG, "supertile must be in [1, 16]");
+    ctx->supertile = supertile;
+    return VT_OK;
+}
+
 int vt_set_first_pass_budget(vt_context* ctx, int32_t nodes)
 {
     if (!ctx) return fail(VT_E_ARG, "ctx is NULL");
@@ -937,6 +945,7 @@ static int render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_p
     a.ray_defer = std::max(ctx->max_defer, ctx->max_defer_late) > 0 ? ctx->ray_defer.p : nullptr;
     a.max_defer = ctx->max_defer;
     a.node_budget = ctx->first_nodes;
+    a.supertile = ctx->supertile;
     a.req_bits = ctx->req_bits.p;
     a.req_list = ctx->req_list.p;
     a.slot_ref = c.d_ref;
