@@ -404,6 +404,15 @@ int vt_server_set_lanes(vt_server *sv, int64_t lane_budget);
  * 0..levelO, bricks of level levelC with cubeInc halo, all resident. */
 int vt_server_add_scene(vt_server *sv, const float *d_vol, int32_t X, int32_t Y, int32_t Z, int32_t levelO,
                         int32_t levelC, int32_t cubeInc, int32_t *scene_out);
+/* The same for requests with isovalues in [iso_lo, iso_hi] only: resident
+ * are the bricks whose level-levelC min/max has min < iso_hi and
+ * max >= iso_lo, which holds every brick such a request can march (a node
+ * is visited only when min < iso <= max); requests outside the range are
+ * refused (VT_E_ARG).  *resident_out (may be NULL) gets the number of
+ * resident bricks. */
+int vt_server_add_scene_range(vt_server *sv, const float *d_vol, int32_t X, int32_t Y, int32_t Z, int32_t levelO,
+                              int32_t levelC, int32_t cubeInc, float iso_lo, float iso_hi, int32_t *scene_out,
+                              int64_t *resident_out);
 int vt_server_run(vt_server *sv, const vt_request *reqs, int64_t n, int32_t mode, vt_request_result *results,
                   vt_serve_stats *stats);
 

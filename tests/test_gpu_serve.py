@@ -96,3 +96,46 @@ def test_admission_respects_the_budget(vt, ctx):
         sv.run([{"scene": s, "cam": random_walk(1024, 1024, 32, 1, seed=0)[0], "iso": 0.3}], mode=0)
     sv.close()
     d.free()
+
+
+def test_isovalue_range_scene(vt, vto, ctx):
+    """A scene for isovalues in [0.25, 0.35] holds only the bricks whose
+    min/max straddles the range, renders every request in the range to the
+    same bytes as a scene holding every brick (and as the oracle), and
+    refuses isovalues outside the range."""
+    from visualtur_amd.prefetch import random_walk
+
+    vol = blobs_volume(64, 16, seed=5)
+    sv = vt.Server(ctx, hbm_budget=64 << 20, max_tiles=96)
+    d = ctx.to_device(vol)
+    full = sv.add_scene(d, vol.shape, 4, 3)
+    part, resident = sv.add_scene_range(d, vol.shape, 4, 3, 0.25, 0.35)
+    # the bricks whose zero-padded samples straddle the range, from the volume
+    pad = np.zeros((65, 65, 65), np.float32)
+    pad[:64, :64, :64] = vol
+    want = 0
+    for bx in range(8):
+        for by in range(8):
+            for bz in range(8):
+                box = pad[8 * bx:8 * bx + 9, 8 * by:8 * by + 9, 8 * bz:8 * bz + 9]
+                want += int(box.min() < 0.35 and box.max() >= 0.25)
+    print(f"resident bricks {resident} of 512 (expected {want})", flush=True)
+    assert resident == want and 0 < resident < 512
+    reqs = []
+    for k, iso in enumerate([0.25, 0.3, 0.35, 0.3]):
+        cam = random_walk(96, 80, 64, 1, seed=300 + k)[0]
+        for scene in (full, part):
+            reqs.append({"scene": scene, "cam": cam, "iso": iso, "arrival_s": 0.0, "keep": True})
+    _, _, out = sv.run(reqs, mode=0)
+    for k in range(0, len(reqs), 2):
+        (rf, hf), (rp, hp) = out[k], out[k + 1]
+        assert hf.tobytes() == hp.tobytes() and np.array_equal(rf.view(np.uint32), rp.view(np.uint32)), k
+    q = reqs[-1]
+    n, runs = vto.build_octree(vol, q["iso"], literal=False)
+    rc, hc, _ = vto.render(vto.Volume(vol), vto.Octree(runs, n, 4), q["cam"], vto.params(4, 3, q["iso"]))
+    assert np.array_equal(out[len(reqs) - 1][1]["node"], hc["node"])
+    assert (hc["node"] != 0).mean() > 0.05
+    with pytest.raises(vt.VTError):
+        sv.run([{"scene": part, "cam": reqs[0]["cam"], "iso": 0.5, "arrival_s": 0.0}], mode=0)
+    sv.close()
+    d.free()

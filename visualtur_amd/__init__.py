@@ -142,6 +142,10 @@ _SIGS = {
     "vt_server_set_lanes": (ctypes.c_int, [_P, ctypes.c_int64]),
     "vt_server_add_scene": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                            ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
+    "vt_server_add_scene_range": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                                 ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                                                 ctypes.c_float, ctypes.POINTER(ctypes.c_int32),
+                                                 ctypes.POINTER(ctypes.c_int64)]),
     "vt_server_run": (ctypes.c_int, [_P, _P, ctypes.c_int64, ctypes.c_int32, _P, ctypes.POINTER(ServeStats)]),
     "vt_mlp_num_params": (ctypes.c_int, []),
     "vt_prefetch": (ctypes.c_int, [_P, _P, ctypes.POINTER(Camera), ctypes.c_int64, ctypes.c_float,
@@ -541,6 +545,15 @@ class Server:
         _check(lib().vt_server_add_scene(self.handle, d_vol.ptr, *[int(d) for d in dims], level_o, level_c,
                                          cube_inc, ctypes.byref(sid)), self.ctx.handle)
         return sid.value
+
+    def add_scene_range(self, d_vol: DeviceBuffer, dims, level_o: int, level_c: int, iso_lo: float, iso_hi: float,
+                        cube_inc: int = 3) -> tuple[int, int]:
+        """A scene for isovalues in [iso_lo, iso_hi]; returns (scene id, resident bricks)."""
+        sid, nres = ctypes.c_int32(), ctypes.c_int64()
+        _check(lib().vt_server_add_scene_range(self.handle, d_vol.ptr, *[int(d) for d in dims], level_o, level_c,
+                                               cube_inc, iso_lo, iso_hi, ctypes.byref(sid), ctypes.byref(nres)),
+               self.ctx.handle)
+        return sid.value, nres.value
 
     def run(self, requests, mode: int = 0):
         """requests: list of dicts {scene, cam, iso, arrival_s, keep (bool)}.  Returns

@@ -32,6 +32,8 @@ struct Scene {
     std::map<uint32_t, int> root;
     std::vector<uint64_t> level_off;
     size_t table_bytes = 0;
+    float iso_lo = -INFINITY, iso_hi = INFINITY;   // isovalues the resident bricks serve
+    int64_t resident = 0;
 };
 
 struct Arena {   // first-fit allocator over one device allocation
@@ -208,10 +210,16 @@ int vt_server_destroy(vt_server* sv)
     return VT_OK;
 }
 
-int vt_server_add_scene(vt_server* sv, const float* d_vol, int32_t X, int32_t Y, int32_t Z, int32_t levelO,
-                        int32_t levelC, int32_t cubeInc, int32_t* scene_out)
+// A scene serves requests with isovalues in [iso_lo, iso_hi].  A request at
+// iso visits only nodes with min < iso <= max (the membership tables below),
+// so every brick it marches has min < iso_hi and max >= iso_lo at levelC:
+// those bricks are the resident ones (all of them for the full range).
+static int add_scene(vt_server* sv, const float* d_vol, int32_t X, int32_t Y, int32_t Z, int32_t levelO,
+                     int32_t levelC, int32_t cubeInc, float iso_lo, float iso_hi, int32_t* scene_out,
+                     int64_t* resident_out)
 {
     if (!sv || !d_vol || !scene_out) return fail(VT_E_ARG, "NULL argument");
+    if (!(iso_lo <= iso_hi)) return fail(VT_E_ARG, "the isovalue range must have iso_lo <= iso_hi");
     if (X <= 0 || Y <= 0 || Z <= 0) return fail(VT_E_ARG, "bad volume dims");
     const int n = tree_levels3(X, Y, Z);
     if (levelC < 0 || levelC > levelO || levelO > n || levelO > 8 || cubeInc < 0)
@@ -222,6 +230,8 @@ int vt_server_add_scene(vt_server* sv, const float* d_vol, int32_t X, int32_t Y,
     sc->level_o = levelO;
     sc->level_c = levelC;
     sc->inc = cubeInc;
+    sc->iso_lo = iso_lo;
+    sc->iso_hi = iso_hi;
     const int cube = 1 << (n - levelC);
     sc->S = cube + 2 * cubeInc;
     sc->slot_elems = (size_t)sc->S * sc->S * sc->S;
@@ -253,36 +263,66 @@ int vt_server_add_scene(vt_server* sv, const float* d_vol, int32_t X, int32_t Y,
         off += 1ull << (3 * (l - 1));
     }
     sc->table_bytes = (size_t)off;
-    // every brick resident: slot = brick index
+    // the resident bricks, in brick order, take slots 0, 1, ...; the others
+    // map to -1
     const int64_t nb = 1ll << (3 * levelC);
-    size_t pool_bytes = (size_t)nb * sc->slot_elems * 4;
+    const auto& bmn = sc->mn[(size_t)levelC];
+    const auto& bmx = sc->mx[(size_t)levelC];
+    std::vector<int32_t> slot_of((size_t)nb, -1);
+    std::vector<int64_t> codes;
+    std::vector<int32_t> slots;
+    for (int64_t b = 0; b < nb; ++b)
+        if (bmn[(size_t)b] < iso_hi && bmx[(size_t)b] >= iso_lo) {
+            slot_of[(size_t)b] = (int32_t)codes.size();
+            slots.push_back((int32_t)codes.size());
+            codes.push_back(b + nb);
+        }
+    const int64_t nr = (int64_t)codes.size();
+    sc->resident = nr;
+    size_t pool_bytes = (size_t)std::max<int64_t>(nr, 1) * sc->slot_elems * 4;
     if (hipMalloc((void**)&sc->d_pool, pool_bytes) != hipSuccess)
         return fail(VT_E_NOMEM, "scene brick pool (" + std::to_string(pool_bytes) + " bytes)");
-    std::vector<int32_t> ident((size_t)nb);
-    std::vector<int64_t> codes((size_t)nb);
-    for (int64_t b = 0; b < nb; ++b) { ident[(size_t)b] = (int32_t)b; codes[(size_t)b] = b + nb; }
     HIP_TRY(hipMalloc((void**)&sc->d_slot, (size_t)nb * 4));
-    HIP_TRY(hipMemcpy(sc->d_slot, ident.data(), (size_t)nb * 4, hipMemcpyHostToDevice));
-    DevBuf<int64_t> dcodes;
-    if (dcodes.ensure((size_t)nb) != hipSuccess) return fail(VT_E_NOMEM, "codes");
-    HIP_TRY(hipMemcpy(dcodes.p, codes.data(), (size_t)nb * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(sc->d_slot, slot_of.data(), (size_t)nb * 4, hipMemcpyHostToDevice));
     if (cube % 4 == 0 && cubeInc >= 1) {
         sc->G = cube / 4;
-        size_t mm = (size_t)nb * sc->G * sc->G * sc->G * sizeof(float2);
+        size_t mm = (size_t)std::max<int64_t>(nr, 1) * sc->G * sc->G * sc->G * sizeof(float2);
         if (hipMalloc((void**)&sc->d_submm, mm) != hipSuccess) return fail(VT_E_NOMEM, "sub-block min/max");
     }
-    for (int64_t o = 0; o < nb; o += 65536) {
-        int cnt = (int)std::min<int64_t>(65536, nb - o);
-        HIP_TRY(launch_bricks_from_volume(d_vol, X, Y, Z, dcodes.p + o, sc->d_slot + o, cnt, levelC, n, cube, cubeInc,
-                                          sc->slot_elems,
-                                          sc->d_pool, s));
-        if (sc->d_submm)
-            HIP_TRY(launch_submm(sc->d_pool, sc->d_slot + o, cnt, sc->S, cubeInc, sc->G, sc->slot_elems, sc->d_submm, s));
+    if (nr > 0) {
+        DevBuf<int64_t> dcodes;
+        DevBuf<int32_t> dslots;
+        if (dcodes.ensure((size_t)nr) != hipSuccess || dslots.ensure((size_t)nr) != hipSuccess)
+            return fail(VT_E_NOMEM, "codes");
+        HIP_TRY(hipMemcpy(dcodes.p, codes.data(), (size_t)nr * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(dslots.p, slots.data(), (size_t)nr * 4, hipMemcpyHostToDevice));
+        for (int64_t o = 0; o < nr; o += 65536) {
+            int cnt = (int)std::min<int64_t>(65536, nr - o);
+            HIP_TRY(launch_bricks_from_volume(d_vol, X, Y, Z, dcodes.p + o, dslots.p + o, cnt, levelC, n, cube,
+                                              cubeInc, sc->slot_elems, sc->d_pool, s));
+            if (sc->d_submm)
+                HIP_TRY(launch_submm(sc->d_pool, dslots.p + o, cnt, sc->S, cubeInc, sc->G, sc->slot_elems,
+                                     sc->d_submm, s));
+        }
+        HIP_TRY(hipStreamSynchronize(s));
     }
-    HIP_TRY(hipStreamSynchronize(s));
     *scene_out = (int32_t)sv->scenes.size();
+    if (resident_out) *resident_out = nr;
     sv->scenes.push_back(std::move(sc));
     return VT_OK;
+}
+
+int vt_server_add_scene(vt_server* sv, const float* d_vol, int32_t X, int32_t Y, int32_t Z, int32_t levelO,
+                        int32_t levelC, int32_t cubeInc, int32_t* scene_out)
+{
+    return add_scene(sv, d_vol, X, Y, Z, levelO, levelC, cubeInc, -INFINITY, INFINITY, scene_out, nullptr);
+}
+
+int vt_server_add_scene_range(vt_server* sv, const float* d_vol, int32_t X, int32_t Y, int32_t Z, int32_t levelO,
+                              int32_t levelC, int32_t cubeInc, float iso_lo, float iso_hi, int32_t* scene_out,
+                              int64_t* resident_out)
+{
+    return add_scene(sv, d_vol, X, Y, Z, levelO, levelC, cubeInc, iso_lo, iso_hi, scene_out, resident_out);
 }
 
 int vt_server_run(vt_server* sv, const vt_request* reqs, int64_t nreq, int32_t mode, vt_request_result* results,
@@ -297,6 +337,9 @@ int vt_server_run(vt_server* sv, const vt_request* reqs, int64_t nreq, int32_t m
     for (int64_t i = 0; i < nreq; ++i) {
         const vt_request& q = reqs[i];
         if (q.scene < 0 || q.scene >= (int32_t)sv->scenes.size()) return fail(VT_E_ARG, "unknown scene");
+        const Scene& qs = *sv->scenes[(size_t)q.scene];
+        if (!(q.iso >= qs.iso_lo && q.iso <= qs.iso_hi))
+            return fail(VT_E_ARG, "isovalue outside the scene's range");
         if (q.cam.W <= 0 || q.cam.H <= 0 || (int64_t)q.cam.W * q.cam.H > (1 << 26)) return fail(VT_E_ARG, "bad screen");
         size_t need = (size_t)q.cam.W * q.cam.H * 48 + 512;
         if ((int64_t)need > sv->budget) return fail(VT_E_ARG, "a request's outputs exceed the HBM budget");
