@@ -453,6 +453,43 @@ def fly_through(W: int, H: int, frames: int, X: int, Y: int, Z: int):
     return cams
 
 
+def c5_viewers(ctx, args, d_vol, dims, level_c, iso):
+    """The concurrent 1024^2 viewers of the large configuration: the volume
+    becomes a serving scene for its one isovalue (only the bricks that hold
+    nodes at it are resident), and seeded random-walk viewers issue requests
+    as a Poisson stream; batched serving against one launch per request."""
+    from visualtur_amd.prefetch import random_walk
+
+    sv = vt.Server(ctx, hbm_budget=2 << 30, max_tiles=65536)
+    t0 = time.perf_counter()
+    scene, resident = sv.add_scene_range(d_vol, dims, level_c, level_c, iso, iso)
+    log(f"c5 viewers: scene of {resident} resident bricks in {time.perf_counter() - t0:.1f}s")
+    rng = np.random.default_rng(0xC5)
+    nv, per = args.c5_viewers, args.c5_viewer_frames
+    walks = {v: random_walk(1024, 1024, dims[0], per, seed=9000 + v) for v in range(nv)}
+    reqs, t = [], 0.0
+    for k in range(nv * per):
+        t += rng.exponential(1.0 / args.c5_viewer_rate)
+        v = k % nv
+        reqs.append({"scene": scene, "cam": walks[v][k // nv], "iso": iso, "arrival_s": t, "keep": k < 8})
+    out = {"viewers": nv, "requests": len(reqs), "rate_per_s": args.c5_viewer_rate, "resident_bricks": resident,
+           "resolution": "1024x1024"}
+    kept = {}
+    for mode, name in ((0, "batched"), (1, "per_request")):
+        res, st, o = sv.run(reqs, mode=mode)
+        kept[name] = o
+        lat = np.array([r["latency_ms"] for r in res])
+        out[name] = {"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                     "rays_per_s": round(st["rays"] / st["wall_s"], 1),
+                     "busy_rays_per_s": round(st["rays"] / max(st["busy_s"], 1e-9), 1), "launches": st["launches"]}
+        log(f"c5 viewers {name}: {out[name]}")
+    out["same_bytes_batched_and_alone"] = all(
+        kept["batched"][k][1].tobytes() == kept["per_request"][k][1].tobytes() and
+        kept["batched"][k][0].tobytes() == kept["per_request"][k][0].tobytes() for k in kept["batched"])
+    sv.close()
+    return out
+
+
 def run_c5(ctx, args):
     """Config 5 with the volume resident in HBM as the brick source:
     4096x4096x2048 float32 (128 GiB), a seeded turbulence-like field (64
@@ -462,6 +499,8 @@ def run_c5(ctx, args):
     with a 3-voxel halo; device brick cache 48 GB; a 3840x2160 fly-through.
     The 128 GB HDF5 file and the 96 GB pinned host tier of the configuration
     are replaced by the volume in HBM (bricks are copied from it on a miss).
+    The 16 concurrent 1024^2 viewers are served from a scene holding the
+    bricks with nodes at the isovalue (batched and one launch per request).
     Size-independent checks at full size: the generator against its float32
     formula and its own slabs, the builder's level-7 nodes against the cell
     test on one layer of bricks, hits unchanged with an 8 GB cache, and
@@ -582,6 +621,9 @@ def run_c5(ctx, args):
                                         "hit_fraction": round(float((last[0]["node"] != 0).mean()), 4),
                                         "passes": int(st.passes), "uploads": int(st.bricks_uploaded)}
     ctx.cache_destroy()
+    viewers = None
+    if args.c5_viewers > 0:
+        viewers = c5_viewers(ctx, args, d_vol, (X, Y, Z), level_c, iso)
     m = np.array(ms)
     octree_bytes = 8 * sum(r.size for r in runs)
     B = [u * Sb + R * 48 + octree_bytes for u in U]
@@ -605,7 +647,7 @@ def run_c5(ctx, args):
                         "kernel": RENDER_KERNELS, "avg_launch_ms": round(sum(kms) / launches, 4),
                         "bytes_per_launch": int(sum(B) / launches), "bricks_marched_per_frame": int(np.mean(U)),
                         "brick_bytes": Sb},
-           "checks": checks, "cpu_baseline": None}
+           "checks": checks, "viewers": viewers, "cpu_baseline": None}
     if not args.no_cpu:
         from oracle import vto
 
@@ -716,6 +758,9 @@ def main() -> int:
     ap.add_argument("--c3-train-paths", type=int, default=512)
     ap.add_argument("--c3-train-frames", type=int, default=4)
     ap.add_argument("--c3-train-steps", type=int, default=1500)
+    ap.add_argument("--c5-viewers", type=int, default=16, help="c5: concurrent 1024^2 viewers served beside (0: none)")
+    ap.add_argument("--c5-viewer-frames", type=int, default=8)
+    ap.add_argument("--c5-viewer-rate", type=float, default=200.0)
     ap.add_argument("--rates", type=float, nargs="*", default=[50, 100, 200, 400, 800])
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
