@@ -164,6 +164,68 @@ __global__ __launch_bounds__(256) void k_leaf_bits(const float* __restrict__ vol
     bits[w] = out;
 }
 
+// The same words for cubes of side >= 32, a 32^3-cell block (512 words) per
+// 256-thread workgroup: its 33^3 corner samples are read once into LDS as
+// below-iso flags (1.10 reads per sample, where a word alone reads 5^3
+// samples for 4^3 cells), then each thread builds two words from them with
+// the flags and bit layout of k_leaf_bits.
+constexpr int kLeafSide = 32, kLeafS = kLeafSide + 1, kLeafZ = kLeafS + 3;   // z rows padded to 36 bytes
+
+__global__ __launch_bounds__(256) void k_leaf_bits32(const float* __restrict__ vol, int X, int Y, int Z, float iso,
+                                                     uint64_t* __restrict__ bits)
+{
+    __shared__ uint8_t below[kLeafS][kLeafS][kLeafZ];
+    const uint64_t c0 = (uint64_t)blockIdx.x << 15;   // first cell of the block, Morton order
+    const int x0 = (int)compact3(c0 >> 2), y0 = (int)compact3(c0 >> 1), z0 = (int)compact3(c0);
+    // rows along z: eight 16-byte loads (z0 + 4q, 4 samples) and the 33rd
+    // sample; a run that leaves the volume, or rows that are not 16-byte
+    // aligned (Z not a multiple of 4, or an unaligned volume), go sample by
+    // sample
+    const bool vec = (Z & 3) == 0 && ((uintptr_t)vol & 15) == 0;
+    for (int e = threadIdx.x; e < kLeafS * kLeafS * 9; e += 256) {
+        const int row = e / 9, q = e - row * 9;
+        const int x = row / kLeafS, y = row - x * kLeafS;
+        const int gx = x0 + x, gy = y0 + y, gz = z0 + 4 * q;
+        uint8_t* dst = &below[x][y][4 * q];
+        if (q < 8 && vec && gx < X && gy < Y && gz + 3 < Z) {
+            const float4 v = *reinterpret_cast<const float4*>(vol + ((size_t)gx * Y + gy) * Z + gz);
+            dst[0] = (v.x - iso) < 0 ? 1 : 0;
+            dst[1] = (v.y - iso) < 0 ? 1 : 0;
+            dst[2] = (v.z - iso) < 0 ? 1 : 0;
+            dst[3] = (v.w - iso) < 0 ? 1 : 0;
+        } else {
+            for (int i = 0; i < (q < 8 ? 4 : 1); ++i)
+                dst[i] = (vox(vol, X, Y, Z, gx, gy, gz + i) - iso) < 0 ? 1 : 0;
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 512; k += 256) {
+        const uint64_t m = (uint64_t)k << 6;   // the word's first cell, relative to the block
+        const int bx = (int)compact3(m >> 2), by = (int)compact3(m >> 1), bz = (int)compact3(m);
+        uint32_t rows[5][5];   // bit z of rows[x][y]: sample (bx+x, by+y, bz+z) below iso
+#pragma unroll
+        for (int x = 0; x < 5; ++x)
+#pragma unroll
+            for (int y = 0; y < 5; ++y) {
+                uint32_t row = 0;
+#pragma unroll
+                for (int z = 0; z < 5; ++z) row |= (uint32_t)below[bx + x][by + y][bz + z] << z;
+                rows[x][y] = row;
+            }
+        uint64_t out = 0;
+#pragma unroll
+        for (int b = 0; b < 64; ++b) {
+            int cx = ((b >> 2) & 1) | (((b >> 5) & 1) << 1);
+            int cy = ((b >> 1) & 1) | (((b >> 4) & 1) << 1);
+            int cz = (b & 1) | (((b >> 3) & 1) << 1);
+            uint32_t f = ((rows[cx][cy] >> cz) & 3u) | (((rows[cx][cy + 1] >> cz) & 3u) << 2) |
+                         (((rows[cx + 1][cy] >> cz) & 3u) << 4) | (((rows[cx + 1][cy + 1] >> cz) & 3u) << 6);
+            out |= (f != 0u && f != 0xffu ? 1ull : 0ull) << b;
+        }
+        bits[((uint64_t)blockIdx.x << 9) + (uint64_t)k] = out;
+    }
+}
+
 // level l (>= 2, 8^l >= 64) word from 8 words of level l+1
 __global__ void k_parent_bits(const uint64_t* __restrict__ child, uint64_t nwords, uint64_t* __restrict__ parent)
 {
@@ -301,8 +363,12 @@ hipError_t launch_generate(float* out, const GenParams& p, int64_t x0, int64_t n
 hipError_t launch_leaf_bits(const float* vol, int X, int Y, int Z, float iso, uint64_t nwords, uint64_t* bits,
                             hipStream_t s)
 {
-    hipLaunchKernelGGL(k_leaf_bits, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, s, vol, X, Y, Z, iso,
-                       nwords, bits);
+    if (nwords >= 512) {   // a cube of side >= 32: whole 32^3 blocks
+        hipLaunchKernelGGL(k_leaf_bits32, dim3((unsigned)(nwords / 512)), dim3(256), 0, s, vol, X, Y, Z, iso, bits);
+    } else {
+        hipLaunchKernelGGL(k_leaf_bits, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, s, vol, X, Y, Z, iso,
+                           nwords, bits);
+    }
     return hipGetLastError();
 }
 
