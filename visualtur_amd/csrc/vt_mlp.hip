@@ -59,7 +59,9 @@ __global__ __launch_bounds__(256) void k_gemm(const bf16* __restrict__ A, int ld
             const int valid = gm < M ? max(0, min(8, kend - gk)) : 0;
             *reinterpret_cast<bf16x8*>(&As[r][c]) = load8(A + (size_t)gm * lda + gk, valid, vec);
         } else {
-            const int kr = tid >> 3, c = (tid & 7) * 8, gk = k0 + kr, gm = m0 + c;
+            // a wave's lanes take 32 k rows x 2 runs, so each of the eight
+            // column stores below spreads over 32 LDS banks
+            const int kr = tid & 31, c = (tid >> 5) * 8, gk = k0 + kr, gm = m0 + c;
             const int valid = gk < kend ? max(0, min(8, M - gm)) : 0;
             const bf16x8 v = load8(A + (size_t)gk * lda + gm, valid, vec);
 #pragma unroll
@@ -70,7 +72,7 @@ __global__ __launch_bounds__(256) void k_gemm(const bf16* __restrict__ A, int ld
             const int valid = gn < N ? max(0, min(8, kend - gk)) : 0;
             *reinterpret_cast<bf16x8*>(&Bs[r][c]) = load8(B + (size_t)gn * ldb + gk, valid, vec);
         } else {
-            const int kr = tid >> 3, c = (tid & 7) * 8, gk = k0 + kr, gn = n0 + c;
+            const int kr = tid & 31, c = (tid >> 5) * 8, gk = k0 + kr, gn = n0 + c;
             const int valid = gk < kend ? max(0, min(8, N - gn)) : 0;
             const bf16x8 v = load8(B + (size_t)gk * ldb + gn, valid, vec);
 #pragma unroll
