@@ -440,7 +440,8 @@ __device__ __forceinline__ void pass_ray(const RenderArgs& a, uint32_t id, bool 
 // tile (the tiles of a workgroup are consecutive in row-major tile order).
 constexpr int kPassBlock = 64;
 
-__global__ __launch_bounds__(kPassBlock) void k_render_pass(RenderArgs a)
+// at most 96 VGPRs: 5 waves per SIMD instead of 4, without spilling
+__global__ __launch_bounds__(kPassBlock, 5) void k_render_pass(RenderArgs a)
 {
     const int W = a.cam.W, H = a.cam.H;
     uint32_t id;
