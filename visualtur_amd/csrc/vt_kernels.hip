@@ -830,7 +830,8 @@ __device__ __forceinline__ void serve_ray(const ServeJob& J, int i, int j, const
 }
 
 template <bool kWide>
-__global__ __launch_bounds__(kPassBlock) void k_serve_pass(const ServeJob* __restrict__ jobs,
+// at most 96 VGPRs: 5 waves per SIMD for both widths, without spilling
+__global__ __launch_bounds__(kPassBlock, 5) void k_serve_pass(const ServeJob* __restrict__ jobs,
                                                            const ServeTile* __restrict__ work, ServeCounters* ctr,
                                                            int lg)
 {
