@@ -69,10 +69,25 @@ def assert_same(hits_g, rgba_g, hits_c, rgba_c, depth_tol=1e-4, rgba_tol=1.0 / 2
     assert np.all(np.abs(rgba_g[~ng].astype(np.float64) - rgba_c[~ng]) <= rgba_tol)
 
 
+def assert_bits(hits_g, rgba_g, hits_c, rgba_c):
+    """Bit-identical hit records and pixels, NaN matching NaN (a 0/0 secant
+    gives NaN on both sides, with a sign bit that differs between the GPU and
+    the host)."""
+    for f in ("node", "brick", "voxel"):
+        assert np.array_equal(hits_g[f], hits_c[f]), f
+    dg, dc = hits_g["depth"], hits_c["depth"]
+    nd = np.isnan(dg)
+    assert np.array_equal(nd, np.isnan(dc))
+    assert np.array_equal(dg[~nd].view(np.uint32), dc[~nd].view(np.uint32))
+    nr = np.isnan(rgba_g)
+    assert np.array_equal(nr, np.isnan(rgba_c))
+    assert np.array_equal(rgba_g[~nr].view(np.uint32), rgba_c[~nr].view(np.uint32))
+
+
 def bitexact_fraction(hits_g, rgba_g, hits_c, rgba_c):
     same = (hits_g["depth"].view(np.uint32) == hits_c["depth"].view(np.uint32)) & \
            np.all(rgba_g.view(np.uint32) == rgba_c.view(np.uint32), axis=1)
     return float(same.mean())
 
 
-__all__ = ["Scene", "orbit_camera", "assert_same", "bitexact_fraction", "blobs_volume", "flat_runs"]
+__all__ = ["Scene", "orbit_camera", "assert_same", "assert_bits", "bitexact_fraction", "blobs_volume", "flat_runs"]

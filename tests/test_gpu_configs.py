@@ -1,0 +1,249 @@
+"""The benchmark workloads at reduced size, end to end on the GPU against the
+oracle: the fBm and SDF generators against their float32 definitions, the
+out-of-core fBm orbit at iso 0 (HDF5 in 32^3 chunks, a device cache of a
+quarter of the bricks, an on-demand host tier), the SDF random walk with the
+learned prefetcher beside plain rendering, and the visited-brick trace
+against the bricks the reference algorithm marches."""
+import math
+
+import numpy as np
+import pytest
+
+from gpu_scenes import assert_bits, assert_same, flat_runs, orbit_camera
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def _same_bytes(a_hits, a_rgba, b_hits, b_rgba):
+    return a_hits.tobytes() == b_hits.tobytes() and a_rgba.tobytes() == b_rgba.tobytes()
+
+
+def _render(vt, ctx, cam, prm):
+    R = cam.W * cam.H
+    d_rgba, d_hits = ctx.alloc(R * 16), ctx.alloc(R * 32)
+    try:
+        st = ctx.render_frame(cam, prm, d_rgba, d_hits)
+        return d_rgba.download(np.float32, (R, 4)), d_hits.download(vt.HIT_DTYPE, (R,)), st
+    finally:
+        d_rgba.free()
+        d_hits.free()
+
+
+FBM = dict(octaves=6, gain=0.5, lacunarity=2.0, scale=1.0)
+
+
+def test_fbm_generator_equals_definition(vt, ctx):
+    """Every voxel of a 64^3 fBm volume (6 octaves, lacunarity 2, gain 0.5,
+    the 1024^3 workload's parameters) equals the float32 definition."""
+    from oracle import gen
+
+    dims = (64, 48, 40)
+    seed = 0x5EED0002
+    d = ctx.alloc(int(np.prod(dims)) * 4)
+    for base in (1.0 / 64.0, 1.0 / 8.0):
+        ctx.volume_generate(vt.GEN_FBM, dims, seed, d, base_freq=base, **FBM)
+        got = d.download(np.float32, dims)
+        x, y, z = np.meshgrid(*(np.arange(s) for s in dims), indexing="ij")
+        ref = gen.fbm(x, y, z, seed, 6, base, 0.5, 2.0, 1.0)
+        bad = np.count_nonzero(got.view(np.uint32) != ref.view(np.uint32))
+        log(f"fbm base {base}: {bad} of {got.size} voxels differ, range [{got.min():.3f}, {got.max():.3f}]")
+        assert bad == 0
+    d.free()
+
+
+def test_sdf_generator_equals_definition(vt, ctx):
+    """A 64^3 slab of the spheres-and-tori SDF plus 0.02-amplitude fBm equals
+    the float32 definition voxel for voxel (shapes of the 2048^3 workload's
+    generator, scaled to the extent)."""
+    from oracle import gen
+    from visualtur_amd.volumes import sdf_shapes
+
+    D = 96
+    shapes = sdf_shapes(40, 0x5EED0003, D, radius=(0.05, 0.2))
+    dims = (D, D, D)
+    kw = dict(n=D, octaves=5, base_freq=1.0 / 16.0, gain=0.5, lacunarity=2.0, scale=0.02)
+    d = ctx.alloc(D * D * 40 * 4)
+    ctx.volume_generate(vt.GEN_SDF, dims, 0x5EED0003, d, shapes=shapes, x0=30, nx=40, **kw)
+    got = d.download(np.float32, (40, D, D))
+    x, y, z = np.meshgrid(np.arange(30, 70), np.arange(D), np.arange(D), indexing="ij")
+    ref = gen.sdf(shapes, D, x, y, z, 0x5EED0003, 5, 1.0 / 16.0, 0.5, 2.0, 0.02)
+    bad = np.count_nonzero(got.view(np.uint32) != ref.view(np.uint32))
+    log(f"sdf: {bad} of {got.size} voxels differ, {int((got < 0).sum())} inside")
+    assert bad == 0
+    assert (got < 0).any() and (got > 0).any()
+    d.free()
+
+
+@pytest.fixture(scope="module")
+def fbm_out_of_core(vt, vto, ctx, tmp_path_factory):
+    """The 1024^3 fBm workload at 128^3: 6 octaves (base frequency scaled to
+    keep the features per extent), iso 0, HDF5 in 32^3 chunks, 32^3 bricks
+    with a 3-voxel halo, 64 bricks of which a device cache holds 16 (25 %),
+    an on-demand host tier smaller than the bricks with nodes."""
+    D = 128
+    dims = (D, D, D)
+    d_vol = ctx.alloc(D ** 3 * 4)
+    ctx.volume_generate(vt.GEN_FBM, dims, 0x5EED0002, d_vol, base_freq=1.0 / 8.0, **FBM)
+    vol = d_vol.download(np.float32, dims)
+    h5 = tmp_path_factory.mktemp("c2r") / "fbm.h5"
+    ctx.volume_write_hdf5(d_vol, dims, h5, "value", 32)
+    iso, level = 0.0, 2
+    n_c, runs_c = vto.build_octree(vol, iso, literal=False)
+    n_g, sizes = ctx.octree_build(d_vol, dims, iso, False, level)
+    assert n_g == n_c == 7
+    for l in range(n_c + 1):
+        assert np.array_equal(ctx.octree_runs(l, sizes[l]), runs_c[l]), l
+    d_vol.free()
+    with_nodes = sum(int(runs_c[level][j + 1] - runs_c[level][j]) + 1 for j in range(0, runs_c[level].size, 2))
+    cams = [orbit_camera(vt, 192, 108, (D / 2,) * 3, 2.2 * D, 5.6 * k * 4, 20.0) for k in range(5)]
+    return dict(vol=vol, h5=h5, iso=iso, level=level, n=n_c, runs=runs_c, cams=cams, with_nodes=with_nodes,
+                V=vto.Volume(vol), O=vto.Octree(runs_c, n_c, level), prm_o=vto.params(level, level, iso),
+                prm=vt.render_params(level, level, iso))
+
+
+def test_fbm_orbit_out_of_core_equals_oracle(vt, vto, ctx, fbm_out_of_core):
+    """Every frame of a short orbit, rendered through the 25 % device cache fed
+    from the HDF5 file through an on-demand host tier, equals the oracle (the
+    iso-0 zero-padding crossings and 0/0 secants included), with loads beside
+    the first pass and without."""
+    s = fbm_out_of_core
+    ctx.octree_upload(*flat_runs(s["runs"], s["level"]), s["n"], s["level"])
+    cpu_slots = max(2, s["with_nodes"] - 6)   # on demand: fewer host slots than bricks with nodes
+    for early in (256, 0):
+        ctx.cache_create(s["level"], 32, 3, 16, cpu_slots)
+        ctx.cache_attach_hdf5(str(s["h5"]), "value")
+        ctx.set_early_loads(early)
+        try:
+            nan_depths = 0
+            for k, cam in enumerate(s["cams"]):
+                rg, hg, st = _render(vt, ctx, cam, s["prm"])
+                rc, hc, _ = vto.render(s["V"], s["O"], cam, s["prm_o"])
+                log(f"fbm orbit early={early} frame {k}: passes {st.passes}, uploads {st.bricks_uploaded}, "
+                    f"host reads {st.host_tier_misses}, hits {(hc['node'] != 0).mean():.3f}")
+                assert st.err == 0
+                assert_same(hg, rg, hc, rc)
+                assert_bits(hg, rg, hc, rc)
+                nan_depths += int(np.isnan(hc["depth"]).sum())
+            cs = ctx.cache_stats()
+            assert cs["gpu_resident"] <= 16 and cs["file_reads"] > 0
+            assert cs["cpu_slots"] == cpu_slots
+        finally:
+            ctx.set_early_loads()
+            ctx.cache_destroy()
+    log(f"0/0 secant depths (NaN) in the orbit: {nan_depths}")
+
+
+def test_brick_trace_equals_reference_marched_bricks(vt, vto, ctx, fbm_out_of_core):
+    """With no deferral (a ray parks at its first absent brick, as in the
+    reference's staged loop) the visited-brick trace of a frame is exactly
+    the set of bricks the reference algorithm marches (brickID = node >>
+    3 (levelO - levelC), lruCache.cu:177), frame by frame through an
+    out-of-core cache.  With deferral on, rays also march nodes past a
+    deferred one whose result a later pass may overrule, so the trace is a
+    superset."""
+    s = fbm_out_of_core
+    ctx.octree_upload(*flat_runs(s["runs"], s["level"]), s["n"], s["level"])
+    ctx.cache_create(s["level"], 32, 3, 16, 64)
+    ctx.cache_attach_hdf5(str(s["h5"]), "value")
+    ctx.set_frame_options(collect_stats=False, record_trace=True)
+    try:
+        for defer in ((0, 0), (1, 2), (4, 4)):
+            ctx.set_march_options(skip_empty=True, max_defer=defer[0], max_defer_late=defer[1])
+            ctx.cache_clear()
+            extra = 0
+            for k, cam in enumerate(s["cams"]):
+                _, _, st = _render(vt, ctx, cam, s["prm"])
+                got = ctx.brick_trace(s["level"]).astype(bool)
+                _, _, seen = vto.render(s["V"], s["O"], cam, s["prm_o"], trace=True)
+                seen = np.asarray(seen, bool)
+                assert seen.any()
+                if defer == (0, 0):
+                    assert np.array_equal(got, seen), (k, np.nonzero(got != seen)[0][:10])
+                else:
+                    assert np.all(got[seen]), k
+                    extra += int((got & ~seen).sum())
+                log(f"trace defer={defer} frame {k}: {int(got.sum())} bricks (oracle {int(seen.sum())}), "
+                    f"passes {st.passes}")
+            if defer != (0, 0):
+                log(f"defer={defer}: {extra} speculative bricks over the frames")
+    finally:
+        ctx.set_march_options()
+        ctx.set_frame_options(False, False)
+        ctx.cache_destroy()
+
+
+def test_sdf_walk_plain_and_prefetch_equal_oracle(vt, vto, ctx, tmp_path):
+    """The 2048^3 SDF workload at 256^3 (200 spheres and tori, radii
+    U[0.01, 0.08] of the extent, plus 0.02-amplitude fBm; iso 0; HDF5 in 32^3
+    chunks; a device cache below the bricks with nodes, filled from a pinned
+    host tier): a random-walk path rendered plain and with the learned
+    prefetcher (trained on the traces of other paths) gives byte-identical
+    frames, and every frame equals the oracle."""
+    from visualtur_amd import prefetch as pf
+    from visualtur_amd.volumes import sdf_shapes
+
+    D = 256
+    dims = (D, D, D)
+    seed = 0x5EED0003
+    d_vol = ctx.alloc(D ** 3 * 4)
+    ctx.volume_generate(vt.GEN_SDF, dims, seed, d_vol, shapes=sdf_shapes(200, seed, D), n=D, octaves=5,
+                        base_freq=1.0 / 4.0, gain=0.5, lacunarity=2.0, scale=0.02)
+    vol = d_vol.download(np.float32, dims)
+    iso, level = 0.0, 3
+    n, sizes = ctx.octree_build(d_vol, dims, iso, False, level)
+    runs = [ctx.octree_runs(l, sizes[l]) for l in range(level + 1)]
+    n_c, runs_c = vto.build_octree(vol, iso, literal=False)
+    assert n == n_c
+    for l in range(level + 1):
+        assert np.array_equal(runs[l], runs_c[l]), l
+    cand = pf.candidate_codes(runs[level])
+    prm = vt.render_params(level, level, iso)
+    # training traces from the volume in HBM
+    ctx.cache_create(level, 32, 3, int(cand.size), 0)
+    ctx.cache_attach_device_volume(d_vol, dims)
+    radius = (1.3, 2.2)
+    tcams = [c for s in range(16) for c in pf.random_walk(96, 54, D, 4, seed=500 + s, radius=radius)]
+    traces = pf.collect_traces(ctx, tcams, prm, level)
+    net = vt.MLP(ctx, seed=13, max_batch=8192)
+    X, Y = pf.build_dataset(ctx, net, tcams, traces, cand, n, level, neg_per_pos=3, seed=3)
+    losses, _ = pf.train(ctx, net, X, Y, steps=300, batch=4096, lr=1e-3)
+    log(f"sdf 256^3: {cand.size} bricks with nodes, trained on {X.shape[0]} samples, "
+        f"loss {losses[:10].mean():.3f} -> {losses[-10:].mean():.3f}")
+    ctx.cache_destroy()
+    h5 = tmp_path / "sdf.h5"
+    ctx.volume_write_hdf5(d_vol, dims, h5, "value", 32)
+    d_vol.free()
+    slots = max(4, int(0.6 * cand.size))
+    cams = pf.random_walk(192, 108, D, 6, seed=seed, radius=radius)
+    V, O, prm_o = vto.Volume(vol), vto.Octree(runs_c, n_c, level), vto.params(level, level, iso)
+    frames = {}
+    try:
+        for form in ("plain", "prefetch"):
+            ctx.cache_create(level, 32, 3, slots, int(cand.size))
+            ctx.cache_attach_hdf5(str(h5), "value")
+            ctx.cache_prefetch_host(wait=True)
+            out = []
+            for k, cam in enumerate(cams):
+                pre = net.prefetch(cam, slots, 0.8) if form == "prefetch" else 0
+                rg, hg, st = _render(vt, ctx, cam, prm)
+                assert st.err == 0
+                out.append((hg, rg))
+                log(f"sdf {form} frame {k}: prefetched {pre}, demand loads {st.bricks_uploaded}, passes {st.passes}")
+            frames[form] = out
+            ctx.cache_destroy()
+    finally:
+        net.close()
+    hit_frac = []
+    for k, cam in enumerate(cams):
+        (hp, rp), (hq, rq) = frames["plain"][k], frames["prefetch"][k]
+        assert _same_bytes(hp, rp, hq, rq), k
+        rc, hc, _ = vto.render(V, O, cam, prm_o)
+        assert_same(hp, rp, hc, rc)
+        assert_bits(hp, rp, hc, rc)
+        hit_frac.append(float((hc["node"] != 0).mean()))
+    log(f"hit fractions {np.round(hit_frac, 3)}")
+    assert max(hit_frac) > 0.05

@@ -267,3 +267,78 @@ def test_fourier_modes_spectrum():
     one = np.array([[0.01, 0.02, -0.03, 0.5, 1.0]], np.float32)
     x, y, z = 10.0, 20.0, 30.0
     assert abs(float(fourier_field(one, x, y, z)) - np.cos(0.01 * x + 0.02 * y - 0.03 * z + 0.5)) < 1e-6
+
+
+def _slab64(o, d, mn, mx):
+    """Float64 slab interval of a ray and a box (or None)."""
+    t0, t1 = -np.inf, np.inf
+    for k in range(3):
+        if d[k] == 0.0:
+            if not (mn[k] <= o[k] <= mx[k]):
+                return None
+            continue
+        a, b = (mn[k] - o[k]) / d[k], (mx[k] - o[k]) / d[k]
+        t0, t1 = max(t0, min(a, b)), min(t1, max(a, b))
+    if t1 < max(t0, 0.0):
+        return None
+    return max(t0, 0.0), t1
+
+
+def test_render_cube_affine_closed_form(vto):
+    """renderCube (rayCaster.cu:267-357) on an affine field f = g.x + c inside
+    a larger volume: trilinear samples and the secant are exact, so a ray
+    whose crossing lies among its samples hits at the closed-form point, with
+    normal -g/|g| and colour scaled by the cube's dimension; a ray whose
+    crossing lies beyond its last sample (the step count is (tfar - tnear)/0.5,
+    not rounded up) or outside the cube stays white."""
+    D, dim = 64, 32
+    mb = np.array([16, 16, 16])
+    g = np.array([0.011, -0.004, 0.02])
+    idx = np.indices((D, D, D)).astype(np.float64)
+    vol = (g[0] * idx[0] + g[1] * idx[1] + g[2] * idx[2] + 0.1).astype(np.float32)
+    V = vto.Volume(vol)
+    light = (0.0, 512.0, 0.0)
+    checked_hit = checked_miss = checked_tail = 0
+    for iso, pos, tgt in [(0.75, (31.3, 30.2, 80.0), (33.0, 31.0, 20.0)), (0.6, (-20.0, 35.1, 30.4), (40.0, 30.0, 33.0)),
+                          (0.9, (70.0, 60.0, 75.0), (30.0, 28.0, 30.0))]:
+        eye = np.array(pos)
+        look = np.array(tgt) - eye
+        look /= np.linalg.norm(look)
+        right = np.cross(look, [0.0, 1.0, 0.0])
+        right /= np.linalg.norm(right)
+        up = np.cross(right, look)
+        h, w = vto.screen_size(40.0, 25.0, 25.0)
+        cam = dict(pos=tuple(eye), look=tuple(look), up=tuple(up), right=tuple(right), h=h, w=w, distance=40.0,
+                   W=23, H=19)
+        rgba = vto.render_cube(V, cam, iso, light, mb, dim)
+        dirs = vto.ray_dirs(cam)
+        o = np.array(cam["pos"], np.float32).astype(np.float64)
+        for r in range(cam["W"] * cam["H"]):
+            d = dirs[r].astype(np.float64)
+            iv = _slab64(o, d, mb, mb + dim)
+            white = np.array_equal(rgba[r], [1.0, 1.0, 1.0, 1.0])
+            gd = float(g @ d)
+            if iv is None or abs(gd) < 1e-4:
+                if iv is None:
+                    assert white, r
+                    checked_miss += 1
+                continue
+            tn, tf = iv
+            ts = (iso - 0.1 - g @ o) / gd          # f(o + t d) = iso
+            last = tn + 0.5 * np.floor((tf - tn) / 0.5)
+            if tn + 0.51 < ts < last - 0.01:
+                X = o + ts * d
+                assert not white, r
+                nrm = -g / np.linalg.norm(g)
+                l = X - np.array(light)
+                l /= np.linalg.norm(l)
+                dif = abs(nrm @ l)
+                a = X[1] / dim
+                np.testing.assert_allclose(rgba[r], [(1 - a) * dif, a * dif, 0.0, 1.0], atol=3e-5)
+                checked_hit += 1
+            elif ts < tn - 0.01 or ts > tf + 0.01 or last + 0.01 < ts < tf - 0.01:
+                # the field is monotone along the ray, so no crossing among the samples
+                assert white, (r, ts, tn, tf, last)
+                checked_miss += 1
+                checked_tail += int(last + 0.01 < ts < tf - 0.01)
+    assert checked_hit > 100 and checked_miss > 50 and checked_tail > 0, (checked_hit, checked_miss, checked_tail)

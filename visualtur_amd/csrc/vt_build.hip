@@ -70,7 +70,7 @@ __device__ __forceinline__ float fbm(float x, float y, float z, const GenParams&
         amp *= p.gain;
         fr *= p.lacunarity;
     }
-    return sum / norm * p.scale;
+    return p.octaves > 0 ? sum / norm * p.scale : 0.0f;   // no octave: no fBm term
 }
 
 __global__ void k_generate(float* __restrict__ out, GenParams p, int64_t x0, int64_t nx)

@@ -1041,6 +1041,9 @@ static int render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_p
             int rc = service_requests(ctx, left.data(), (int64_t)left.size(), &st, &reset);
             if (rc != VT_OK) return rc;
         } else {
+            // rays parked on their walk budget only: keep the reference bits
+            // of the bricks this pass marched before the service clears them
+            if (int rc = fold_ref_bits(ctx, true)) return rc;
             HIP_TRY(launch_finish_service(c.d_table, nullptr, nullptr, 0, reset, ctx->stream));
         }
         ms_service += now_ms() - t0;
