@@ -8,16 +8,22 @@ render-pass kernel, a bounded CPU baseline of the reference algorithm, and a
 parity check of sampled rays against the CPU oracle.
 
 Workloads
-  c2  (default) 1024^3 value-noise fBm (6 octaves, lacunarity 2, gain 0.5),
-      iso 0, written to HDF5 in 32^3 chunks; 32^3 bricks with a 3-voxel halo
-      (levels 0..5, 32768 bricks); device brick cache capped at 25% (8192
-      slots), pinned host tier filled from the HDF5 file by the reader
-      thread before timing; 1920x1080, 45 deg fov, orbit at 2.2 extents
-      from the centre, 5.6 deg per frame.  Device misses are served over
-      PCIe from the pinned host tier inside the timed frames.
+  c3  (default) 2048^3 signed-distance field of 200 seeded spheres and tori
+      plus 0.02-amplitude fBm, iso 0, written to HDF5 in 32^3 chunks; 32^3
+      bricks with a 3-voxel halo (levels 0..6, 262144 bricks); device brick
+      cache 8 GB, pinned host tier 32 GB filled from the file before timing;
+      1920x1080 frames along a seeded random-walk camera path.  Legs: plain,
+      and with the learned prefetcher; each leg's sampled frames must equal
+      the plain leg's bytes.
+  c2  1024^3 value-noise fBm (6 octaves, lacunarity 2, gain 0.5), iso 0,
+      HDF5 in 32^3 chunks; device cache capped at 25% (8192 of 32768
+      bricks), pinned host tier; 1920x1080, orbit at 2.2 extents from the
+      centre, 5.6 deg per frame.
   c1  256^3 Gaussian blobs (16, seeded), iso 0.5, 32^3 bricks (levels 0..3,
       all 512 resident), 512x512, 45 deg fov, camera 2.5 extents from the
       centre looking at it.
+  c4  serving: the c1 and c2 volumes resident, Poisson request streams.
+  c5  4096x4096x2048 turbulence, 3840x2160 fly-through.
 """
 from __future__ import annotations
 
@@ -49,6 +55,7 @@ HBM_PEAK_GBS = 8000.0
 # dispatch, each scaled by what the same counter reads for a 256 MiB copy
 PROFILES = {"c1": ROOT / "profiles" / "render_passes_256cubed_blobs.md",
             "c2": ROOT / "profiles" / "render_passes_1024cubed_fbm_orbit.md",
+            "c3": ROOT / "profiles" / "render_passes_2048cubed_sdf_walk.md",
             "c5": ROOT / "profiles" / "render_passes_4096x4096x2048_turbulence_flythrough.md"}
 RENDER_KERNELS = "k_render_pass+k_render_wide"
 
@@ -67,6 +74,44 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def host_cpu() -> dict:
+    """CPU model, logical CPUs of the machine and CPUs this process may use."""
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(), "usable": len(os.sched_getaffinity(0))}
+
+
+def march_options(ctx, args) -> None:
+    ctx.set_march_options(skip_empty=True, max_defer=args.max_defer, persistent=bool(args.persistent),
+                          max_defer_late=None if args.max_defer_late < 0 else args.max_defer_late)
+
+
+def trace_bricks(ctx, args, cams, prm, level_c, d_rgba, d_hits, collect_stats=False):
+    """Distinct bricks marched per frame under the reference's pass protocol
+    (and, with collect_stats, the trilinear samples per frame): a replay with
+    no deferral (a ray parks at its first absent brick), whose visited-brick
+    trace equals the set the reference algorithm marches.  The timed runs'
+    march options are restored afterwards."""
+    ctx.set_frame_options(collect_stats=collect_stats, record_trace=True)
+    ctx.set_march_options(skip_empty=True, max_defer=0, max_defer_late=0)
+    U, samples = [], []
+    try:
+        for cam in cams:
+            st = ctx.render_frame(cam, prm, d_rgba, d_hits)
+            U.append(int(ctx.brick_trace(level_c).sum()))
+            samples.append(int(st.samples))
+    finally:
+        ctx.set_frame_options(False, False)
+        march_options(ctx, args)
+    return (U, samples) if collect_stats else U
+
+
 def brick_bytes(cube: int, inc: int) -> int:
     return (cube + 2 * inc) ** 3 * 4
 
@@ -76,19 +121,16 @@ class Workload:
 
 
 def setup_c1(ctx, args) -> Workload:
-    from oracle import vto  # the checker: builds the octree the reference way
-
     w = Workload()
     w.name = "c1"
     w.desc = "256^3 gaussian blobs x16, iso 0.5, 32^3 bricks all resident, 512x512, 45deg, d=2.5 extents"
     w.vol = gaussian_blobs(256, 16, C1_SEED)
     w.iso, w.level_o, w.level_c, w.inc = 0.5, 3, 3, 3
-    t0 = time.perf_counter()
-    w.n, w.runs = vto.build_octree(w.vol, w.iso, literal=False)
-    log(f"octree built in {time.perf_counter() - t0:.2f}s (n={w.n})")
-    flat = np.concatenate(w.runs[: w.level_o + 1]).astype(np.uint64)
-    ctx.octree_upload(flat, [r.size for r in w.runs[: w.level_o + 1]], w.n, w.level_o)
     w.d_vol = ctx.to_device(w.vol)
+    t0 = time.perf_counter()
+    w.n, sizes = ctx.octree_build(w.d_vol, w.vol.shape, w.iso, False, w.level_o)
+    w.runs = [ctx.octree_runs(l, sizes[l]) for l in range(w.level_o + 1)]
+    log(f"octree built on the GPU in {time.perf_counter() - t0:.2f}s (n={w.n})")
     w.cube = 1 << (w.n - w.level_c)
     w.slots = 1 << (3 * w.level_c)
     ctx.cache_create(w.level_c, w.cube, w.inc, w.slots, 0)
@@ -271,14 +313,16 @@ def serving_sweep(ctx, args):
 
 
 def run_c3(ctx, args):
-    """Config 3: 2048^3 signed-distance field (seeded union of 200 spheres and
-    tori, radii U[0.01, 0.08] of the extent, plus 0.02-amplitude fBm), iso 0,
-    32^3 bricks (levels 0..6, 262144 bricks), HDF5 in 32^3 chunks; device
-    cache 8 GB, pinned host tier 32 GB; a seeded random-walk camera path at
-    1920x1080.  The prefetcher is trained on the visited-brick traces of
-    other seeded paths (rendered at 480x270 from the volume in HBM before
-    the file is written), then the path is rendered plain and with a
-    prefetch before every frame, each from a cold device cache."""
+    """Config 3, the headline: 2048^3 signed-distance field (seeded union of
+    200 spheres and tori, radii U[0.01, 0.08] of the extent, plus
+    0.02-amplitude fBm), iso 0, 32^3 bricks (levels 0..6, 262144 bricks),
+    HDF5 in 32^3 chunks; device cache 8 GB, pinned host tier 32 GB; a seeded
+    random-walk camera path at 1920x1080.  The prefetcher is trained on the
+    visited-brick traces of other seeded paths (rendered at 480x270 from the
+    volume in HBM before the file is written).  The path is then rendered,
+    each leg from a cold device cache: plain (one frame at a time), prefetch
+    (a prefetch before every frame), and the rest as FORMS lists.  Frames of
+    every leg are compared with the plain leg's bytes on sampled frames."""
     from visualtur_amd import prefetch as pf
     from visualtur_amd.volumes import C3_SEED, sdf_shapes
 
@@ -322,14 +366,17 @@ def run_c3(ctx, args):
     gpu_slots, cpu_slots = int(8e9 // Sb), int(32e9 // Sb)
     W, H = 1920, 1080
     R = W * H
-    cams = pf.random_walk(W, H, D, args.warmup + args.steps, seed=C3_SEED, radius=radius)
+    nfr = args.warmup + args.steps
+    cams = pf.random_walk(W, H, D, nfr, seed=C3_SEED, radius=radius)
+    timed = list(range(args.warmup, nfr))
+    sampled = sorted(set(timed[::8] + [timed[-1]]))   # frames whose bytes every leg must reproduce
+    out_bufs = {k: (ctx.alloc(R * 16), ctx.alloc(R * 32)) for k in sampled}
     d_rgba, d_hits = ctx.alloc(R * 16), ctx.alloc(R * 32)
-    legs, last = {}, {}
-    for form in ("plain", "prefetch"):
+
+    def cache_setup():
         ctx.cache_create(level_c, 32, inc, gpu_slots, cpu_slots)
         ctx.cache_attach_hdf5(h5, "value")
-        ctx.set_march_options(skip_empty=True, max_defer=args.max_defer, persistent=bool(args.persistent),
-                          max_defer_late=None if args.max_defer_late < 0 else args.max_defer_late)
+        march_options(ctx, args)
         if args.wide_lanes is not None or args.wide_first is not None:
             ctx.set_wide_march(*wide_budgets(args))
         if args.early_reserve is not None:
@@ -338,17 +385,22 @@ def run_c3(ctx, args):
             ctx.set_first_pass_budget(args.first_nodes)
         if args.supertile is not None:
             ctx.set_first_pass_tiles(args.supertile)
-        t0 = time.perf_counter()
+        t = time.perf_counter()
         ctx.cache_prefetch_host(wait=True)
-        log(f"c3 {form}: host tier filled from hdf5 in {time.perf_counter() - t0:.2f}s, "
-            f"{ctx.cache_stats()['cpu_resident']} bricks")
+        return time.perf_counter() - t
+
+    legs, frames = {}, {}
+    for form in ("plain", "prefetch"):
+        fill_s = cache_setup()
+        log(f"c3 {form}: host tier filled from hdf5 in {fill_s:.2f}s, {ctx.cache_stats()['cpu_resident']} bricks")
         use = net if form == "prefetch" else None
-        ms, pre_ms, uploads, passes, pre_n, kms = [], [], [], [], [], []
-        for k in range(args.warmup + args.steps):
+        ms, pre_ms, uploads, passes, pre_n, kms, launches = [], [], [], [], [], [], []
+        for k in range(nfr):
+            rg, hi = out_bufs.get(k, (d_rgba, d_hits))
             f0 = time.perf_counter()
-            pre_n.append(use.prefetch(cams[k], gpu_slots, args.prefetch_min_prob) if use is not None else 0)
+            pre = use.prefetch(cams[k], gpu_slots, args.prefetch_min_prob) if use is not None else 0
             f1 = time.perf_counter()
-            st = ctx.render_frame(cams[k], prm, d_rgba, d_hits)
+            st = ctx.render_frame(cams[k], prm, rg, hi)
             f2 = time.perf_counter()
             if k >= args.warmup:
                 ms.append((f2 - f0) * 1e3)
@@ -356,64 +408,82 @@ def run_c3(ctx, args):
                 uploads.append(st.bricks_uploaded)
                 passes.append(st.passes)
                 kms.append(st.ms_kernel)
+                pre_n.append(pre)
         ctx.synchronize()
         m = np.array(ms)
         legs[form] = {"value": round(R * len(ms) / (m.sum() * 1e-3), 1), "p50_ms": round(float(np.percentile(m, 50)), 3),
                       "p99_ms": round(float(np.percentile(m, 99)), 3),
                       "demand_misses_per_frame": float(np.mean(uploads)), "passes_per_frame": float(np.mean(passes)),
-                      "prefetched_per_frame": float(np.mean(pre_n[args.warmup:])),
+                      "prefetched_per_frame": float(np.mean(pre_n)),
                       "prefetch_ms_per_frame": round(float(np.mean(pre_ms)), 3),
-                      "kernel_ms_per_frame": round(float(np.mean(kms)), 4)}
-        log(f"c3 {form}: {legs[form]}")
-        last[form] = (d_hits.download(vt.HIT_DTYPE, (R,)), d_rgba.download(np.float32, (R, 4)))
+                      "kernel_ms_per_frame": round(float(np.mean(kms)), 4),
+                      "_kernel_ms": float(np.sum(kms)), "_launches": int(np.sum(passes))}
+        log(f"c3 {form}: { {k: v for k, v in legs[form].items() if not k.startswith('_')} }")
+        frames[form] = {k: (out_bufs[k][1].download(vt.HIT_DTYPE, (R,)), out_bufs[k][0].download(np.float32, (R, 4)))
+                        for k in sampled}
         ctx.cache_destroy()
-    # visited bricks of the timed frames (trace replay, untimed) for the miss rates
-    ctx.cache_create(level_c, 32, inc, gpu_slots, cpu_slots)
-    ctx.cache_attach_hdf5(h5, "value")
-    ctx.cache_prefetch_host(wait=True)
-    ctx.set_frame_options(collect_stats=False, record_trace=True)
-    U = []
-    for k in range(args.warmup, args.warmup + args.steps):
-        ctx.render_frame(cams[k], prm, d_rgba, d_hits)
-        U.append(int(ctx.brick_trace(level_c).sum()))
-    ctx.set_frame_options(False, False)
+    # bricks the reference algorithm marches in the timed frames (untimed replay, no deferral)
+    cache_setup()
+    U = trace_bricks(ctx, args, [cams[k] for k in timed], prm, level_c, d_rgba, d_hits)
+    ctx.cache_destroy()
     for form in legs:
         legs[form]["device_miss_rate"] = round(legs[form]["demand_misses_per_frame"] / max(1.0, float(np.mean(U))), 5)
-    same = (last["plain"][0].tobytes() == last["prefetch"][0].tobytes() and
-            last["plain"][1].tobytes() == last["prefetch"][1].tobytes())
-    out = {"metric": METRIC, "value": legs["prefetch"]["value"], "unit": "rays/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": round(float(R / legs["prefetch"]["value"] * 1e3), 4),
+    same = {form: all(frames[form][k][0].tobytes() == frames["plain"][k][0].tobytes() and
+                      frames[form][k][1].tobytes() == frames["plain"][k][1].tobytes() for k in sampled)
+            for form in legs}
+    head = "prefetch"
+    octree_bytes = 8 * sum(r.size for r in runs)
+    B = sum(u * Sb + R * 48 + octree_bytes for u in U)
+    achieved = B / (legs[head]["_kernel_ms"] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic("c3"),
+                "kernel": RENDER_KERNELS, "leg": head,
+                "avg_launch_ms": round(legs[head]["_kernel_ms"] / legs[head]["_launches"], 4),
+                "bytes_per_launch": int(B / legs[head]["_launches"]),
+                "bricks_marched_per_frame": int(np.mean(U)), "brick_bytes": Sb}
+    for form in legs:
+        for key in [k for k in legs[form] if k.startswith("_")]:
+            del legs[form][key]
+    out = {"metric": METRIC, "value": legs[head]["value"], "unit": "rays/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(float(R / legs[head]["value"] * 1e3), 4),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": "2048^3 SDF (200 spheres/tori + 0.02 fBm), iso 0, HDF5 32^3 chunks, 32^3 bricks + "
                                   "3-voxel halo, device cache 8 GB, pinned host tier 32 GB, 1920x1080 random walk",
-                      "form": "prefetch (plain beside)", "bricks_with_nodes": int(cand.size),
+                      "form": f"{head} (plain beside)", "bricks_with_nodes": int(cand.size),
                       "gpu_slots": gpu_slots, "cpu_slots": cpu_slots},
-           "prefetch": legs["prefetch"], "plain": legs["plain"], "speedup": round(legs["prefetch"]["value"] /
-                                                                                   legs["plain"]["value"], 3),
-           "bricks_marched_per_frame": float(np.mean(U)), "identical_last_frame": same,
+           "p50_ms": legs[head]["p50_ms"], "p99_ms": legs[head]["p99_ms"],
+           "device_miss_rate": legs[head]["device_miss_rate"],
+           "legs": legs, "speedup": {f: round(legs[f]["value"] / legs["plain"]["value"], 3) for f in legs},
+           "identical_sampled_frames": same, "sampled_frames": len(sampled),
+           "bricks_marched_per_frame": float(np.mean(U)), "roofline": roofline,
            "train": {"paths": args.c3_train_paths, "frames": len(tcams), "seconds": round(train_s, 2),
-                     "final_loss": round(float(losses[-10:].mean()), 4)}}
+                     "final_loss": round(float(losses[-10:].mean()), 4)}, "cpu_baseline": None}
     if vol_host is not None:
         from oracle import vto
 
         w = Workload()
         w.prm = prm
         objs = (vto.Volume(vol_host), vto.Octree(runs, n, level_c), vto.params(level_c, level_c, iso))
+        cache_setup()
         out["parity"] = parity_sample(w, ctx, cams[-1], objs, args.parity_rays, seed=3)
-        # the reference algorithm on this host's cores: rays of the last frame, about cpu_seconds of work
-        threads = len(os.sched_getaffinity(0))
-        ids = np.arange(0, R, max(1, R // 65536), dtype=np.int64)
+        ctx.cache_destroy()
+        # the reference algorithm on this host's cores: whole frames of the
+        # timed path, in order, for about cpu_seconds (all bricks in memory)
+        cpu = host_cpu()
         t0 = time.perf_counter()
         done = 0
-        while time.perf_counter() - t0 < args.cpu_seconds:
-            vto.render_list(*objs[:2], cams[-1], objs[2], ids)
-            done += ids.size
+        for k in timed:
+            vto.render(*objs[:2], cams[k], objs[2], nthreads=cpu["usable"])
+            done += 1
+            if time.perf_counter() - t0 >= args.cpu_seconds:
+                break
         dt = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": done / dt, "unit": "rays/s", "cores": threads, "kind": "port",
-                               "sample": f"{done} rays ({ids.size} evenly spaced rays of the last frame, repeated), "
-                                         f"{dt:.1f}s wall, all bricks in memory"}
-    d_rgba.free()
-    d_hits.free()
+        out["cpu_baseline"] = {"value": round(done * R / dt, 1), "unit": "rays/s", "cores": cpu["usable"],
+                               "kind": "port", "host": cpu["model"], "nproc": cpu["nproc"],
+                               "sample": f"the first {done} timed frames of the path (1920x1080 each), {dt:.1f}s "
+                                         f"wall, OpenMP over 64-ray chunks, all bricks in memory"}
+    for b in [d_rgba, d_hits] + [x for p in out_bufs.values() for x in p]:
+        b.free()
     net.close()
     shutil.rmtree(tmpdir, ignore_errors=True)
     return out
@@ -589,8 +659,7 @@ def run_c5(ctx, args):
     gpu_slots = int(48e9 // Sb)
     ctx.cache_create(level_c, 32, inc, gpu_slots, 0)
     ctx.cache_attach_device_volume(d_vol, (X, Y, Z))
-    ctx.set_march_options(skip_empty=True, max_defer=args.max_defer, persistent=bool(args.persistent),
-                          max_defer_late=None if args.max_defer_late < 0 else args.max_defer_late)
+    march_options(ctx, args)
     if args.supertile is not None:
         ctx.set_first_pass_tiles(args.supertile)
     for k in range(args.warmup):
@@ -609,13 +678,8 @@ def run_c5(ctx, args):
     total_s = time.perf_counter() - t0
     last = (d_hits.download(vt.HIT_DTYPE, (R,)), d_rgba.download(np.float32, (R, 4)))
     resident = ctx.cache_stats()["gpu_resident"]
-    # visited bricks of the timed frames (trace replay, untimed)
-    ctx.set_frame_options(collect_stats=False, record_trace=True)
-    U = []
-    for k in range(args.warmup, args.warmup + args.steps):
-        ctx.render_frame(cams[k], prm, d_rgba, d_hits)
-        U.append(int(ctx.brick_trace(level_c).sum()))
-    ctx.set_frame_options(False, False)
+    # bricks the reference algorithm marches in the timed frames (trace replay, untimed)
+    U = trace_bricks(ctx, args, cams[args.warmup: args.warmup + args.steps], prm, level_c, d_rgba, d_hits)
     # the last frame again through an 8 GB cache, from cold: the same bytes
     ctx.cache_destroy()
     ctx.cache_create(level_c, 32, inc, int(8e9 // Sb), 0)
@@ -693,12 +757,14 @@ def run_c5(ctx, args):
     return out
 
 
-def cpu_baseline(w, cams, seconds: float):
-    """Reference algorithm (the C oracle) on this host's cores, over whole
-    frames of the same workload until `seconds` of wall time have elapsed."""
+def cpu_baseline(w, cams, seconds: float, threads: int | None = None):
+    """Reference algorithm (the C oracle) on this host's cores (or `threads`
+    of them), over whole frames of the same workload until `seconds` of wall
+    time have elapsed."""
     from oracle import vto
 
-    threads = len(os.sched_getaffinity(0))
+    cpu = host_cpu()
+    threads = cpu["usable"] if threads is None else threads
     V = vto.Volume(w.oracle_volume())
     n, runs = w.oracle_runs()
     O = vto.Octree(runs, n, w.level_o)
@@ -714,9 +780,10 @@ def cpu_baseline(w, cams, seconds: float):
         if time.perf_counter() - t0 >= seconds:
             break
     dt = time.perf_counter() - t0
-    return {"value": rays / dt, "unit": "rays/s", "cores": threads, "kind": "port",
+    return {"value": rays / dt, "unit": "rays/s", "cores": threads, "kind": "port", "host": cpu["model"],
+            "nproc": cpu["nproc"],
             "sample": f"{frames} full {cams[0].W}x{cams[0].H} frames of the timed camera path, "
-                      f"{dt:.1f}s wall, OpenMP over 64-ray chunks"}, (V, O, prm)
+                      f"{dt:.1f}s wall, OpenMP over 64-ray chunks, all bricks in memory"}, (V, O, prm)
 
 
 def parity_sample(w, ctx, cam, oracle_objs, nrays: int, seed: int):
@@ -767,7 +834,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--c3-train-paths", type=int, default=512)
     ap.add_argument("--c3-train-frames", type=int, default=4)
     ap.add_argument("--c3-train-steps", type=int, default=1500)
@@ -834,8 +901,7 @@ def main() -> int:
         print(json.dumps(out), flush=True)
         return 0
     w = setup_c2(ctx, args) if args.config == "c2" else setup_c1(ctx, args)
-    ctx.set_march_options(skip_empty=True, max_defer=args.max_defer, persistent=bool(args.persistent),
-                          max_defer_late=None if args.max_defer_late < 0 else args.max_defer_late)
+    march_options(ctx, args)
     if args.wide_lanes is not None or args.wide_first is not None:
         ctx.set_wide_march(*wide_budgets(args))
     if args.early_reserve is not None:
@@ -867,15 +933,11 @@ def main() -> int:
     total_s = time.perf_counter() - t0
     rays_per_s = R * args.steps / total_s
 
-    # algorithmic bytes per frame: distinct bricks marched (trace replay,
-    # untimed) x brick payload + 48 B per ray + octree levels 0..levelO
-    ctx.set_frame_options(collect_stats=True, record_trace=True)
-    U, samples = [], []
-    for k in range(args.warmup, args.warmup + args.steps):
-        st = ctx.render_frame(cams[k], w.prm, d_rgba, d_hits)
-        U.append(int(ctx.brick_trace(w.level_c).sum()))
-        samples.append(int(st.samples))
-    ctx.set_frame_options(False, False)
+    # algorithmic bytes per frame: distinct bricks the reference algorithm
+    # marches (trace replay without deferral, untimed) x brick payload + 48 B
+    # per ray + octree levels 0..levelO
+    U, samples = trace_bricks(ctx, args, cams[args.warmup: args.warmup + args.steps], w.prm, w.level_c, d_rgba,
+                              d_hits, collect_stats=True)
     octree_bytes = 8 * sum(r.size for r in w.runs[: w.level_o + 1])
     Sb = brick_bytes(w.cube, w.inc)
     B = [u * Sb + R * 48 + octree_bytes for u in U]
@@ -910,6 +972,10 @@ def main() -> int:
     if not args.no_cpu:
         log("cpu baseline")
         cb, objs = cpu_baseline(w, cams[args.warmup: args.warmup + args.steps], args.cpu_seconds)
+        if w.name == "c1":   # the per-core figure: one thread on the same frames
+            one, _ = cpu_baseline(w, cams[args.warmup: args.warmup + args.steps], min(args.cpu_seconds, 6.0),
+                                  threads=1)
+            cb["one_thread"] = {"value": one["value"], "sample": one["sample"]}
         out["cpu_baseline"] = cb
         log("parity sample")
         out["parity"] = parity_sample(w, ctx, cams[-1], objs, args.parity_rays, seed=1)

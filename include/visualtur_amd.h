@@ -366,12 +366,16 @@ int vt_prefetch(vt_context *ctx, vt_mlp *m, const vt_camera *cam, int64_t budget
 /* Render-request serving (no reference counterpart).  Scenes keep every
  * brick resident in HBM plus a min/max pyramid over the octree levels, so a
  * request may carry its own isovalue (a node is valid for iso t exactly
- * when min < t <= max over its zero-padded samples).  vt_server_run admits
- * requests as they arrive while their outputs (48 B per ray) fit the HBM
- * budget, and renders 16x16 tiles of all admitted requests in one launch
- * (a hipGraph of fixed shape, replayed per batch): continuous batching.
- * Mode 1 renders one request per launch in arrival order (the plain
- * per-request baseline). */
+ * when min < t <= max over its zero-padded samples).  A worker thread serves
+ * requests as they are submitted (continuous batching): it admits a request
+ * while its outputs (48 B per ray) and its isovalue's membership tables fit
+ * one HBM arena of hbm_budget_bytes (tables no admitted request uses are
+ * evicted, least recently used first), renders 16x16 tiles of all admitted
+ * requests in one launch (a captured hipGraph, replayed per batch), and
+ * keeps two batches in flight, so admission and the next batch's set-up
+ * overlap the device work.  Mode 1 renders one request per launch in
+ * arrival order (the plain per-request baseline).  The server has its own
+ * HIP streams; scenes are added while it is stopped. */
 typedef struct vt_server vt_server;
 typedef struct {
     int32_t scene;          /* from vt_server_add_scene */
@@ -391,7 +395,9 @@ typedef struct {
 } vt_request_result;
 typedef struct {
     int64_t requests, launches, tiles, rays;
-    double wall_s, busy_s;
+    double wall_s, busy_s;      /* busy: time with at least one batch on the device */
+    int64_t tables_built;       /* isovalue membership tables built since the server was created */
+    int64_t tables_held;        /* ... and held in the arena when the run stopped */
 } vt_serve_stats;
 int vt_server_create(vt_context *ctx, int64_t hbm_budget_bytes, int32_t max_tiles, vt_server **out);
 int vt_server_destroy(vt_server *sv);
@@ -413,6 +419,22 @@ int vt_server_add_scene(vt_server *sv, const float *d_vol, int32_t X, int32_t Y,
 int vt_server_add_scene_range(vt_server *sv, const float *d_vol, int32_t X, int32_t Y, int32_t Z, int32_t levelO,
                               int32_t levelC, int32_t cubeInc, float iso_lo, float iso_hi, int32_t *scene_out,
                               int64_t *resident_out);
+/* Online form.  vt_server_start runs the worker in `mode`; vt_server_submit
+ * queues a request from any host thread (arrival_s is ignored: it arrives
+ * now; h_rgba/h_hits must stay valid until it is done) and returns a ticket;
+ * vt_server_poll reports it (wait=1 blocks): *done = 1 with the request's
+ * status and times in seconds after its submission (the ticket is then
+ * forgotten), or *done = 0.  vt_server_stop serves what was submitted, then
+ * stops the worker and returns the run's statistics. */
+int vt_server_start(vt_server *sv, int32_t mode);
+int vt_server_submit(vt_server *sv, const vt_request *req, int64_t *ticket);
+int vt_server_poll(vt_server *sv, int64_t ticket, int32_t wait, int32_t *done, vt_request_result *result);
+int vt_server_stop(vt_server *sv, vt_serve_stats *stats);
+/* Replay form: starts the worker, submits each request at its arrival time
+ * (seconds after the call) from the calling thread, waits for all of them
+ * and stops.  Result times are seconds after the call; a request that fails
+ * (e.g. VT_E_RANGE: a sample outside its slot) has its status returned
+ * after every request has finished, and the server stays usable. */
 int vt_server_run(vt_server *sv, const vt_request *reqs, int64_t n, int32_t mode, vt_request_result *results,
                   vt_serve_stats *stats);
 

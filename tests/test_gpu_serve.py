@@ -139,3 +139,115 @@ def test_isovalue_range_scene(vt, vto, ctx):
         sv.run([{"scene": part, "cam": reqs[0]["cam"], "iso": 0.5, "arrival_s": 0.0}], mode=0)
     sv.close()
     d.free()
+
+
+def test_online_submissions_from_another_thread(vt, vto, ctx):
+    """Requests submitted from a second host thread while the worker renders
+    earlier ones are admitted into the running stream of batches (continuous
+    batching: they are admitted before the earlier requests finish) and give
+    the same bytes as when each is rendered alone."""
+    import threading
+    import time
+
+    from visualtur_amd.prefetch import random_walk
+
+    vol = blobs_volume(64, 16, seed=25)
+    sv = vt.Server(ctx, hbm_budget=512 << 20, max_tiles=256)
+    d = ctx.to_device(vol)
+    sc = sv.add_scene(d, vol.shape, 4, 2)
+    heavy = [{"scene": sc, "cam": random_walk(1024, 1024, 64, 1, seed=400 + k)[0], "iso": 0.25, "keep": True}
+             for k in range(6)]
+    light = [{"scene": sc, "cam": random_walk(*[(64, 64), (96, 80), (37, 23)][k % 3], 64, 1, seed=500 + k)[0],
+              "iso": [0.2, 0.3][k % 2], "keep": True} for k in range(9)]
+    alone, _, out_alone = sv.run(heavy + light, mode=1)
+    sv.start(0)
+    t_sub, tickets = {}, {}
+    try:
+        for k, q in enumerate(heavy):
+            t_sub[k] = time.perf_counter()
+            tickets[k] = sv.submit(q)
+
+        def other():
+            for k, q in enumerate(light):
+                t_sub[len(heavy) + k] = time.perf_counter()
+                tickets[len(heavy) + k] = sv.submit(q)
+
+        th = threading.Thread(target=other)
+        th.start()
+        th.join()
+        res, outs = {}, {}
+        for k, t in tickets.items():
+            done, r, o = sv.poll(t, wait=True)
+            assert done
+            res[k], outs[k] = r, o
+    finally:
+        st = sv.stop()
+    first_heavy_done = max(t_sub[k] + res[k]["done_s"] for k in range(len(heavy)))
+    light_admitted = [t_sub[k] + res[k]["admit_s"] for k in range(len(heavy), len(heavy) + len(light))]
+    print(f"online: {st['launches']} launches for {st['requests']} requests, light requests admitted "
+          f"{(first_heavy_done - min(light_admitted)) * 1e3:.2f} ms before the heavy ones finished", flush=True)
+    assert st["requests"] == len(tickets) and st["launches"] < len(alone)
+    assert min(light_admitted) < first_heavy_done
+    for k in range(len(heavy) + len(light)):
+        (rg, hg), (ra, ha) = outs[k], out_alone[k]
+        assert hg.tobytes() == ha.tobytes() and np.array_equal(rg.view(np.uint32), ra.view(np.uint32)), k
+    q = light[0]
+    n, runs = vto.build_octree(vol, q["iso"], literal=False)
+    rc, hc, _ = vto.render(vto.Volume(vol), vto.Octree(runs, n, 4), q["cam"], vto.params(4, 2, q["iso"]))
+    assert np.array_equal(outs[len(heavy)][1]["node"], hc["node"])
+    sv.close()
+    d.free()
+
+
+def test_failed_request_leaves_the_server_usable(vt, ctx):
+    """A scene without a halo makes the march read outside its slots: its
+    requests fail with VT_E_RANGE, their outputs are released, and a request
+    needing nearly the whole HBM budget is then served on the same server."""
+    from visualtur_amd.prefetch import random_walk
+
+    vol = blobs_volume(64, 16, seed=35)
+    budget = 2 << 20
+    sv = vt.Server(ctx, hbm_budget=budget, max_tiles=64)
+    d = ctx.to_device(vol)
+    bad = sv.add_scene(d, vol.shape, 4, 2, cube_inc=0)
+    good = sv.add_scene(d, vol.shape, 4, 2)
+    reqs = [{"scene": bad, "cam": random_walk(96, 96, 64, 1, seed=600 + k)[0], "iso": 0.25} for k in range(4)]
+    with pytest.raises(vt.VTError) as e:
+        sv.run(reqs, mode=0)
+    assert e.value.code == -7   # VT_E_RANGE
+    side = int(np.sqrt((budget - 4096) / 48)) - 1   # outputs of nearly the whole budget
+    big = {"scene": good, "cam": random_walk(side, side, 64, 1, seed=7)[0], "iso": 0.25, "keep": True}
+    res, st, out = sv.run([big], mode=0)
+    assert st["rays"] == side * side and (out[0][1]["node"] != 0).any()
+    sv.close()
+    d.free()
+
+
+def test_many_isovalues_stay_within_the_budget(vt, vto, ctx):
+    """Each new isovalue's membership tables come from the HBM arena: with
+    more distinct isovalues than the budget can hold tables for, unused tables
+    are evicted (least recently used first), every request is served, and
+    the tables held never exceed what the budget leaves."""
+    from visualtur_amd.prefetch import random_walk
+
+    vol = blobs_volume(128, 20, seed=45)
+    table_bytes = sum(8 ** (l - 1) for l in range(1, 7))   # levels 1..6, one byte per parent
+    budget = 1 << 20
+    sv = vt.Server(ctx, hbm_budget=budget, max_tiles=64)
+    d = ctx.to_device(vol)
+    sc = sv.add_scene(d, vol.shape, 6, 4)
+    isos = [0.05 + 0.01 * k for k in range(48)]
+    reqs = [{"scene": sc, "cam": random_walk(48, 40, 128, 1, seed=700 + k)[0], "iso": iso, "arrival_s": 0.0,
+             "keep": k % 12 == 0} for k, iso in enumerate(isos)]
+    res, st, out = sv.run(reqs, mode=0)
+    held_max = budget // ((table_bytes + 255) // 256 * 256)
+    print(f"tables built {st['tables_built']}, held {st['tables_held']} (at most {held_max})", flush=True)
+    assert st["requests"] == len(reqs) and st["rays"] == 48 * 40 * len(reqs)
+    assert st["tables_built"] >= len(isos) and st["tables_held"] < len(isos) and st["tables_held"] <= held_max
+    for k in out:
+        q = reqs[k]
+        n, runs = vto.build_octree(vol, q["iso"], literal=False)
+        rc, hc, _ = vto.render(vto.Volume(vol), vto.Octree(runs, n, 6), q["cam"], vto.params(6, 4, q["iso"]))
+        assert np.array_equal(out[k][1]["node"], hc["node"]) and np.array_equal(out[k][1]["voxel"], hc["voxel"]), k
+    sv.close()
+    d.free()

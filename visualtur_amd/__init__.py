@@ -80,7 +80,8 @@ class RequestResult(ctypes.Structure):
 
 class ServeStats(ctypes.Structure):
     _fields_ = [("requests", ctypes.c_int64), ("launches", ctypes.c_int64), ("tiles", ctypes.c_int64),
-                ("rays", ctypes.c_int64), ("wall_s", ctypes.c_double), ("busy_s", ctypes.c_double)]
+                ("rays", ctypes.c_int64), ("wall_s", ctypes.c_double), ("busy_s", ctypes.c_double),
+                ("tables_built", ctypes.c_int64), ("tables_held", ctypes.c_int64)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -147,6 +148,11 @@ _SIGS = {
                                                  ctypes.c_float, ctypes.POINTER(ctypes.c_int32),
                                                  ctypes.POINTER(ctypes.c_int64)]),
     "vt_server_run": (ctypes.c_int, [_P, _P, ctypes.c_int64, ctypes.c_int32, _P, ctypes.POINTER(ServeStats)]),
+    "vt_server_start": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "vt_server_stop": (ctypes.c_int, [_P, ctypes.POINTER(ServeStats)]),
+    "vt_server_submit": (ctypes.c_int, [_P, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_int64)]),
+    "vt_server_poll": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
+                                      ctypes.POINTER(RequestResult)]),
     "vt_mlp_num_params": (ctypes.c_int, []),
     "vt_prefetch": (ctypes.c_int, [_P, _P, ctypes.POINTER(Camera), ctypes.c_int64, ctypes.c_float,
                                    ctypes.POINTER(ctypes.c_int64)]),
@@ -555,6 +561,52 @@ class Server:
                self.ctx.handle)
         return sid.value, nres.value
 
+    @staticmethod
+    def _request(r: Request, q: dict):
+        """Fills `r` from the dict `q`; returns the host outputs when q["keep"]."""
+        r.scene = q["scene"]
+        r.cam = q["cam"]
+        r.iso = q["iso"]
+        r.light[:] = list(q.get("light", (0.0, 512.0, 0.0)))
+        r.color_scale = q.get("color_scale", 256.0)
+        r.arrival_s = q.get("arrival_s", 0.0)
+        if not q.get("keep"):
+            return None
+        R = q["cam"].W * q["cam"].H
+        rgba, hits = np.zeros((R, 4), np.float32), np.zeros(R, HIT_DTYPE)
+        r.h_rgba, r.h_hits = rgba.ctypes.data, hits.ctypes.data
+        return rgba, hits
+
+    def start(self, mode: int = 0) -> None:
+        """Run the serving worker (online form); mode 1 = one request per launch."""
+        self._pending = {}
+        _check(lib().vt_server_start(self.handle, mode), self.ctx.handle)
+
+    def submit(self, q: dict) -> int:
+        """Queue request `q` (as for run); returns its ticket."""
+        r = Request()
+        outs = self._request(r, q)
+        t = ctypes.c_int64()
+        _check(lib().vt_server_submit(self.handle, ctypes.byref(r), ctypes.byref(t)), self.ctx.handle)
+        self._pending[t.value] = outs   # keeps the host outputs alive until done
+        return t.value
+
+    def poll(self, ticket: int, wait: bool = True):
+        """(done, result dict, outputs or None); raises if the request failed."""
+        done, res = ctypes.c_int32(), RequestResult()
+        rc = lib().vt_server_poll(self.handle, ticket, 1 if wait else 0, ctypes.byref(done), ctypes.byref(res))
+        outs = self._pending.pop(ticket, None) if done.value else None
+        _check(rc, self.ctx.handle)
+        if not done.value:
+            return False, None, None
+        return True, {"admit_s": res.admit_s, "done_s": res.done_s, "latency_ms": res.latency_ms,
+                      "launches": res.launches}, outs
+
+    def stop(self) -> dict:
+        st = ServeStats()
+        _check(lib().vt_server_stop(self.handle, ctypes.byref(st)), self.ctx.handle)
+        return st.as_dict()
+
     def run(self, requests, mode: int = 0):
         """requests: list of dicts {scene, cam, iso, arrival_s, keep (bool)}.  Returns
         (results as a list of dicts, stats dict, outputs {index: (rgba, hits)})."""
@@ -562,18 +614,9 @@ class Server:
         arr = (Request * max(n, 1))()
         outs = {}
         for k, q in enumerate(requests):
-            r = arr[k]
-            r.scene = q["scene"]
-            r.cam = q["cam"]
-            r.iso = q["iso"]
-            r.light[:] = list(q.get("light", (0.0, 512.0, 0.0)))
-            r.color_scale = q.get("color_scale", 256.0)
-            r.arrival_s = q.get("arrival_s", 0.0)
-            if q.get("keep"):
-                R = q["cam"].W * q["cam"].H
-                rgba, hits = np.zeros((R, 4), np.float32), np.zeros(R, HIT_DTYPE)
-                outs[k] = (rgba, hits)
-                r.h_rgba, r.h_hits = rgba.ctypes.data, hits.ctypes.data
+            o = self._request(arr[k], q)
+            if o is not None:
+                outs[k] = o
         res = (RequestResult * max(n, 1))()
         st = ServeStats()
         _check(lib().vt_server_run(self.handle, arr, n, mode, res, ctypes.byref(st)), self.ctx.handle)

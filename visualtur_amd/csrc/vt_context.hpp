@@ -23,6 +23,7 @@ static_assert(sizeof(vt_hit) == sizeof(vt::HitRecord), "hit layout");
 
 namespace vt_rt {
 int fail(int code, const std::string& msg);
+extern thread_local std::string g_err;   // message of the calling thread's last failure
 
 #define HIP_TRY(expr)                                                                         \
     do {                                                                                      \

@@ -415,6 +415,16 @@ bool runs_well_formed(const uint64_t* E, int size, int level, std::string& why)
 
 // ===================================================================== ABI
 
+// Per-ray state of the render passes: resume node, deferral lists
+// ([max_defer][R], pass-major), the parked-ray lists of consecutive passes,
+// and the phase byte.
+struct RayState {
+    uint64_t* node;
+    uint64_t* defer;
+    uint32_t* list[2];
+    uint8_t* phase;
+};
+
 extern "C" {
 
 int vt_last_error(vt_context*, char* buf, size_t len)
@@ -667,6 +677,10 @@ int vt_cache_create(vt_context* ctx, int32_t levelC, int32_t cubeDim, int32_t cu
         ctx->req_list.ensure((size_t)ctx->cache->nbricks, hipHostMallocCoherent) != hipSuccess ||
         ctx->astage.ensure(ctx->cache->gpu_slots) != hipSuccess)
         return fail(VT_E_NOMEM, "request buffers");
+    // staging of the largest load (every slot), so that no frame allocates it
+    const StageLayout L((size_t)ctx->cache->gpu_slots);
+    if (ctx->cache->h_stage.ensure(L.bytes) != hipSuccess || ctx->cache->d_stage.ensure(L.bytes) != hipSuccess)
+        return fail(VT_E_NOMEM, "staging allocation failed");
     memset(ctx->req_list.p, 0, (size_t)ctx->cache->nbricks * 4);
     return VT_OK;
 }
@@ -886,7 +900,7 @@ int vt_get_brick_trace(vt_context* ctx, uint8_t* out, int64_t n)
 }
 
 static int render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_params* p, float* d_rgba,
-                        vt_hit* d_hits, vt_frame_stats* stats)
+                        vt_hit* d_hits, vt_frame_stats* stats, const RayState* state = nullptr)
 {
     double t_start = now_ms();
     if (!ctx || !p || !d_rgba) return fail(VT_E_ARG, "NULL argument");
@@ -903,11 +917,16 @@ static int render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_p
     host_tier_start(ctx);
 
     const uint32_t R = (uint32_t)cam->W * (uint32_t)cam->H;
-    if (ctx->ray_node.ensure(R) != hipSuccess || ctx->ray_phase.ensure(R) != hipSuccess ||
-        ctx->list[0].ensure(R) != hipSuccess || ctx->list[1].ensure(R) != hipSuccess ||
-        (std::max(ctx->max_defer, ctx->max_defer_late) > 0 &&
-         ctx->ray_defer.ensure((size_t)R * (size_t)std::max(ctx->max_defer, ctx->max_defer_late)) != hipSuccess))
-        return fail(VT_E_NOMEM, "ray state allocation failed");
+    RayState own;
+    if (!state) {   // the context's buffers, grown to the frame
+        if (ctx->ray_node.ensure(R) != hipSuccess || ctx->ray_phase.ensure(R) != hipSuccess ||
+            ctx->list[0].ensure(R) != hipSuccess || ctx->list[1].ensure(R) != hipSuccess ||
+            (std::max(ctx->max_defer, ctx->max_defer_late) > 0 &&
+             ctx->ray_defer.ensure((size_t)R * (size_t)std::max(ctx->max_defer, ctx->max_defer_late)) != hipSuccess))
+            return fail(VT_E_NOMEM, "ray state allocation failed");
+        own = RayState{ctx->ray_node.p, ctx->ray_defer.p, {ctx->list[0].p, ctx->list[1].p}, ctx->ray_phase.p};
+        state = &own;
+    }
     HitRecord* hits = reinterpret_cast<HitRecord*>(d_hits);
     if (!hits) {
         if (ctx->hits_scratch.ensure(R) != hipSuccess) return fail(VT_E_NOMEM, "hit buffer");
@@ -940,9 +959,9 @@ static int render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_p
     a.iso = p->iso;
     a.lx = p->light[0]; a.ly = p->light[1]; a.lz = p->light[2];
     a.color_scale = p->color_scale;
-    a.ray_node = ctx->ray_node.p;
-    a.ray_phase = ctx->ray_phase.p;
-    a.ray_defer = std::max(ctx->max_defer, ctx->max_defer_late) > 0 ? ctx->ray_defer.p : nullptr;
+    a.ray_node = state->node;
+    a.ray_phase = state->phase;
+    a.ray_defer = std::max(ctx->max_defer, ctx->max_defer_late) > 0 ? state->defer : nullptr;
     a.max_defer = ctx->max_defer;
     a.node_budget = ctx->first_nodes;
     a.supertile = ctx->supertile;
@@ -965,7 +984,7 @@ static int render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_p
     int cur = 0;
     a.in_list = nullptr;
     a.in_count = 0;
-    a.out_list = ctx->list[cur].p;
+    a.out_list = state->list[cur];
     // bricks still arriving from a prefetch land before the first pass
     if (int rc = copy_stream_idle(ctx)) return rc;
     // free slots for the loads issued while the first pass runs: at most as
@@ -1047,12 +1066,12 @@ static int render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_p
             HIP_TRY(launch_finish_service(c.d_table, nullptr, nullptr, 0, reset, ctx->stream));
         }
         ms_service += now_ms() - t0;
-        a.in_list = ctx->list[cur].p;
+        a.in_list = state->list[cur];
         a.in_count = parked;
         a.max_defer = ctx->max_defer_late;   // few rays remain: let them walk further per pass
         a.node_budget = 0;                   // and to the end of their walk
         cur ^= 1;
-        a.out_list = ctx->list[cur].p;
+        a.out_list = state->list[cur];
         if (pass > 1000000) return fail(VT_E_STATE, "frame did not converge");
     }
     c.epoch++;
