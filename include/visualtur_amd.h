@@ -361,6 +361,46 @@ int vt_mlp_score(vt_mlp *m, const vt_camera *cam, int32_t nLevels, int32_t level
  * *loaded (may be NULL) receives the number of bricks loaded. */
 int vt_prefetch(vt_context *ctx, vt_mlp *m, const vt_camera *cam, int64_t budget, float min_prob, int64_t *loaded);
 
+/* ------------------------------------------- frames in flight (serving) */
+
+/* Frame requests served through the context's octree and out-of-core brick
+ * cache with several frames in flight (no reference counterpart: the
+ * reference renders one frame at a time, visualTur.cu:167-228).  A worker
+ * thread takes submitted frames into up to max_frames slots; each launch
+ * carries the parked rays of the frames already started (their later
+ * passes) and the first pass of the frames admitted since the last one, and
+ * the service between launches loads bricks for all of them.  With a
+ * prefetcher, the next waiting frame's likely bricks are scored and copied
+ * into free slots while the launch runs.  Each frame's pixels and hit
+ * records equal vt_render_frame's for the same camera.  While a vt_frames
+ * exists the context belongs to it: call no other vt_* function on that
+ * context (the MLP given to vt_frames_set_prefetch included).  The march
+ * options of the context at creation apply. */
+typedef struct vt_frames vt_frames;
+typedef struct {
+    int64_t frames, launches;
+    int64_t bricks_requested, bricks_uploaded, bricks_early, bricks_prefetched, host_tier_misses;
+    double ms_kernel;           /* device time of the launches */
+    double wall_s, busy_s;      /* since creation or the last vt_frames_get_stats; busy: frames in flight */
+} vt_frames_stats;
+/* max_rays: the largest W*H a frame may have. */
+int vt_frames_create(vt_context *ctx, int32_t max_frames, int64_t max_rays, vt_frames **out);
+/* m = NULL turns the prefetcher off; budget and min_prob as in vt_prefetch. */
+int vt_frames_set_prefetch(vt_frames *f, vt_mlp *m, int64_t budget, float min_prob);
+/* Queue one frame (outputs: caller-owned device memory, d_hits required,
+ * untouched by the caller until the frame is done); every frame uses the
+ * same levelO and levelC.  Returns a ticket. */
+int vt_frames_submit(vt_frames *f, const vt_camera *cam, const vt_render_params *p, float *d_rgba,
+                     vt_hit *d_hits, int64_t *ticket);
+/* *done = 1 with the frame's status and statistics (passes: launches it took
+ * part in; ms_total: submission to completion) and the ticket forgotten, or
+ * *done = 0; wait = 1 blocks. */
+int vt_frames_poll(vt_frames *f, int64_t ticket, int32_t wait, int32_t *done, vt_frame_stats *stats);
+/* Totals since creation or the last call (which resets them). */
+int vt_frames_get_stats(vt_frames *f, vt_frames_stats *out);
+/* Serves what was submitted, then stops the worker. */
+int vt_frames_destroy(vt_frames *f);
+
 /* --------------------------------------------------------- serving loop */
 
 /* Render-request serving (no reference counterpart).  Scenes keep every

@@ -180,9 +180,11 @@ def test_sdf_walk_plain_and_prefetch_equal_oracle(vt, vto, ctx, tmp_path):
     """The 2048^3 SDF workload at 256^3 (200 spheres and tori, radii
     U[0.01, 0.08] of the extent, plus 0.02-amplitude fBm; iso 0; HDF5 in 32^3
     chunks; a device cache below the bricks with nodes, filled from a pinned
-    host tier): a random-walk path rendered plain and with the learned
-    prefetcher (trained on the traces of other paths) gives byte-identical
-    frames, and every frame equals the oracle."""
+    host tier): a random-walk path rendered plain, with the learned
+    prefetcher (trained on the traces of other paths) before each frame, and
+    as frames in flight with the prefetcher loading the next frame's bricks
+    (the prefetch+serving form) gives byte-identical frames, and every frame
+    equals the oracle."""
     from visualtur_amd import prefetch as pf
     from visualtur_amd.volumes import sdf_shapes
 
@@ -222,6 +224,24 @@ def test_sdf_walk_plain_and_prefetch_equal_oracle(vt, vto, ctx, tmp_path):
     V, O, prm_o = vto.Volume(vol), vto.Octree(runs_c, n_c, level), vto.params(level, level, iso)
     frames = {}
     try:
+        # frames in flight (two at a time) with the prefetcher: the serving form
+        ctx.cache_create(level, 32, 3, slots, int(cand.size))
+        ctx.cache_attach_hdf5(str(h5), "value")
+        ctx.cache_prefetch_host(wait=True)
+        bufs = [(ctx.alloc(c.W * c.H * 16), ctx.alloc(c.W * c.H * 32)) for c in cams]
+        with vt.Frames(ctx, max_frames=2, max_rays=cams[0].W * cams[0].H) as fr:
+            fr.set_prefetch(net, slots, 0.8)
+            tickets = [fr.submit(c, prm, *bufs[k]) for k, c in enumerate(cams)]
+            for t in tickets:
+                assert fr.poll(t)[0]
+            agg = fr.stats()
+        log(f"sdf frames in flight: {agg}")
+        R = cams[0].W * cams[0].H
+        frames["serving"] = [(h.download(vt.HIT_DTYPE, (R,)), r.download(np.float32, (R, 4))) for r, h in bufs]
+        for r, h in bufs:
+            r.free()
+            h.free()
+        ctx.cache_destroy()
         for form in ("plain", "prefetch"):
             ctx.cache_create(level, 32, 3, slots, int(cand.size))
             ctx.cache_attach_hdf5(str(h5), "value")
@@ -241,6 +261,7 @@ def test_sdf_walk_plain_and_prefetch_equal_oracle(vt, vto, ctx, tmp_path):
     for k, cam in enumerate(cams):
         (hp, rp), (hq, rq) = frames["plain"][k], frames["prefetch"][k]
         assert _same_bytes(hp, rp, hq, rq), k
+        assert _same_bytes(hp, rp, *frames["serving"][k]), k
         rc, hc, _ = vto.render(V, O, cam, prm_o)
         assert_same(hp, rp, hc, rc)
         assert_bits(hp, rp, hc, rc)

@@ -87,6 +87,16 @@ class ServeStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class FramesStats(ctypes.Structure):
+    _fields_ = [("frames", ctypes.c_int64), ("launches", ctypes.c_int64), ("bricks_requested", ctypes.c_int64),
+                ("bricks_uploaded", ctypes.c_int64), ("bricks_early", ctypes.c_int64),
+                ("bricks_prefetched", ctypes.c_int64), ("host_tier_misses", ctypes.c_int64),
+                ("ms_kernel", ctypes.c_double), ("wall_s", ctypes.c_double), ("busy_s", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 HIT_DTYPE = np.dtype([("node", "<u8"), ("brick", "<u8"), ("voxel", "<i4", (3,)), ("depth", "<f4")])
 assert HIT_DTYPE.itemsize == 32
 
@@ -138,6 +148,14 @@ _SIGS = {
                                             ctypes.c_char_p, ctypes.c_int32]),
     "vt_image_write_ppm": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_char_p]),
     "vt_copy_probe": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
+    "vt_frames_create": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(_P)]),
+    "vt_frames_set_prefetch": (ctypes.c_int, [_P, _P, ctypes.c_int64, ctypes.c_float]),
+    "vt_frames_submit": (ctypes.c_int, [_P, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams), _P, _P,
+                                        ctypes.POINTER(ctypes.c_int64)]),
+    "vt_frames_poll": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
+                                      ctypes.POINTER(FrameStats)]),
+    "vt_frames_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(FramesStats)]),
+    "vt_frames_destroy": (ctypes.c_int, [_P]),
     "vt_server_create": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(_P)]),
     "vt_server_destroy": (ctypes.c_int, [_P]),
     "vt_server_set_lanes": (ctypes.c_int, [_P, ctypes.c_int64]),
@@ -531,6 +549,50 @@ class MLP:
               d_logits: DeviceBuffer) -> None:
         """Logits of n bricks for the camera in one fused launch (vt_mlp_score)."""
         _check(lib().vt_mlp_score(self.handle, ctypes.byref(cam), n_levels, level_c, d_codes.ptr, n, d_logits.ptr))
+
+
+class Frames:
+    """Frames in flight through the context's out-of-core brick cache
+    (vt_frames_*): submit frames, poll them, optionally with a prefetcher.
+    While it exists, the context belongs to it."""
+
+    def __init__(self, ctx: Context, max_frames: int = 2, max_rays: int = 1920 * 1080):
+        h = ctypes.c_void_p()
+        _check(lib().vt_frames_create(ctx.handle, max_frames, max_rays, ctypes.byref(h)), ctx.handle)
+        self.ctx, self.handle = ctx, h.value
+
+    def set_prefetch(self, mlp: "MLP | None", budget: int = 0, min_prob: float = 0.8) -> None:
+        _check(lib().vt_frames_set_prefetch(self.handle, mlp.handle if mlp is not None else None, budget, min_prob),
+               self.ctx.handle)
+
+    def submit(self, cam: Camera, params: RenderParams, d_rgba: DeviceBuffer, d_hits: DeviceBuffer) -> int:
+        t = ctypes.c_int64()
+        _check(lib().vt_frames_submit(self.handle, ctypes.byref(cam), ctypes.byref(params), d_rgba.ptr, d_hits.ptr,
+                                      ctypes.byref(t)), self.ctx.handle)
+        return t.value
+
+    def poll(self, ticket: int, wait: bool = True):
+        """(done, FrameStats or None); raises if the frame failed."""
+        done, st = ctypes.c_int32(), FrameStats()
+        _check(lib().vt_frames_poll(self.handle, ticket, 1 if wait else 0, ctypes.byref(done), ctypes.byref(st)),
+               self.ctx.handle)
+        return (True, st) if done.value else (False, None)
+
+    def stats(self) -> dict:
+        st = FramesStats()
+        _check(lib().vt_frames_get_stats(self.handle, ctypes.byref(st)), self.ctx.handle)
+        return st.as_dict()
+
+    def close(self) -> None:
+        if self.handle:
+            lib().vt_frames_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class Server:

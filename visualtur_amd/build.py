@@ -29,7 +29,7 @@ HIP = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-pack
 INCS = [f"-I{INCLUDE}", f"-I{CSRC}", f"-I{HDF5_INC}"]
 
 HIP_SOURCES = ["vt_kernels.hip", "vt_build.hip", "vt_mlp.hip", "vt_runtime.cpp", "vt_prefetch.cpp",
-               "vt_serve.cpp"]
+               "vt_serve.cpp", "vt_frames.cpp"]
 HOST_SOURCES = ["vt_hdf5.cpp"]
 CLASS_SOURCES = ["vt_classes.cpp"]
 

@@ -419,6 +419,32 @@ struct BrickCache {
 using vt_rt::fail;
 using namespace vt_rt;
 
+// Runtime steps shared by the single-frame loop (vt_runtime.cpp), the
+// multi-frame loop (vt_frames.cpp) and the prefetcher (vt_prefetch.cpp).
+namespace vt_rt {
+int set_device(vt_context* ctx);
+OctreeView make_view(const vt_context* ctx, int final_level);
+Camera to_cam(const vt_camera* c);
+int check_camera(const vt_camera* cam);
+}  // namespace vt_rt
+// Device reference bits into the host clock (copied: already read back).
+int fold_ref_bits(vt_context* ctx, bool copied = false);
+// Loads requested bricks (indices relative to 8^levelC) between passes; with
+// `reset`, only enqueued, its last kernel resetting the per-pass state.
+int service_requests(vt_context* ctx, const uint32_t* reqs, int64_t count, vt_frame_stats* st,
+                     const PassReset* reset = nullptr);
+int copy_stream_idle(vt_context* ctx);
+int publish_loads(vt_context* ctx, hipStream_t s);
+int make_reserve(vt_context* ctx, int64_t reserve);
+int load_during_pass(vt_context* ctx, hipEvent_t done, int64_t* loaded, int64_t* misses);
+void host_tier_start(vt_context* ctx);
+struct vt_mlp;
+// vt_prefetch's scoring, selection and loads; without `publish` the copies'
+// sub-block min/max and page-table entries wait for publish_loads, so a
+// render pass may run while they land.
+int prefetch_bricks(vt_context* ctx, vt_mlp* m, const vt_camera* cam, int64_t budget, float min_prob, bool publish,
+                    int64_t* loaded);
+
 struct vt_file {
     H5Volume vol;
 };

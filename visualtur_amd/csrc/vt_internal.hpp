@@ -51,6 +51,17 @@ struct FrameCounters {
     uint32_t pad[8];
 };
 
+// One frame of a multi-frame pass (vt_frames): its camera, shading and
+// outputs.  Rays of several frames share the launches, the per-ray state
+// arrays (ray id = frame slot << rshift | pixel) and the brick cache.
+struct RenderJob {
+    Camera cam;
+    Shading sh;
+    float4* rgba;
+    HitRecord* hits;
+};
+constexpr int kMaxFrames = 8;
+
 struct RenderArgs {
     Camera cam;
     OctreeView oct;
@@ -86,6 +97,17 @@ struct RenderArgs {
     int tiles_x;                 // 16x16 pixel tiles per row
     int tiles_total;
     int collect_stats;
+    // multi-frame passes (jobs != null): in_list holds the parked rays of
+    // frames already in flight (their later passes); the first pass of the
+    // nfresh frames fresh_slot[] follows in the same launch, tile by tile
+    const RenderJob* jobs;
+    int rshift;                  // ray id = frame slot << rshift | pixel
+    uint32_t defer_stride;       // rays per deferral list (frame slots << rshift)
+    int max_defer_late;          // deferral of the resumed rays (max_defer: of the first passes)
+    uint32_t* job_parked;        // per frame slot: rays parked in this launch
+    int nfresh;
+    uint32_t fresh_slot[kMaxFrames];
+    uint32_t fresh_tile0[kMaxFrames + 1];   // first 8x8 tile of each fresh frame in the launch (prefix sums)
 };
 
 struct TraceArgs {
@@ -173,6 +195,9 @@ int render_persistent_workgroups(int device);   // resident workgroups of the pe
 hipError_t launch_render_pass(const RenderArgs& a, hipStream_t s);
 // a pass with 2^lg lanes per ray, lg in 0..6 (first pass: in_list null)
 hipError_t launch_render_wide(const RenderArgs& a, int lg, hipStream_t s);
+// multi-frame pass: the in_list rays (one lane each), then the tiles of the
+// fresh frames; with no fresh frame and lg > 0, the in_list rays with 2^lg lanes each
+hipError_t launch_render_multi(const RenderArgs& a, int lg, hipStream_t s);
 hipError_t launch_trace_nodes(const TraceArgs& a, hipStream_t s);
 // Assemble bricks with halo from a device volume [X][Y][Z] (zero outside).
 hipError_t launch_bricks_from_volume(const float* vol, int X, int Y, int Z,

@@ -286,21 +286,33 @@ __device__ __forceinline__ bool visit_node_g(const Ray& r, const OctreeView& oct
     return true;
 }
 
-// One ray of a render pass (ray `id`, when `active`; group-uniform).
-template <bool kWide>
-__device__ __forceinline__ void pass_ray(const RenderArgs& a, uint32_t id, bool active, const Lanes& g)
+// One ray of a render pass (ray `id`, when `active`; group-uniform).  With
+// kMulti the ray belongs to frame slot id >> a.rshift (its camera, shading
+// and outputs in a.jobs), `first` says whether this is its frame's first
+// pass, and the per-ray state is indexed by id with deferral lists of
+// a.defer_stride rays; otherwise the frame is a.cam and the first pass is
+// the one without an input list.
+template <bool kWide, bool kMulti>
+__device__ __forceinline__ void pass_ray(const RenderArgs& a, uint32_t id, bool active, bool first_region,
+                                         const Lanes& g)
 {
-    const int W = a.cam.W, H = a.cam.H;
-    const uint32_t R = (uint32_t)W * (uint32_t)H;
+    const uint32_t fslot = kMulti ? (id >> a.rshift) : 0u;
+    const uint32_t pix = kMulti ? (id & ((1u << a.rshift) - 1u)) : id;
+    const Camera& cam = kMulti ? a.jobs[fslot].cam : a.cam;
+    const int W = cam.W, H = cam.H;
+    const uint32_t R = kMulti ? a.defer_stride : (uint32_t)W * (uint32_t)H;
+    const bool first = kMulti ? first_region : a.in_list == nullptr;
+    const int max_defer = kMulti && !first ? a.max_defer_late : a.max_defer;
+    const int node_budget = kMulti && !first ? 0 : a.node_budget;
     uint32_t err = 0, nsamples = 0, nnodes = 0;
     bool park = false;
 
     if (active) {
-        int i = (int)(id / (uint32_t)W), j = (int)(id % (uint32_t)W);
+        int i = (int)(pix / (uint32_t)W), j = (int)(pix % (uint32_t)W);
         Ray r;
-        make_ray(a.cam, i, j, r);
+        make_ray(cam, i, j, r);
         BrickView bv{a.brick_slot, a.pool, a.S, a.inc, a.slot_elems, a.level_c, a.shift, a.brick_base, a.submm, a.G};
-        Shading sh{a.iso, a.lx, a.ly, a.lz, a.color_scale};
+        const Shading sh = kMulti ? a.jobs[fslot].sh : Shading{a.iso, a.lx, a.ly, a.lz, a.color_scale};
         float4 c;
         HitRecord h;
         uint64_t bidx = 0, node = 0;
@@ -325,7 +337,7 @@ __device__ __forceinline__ void pass_ray(const RenderArgs& a, uint32_t id, bool 
         int nprev = 0, k = 0;
         uint8_t tent = 0;
         bool adv, descend;
-        if (a.in_list == nullptr) {
+        if (first) {
             adv = descend = root_ok(r, a.oct);
             node = adv ? 1ull : 0ull;
         } else {
@@ -358,7 +370,7 @@ __device__ __forceinline__ void pass_ray(const RenderArgs& a, uint32_t id, bool 
                 }
                 // a ray that has walked its budget parks on its next node,
                 // requesting nothing, and goes on in a later pass
-                if (a.node_budget > 0 && walked >= a.node_budget) {
+                if (node_budget > 0 && walked >= node_budget) {
                     st = kBudget;
                     break;
                 }
@@ -381,7 +393,7 @@ __device__ __forceinline__ void pass_ray(const RenderArgs& a, uint32_t id, bool 
                     st = kBlocked;
                     break;
                 }
-                if (nd >= a.max_defer) {
+                if (nd >= max_defer) {
                     st = WALK_ABSENT;
                     break;
                 }
@@ -408,8 +420,8 @@ __device__ __forceinline__ void pass_ray(const RenderArgs& a, uint32_t id, bool 
                 a.ray_phase[id] = (uint8_t)nd;
                 park = true;
             } else if (st == WALK_HIT || st == WALK_MISS) {
-                a.rgba[id] = c;
-                a.hits[id] = h;
+                (kMulti ? a.jobs[fslot].rgba : a.rgba)[pix] = c;
+                (kMulti ? a.jobs[fslot].hits : a.hits)[pix] = h;
                 if (nd > 0) {
                     a.ray_phase[id] = (uint8_t)(PHASE_TENT | nd);
                     park = true;
@@ -422,6 +434,17 @@ __device__ __forceinline__ void pass_ray(const RenderArgs& a, uint32_t id, bool 
     park = park && g.leader;
     uint32_t pslot = wave_atomic_add(&a.ctr->parked_count, park);
     if (park) a.out_list[pslot] = id;
+    if (kMulti) {
+        // parked rays per frame: one atomic per frame present in the wave
+        unsigned long long left = __ballot(park);
+        while (left) {
+            const int l = __ffsll((long long)left) - 1;
+            const uint32_t s = (uint32_t)__shfl((int)fslot, l);
+            const unsigned long long m = __ballot(park && fslot == s);
+            if ((int)__lane_id() == l) atomicAdd(&a.job_parked[s], (uint32_t)__popcll(m));
+            left &= ~m;
+        }
+    }
     if (err) atomicOr(&a.ctr->err, err);
     if (a.collect_stats) {
         unsigned long long s = g.leader ? nsamples : 0u, nn = g.leader ? nnodes : 0u;
@@ -481,7 +504,7 @@ __global__ __launch_bounds__(kPassBlock, 5) void k_render_pass(RenderArgs a)
         id = active ? (uint32_t)(i * W + j) : 0;
     }
     const Lanes g{1, 0, (int)(threadIdx.x & 63), true};
-    pass_ray<false>(a, id, active, g);
+    pass_ray<false, false>(a, id, active, false, g);
 }
 
 // A pass with 2^lg lanes per ray over groups of 64 rays: the first pass's
@@ -515,7 +538,59 @@ __global__ __launch_bounds__(kPassBlock) void k_render_wide(RenderArgs a, int lg
     }
     const int L = 1 << lg;
     const Lanes g{L, lane & (L - 1), lane & ~(L - 1), (lane & (L - 1)) == 0};
-    pass_ray<true>(a, id, active, g);
+    pass_ray<true, false>(a, id, active, false, g);
+}
+
+// Multi-frame pass, one lane per ray: workgroups [0, list_blocks) take 64
+// consecutive in_list entries (the parked rays of frames in flight, started
+// first since they are the long walks), the rest take the 8x8 tiles of the
+// fresh frames in order (fresh_tile0 prefix sums), dealt round-robin to the
+// XCDs as in k_render_pass.
+__global__ __launch_bounds__(kPassBlock, 5) void k_render_multi(RenderArgs a)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t list_blocks = (a.in_count + 63) / 64;
+    uint32_t id = 0;
+    bool active, first;
+    if (blockIdx.x < list_blocks) {
+        const uint32_t idx = blockIdx.x * 64 + lane;
+        active = idx < a.in_count;
+        id = active ? a.in_list[idx] : 0u;
+        first = false;
+    } else {
+        const uint32_t t = blockIdx.x - list_blocks;
+        int k = 0;
+        while (k + 1 < a.nfresh && t >= a.fresh_tile0[k + 1]) ++k;
+        const uint32_t fs = a.fresh_slot[k];
+        const Camera& cam = a.jobs[fs].cam;
+        const int tx8 = (cam.W + 7) >> 3;
+        const uint32_t tt = t - a.fresh_tile0[k];
+        active = t < a.fresh_tile0[a.nfresh];
+        const int i = (int)(tt / (uint32_t)tx8) * 8 + (int)(lane >> 3);
+        const int j = (int)(tt % (uint32_t)tx8) * 8 + (int)(lane & 7);
+        active = active && i < cam.H && j < cam.W;
+        id = active ? (fs << a.rshift) | (uint32_t)(i * cam.W + j) : 0u;
+        first = true;
+    }
+    const Lanes g{1, 0, (int)lane, true};
+    pass_ray<false, true>(a, id, active, first, g);
+}
+
+// Multi-frame pass over in_list only, 2^lg lanes per ray (the later passes
+// of frames in flight when no frame starts), grouped as in k_render_wide.
+__global__ __launch_bounds__(kPassBlock) void k_render_multi_wide(RenderArgs a, int lg)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    const uint32_t b = blockIdx.x;
+    const uint32_t grp = ((b >> 3) >> lg) * 8u + (b & 7u);
+    const uint32_t w = (b >> 3) & ((1u << lg) - 1u);
+    const uint32_t p = (w << (6 - lg)) + ((uint32_t)lane >> lg);
+    const uint32_t idx = grp * 64u + p;
+    const bool active = idx < a.in_count;
+    const uint32_t id = active ? a.in_list[idx] : 0u;
+    const int L = 1 << lg;
+    const Lanes g{L, lane & (L - 1), lane & ~(L - 1), (lane & (L - 1)) == 0};
+    pass_ray<true, true>(a, id, active, false, g);
 }
 
 // ------------------------------------------------- persistent render pass
@@ -785,6 +860,23 @@ hipError_t launch_render_wide(const RenderArgs& a, int lg, hipStream_t s)
     const uint64_t groups = (rays + 63) / 64;
     const unsigned grid = (unsigned)(((groups + 7) / 8 * 8) << lg);
     hipLaunchKernelGGL(k_render_wide, dim3(grid), dim3(kPassBlock), 0, s, a, lg);
+    return hipGetLastError();
+}
+
+hipError_t launch_render_multi(const RenderArgs& a, int lg, hipStream_t s)
+{
+    if (!a.jobs || a.nfresh < 0 || a.nfresh > kMaxFrames || lg < 0 || lg > 6) return hipErrorInvalidValue;
+    const uint32_t tiles = a.nfresh > 0 ? a.fresh_tile0[a.nfresh] : 0u;
+    if (tiles == 0 && lg > 0) {
+        if (a.in_count == 0) return hipSuccess;
+        const uint64_t groups = ((uint64_t)a.in_count + 63) / 64;
+        const unsigned grid = (unsigned)(((groups + 7) / 8 * 8) << lg);
+        hipLaunchKernelGGL(k_render_multi_wide, dim3(grid), dim3(kPassBlock), 0, s, a, lg);
+        return hipGetLastError();
+    }
+    const uint64_t grid = ((uint64_t)a.in_count + 63) / 64 + tiles;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_render_multi, dim3((unsigned)grid), dim3(kPassBlock), 0, s, a);
     return hipGetLastError();
 }
 

@@ -316,11 +316,15 @@ int vt_mlp_score(vt_mlp* m, const vt_camera* cam, int32_t nLevels, int32_t level
 
 int load_async(vt_context* ctx, const uint32_t* req, size_t n, std::vector<uint32_t>& left, int64_t* loaded,
                int64_t* misses, bool publish);
-int copy_stream_idle(vt_context* ctx);
-void host_tier_start(vt_context* ctx);
 
 extern "C" int vt_prefetch(vt_context* ctx, vt_mlp* m, const vt_camera* cam, int64_t budget, float min_prob,
                            int64_t* loaded)
+{
+    return prefetch_bricks(ctx, m, cam, budget, min_prob, true, loaded);
+}
+
+int prefetch_bricks(vt_context* ctx, vt_mlp* m, const vt_camera* cam, int64_t budget, float min_prob, bool publish,
+                    int64_t* loaded)
 {
     if (!ctx || !m || !cam) return fail(VT_E_ARG, "NULL argument");
     if (!ctx->octree_ready) return fail(VT_E_STATE, "no octree uploaded");
@@ -472,7 +476,7 @@ extern "C" int vt_prefetch(vt_context* ctx, vt_mlp* m, const vt_camera* cam, int
         for (int64_t q = 0; q < n; ++q)
             load[(size_t)q] = (uint32_t)(ctx->cand_codes[(size_t)best[(size_t)q].second] - base);
         int64_t misses = 0;
-        if (int rc = load_async(ctx, load.data(), load.size(), left, &nloaded, &misses, true)) return rc;
+        if (int rc = load_async(ctx, load.data(), load.size(), left, &nloaded, &misses, publish)) return rc;
     }
     if (loaded) *loaded = nloaded;
     return VT_OK;

@@ -33,7 +33,7 @@ int fail(int code, const std::string& msg)
 }  // namespace vt_rt
 using vt_rt::g_err;
 
-namespace {
+namespace vt_rt {
 
 int set_device(vt_context* ctx)
 {
@@ -67,13 +67,13 @@ int check_camera(const vt_camera* cam)
     return VT_OK;
 }
 
-}  // namespace
+}  // namespace vt_rt
 
 // Fold the device reference bits (set by the render kernel on every brick
 // it marches) into the host clock.  With `copied`, the bytes were already
 // read back into h_ref after the last pass, and the service clears them on
 // the device; otherwise they are read and cleared here.
-int fold_ref_bits(vt_context* ctx, bool copied = false)
+int fold_ref_bits(vt_context* ctx, bool copied)
 {
     BrickCache& c = *ctx->cache;
     if (c.used == 0) return VT_OK;
@@ -88,8 +88,6 @@ int fold_ref_bits(vt_context* ctx, bool copied = false)
 
 int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t* misses, int32_t* slots_out,
                 const int64_t* victims = nullptr, const PassReset* reset = nullptr);
-int copy_stream_idle(vt_context* ctx);
-int publish_loads(vt_context* ctx, hipStream_t s);
 
 // Starts the host tier once the brick geometry is known; with the octree
 // uploaded down to levelC, a tier that holds every brick with octree nodes
@@ -109,7 +107,7 @@ void host_tier_start(vt_context* ctx)
 // the work is only enqueued (the next pass follows on the same stream), and
 // the last kernel of the service also resets the per-pass state.
 int service_requests(vt_context* ctx, const uint32_t* reqs, int64_t count, vt_frame_stats* st,
-                     const PassReset* reset = nullptr)
+                     const PassReset* reset)
 {
     BrickCache& c = *ctx->cache;
     if (int rc = fold_ref_bits(ctx, reset != nullptr)) return rc;
