@@ -332,6 +332,9 @@ void vto_ray_dir(const vto_camera *c, int i, int j, float out[3])
  * outside the dataset before the hyperslab read). */
 float vto_voxel(const vto_volume *v, int x, int y, int z)
 {
+    x -= v->origin[0];
+    y -= v->origin[1];
+    z -= v->origin[2];
     if (x < 0 || y < 0 || z < 0 || x >= v->dims[0] || y >= v->dims[1] || z >= v->dims[2])
         return 0.0f;
     return v->data[((size_t)x * (size_t)v->dims[1] + (size_t)y) * (size_t)v->dims[2] + (size_t)z];
@@ -420,19 +423,21 @@ typedef struct {
     int top;
 } trav_stack;
 
-static int trav_next(const trav_ctx *t, trav_stack *st, int finalLevel, uint64_t *out)
+/* The DFS over a stack of `cap` entries (sidx, slvl, *stop). */
+static int trav_next_s(const trav_ctx *t, uint64_t *sidx, int *slvl, int *stop, int cap, int finalLevel,
+                       uint64_t *out)
 {
     const vto_octree *oct = t->oct;
     int n = oct->nLevels;
-    while (st->top >= 0) {
-        if (st->lvl[st->top] == finalLevel) {
-            *out = st->idx[st->top];
-            st->top--;
+    while (*stop >= 0) {
+        if (slvl[*stop] == finalLevel) {
+            *out = sidx[*stop];
+            (*stop)--;
             return 1;
         }
-        uint64_t current = st->idx[st->top];
-        int clevel = st->lvl[st->top];
-        st->top--;
+        uint64_t current = sidx[*stop];
+        int clevel = slvl[*stop];
+        (*stop)--;
         int32_t minBox[3];
         vto_decode(current, clevel, n, minBox);
         int level = clevel + 1;
@@ -474,13 +479,18 @@ static int trav_next(const trav_ctx *t, trav_stack *st, int finalLevel, uint64_t
         }
         /* Octree.cu:707-712: push in reverse so the nearest is on top. */
         for (int i = nValid - 1; i >= 0; --i) {
-            if (st->top + 1 >= VTO_STACK) { fprintf(stderr, "vt_oracle: stack overflow\n"); abort(); }
-            st->top++;
-            st->idx[st->top] = children[i];
-            st->lvl[st->top] = level;
+            if (*stop + 1 >= cap) { fprintf(stderr, "vt_oracle: stack overflow\n"); abort(); }
+            (*stop)++;
+            sidx[*stop] = children[i];
+            slvl[*stop] = level;
         }
     }
     return 0;
+}
+
+static int trav_next(const trav_ctx *t, trav_stack *st, int finalLevel, uint64_t *out)
+{
+    return trav_next_s(t, st->idx, st->lvl, &st->top, VTO_STACK, finalLevel, out);
 }
 
 static void trav_reset(trav_stack *st)
@@ -668,6 +678,239 @@ int vto_render_list(const vto_volume *vol, const vto_octree *oct,
             render_ray(vol, oct, cam, p, ids[k], st, rgba + 4 * k, hits + k, NULL);
         free(st);
     }
+    return 0;
+}
+
+/* ========================================================= cached frame */
+
+/* visualTur.cu:167-228 with its brick cache, for the CPU baseline: every
+ * pass advances each waiting ray's DFS to its next node (Octree.cu:883-901,
+ * the rays in parallel), then updates an LRU cache of `capacity` bricks in
+ * ray order (lruCache.cu:168-313: a hit moves the brick to the back of the
+ * list, a miss takes the front brick's slot and reads the zero-padded box of
+ * the missing brick from the source, lruCache.cu:226, at most `capacity`
+ * new bricks per pass), then marches every ray whose brick is cached
+ * (rayCaster.cu:139-265, in parallel) from the slot.  One policy departs
+ * from the reference: a slot whose brick a ray of the current pass uses is
+ * never taken (the reference's GPU tier could overwrite it, Q2); a ray
+ * whose brick finds no slot waits for the next pass.  Pixels and hit
+ * records are those of vto_render. */
+#define VTO_CSTACK 48   /* a line meets at most 4 children per level: 3 net pushes per level */
+
+static void lru_unlink(int32_t *prev, int32_t *next, int32_t *head, int32_t *tail, int32_t s)
+{
+    if (prev[s] >= 0) next[prev[s]] = next[s]; else *head = next[s];
+    if (next[s] >= 0) prev[next[s]] = prev[s]; else *tail = prev[s];
+    prev[s] = next[s] = -1;
+}
+
+static void lru_push_back(int32_t *prev, int32_t *next, int32_t *head, int32_t *tail, int32_t s)
+{
+    prev[s] = *tail;
+    next[s] = -1;
+    if (*tail >= 0) next[*tail] = s; else *head = s;
+    *tail = s;
+}
+
+struct vto_cache {
+    int level_c, cube, inc, S;
+    int64_t capacity, nb, pass;   /* pass: stamp of the current pass, across frames */
+    size_t slot_elems;
+    float *pool;
+    int32_t *slot_of, *prev, *next;
+    int64_t *brick_of, *used_in;
+    int32_t head, tail;
+};
+
+vto_cache *vto_cache_new(int level_c, int cube, int inc, int64_t capacity)
+{
+    if (level_c < 0 || level_c > 10 || cube < 1 || inc < 0 || capacity < 1) return NULL;
+    vto_cache *c = (vto_cache *)calloc(1, sizeof *c);
+    if (!c) return NULL;
+    c->level_c = level_c;
+    c->cube = cube;
+    c->inc = inc;
+    c->S = cube + 2 * inc;
+    c->nb = (int64_t)1 << (3 * level_c);
+    c->capacity = capacity < c->nb ? capacity : c->nb;
+    c->slot_elems = (size_t)c->S * c->S * c->S;
+    c->pool = (float *)malloc((size_t)c->capacity * c->slot_elems * sizeof(float));
+    c->slot_of = (int32_t *)malloc((size_t)c->nb * sizeof(int32_t));
+    c->brick_of = (int64_t *)malloc((size_t)c->capacity * sizeof(int64_t));
+    c->used_in = (int64_t *)malloc((size_t)c->capacity * sizeof(int64_t));
+    c->prev = (int32_t *)malloc((size_t)c->capacity * sizeof(int32_t));
+    c->next = (int32_t *)malloc((size_t)c->capacity * sizeof(int32_t));
+    if (!c->pool || !c->slot_of || !c->brick_of || !c->used_in || !c->prev || !c->next) {
+        vto_cache_free(c);
+        return NULL;
+    }
+    for (int64_t b = 0; b < c->nb; ++b) c->slot_of[b] = -1;
+    c->head = c->tail = -1;
+    for (int64_t s = 0; s < c->capacity; ++s) {   /* every slot starts empty, in the list */
+        c->brick_of[s] = -1;
+        c->used_in[s] = -1;
+        lru_push_back(c->prev, c->next, &c->head, &c->tail, (int32_t)s);
+    }
+    return c;
+}
+
+void vto_cache_free(vto_cache *c)
+{
+    if (!c) return;
+    free(c->pool); free(c->slot_of); free(c->brick_of); free(c->used_in); free(c->prev); free(c->next);
+    free(c);
+}
+
+int vto_render_cached(const vto_octree *oct, const vto_camera *cam, const vto_params *p, vto_cache *c,
+                      vto_box_reader read, void *user, float *rgba, vto_hit *hits, vto_cache_stats *stats,
+                      int nthreads)
+{
+    if (!c || p->levelC != c->level_c || p->levelC > p->levelO || p->levelO > oct->maxLevel) return -1;
+    const int n = oct->nLevels;
+    if ((1 << (n - p->levelC)) != c->cube) return -1;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#endif
+    const int64_t R = (int64_t)cam->W * cam->H;
+    const int S = c->S, inc = c->inc;
+    const uint64_t base = (uint64_t)1 << (3 * p->levelC);
+    const int shift = 3 * (p->levelO - p->levelC);
+    const int side = 1 << (n - p->levelO);
+    uint8_t *state = (uint8_t *)malloc((size_t)R);        /* 0 advance, 1 wait for its brick, 2 cached, 3 done */
+    uint64_t *node = (uint64_t *)malloc((size_t)R * sizeof(uint64_t));
+    int32_t *slot = (int32_t *)malloc((size_t)R * sizeof(int32_t));
+    float *dirs = (float *)malloc((size_t)R * 3 * sizeof(float));
+    uint64_t *sidx = (uint64_t *)malloc((size_t)R * VTO_CSTACK * sizeof(uint64_t));
+    int *slvl = (int *)malloc((size_t)R * VTO_CSTACK * sizeof(int));
+    int *stop = (int *)malloc((size_t)R * sizeof(int));
+    int rc = 0;
+    if (!state || !node || !slot || !dirs || !sidx || !slvl || !stop) {
+        rc = -2;
+        goto done;
+    }
+    vto_cache_stats st;
+    memset(&st, 0, sizeof st);
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+#endif
+    for (int64_t id = 0; id < R; ++id) {   /* visualTur.cu:59-81 and Octree.cu:750-760: every ray at the root */
+        float *d = dirs + 3 * id;
+        vto_ray_dir(cam, (int)(id / cam->W), (int)(id % cam->W), d);
+        float *px = rgba + 4 * id;
+        memset(&hits[id], 0, sizeof hits[id]);
+        px[0] = px[1] = px[2] = px[3] = 1.0f;
+        trav_ctx t;
+        trav_init(&t, oct, cam->pos, d);
+        state[id] = root_ok(&t) ? 0 : 3;
+        sidx[id * VTO_CSTACK] = 1;
+        slvl[id * VTO_CSTACK] = 0;
+        stop[id] = 0;
+    }
+    for (;;) {
+        int64_t waiting = 0;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 256) reduction(+ : waiting)
+#endif
+        for (int64_t id = 0; id < R; ++id) {   /* getBoxIntersected */
+            if (state[id] == 0) {
+                trav_ctx t;
+                trav_init(&t, oct, cam->pos, dirs + 3 * id);
+                uint64_t nd;
+                if (trav_next_s(&t, sidx + id * VTO_CSTACK, slvl + id * VTO_CSTACK, &stop[id], VTO_CSTACK,
+                                p->levelO, &nd)) {
+                    node[id] = nd;
+                    state[id] = 1;
+                } else {
+                    state[id] = 3;   /* exhausted: stays white */
+                }
+            }
+            if (state[id] == 1) waiting++;
+        }
+        if (waiting == 0) break;
+        st.passes++;
+        const int64_t pass = ++c->pass;
+        /* updateCache, in ray order */
+        int64_t loaded = 0;
+        for (int64_t id = 0; id < R; ++id) {
+            if (state[id] != 1) continue;
+            const int64_t b = (int64_t)((node[id] >> shift) - base);
+            int32_t sl = c->slot_of[b];
+            if (sl >= 0) {
+                st.hits++;
+            } else {
+                if (loaded >= c->capacity) continue;   /* the pass's admission is used up: wait */
+                int32_t v = c->head;   /* least recently used slot no ray of this pass uses */
+                while (v >= 0 && c->used_in[v] == pass) v = c->next[v];
+                if (v < 0) continue;   /* every slot serves this pass: wait */
+                if (c->brick_of[v] >= 0) {
+                    c->slot_of[c->brick_of[v]] = -1;
+                    st.evictions++;
+                }
+                int32_t mn[3];
+                vto_decode((uint64_t)b + base, p->levelC, n, mn);
+                int32_t s3[3] = {mn[0] - inc, mn[1] - inc, mn[2] - inc};
+                int32_t e3[3] = {s3[0] + S, s3[1] + S, s3[2] + S};
+                if (read(user, s3, e3, c->pool + (size_t)v * c->slot_elems) != 0) {
+                    c->brick_of[v] = -1;
+                    rc = -3;
+                    goto done;
+                }
+                c->brick_of[v] = b;
+                c->slot_of[b] = v;
+                sl = v;
+                loaded++;
+                st.misses++;
+            }
+            c->used_in[sl] = pass;
+            lru_unlink(c->prev, c->next, &c->head, &c->tail, sl);
+            lru_push_back(c->prev, c->next, &c->head, &c->tail, sl);
+            slot[id] = sl;
+            state[id] = 2;
+        }
+        /* render: march the cached rays from their slots */
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 256)
+#endif
+        for (int64_t id = 0; id < R; ++id) {
+            if (state[id] != 2) continue;
+            const uint64_t b = node[id] >> shift;
+            int32_t bmn[3], nmn[3];
+            vto_decode(b, p->levelC, n, bmn);
+            vto_decode(node[id], p->levelO, n, nmn);
+            vto_volume sv;
+            sv.data = c->pool + (size_t)slot[id] * c->slot_elems;
+            sv.dims[0] = sv.dims[1] = sv.dims[2] = S;
+            sv.origin[0] = bmn[0] - inc;
+            sv.origin[1] = bmn[1] - inc;
+            sv.origin[2] = bmn[2] - inc;
+            const float *d = dirs + 3 * id;
+            float Xn[3];
+            if (march_node(&sv, cam->pos, d, nmn, side, p->iso, Xn)) {
+                vto_hit *h = &hits[id];
+                shade(&sv, Xn, p, rgba + 4 * id);
+                h->node = node[id];
+                h->brick = b;
+                h->voxel[0] = floor_to_int(Xn[0]);
+                h->voxel[1] = floor_to_int(Xn[1]);
+                h->voxel[2] = floor_to_int(Xn[2]);
+                float dx = Xn[0] - cam->pos[0], dy = Xn[1] - cam->pos[1], dz = Xn[2] - cam->pos[2];
+                h->depth = sqrtf(dx * dx + dy * dy + dz * dz);
+                state[id] = 3;
+            } else {
+                state[id] = 0;
+            }
+        }
+    }
+    if (stats) *stats = st;
+done:
+    free(state); free(node); free(slot); free(dirs); free(sidx); free(slvl); free(stop);
+    return rc;
+}
+
+/* FileManager.cu:86-167 on an in-memory volume, as a brick source. */
+int vto_volume_box_reader(void *user, const int32_t s[3], const int32_t e[3], float *out)
+{
+    vto_read_box((const vto_volume *)user, s, e, out);
     return 0;
 }
 

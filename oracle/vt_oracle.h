@@ -79,6 +79,7 @@ void vto_ray_dir(const vto_camera *cam, int i, int j, float out[3]);
 typedef struct {
     const float *data;   /* C order [X][Y][Z], z fastest */
     int32_t dims[3];
+    int32_t origin[3];   /* voxel (x,y,z) is data[x - origin[0]][...]; 0 for a whole volume */
 } vto_volume;
 
 float vto_voxel(const vto_volume *v, int x, int y, int z);
@@ -132,6 +133,35 @@ int vto_render_list(const vto_volume *vol, const vto_octree *oct,
                     const vto_camera *cam, const vto_params *p,
                     const int64_t *ids, int64_t count,
                     float *rgba, vto_hit *hits, int nthreads);
+
+/* A brick source: the zero-padded box [s, e) into out (C order); 0 = ok. */
+typedef int (*vto_box_reader)(void *user, const int32_t s[3], const int32_t e[3], float *out);
+int vto_volume_box_reader(void *user, const int32_t s[3], const int32_t e[3], float *out);
+
+typedef struct {
+    int64_t passes, hits, misses, evictions;
+} vto_cache_stats;
+
+/* FileManager (FileManager.cu:11-37, 86-167) over libhdf5, as a brick source
+ * (vt_oracle_h5.c); NULL when the file or dataset cannot be opened. */
+typedef struct vto_h5 vto_h5;
+vto_h5 *vto_h5_open(const char *file, const char *dataset);
+void vto_h5_close(vto_h5 *h);
+int vto_h5_box_reader(void *user, const int32_t s[3], const int32_t e[3], float *out);
+
+/* An LRU brick cache of `capacity` bricks of level level_c (cube^3 cells
+ * plus an `inc` halo), kept across frames (lruCache.cu:101-129). */
+typedef struct vto_cache vto_cache;
+vto_cache *vto_cache_new(int level_c, int cube, int inc, int64_t capacity);
+void vto_cache_free(vto_cache *c);
+
+/* The reference frame loop (visualTur.cu:167-228) through cache `c`, filled
+ * from `read` (lruCache.cu:168-313); the march samples the cached slots.
+ * Same pixels and hit records as vto_render.  0 ok, -1 bad arguments,
+ * -2 no memory, -3 the source failed. */
+int vto_render_cached(const vto_octree *oct, const vto_camera *cam, const vto_params *p, vto_cache *c,
+                      vto_box_reader read, void *user, float *rgba, vto_hit *hits, vto_cache_stats *stats,
+                      int nthreads);
 
 /* rayCaster::renderCube (rayCaster.cu:267-357): one brick, every pixel. */
 int vto_render_cube(const vto_volume *vol, const vto_camera *cam, float iso, const float light[3],

@@ -25,7 +25,7 @@ class OCamera(ctypes.Structure):
 
 
 class OVolume(ctypes.Structure):
-    _fields_ = [("data", _P), ("dims", ctypes.c_int32 * 3)]
+    _fields_ = [("data", _P), ("dims", ctypes.c_int32 * 3), ("origin", ctypes.c_int32 * 3)]
 
 
 class OOctree(ctypes.Structure):
@@ -65,6 +65,12 @@ _SIGS = {
                                         ctypes.c_int]),
     "vto_free": (None, [_P]),
     "vto_render_cube": (ctypes.c_int, [ctypes.POINTER(OVolume), ctypes.POINTER(OCamera), _F, _P, _P, ctypes.c_int, _P]),
+    "vto_render_cached": (ctypes.c_int, [ctypes.POINTER(OOctree), ctypes.POINTER(OCamera), ctypes.POINTER(OParams),
+                                         _P, _P, _P, _P, _P, _P, ctypes.c_int]),
+    "vto_cache_new": (_P, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64]),
+    "vto_cache_free": (None, [_P]),
+    "vto_h5_open": (_P, [ctypes.c_char_p, ctypes.c_char_p]),
+    "vto_h5_close": (None, [_P]),
     "vto_octree_write": (ctypes.c_int, [ctypes.c_char_p, _F, _I32, _P, _I32, _P, _P]),
     "vto_octree_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_F), ctypes.POINTER(_I32), _P,
                                        ctypes.POINTER(_I32), ctypes.POINTER(_P), _P, ctypes.c_int]),
@@ -330,6 +336,58 @@ def render_list(vol: Volume, oct: Octree, cam, prm: OParams, ids, nthreads=0):
     if rc != 0:
         raise ValueError("oracle render: bad parameters")
     return rgba, hits
+
+
+class CacheStats(ctypes.Structure):
+    _fields_ = [("passes", ctypes.c_int64), ("hits", ctypes.c_int64), ("misses", ctypes.c_int64),
+                ("evictions", ctypes.c_int64)]
+
+
+class H5Source:
+    """An HDF5 dataset as the cached render's brick source (FileManager's box read)."""
+
+    def __init__(self, path, dataset="value"):
+        self.h = lib().vto_h5_open(str(path).encode(), dataset.encode())
+        if not self.h:
+            raise OSError(f"oracle: cannot open {path}:{dataset}")
+
+    def close(self):
+        if self.h:
+            lib().vto_h5_close(self.h)
+            self.h = None
+
+
+class BrickLRU:
+    """The cached render's LRU brick cache, kept across frames."""
+
+    def __init__(self, level_c: int, cube: int, inc: int, capacity: int):
+        self.h = lib().vto_cache_new(level_c, cube, inc, capacity)
+        if not self.h:
+            raise MemoryError("oracle: brick cache")
+
+    def close(self):
+        if self.h:
+            lib().vto_cache_free(self.h)
+            self.h = None
+
+
+def render_cached(oct: Octree, cam, prm: OParams, cache: BrickLRU, source, nthreads=0):
+    """The reference frame loop through `cache` fed from `source` (a Volume
+    or an H5Source); returns rgba[R,4], hits[R], stats."""
+    oc = camera_from(cam)
+    R = oc.W * oc.H
+    rgba = np.zeros((R, 4), np.float32)
+    hits = np.zeros(R, HIT_DTYPE)
+    st = CacheStats()
+    if isinstance(source, H5Source):
+        fn, user = ctypes.cast(lib().vto_h5_box_reader, _P), source.h
+    else:
+        fn, user = ctypes.cast(lib().vto_volume_box_reader, _P), ctypes.addressof(source.c)
+    rc = lib().vto_render_cached(ctypes.byref(oct.c), ctypes.byref(oc), ctypes.byref(prm), cache.h, fn, user,
+                                 rgba.ctypes.data, hits.ctypes.data, ctypes.byref(st), nthreads)
+    if rc != 0:
+        raise ValueError(f"oracle render_cached: status {rc}")
+    return rgba, hits, {k: getattr(st, k) for k, _ in st._fields_}
 
 
 def render_cube(vol: Volume, cam, iso, light, min_box, dim):
