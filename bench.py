@@ -482,7 +482,7 @@ def run_c3(ctx, args):
         ctx.cache_destroy()
     # bricks the reference algorithm marches in the timed frames (untimed replay, no deferral)
     cache_setup()
-    U = trace_bricks(ctx, args, [cams[k] for k in timed], prm, level_c, d_rgba, d_hits)
+    U, samples = trace_bricks(ctx, args, [cams[k] for k in timed], prm, level_c, d_rgba, d_hits, collect_stats=True)
     ctx.cache_destroy()
     for form in legs:
         legs[form]["device_miss_rate"] = round(legs[form]["demand_misses_per_frame"] / max(1.0, float(np.mean(U))), 5)
@@ -495,7 +495,8 @@ def run_c3(ctx, args):
     achieved = B / (legs[head]["_kernel_ms"] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic("c3"),
-                "kernel": RENDER_KERNELS, "leg": head,
+                "kernel": "k_render_multi+k_render_multi_wide" if head == "serving" else RENDER_KERNELS, "leg": head,
+                "samples_per_frame": int(np.mean(samples)),
                 "avg_launch_ms": round(legs[head]["_kernel_ms"] / legs[head]["_launches"], 4),
                 "bytes_per_launch": int(B / legs[head]["_launches"]),
                 "bricks_marched_per_frame": int(np.mean(U)), "brick_bytes": Sb}
@@ -566,13 +567,16 @@ def in_runs(runs: np.ndarray, codes: np.ndarray) -> np.ndarray:
     return (i >= 0) & (codes <= ends[np.maximum(i, 0)])
 
 
-def fly_through(W: int, H: int, frames: int, X: int, Y: int, Z: int):
+def fly_through(W: int, H: int, frames: int, X: int, Y: int, Z: int, dy: float = 0.0, dz: float = 0.0,
+                phase: float = 0.0):
     """A fly-through: enters the volume through the x = 0 face at mid
-    height and depth and flies along +x to 85 % of its length with a slow
-    sideways and vertical weave, looking along the path."""
+    height and depth (offset by dy, dz of the extents) and flies along +x to
+    85 % of its length with a slow sideways and vertical weave (shifted by
+    `phase` of a period), looking along the path."""
     def pos(t):
-        return np.array([-0.15 * X + 1.0 * X * t, 0.5 * Y + 0.15 * Y * math.sin(2 * math.pi * t),
-                         0.5 * Z + 0.1 * Z * math.sin(4 * math.pi * t)], np.float64)
+        return np.array([-0.15 * X + 1.0 * X * t,
+                         (0.5 + dy) * Y + 0.15 * Y * math.sin(2 * math.pi * (t + phase)),
+                         (0.5 + dz) * Z + 0.1 * Z * math.sin(4 * math.pi * (t + phase))], np.float64)
     cams = []
     for k in range(frames):
         t = k / max(1, frames - 1)
@@ -582,11 +586,13 @@ def fly_through(W: int, H: int, frames: int, X: int, Y: int, Z: int):
     return cams
 
 
-def c5_viewers(ctx, args, d_vol, dims, level_c, iso):
+def c5_viewer_server(ctx, args, d_vol, dims, level_c, iso):
     """The concurrent 1024^2 viewers of the large configuration: the volume
     becomes a serving scene for its one isovalue (only the bricks that hold
     nodes at it are resident), and seeded random-walk viewers issue requests
-    as a Poisson stream; batched serving against one launch per request."""
+    as a Poisson stream.  Returns the server, the requests and the scene's
+    size, after an untimed warm-up that builds the isovalue's tables and the
+    captured launches."""
     from visualtur_amd.prefetch import random_walk
 
     sv = vt.Server(ctx, hbm_budget=2 << 30, max_tiles=65536)
@@ -601,28 +607,19 @@ def c5_viewers(ctx, args, d_vol, dims, level_c, iso):
         t += rng.exponential(1.0 / args.c5_viewer_rate)
         v = k % nv
         reqs.append({"scene": scene, "cam": walks[v][k // nv], "iso": iso, "arrival_s": t, "keep": k < 8})
-    out = {"viewers": nv, "requests": len(reqs), "rate_per_s": args.c5_viewer_rate, "resident_bricks": resident,
-           "resolution": "1024x1024"}
-    # untimed warm-up: the isovalue's membership tables and the captured
-    # launches are built once per scene and width, before the stream
     t0 = time.perf_counter()
     for mode in (0, 1):
         sv.run([dict(q, arrival_s=0.0, keep=False) for q in reqs[:nv]], mode=mode)
     log(f"c5 viewers: warm-up in {time.perf_counter() - t0:.2f}s")
-    kept = {}
-    for mode, name in ((0, "batched"), (1, "per_request")):
-        res, st, o = sv.run(reqs, mode=mode)
-        kept[name] = o
-        lat = np.array([r["latency_ms"] for r in res])
-        out[name] = {"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
-                     "rays_per_s": round(st["rays"] / st["wall_s"], 1),
-                     "busy_rays_per_s": round(st["rays"] / max(st["busy_s"], 1e-9), 1), "launches": st["launches"]}
-        log(f"c5 viewers {name}: {out[name]}")
-    out["same_bytes_batched_and_alone"] = all(
-        kept["batched"][k][1].tobytes() == kept["per_request"][k][1].tobytes() and
-        kept["batched"][k][0].tobytes() == kept["per_request"][k][0].tobytes() for k in kept["batched"])
-    sv.close()
-    return out
+    return sv, reqs, {"viewers": nv, "requests": len(reqs), "rate_per_s": args.c5_viewer_rate,
+                      "resident_bricks": resident, "resolution": "1024x1024"}
+
+
+def viewer_row(res, st):
+    lat = np.array([r["latency_ms"] for r in res])
+    return {"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "rays_per_s": round(st["rays"] / st["wall_s"], 1),
+            "busy_rays_per_s": round(st["rays"] / max(st["busy_s"], 1e-9), 1), "launches": st["launches"]}
 
 
 def run_c5(ctx, args):
@@ -713,36 +710,74 @@ def run_c5(ctx, args):
     prm = vt.render_params(level_c, level_c, iso)
     W, H = 3840, 2160
     R = W * H
-    cams = fly_through(W, H, args.warmup + args.steps, X, Y, Z)
+    nfr = args.warmup + args.steps
+    cams = fly_through(W, H, nfr, X, Y, Z)
+    timed = list(range(args.warmup, nfr))
+    sampled = sorted(set(timed[::16] + [timed[-1]]))
+    out_bufs = {k: (ctx.alloc(R * 16), ctx.alloc(R * 32)) for k in sampled}
     d_rgba, d_hits = ctx.alloc(R * 16), ctx.alloc(R * 32)
-    gpu_slots = int(48e9 // Sb)
-    ctx.cache_create(level_c, 32, inc, gpu_slots, 0)
-    ctx.cache_attach_device_volume(d_vol, (X, Y, Z))
-    march_options(ctx, args)
-    if args.supertile is not None:
-        ctx.set_first_pass_tiles(args.supertile)
+    gpu_slots, cpu_slots = int(48e9 // Sb), int(96e9 // Sb)
+    with_nodes = int(sum(int(runs[level_c][j + 1] - runs[level_c][j]) + 1 for j in range(0, runs[level_c].size, 2)))
+
+    def cache_setup(slots):
+        # brick source: a pinned host tier holding every brick with nodes,
+        # filled once from the volume in HBM, so device misses cross PCIe
+        t = time.perf_counter()
+        ctx.cache_create(level_c, 32, inc, slots, cpu_slots)
+        ctx.cache_attach_host_from_device(d_vol, (X, Y, Z))
+        march_options(ctx, args)
+        if args.supertile is not None:
+            ctx.set_first_pass_tiles(args.supertile)
+        return time.perf_counter() - t
+
+    # the learned prefetcher, trained on other fly-throughs (other heights,
+    # depths and weave phases) rendered at 960x540 from the volume in HBM
+    net, train = None, None
+    if args.prefetch:
+        from visualtur_amd import prefetch as pf
+
+        t0 = time.perf_counter()
+        cand = pf.candidate_codes(runs[level_c])
+        ctx.cache_create(level_c, 32, inc, int(cand.size), 0)
+        ctx.cache_attach_device_volume(d_vol, (X, Y, Z))
+        tcams = [c for k, (dy, dz, ph) in enumerate([(-0.2, -0.15, 0.1), (0.2, 0.1, 0.3), (-0.1, 0.2, 0.55),
+                                                     (0.15, -0.2, 0.75), (0.05, 0.05, 0.9), (-0.25, 0.0, 0.4)])
+                 for c in fly_through(960, 540, 32, X, Y, Z, dy, dz, ph)]
+        traces = pf.collect_traces(ctx, tcams, prm, level_c)
+        net = vt.MLP(ctx, seed=17, max_batch=65536)
+        Xf, Yf = pf.build_dataset(ctx, net, tcams, traces, cand, n, level_c, neg_per_pos=3, seed=5, max_pos=4096)
+        losses, _ = pf.train(ctx, net, Xf, Yf, steps=args.c3_train_steps, batch=8192, lr=1e-3)
+        ctx.cache_destroy()
+        train = {"frames": len(tcams), "samples": int(Xf.shape[0]), "seconds": round(time.perf_counter() - t0, 2),
+                 "final_loss": round(float(losses[-10:].mean()), 4)}
+        log(f"c5 prefetcher trained: {train}")
+        del Xf, Yf
+    # plain: one frame at a time
+    fill_s = cache_setup(gpu_slots)
+    log(f"c5 plain: pinned host tier of {ctx.cache_stats()['cpu_resident']} bricks filled from HBM in {fill_s:.1f}s")
     for k in range(args.warmup):
-        ctx.render_frame(cams[k], prm, d_rgba, d_hits)
+        ctx.render_frame(cams[k], prm, *out_bufs.get(k, (d_rgba, d_hits)))
     ctx.synchronize()
     ms, kms, passes, uploads = [], [], [], []
     t0 = time.perf_counter()
-    for k in range(args.warmup, args.warmup + args.steps):
+    for k in timed:
         f0 = time.perf_counter()
-        st = ctx.render_frame(cams[k], prm, d_rgba, d_hits)
+        st = ctx.render_frame(cams[k], prm, *out_bufs.get(k, (d_rgba, d_hits)))
         ms.append((time.perf_counter() - f0) * 1e3)
         kms.append(st.ms_kernel)
         passes.append(st.passes)
         uploads.append(st.bricks_uploaded)
     ctx.synchronize()
     total_s = time.perf_counter() - t0
-    last = (d_hits.download(vt.HIT_DTYPE, (R,)), d_rgba.download(np.float32, (R, 4)))
+    frames = {"plain": {k: (out_bufs[k][1].download(vt.HIT_DTYPE, (R,)), out_bufs[k][0].download(np.float32, (R, 4)))
+                        for k in sampled}}
+    last = frames["plain"][timed[-1]]
     resident = ctx.cache_stats()["gpu_resident"]
     # bricks the reference algorithm marches in the timed frames (trace replay, untimed)
-    U = trace_bricks(ctx, args, cams[args.warmup: args.warmup + args.steps], prm, level_c, d_rgba, d_hits)
+    U = trace_bricks(ctx, args, [cams[k] for k in timed], prm, level_c, d_rgba, d_hits)
     # the last frame again through an 8 GB cache, from cold: the same bytes
     ctx.cache_destroy()
-    ctx.cache_create(level_c, 32, inc, int(8e9 // Sb), 0)
-    ctx.cache_attach_device_volume(d_vol, (X, Y, Z))
+    cache_setup(int(8e9 // Sb))
     st = ctx.render_frame(cams[-1], prm, d_rgba, d_hits)
     small = (d_hits.download(vt.HIT_DTYPE, (R,)), d_rgba.download(np.float32, (R, 4)))
     checks["small_cache_same_frame"] = {"identical": bool(small[0].tobytes() == last[0].tobytes() and
@@ -750,33 +785,108 @@ def run_c5(ctx, args):
                                         "hit_fraction": round(float((last[0]["node"] != 0).mean()), 4),
                                         "passes": int(st.passes), "uploads": int(st.bricks_uploaded)}
     ctx.cache_destroy()
-    viewers = None
+    plain = {"value": round(R * len(timed) / total_s, 1), "p50_ms": round(float(np.percentile(ms, 50)), 3),
+             "p99_ms": round(float(np.percentile(ms, 99)), 3), "kernel_ms_per_frame": round(float(np.mean(kms)), 4),
+             "passes_per_frame": float(np.mean(passes)), "bricks_uploaded_per_frame": float(np.mean(uploads)),
+             "device_miss_rate": round(float(sum(uploads)) / max(1, sum(U)), 5), "resident_after": int(resident)}
+    # the viewers alone (batched and one launch per request), then the
+    # serving form: the fly-through as frames in flight with the prefetcher
+    # and the viewers' request stream served beside it, both on the GPU at once
+    viewers, serving = None, None
+    sv = vreqs = None
     if args.c5_viewers > 0:
-        viewers = c5_viewers(ctx, args, d_vol, (X, Y, Z), level_c, iso)
-    m = np.array(ms)
+        sv, vreqs, viewers = c5_viewer_server(ctx, args, d_vol, (X, Y, Z), level_c, iso)
+        kept = {}
+        for mode, name in ((0, "batched_alone"), (1, "per_request_alone")):
+            res, vst, o = sv.run(vreqs, mode=mode)
+            kept[name] = o
+            viewers[name] = viewer_row(res, vst)
+            log(f"c5 viewers {name}: {viewers[name]}")
+        viewers["same_bytes_batched_and_alone"] = all(
+            kept["batched_alone"][k][1].tobytes() == kept["per_request_alone"][k][1].tobytes() and
+            kept["batched_alone"][k][0].tobytes() == kept["per_request_alone"][k][0].tobytes()
+            for k in kept["batched_alone"])
+    if args.frames_in_flight > 0:
+        import threading
+
+        cache_setup(gpu_slots)
+        J = args.frames_in_flight
+        win = J + 2
+        ring = [(ctx.alloc(R * 16), ctx.alloc(R * 32)) for _ in range(win)]
+        beside = {}
+        th = threading.Thread(target=lambda: beside.update(zip(("res", "st", "out"), sv.run(vreqs, mode=0))))
+        with vt.Frames(ctx, max_frames=J, max_rays=R) as fr:
+            if net is not None:
+                fr.set_prefetch(net, gpu_slots, args.prefetch_min_prob)
+            for k in range(args.warmup):
+                fr.poll(fr.submit(cams[k], prm, *out_bufs.get(k, ring[k % win])))
+            fr.stats()
+            if sv is not None:
+                th.start()
+            t0 = time.perf_counter()
+            pending, lat = [], []
+            for k in timed:
+                if len(pending) == win:
+                    lat.append(fr.poll(pending.pop(0))[1].ms_total)
+                pending.append(fr.submit(cams[k], prm, *out_bufs.get(k, ring[k % win])))
+            for t in pending:
+                lat.append(fr.poll(t)[1].ms_total)
+            wall = time.perf_counter() - t0
+            agg = fr.stats()
+        if sv is not None:
+            th.join()
+            viewers["batched_beside_flythrough"] = viewer_row(beside["res"], beside["st"])
+            viewers["beside_same_bytes"] = all(
+                beside["out"][k][1].tobytes() == kept["per_request_alone"][k][1].tobytes() for k in beside["out"])
+            log(f"c5 viewers beside the fly-through: {viewers['batched_beside_flythrough']}")
+        for r_, h_ in ring:
+            r_.free()
+            h_.free()
+        n_t = len(timed)
+        serving = {"value": round(R * n_t / wall, 1), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                   "p99_ms": round(float(np.percentile(lat, 99)), 3), "ms_per_frame": round(wall * 1e3 / n_t, 4),
+                   "kernel_ms_per_frame": round(agg["ms_kernel"] / n_t, 4),
+                   "launches_per_frame": agg["launches"] / n_t,
+                   "bricks_uploaded_per_frame": agg["bricks_uploaded"] / n_t,
+                   "prefetched_per_frame": agg["bricks_prefetched"] / n_t, "frames_in_flight": J,
+                   "viewers_beside": sv is not None, "latency": "submission to completion (window of J + 2 frames)",
+                   "identical_sampled_frames": all(
+                       out_bufs[k][1].download(vt.HIT_DTYPE, (R,)).tobytes() == frames["plain"][k][0].tobytes() and
+                       out_bufs[k][0].download(np.float32, (R, 4)).tobytes() == frames["plain"][k][1].tobytes()
+                       for k in sampled),
+                   "_kernel_ms": agg["ms_kernel"], "_launches": agg["launches"]}
+        serving["speedup"] = round(serving["value"] / plain["value"], 3)
+        log(f"c5 serving: { {k: v for k, v in serving.items() if not k.startswith('_')} }")
+        ctx.cache_destroy()
+    if sv is not None:
+        sv.close()
+    if net is not None:
+        net.close()
+    head = serving if serving is not None else plain
     octree_bytes = 8 * sum(r.size for r in runs)
-    B = [u * Sb + R * 48 + octree_bytes for u in U]
-    achieved = sum(B) / (sum(kms) * 1e-3) / 1e9
-    launches = sum(passes)
-    out = {"metric": METRIC, "value": round(R * args.steps / total_s, 1), "unit": "rays/s", "n_gpus": 1,
-           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(float(m.mean()), 4),
+    B = sum(u * Sb + R * 48 + octree_bytes for u in U)
+    h_kms = head["_kernel_ms"] if serving is not None else sum(kms)
+    h_launches = head["_launches"] if serving is not None else sum(passes)
+    achieved = B / (h_kms * 1e-3) / 1e9
+    if serving is not None:
+        del serving["_kernel_ms"], serving["_launches"]
+    out = {"metric": METRIC, "value": head["value"], "unit": "rays/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(R / head["value"] * 1e3, 4),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": "4096x4096x2048 turbulence (64 Fourier modes, k^-5/3), iso = 70th percentile, "
-                                  "resident in HBM as the brick source, 32^3 bricks + 3-voxel halo, device cache "
-                                  "48 GB, 3840x2160 fly-through",
-                      "form": "plain", "iso": iso, "gpu_slots": gpu_slots, "bricks": 1 << (3 * level_c),
-                      "bricks_with_nodes": int(sum(int(runs[level_c][j + 1] - runs[level_c][j]) + 1
-                                                   for j in range(0, runs[level_c].size, 2)))},
-           "p50_ms": round(float(np.percentile(m, 50)), 3), "p99_ms": round(float(np.percentile(m, 99)), 3),
-           "kernel_ms_per_frame": round(float(np.mean(kms)), 4), "passes_per_frame": float(np.mean(passes)),
-           "bricks_uploaded_per_frame": float(np.mean(uploads)), "bricks_marched_per_frame": float(np.mean(U)),
-           "device_miss_rate": round(float(sum(uploads)) / max(1, sum(U)), 5), "resident_after": int(resident),
+                                  "32^3 bricks + 3-voxel halo, device cache 48 GB, pinned host tier (the bricks with "
+                                  "nodes, filled from the volume in HBM), 3840x2160 fly-through, 16 1024^2 viewers",
+                      "form": "serving: prefetch + frames in flight, viewers beside" if serving else "plain",
+                      "iso": iso, "gpu_slots": gpu_slots, "cpu_slots": cpu_slots, "bricks": 1 << (3 * level_c),
+                      "bricks_with_nodes": with_nodes},
+           "p50_ms": head["p50_ms"], "p99_ms": head["p99_ms"], "plain": plain, "serving": serving,
+           "bricks_marched_per_frame": float(np.mean(U)),
            "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic("c5"),
-                        "kernel": RENDER_KERNELS, "avg_launch_ms": round(sum(kms) / launches, 4),
-                        "bytes_per_launch": int(sum(B) / launches), "bricks_marched_per_frame": int(np.mean(U)),
-                        "brick_bytes": Sb},
-           "checks": checks, "viewers": viewers, "cpu_baseline": None}
+                        "kernel": "k_render_multi+k_render_multi_wide" if serving else RENDER_KERNELS,
+                        "avg_launch_ms": round(h_kms / h_launches, 4), "bytes_per_launch": int(B / h_launches),
+                        "bricks_marched_per_frame": int(np.mean(U)), "brick_bytes": Sb},
+           "checks": checks, "viewers": viewers, "train": train, "cpu_baseline": None}
     if not args.no_cpu:
         from oracle import vto
 
@@ -789,8 +899,7 @@ def run_c5(ctx, args):
             vt._check(vt.lib().vt_memcpy_d2h(ctx.handle, part.ctypes.data, d_vol.ptr + xa * slab_n * 4,
                                              part.nbytes), ctx.handle)
         log(f"c5 volume copied to the host in {time.perf_counter() - t0:.1f}s")
-        ctx.cache_create(level_c, 32, inc, gpu_slots, 0)
-        ctx.cache_attach_device_volume(d_vol, (X, Y, Z))
+        cache_setup(gpu_slots)
         w = Workload()
         w.prm = prm
         objs = (vto.Volume(vol_host), vto.Octree(runs, n, level_c), vto.params(level_c, level_c, iso))
@@ -809,8 +918,8 @@ def run_c5(ctx, args):
                                          f"{dt:.1f}s wall, all bricks in memory"}
         ctx.cache_destroy()
         del objs, vol_host
-    d_rgba.free()
-    d_hits.free()
+    for b in [d_rgba, d_hits] + [x for pr in out_bufs.values() for x in pr]:
+        b.free()
     slab.free()
     d_vol.free()
     return out

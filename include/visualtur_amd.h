@@ -152,6 +152,14 @@ int vt_cache_attach_hdf5(vt_context *ctx, const char *file, const char *dataset)
  * assembled on the device; no host tier involved). */
 int vt_cache_attach_device_volume(vt_context *ctx, const float *d_vol, int32_t X, int32_t Y,
                                   int32_t Z);
+/* Brick source = a pinned host tier filled once from a volume in HBM (C
+ * order [X][Y][Z]): every brick holding octree nodes (the octree must be
+ * uploaded down to levelC; at most cpuSlots of them, VT_E_ARG otherwise) is
+ * assembled on the device with its zero-padded halo and copied to pinned
+ * host memory; device misses then cross the host link as from a tier filled
+ * from HDF5 (lruCache.cu:226-266 without the file).  The tier holds exactly
+ * the bricks with nodes, the only ones a ray can request. */
+int vt_cache_attach_host_from_device(vt_context *ctx, const float *d_vol, int32_t X, int32_t Y, int32_t Z);
 /* Start filling the host tier from the file on the reader thread
  * (wait=1 blocks until done). */
 int vt_cache_prefetch_host(vt_context *ctx, int wait);

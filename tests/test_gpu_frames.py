@@ -47,7 +47,9 @@ def test_frames_in_flight_equal_single_frames(vt, vto, ctx):
     try:
         for max_frames, pre in ((1, False), (2, False), (4, False), (8, False), (3, True)):
             ctx.cache_clear()
-            out, stats, agg = _frames(vt, ctx, sc, cams, max_frames, net if pre else None, budget=4, min_prob=0.5)
+            # min_prob 0 with the untrained network: every absent candidate qualifies, so
+            # the prefetcher loads at every boundary where a frame waits
+            out, stats, agg = _frames(vt, ctx, sc, cams, max_frames, net if pre else None, budget=4, min_prob=0.0)
             print(f"frames in flight {max_frames} prefetch {pre}: {agg['launches']} launches for {len(cams)} frames "
                   f"({passes} passes one frame at a time), {agg['bricks_uploaded']} uploads, "
                   f"{agg['bricks_prefetched']} prefetched", flush=True)

@@ -156,6 +156,7 @@ _SIGS = {
                                       ctypes.POINTER(FrameStats)]),
     "vt_frames_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(FramesStats)]),
     "vt_frames_destroy": (ctypes.c_int, [_P]),
+    "vt_cache_attach_host_from_device": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "vt_server_create": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(_P)]),
     "vt_server_destroy": (ctypes.c_int, [_P]),
     "vt_server_set_lanes": (ctypes.c_int, [_P, ctypes.c_int64]),
@@ -402,6 +403,10 @@ class Context:
     def cache_attach_device_volume(self, vol: DeviceBuffer, dims) -> None:
         self._keep.append(vol)
         self._c(lib().vt_cache_attach_device_volume(self.handle, vol.ptr, *[int(d) for d in dims]))
+
+    def cache_attach_host_from_device(self, vol: DeviceBuffer, dims) -> None:
+        """A pinned host tier holding the bricks with nodes, filled from a device volume."""
+        self._c(lib().vt_cache_attach_host_from_device(self.handle, vol.ptr, *[int(d) for d in dims]))
 
     def cache_attach_hdf5(self, file: str, dataset: str) -> None:
         self._c(lib().vt_cache_attach_hdf5(self.handle, str(file).encode(), dataset.encode()))

@@ -739,6 +739,78 @@ int vt_cache_attach_hdf5(vt_context* ctx, const char* file, const char* dataset)
     return VT_OK;
 }
 
+int vt_cache_attach_host_from_device(vt_context* ctx, const float* d_vol, int32_t X, int32_t Y, int32_t Z)
+{
+    if (!ctx || !d_vol) return fail(VT_E_ARG, "NULL argument");
+    if (!ctx->cache) return fail(VT_E_STATE, "no cache");
+    if (X <= 0 || Y <= 0 || Z <= 0) return fail(VT_E_ARG, "bad volume dims");
+    BrickCache& c = *ctx->cache;
+    if (!ctx->octree_ready || ctx->max_level < c.level_c || (int)ctx->up_runs.size() <= c.level_c)
+        return fail(VT_E_STATE, "upload the octree down to levelC first (the tier holds the bricks with nodes)");
+    if ((1 << (ctx->n - c.level_c)) != c.cube) return fail(VT_E_ARG, "cubeDim must equal 2^(nLevels-levelC)");
+    if (int rc = set_device(ctx)) return rc;
+    const auto& runs = ctx->up_runs[(size_t)c.level_c];
+    const uint64_t base = 1ull << (3 * c.level_c);
+    std::vector<int64_t> codes;
+    for (size_t j = 0; j + 1 < runs.size(); j += 2)
+        for (uint64_t code = runs[j]; code <= runs[j + 1]; ++code) codes.push_back((int64_t)code);
+    const int64_t K = (int64_t)codes.size();
+    if (K > c.cpu_slots) return fail(VT_E_ARG, "the host tier cannot hold every brick with octree nodes");
+    std::unique_ptr<HostTier> h(new HostTier);
+    h->level_c = c.level_c;
+    h->n = ctx->n;
+    h->cube = c.cube;
+    h->inc = c.inc;
+    h->S = c.S;
+    h->slot_elems = c.brick_elems;   // bricks back to back
+    h->nbricks = c.nbricks;
+    h->capacity = std::max<int64_t>(K, 1);
+    const size_t bytes = (size_t)h->capacity * c.brick_elems * sizeof(float);
+    if (hipHostMalloc((void**)&h->pool, bytes, hipHostMallocMapped) != hipSuccess)
+        return fail(VT_E_NOMEM, "pinned host tier allocation failed (" + std::to_string(bytes) + " bytes)");
+    HIP_TRY(hipHostGetDevicePointer((void**)&h->pool_dev, h->pool, 0));
+    h->slot_of.assign((size_t)c.nbricks, -1);
+    h->brick_of.assign((size_t)h->capacity, -1);
+    h->ref.assign((size_t)h->capacity, 0);
+    h->pin.assign((size_t)h->capacity, 0);
+    for (int64_t k = 0; k < K; ++k) {
+        h->slot_of[(size_t)(codes[(size_t)k] - (int64_t)base)] = (int32_t)k;
+        h->brick_of[(size_t)k] = codes[(size_t)k] - (int64_t)base;
+    }
+    // assembled on the device (zero-padded halo boxes), a batch at a time,
+    // then copied to the pinned slots
+    const int64_t B = std::min<int64_t>(std::max<int64_t>(K, 1), 1024);
+    DevBuf<float> stage;
+    DevBuf<int64_t> d_codes;
+    DevBuf<int32_t> d_slots;
+    if (stage.ensure((size_t)B * c.brick_elems) != hipSuccess || d_codes.ensure((size_t)B) != hipSuccess ||
+        d_slots.ensure((size_t)B) != hipSuccess)
+        return fail(VT_E_NOMEM, "staging of the host tier fill");
+    std::vector<int32_t> idx((size_t)B);
+    for (int64_t q = 0; q < B; ++q) idx[(size_t)q] = (int32_t)q;
+    HIP_TRY(hipMemcpy(d_slots.p, idx.data(), (size_t)B * 4, hipMemcpyHostToDevice));
+    for (int64_t k0 = 0; k0 < K; k0 += B) {
+        const int cnt = (int)std::min<int64_t>(B, K - k0);
+        HIP_TRY(hipMemcpyAsync(d_codes.p, codes.data() + k0, (size_t)cnt * 8, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(launch_bricks_from_volume(d_vol, X, Y, Z, d_codes.p, d_slots.p, cnt, c.level_c, ctx->n, c.cube, c.inc,
+                                          c.brick_elems, stage.p, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(h->pool + (size_t)k0 * c.brick_elems, stage.p, (size_t)cnt * c.brick_elems * 4,
+                               hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    stage.release();
+    h->used = K;
+    h->full_mode = true;
+    h->started = true;
+    h->slabs_done = (int)h->side_bricks();
+    h->filled = K;
+    h->complete.store(true, std::memory_order_release);
+    c.n = ctx->n;
+    c.d_vol = nullptr;
+    c.host = std::move(h);
+    return VT_OK;
+}
+
 int vt_cache_prefetch_host(vt_context* ctx, int wait)
 {
     if (!ctx || !ctx->cache || !ctx->cache->host) return fail(VT_E_STATE, "no HDF5-backed cache");
