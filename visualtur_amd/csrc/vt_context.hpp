@@ -384,17 +384,23 @@ struct BrickCache {
         return used < gpu_slots ? used++ : -1;
     }
 
+    // Frames in flight hold the bricks loaded for rays that have not yet run
+    // the launch that uses them: slots pinned in epochs >= hold_from are
+    // skipped by the clock as long as another victim exists (~0: no hold).
+    uint64_t hold_from = ~0ull;
+
     // clock over the slots that hold bricks, skipping those pinned in the
-    // current epoch (and, when `recent`, those loaded since the last frame
-    // ended: prefetched or requested for the frame in progress) and giving
-    // referenced ones a second chance
-    int64_t clock_victim(bool recent)
+    // current epoch (and those held, with `hold`; and, when `recent`, those
+    // loaded since the last frame ended: prefetched or requested for the
+    // frame in progress) and giving referenced ones a second chance
+    int64_t clock_victim(bool recent, bool hold = true)
     {
         for (int64_t it = 0; it < 3 * gpu_slots; ++it) {
             int64_t c = hand;
             hand = (hand + 1) % gpu_slots;
             if (c >= used || slot_brick[(size_t)c] < 0) continue;
-            if (pin[(size_t)c] == epoch || (recent && pin[(size_t)c] >= protect_from)) continue;
+            const uint64_t p = pin[(size_t)c];
+            if (p == epoch || (hold && p >= hold_from) || (recent && p >= protect_from)) continue;
             if (ref[(size_t)c]) { ref[(size_t)c] = 0; continue; }
             return c;
         }
@@ -403,6 +409,9 @@ struct BrickCache {
 
     uint64_t recent_exhausted = ~0ull;   // epoch in which every unprotected slot was taken
 
+    // A slot for a load between passes: free, else a clock victim, held
+    // slots last (a load of the current epoch always finds one, so every
+    // service makes progress).
     int64_t pick_victim()
     {
         int64_t s = take_free();
@@ -410,7 +419,8 @@ struct BrickCache {
             s = clock_victim(true);
             if (s < 0) recent_exhausted = epoch;   // do not rescan the ring for every victim
         }
-        return s >= 0 ? s : clock_victim(false);
+        if (s < 0) s = clock_victim(false);
+        return s >= 0 || hold_from == ~0ull ? s : clock_victim(false, false);
     }
 };
 
@@ -441,7 +451,8 @@ void host_tier_start(vt_context* ctx);
 struct vt_mlp;
 // vt_prefetch's scoring, selection and loads; without `publish` the copies'
 // sub-block min/max and page-table entries wait for publish_loads, so a
-// render pass may run while they land.
+// render pass may run while they land, and only free slots are filled (no
+// resident brick is evicted for a prefetch).
 int prefetch_bricks(vt_context* ctx, vt_mlp* m, const vt_camera* cam, int64_t budget, float min_prob, bool publish,
                     int64_t* loaded);
 

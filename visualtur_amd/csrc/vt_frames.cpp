@@ -39,6 +39,10 @@ struct Ticket {
     double t_submit = 0.0;
 };
 
+// A frame still parked after this many launches has stopped progressing
+// (its bricks cannot be held): it fails instead of the worker spinning.
+constexpr int kMaxLaunchesPerFrame = 100000;
+
 struct Slot {
     bool used = false;
     bool fresh = false;     // first pass not launched yet
@@ -158,10 +162,17 @@ int frames_loop(vt_frames* f, std::deque<Pending>& waiting, std::vector<Slot>& s
             if (int rc = admit(f, waiting.front(), k, slots[(size_t)k])) return rc;
             waiting.pop_front();
         }
-        // the next waiting frame's bricks, landing while this launch runs
+        // the next waiting frame's bricks, landing while this launch runs:
+        // into free slots only, which the clock provides (it never takes a
+        // brick loaded at this boundary for the parked rays: those are
+        // pinned in the current epoch)
         if (f->mlp && !waiting.empty() && f->pf_budget > 0) {
             int64_t loaded = 0;
             if (int rc = copy_stream_idle(ctx)) return rc;   // the copies of the last boundary have landed
+            // at most a quarter of the cache for speculative loads
+            const int64_t room = std::min<int64_t>({f->pf_budget, std::max<int64_t>(ctx->early_reserve, 1),
+                                                    std::max<int64_t>(c.gpu_slots / 4, 1)});
+            if (int rc = make_reserve(ctx, room)) return rc;
             if (int rc = prefetch_bricks(ctx, f->mlp, &waiting.front().cam, f->pf_budget, f->pf_prob, false, &loaded))
                 return rc;
             f->stats.bricks_prefetched += loaded;
@@ -237,6 +248,9 @@ int frames_loop(vt_frames* f, std::deque<Pending>& waiting, std::vector<Slot>& s
         if (early) {
             if (int rc = make_reserve(ctx, std::min(ctx->early_reserve, ctx->early_last))) return rc;
         }
+        // bricks loaded from here until the service after this launch (beside
+        // it, and by that service) are held until the launch after it has run
+        const uint64_t launch_epoch = c.epoch;
         HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
         HIP_TRY(launch_render_multi(a, lg, ctx->stream));
         HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
@@ -277,7 +291,10 @@ int frames_loop(vt_frames* f, std::deque<Pending>& waiting, std::vector<Slot>& s
         }
         memset(rq, 0, (size_t)nreq * 4);
         // the service between launches: loads for every frame in flight; its
-        // last kernel clears the request bitmap, counters and reference bytes
+        // last kernel clears the request bitmap, counters and reference bytes.
+        // The rays parked in this launch wait for bricks loaded beside it and
+        // by this service: the clock holds both until the next launch ran.
+        c.hold_from = launch_epoch;
         const PassReset reset{ctx->req_bits.p, (int64_t)ctx->req_bits.n, &ctx->d_ctr->req_count, 2, c.d_ref,
                               c.gpu_slots};
         if (!left.empty()) {
@@ -295,6 +312,9 @@ int frames_loop(vt_frames* f, std::deque<Pending>& waiting, std::vector<Slot>& s
         for (int k = 0; k < f->max_frames; ++k) {
             Slot& s = slots[(size_t)k];
             if (!s.used || s.fresh) continue;
+            if (s.st.passes >= kMaxLaunchesPerFrame)
+                return fail(VT_E_STATE, "a frame in flight made no progress in " +
+                                            std::to_string(kMaxLaunchesPerFrame) + " launches");
             s.st.passes++;
             s.st.ms_kernel += ms;
             s.st.rays_parked += f->h_parked.p[k];
@@ -321,6 +341,7 @@ void frames_worker(vt_frames* f)
     std::deque<Pending> waiting;
     std::vector<Slot> slots((size_t)f->max_frames);
     const int rc = frames_loop(f, waiting, slots);
+    if (f->ctx->cache) f->ctx->cache->hold_from = ~0ull;   // one frame at a time again
     if (rc != VT_OK) {
         const std::string msg = vt_rt::g_err;
         (void)hipStreamSynchronize(f->ctx->stream);

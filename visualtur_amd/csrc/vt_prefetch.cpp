@@ -323,6 +323,13 @@ extern "C" int vt_prefetch(vt_context* ctx, vt_mlp* m, const vt_camera* cam, int
     return prefetch_bricks(ctx, m, cam, budget, min_prob, true, loaded);
 }
 
+namespace {
+// With `publish` (between frames) the prefetcher may evict lower-scoring
+// residents for its loads; without it (beside the render launches of frames
+// in flight) it only fills free slots, since a resident brick may be one
+// that a parked ray of a frame in flight is waiting for.
+}  // namespace
+
 int prefetch_bricks(vt_context* ctx, vt_mlp* m, const vt_camera* cam, int64_t budget, float min_prob, bool publish,
                     int64_t* loaded)
 {
@@ -422,6 +429,7 @@ int prefetch_bricks(vt_context* ctx, vt_mlp* m, const vt_camera* cam, int64_t bu
             }
         };
         int64_t n = std::min<int64_t>((int64_t)want.size(), budget);
+        if (!publish) n = std::min<int64_t>(n, c.available());   // frames in flight: free slots only
         if (n > c.available()) {
             // make room: evict resident bricks, lowest logit first (ties by
             // slot; bricks that are not candidates score -inf), each only for
