@@ -189,6 +189,18 @@ int vt_cache_destroy(vt_context *ctx);
  * loading missing bricks between passes.  stats may be NULL. */
 int vt_render_frame(vt_context *ctx, const vt_camera *cam, const vt_render_params *p,
                     float *d_rgba, vt_hit *d_hits, vt_frame_stats *stats);
+/* vt_render_frame for callers that cannot allocate device memory (code
+ * that must not call hipMalloc: tight serving loops, harnesses that forbid
+ * it).  No counterpart in the reference.  The per-ray state of the render
+ * passes lives in a file-scope device array of the library (66 MiB on every
+ * GPU it runs on, held until the process ends) instead of buffers the
+ * context grows; the brick cache and its staging are allocated beforehand
+ * by vt_cache_create.  Limits: at most 2097152 rays (1920x1080 fits),
+ * max_defer and max_defer_late <= 2, d_hits required, trace recording off
+ * (VT_E_ARG / VT_E_STATE otherwise, before any device work).  Calls on one
+ * GPU share the array; within one process they are serialized. */
+int vt_render_frame_noalloc(vt_context *ctx, const vt_camera *cam, const vt_render_params *p,
+                            float *d_rgba, vt_hit *d_hits, vt_frame_stats *stats);
 /* Enable per-frame sample/node counters and the visited-brick trace. */
 int vt_set_frame_options(vt_context *ctx, int collect_stats, int record_trace);
 /* Frame options with no reference counterpart; results are identical for

@@ -129,6 +129,8 @@ _SIGS = {
     "vt_cache_destroy": (ctypes.c_int, [_P]),
     "vt_render_frame": (ctypes.c_int, [_P, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams), _P, _P,
                                        ctypes.POINTER(FrameStats)]),
+    "vt_render_frame_noalloc": (ctypes.c_int, [_P, ctypes.POINTER(Camera), ctypes.POINTER(RenderParams), _P, _P,
+                                               ctypes.POINTER(FrameStats)]),
     "vt_set_frame_options": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
     "vt_get_brick_trace": (ctypes.c_int, [_P, _P, ctypes.c_int64]),
     "vt_set_march_options": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),

@@ -423,6 +423,39 @@ struct RayState {
     uint8_t* phase;
 };
 
+// ------------------------------------------------- allocation-free frames
+// vt_render_frame_noalloc keeps the per-ray state of its render passes in
+// this array instead of buffers the context allocates, for frames of at most
+// kScratchRays rays (1920x1080 fits) with deferral lists of at most
+// kScratchDefer entries: 33 B per ray, 66 MiB.  Every GPU the library runs
+// on holds its own copy until the process ends.  Host code never passes the
+// array's name to a kernel: it uses the device address looked up once per
+// device at context creation (scratch_address).
+constexpr uint32_t kScratchRays = 1u << 21;
+constexpr int kScratchDefer = 2;
+constexpr size_t kScratchBytes = (size_t)kScratchRays * (8 + 8 * kScratchDefer + 4 + 4 + 1);
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_m;
+uint8_t* g_scratch[kMaxDevices] = {};    // device address of `scratch` on each device
+std::mutex g_scratch_use[kMaxDevices];   // one allocation-free frame at a time per device
+
+// Looks up the current device's copy of `scratch` (once per device).
+int scratch_address(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(VT_E_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> g(g_scratch_m);
+    if (g_scratch[device]) return VT_OK;
+    void* ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(VT_E_HIP, "hipGetSymbolAddress returned no address for the frame scratch");
+    g_scratch[device] = static_cast<uint8_t*>(ptr);
+    return VT_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int vt_last_error(vt_context*, char* buf, size_t len)
@@ -450,6 +483,7 @@ int vt_context_create(int device, vt_context** out)
     HIP_TRY(hipMalloc((void**)&ctx->d_ctr, sizeof(FrameCounters)));
     if (ctx->d_next.ensure(8) != hipSuccess) return fail(VT_E_NOMEM, "work counters");
     ctx->persistent_wgs = render_persistent_workgroups(device);
+    if (int rc = scratch_address(device)) return rc;
     HIP_TRY(hipHostMalloc((void**)&ctx->h_ctr, sizeof(FrameCounters), hipHostMallocDefault));
     *out = ctx.release();
     return VT_OK;
@@ -1165,6 +1199,37 @@ int vt_render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_param
         // a frame that fails part-way (a brick source that fails while the
         // first pass runs, say) leaves no kernel or copy running: the caller
         // may free its buffers or the cache at once
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamSynchronize(ctx->cstream);
+    }
+    return rc;
+}
+
+int vt_render_frame_noalloc(vt_context* ctx, const vt_camera* cam, const vt_render_params* p, float* d_rgba,
+                            vt_hit* d_hits, vt_frame_stats* stats)
+{
+    if (!ctx || !p || !d_rgba || !d_hits) return fail(VT_E_ARG, "NULL argument (d_hits is required)");
+    if (int rc = check_camera(cam)) return rc;
+    const uint64_t R = (uint64_t)cam->W * (uint64_t)cam->H;
+    if (R > kScratchRays)
+        return fail(VT_E_ARG, "a frame of " + std::to_string(R) + " rays exceeds the allocation-free maximum of " +
+                                  std::to_string(kScratchRays));
+    if (std::max(ctx->max_defer, ctx->max_defer_late) > kScratchDefer)
+        return fail(VT_E_ARG, "max_defer and max_defer_late must be <= 2 for allocation-free frames");
+    if (ctx->record_trace) return fail(VT_E_STATE, "the visited-brick trace allocates: turn it off");
+    if (ctx->device < 0 || ctx->device >= kMaxDevices || !g_scratch[ctx->device])
+        return fail(VT_E_STATE, "no frame scratch on this device");
+    uint8_t* base = g_scratch[ctx->device];
+    const size_t M = kScratchRays;
+    const RayState st{reinterpret_cast<uint64_t*>(base), reinterpret_cast<uint64_t*>(base + 8 * M),
+                      {reinterpret_cast<uint32_t*>(base + (8 + 8 * kScratchDefer) * M),
+                       reinterpret_cast<uint32_t*>(base + (12 + 8 * kScratchDefer) * M)},
+                      base + (16 + 8 * kScratchDefer) * M};
+    // calls on one device share the array: one frame at a time (the frame
+    // has waited for its last pass when render_frame returns)
+    std::lock_guard<std::mutex> g(g_scratch_use[ctx->device]);
+    const int rc = render_frame(ctx, cam, p, d_rgba, d_hits, stats, &st);
+    if (rc != VT_OK) {
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipStreamSynchronize(ctx->cstream);
     }
