@@ -520,6 +520,11 @@ int vt_file_read_box(vt_file *f, const int32_t s[3], const int32_t e[3], float *
 /* FileManager::CreateFile (FileManager.cu:169-243); chunk 0 = contiguous. */
 int vt_file_create(const char *file, const char *dataset, const float *data,
                    int64_t X, int64_t Y, int64_t Z, int32_t chunk);
+/* The cutFile tool (cutFile.cu:13-21): the box [x, x+dim) x [y, y+dim) x
+ * [z, z+dim) of `dataset` (zero outside it, as readHDF5_Voxel_Array reads)
+ * written to `out_file` as a contiguous dim^3 dataset named "value". */
+int vt_file_cut(const char *file, const char *dataset, const char *out_file, int32_t x, int32_t y, int32_t z,
+                int32_t dim);
 
 #ifdef __cplusplus
 }

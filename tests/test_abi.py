@@ -65,6 +65,28 @@ def test_file_manager_host_paths(vt, vto, tmp_path):
         vt.File(tmp_path / "f0.h5", "no_such_dataset")
 
 
+def test_file_cut_is_the_box_read(vt, vto, tmp_path):
+    """cutFile.cu:13-21: the dim^3 box at (x, y, z), zero outside the
+    dataset, written as a dim^3 dataset "value"."""
+    rng = np.random.default_rng(9)
+    vol = rng.standard_normal((19, 23, 17)).astype(np.float32)
+    V = vto.Volume(vol)
+    src = tmp_path / "src.h5"
+    vt.file_create(src, "density", vol, chunk=8)
+    for x, y, z, dim in [(0, 0, 0, 8), (5, 7, 3, 10), (12, 18, 10, 9), (-3, -2, -5, 6), (30, 0, 0, 4), (0, 0, 0, 32)]:
+        out = tmp_path / f"cut_{x}_{y}_{z}_{dim}.h5"
+        vt.file_cut(src, "density", out, x, y, z, dim)
+        with vt.File(out, "value") as f:
+            assert f.dims == (dim, dim, dim)
+            got = f.read_box((0, 0, 0), (dim, dim, dim))
+        assert np.array_equal(got, V.read_box((x, y, z), (x + dim, y + dim, z + dim))), (x, y, z, dim)
+    with pytest.raises(vt.VTError) as e:
+        vt.file_cut(tmp_path / "nope.h5", "density", tmp_path / "o.h5", 0, 0, 0, 4)
+    assert e.value.code == -3
+    with pytest.raises(vt.VTError):
+        vt.file_cut(src, "density", tmp_path / "o.h5", 0, 0, 0, 0)
+
+
 def test_ppm_writer_layout(vt, tmp_path):
     """vt_image_write_ppm lays the frame out as testVisualTur.cu:67-107 lays
     out its bitmap: H wide, W tall, pixel (x=i, y=j from the bottom) = ray

@@ -8,7 +8,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("shape,iso", [((16, 16, 16), 0.5), ((13, 16, 9), 0.3), ((64, 64, 64), 0.4),
-                                       ((100, 128, 77), 0.25), ((32, 32, 32), 0.0)])
+                                       ((100, 128, 77), 0.25), ((32, 32, 32), 0.0),
+                                       ((16, 16, 16), -0.2)])   # below every value: empty run lists
 @pytest.mark.parametrize("literal", [False, True])
 def test_builder_runs_match_oracle(vt, vto, ctx, tmp_path, shape, iso, literal):
     vol = blobs_volume(max(shape), 10, seed=sum(shape), dims=shape)

@@ -68,6 +68,53 @@ def test_frames_in_flight_equal_single_frames(vt, vto, ctx):
     assert_bits(single[0][1], single[0][0], hc, rc)
 
 
+def test_frames_with_mixed_isos_equal_single_frames(vt, vto, ctx):
+    """Frames of two isovalues in flight together over one octree: the
+    proven sub-block sides are kept for one iso at a time (the newest
+    frame's) and rays of the other iso read the sub-block (min, max).  Every
+    frame has the bytes of the same frame rendered alone with no sample
+    skipped."""
+    from visualtur_amd.prefetch import random_walk
+
+    vol = blobs_volume(64, 14, seed=73)
+    sc = Scene(vto, vol, 0.3, 3, 2)
+    sc.setup_gpu(vt, ctx, slots=9)
+    cams = [random_walk(72, 60, 64, 1, seed=950 + k, radius=(1.4, 2.2))[0] for k in range(7)]
+    isos = [0.3, 0.45, 0.3, 0.45, 0.45, 0.3, 0.3]
+    prms = [vt.render_params(sc.level_o, sc.level_c, iso) for iso in isos]
+    ref = []
+    ctx.set_march_options(skip_empty=False)
+    try:
+        for c, p in zip(cams, prms):
+            sc.prm_g = p
+            rg, hg, _ = sc.render_gpu(vt, ctx, c)
+            ref.append((rg, hg))
+    finally:
+        ctx.set_march_options(skip_empty=True)
+    try:
+        for max_frames in (1, 3):
+            ctx.cache_clear()
+            bufs = [(ctx.alloc(c.W * c.H * 16), ctx.alloc(c.W * c.H * 32)) for c in cams]
+            with vt.Frames(ctx, max_frames=max_frames, max_rays=72 * 60) as fr:
+                tickets = [fr.submit(c, p, *bufs[k]) for k, (c, p) in enumerate(zip(cams, prms))]
+                for t in tickets:
+                    assert fr.poll(t)[0]
+            for k, ((r, h), c) in enumerate(zip(bufs, cams)):
+                R = c.W * c.H
+                assert h.download(vt.HIT_DTYPE, (R,)).tobytes() == ref[k][1].tobytes(), (max_frames, k)
+                assert r.download(np.float32, (R, 4)).tobytes() == ref[k][0].tobytes(), (max_frames, k)
+                r.free()
+                h.free()
+            # and one frame at a time, the sides following each frame's iso
+            for k, (c, p) in enumerate(zip(cams, prms)):
+                sc.prm_g = p
+                rg, hg, _ = sc.render_gpu(vt, ctx, c)
+                assert hg.tobytes() == ref[k][1].tobytes() and rg.tobytes() == ref[k][0].tobytes(), k
+    finally:
+        ctx.cache_destroy()
+    assert (ref[1][1]["node"] != 0).mean() > 0.01
+
+
 def test_frames_refuse_bad_requests(vt, vto, ctx):
     vol = blobs_volume(32, 6, seed=72)
     sc = Scene(vto, vol, 0.3, 3, 2)

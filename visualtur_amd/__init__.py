@@ -196,6 +196,8 @@ _SIGS = {
     "vt_file_read_box": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), _P]),
     "vt_file_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _P, ctypes.c_int64, ctypes.c_int64,
                                       ctypes.c_int64, ctypes.c_int32]),
+    "vt_file_cut": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32,
+                                   ctypes.c_int32, ctypes.c_int32]),
 }
 
 
@@ -713,6 +715,13 @@ def file_create(path: str, dataset: str, vol: np.ndarray, chunk: int = 0) -> Non
     v = np.ascontiguousarray(vol, np.float32)
     X, Y, Z = v.shape
     _check(lib().vt_file_create(str(path).encode(), dataset.encode(), v.ctypes.data, X, Y, Z, chunk))
+
+
+def file_cut(path: str, dataset: str, out_path: str, x: int, y: int, z: int, dim: int) -> None:
+    """The cutFile tool (cutFile.cu:13-21): the dim^3 box at (x, y, z) of
+    `dataset`, zero outside it, written to out_path as dataset "value"."""
+    _check(lib().vt_file_cut(str(path).encode(), dataset.encode(), str(out_path).encode(), int(x), int(y), int(z),
+                             int(dim)))
 
 
 class File:

@@ -335,6 +335,12 @@ struct BrickCache {
     uint8_t* d_ref = nullptr;
     float2* d_submm = nullptr;   // G^3 sub-block (min, max) per slot, null when bricks are not 4k wide
     int G = 0;
+    // per slot, the sides the sub-blocks' (min, max) prove for side_iso, 2
+    // bits a sub-block, side_words words a slot (a multiple of 32: one
+    // 128-byte line per 32^3 brick); NaN side_iso: not computed yet
+    uint32_t* d_sides = nullptr;
+    int side_words = 0;
+    float side_iso = __builtin_nanf("");
     std::vector<int32_t> table;
     std::vector<int64_t> slot_brick;
     std::vector<uint8_t> ref;
@@ -365,6 +371,7 @@ struct BrickCache {
         if (d_table) (void)hipFree(d_table);
         if (d_ref) (void)hipFree(d_ref);
         if (d_submm) (void)hipFree(d_submm);
+        if (d_sides) (void)hipFree(d_sides);
         h_stage.release();
         d_stage.release();
         h_ref.release();
@@ -444,6 +451,7 @@ int fold_ref_bits(vt_context* ctx, bool copied = false);
 int service_requests(vt_context* ctx, const uint32_t* reqs, int64_t count, vt_frame_stats* st,
                      const PassReset* reset = nullptr);
 int copy_stream_idle(vt_context* ctx);
+int ensure_sides(vt_context* ctx, float iso);
 int publish_loads(vt_context* ctx, hipStream_t s);
 int make_reserve(vt_context* ctx, int64_t reserve);
 int load_during_pass(vt_context* ctx, hipEvent_t done, int64_t* loaded, int64_t* misses);

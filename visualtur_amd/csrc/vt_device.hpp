@@ -426,6 +426,10 @@ struct SubGrid {
     const float2* mm;   // G^3 (min, max) of this brick, x-major, or null
     int G;              // sub-blocks per axis (cube / 4)
     int cx0, cy0, cz0;  // global voxel of the brick's first core cell
+    // the sides side_from_minmax gives for this pass's iso, 2 bits per
+    // sub-block (sub-block sb in bits 2*(sb & 15) of word sb >> 4), or null:
+    // one 128-byte line per 32^3 brick instead of a 4 KB (min, max) grid
+    const uint32_t* sides = nullptr;
 };
 
 // Side of iso that a sub-block's (min, max) proves for every trilinear value
@@ -451,15 +455,21 @@ struct SideCache {
 __device__ __forceinline__ int proven_side(const SubGrid& g, float iso, float px, float py, float pz, SideCache& sc)
 {
     if (px >= sc.x0 && px < sc.x1 && py >= sc.y0 && py < sc.y1 && pz >= sc.z0 && pz < sc.z1) return sc.side;
-    if (!g.mm || !(px >= 0.0f) || !(py >= 0.0f) || !(pz >= 0.0f) || !coord_ok(px) || !coord_ok(py) || !coord_ok(pz))
+    if ((!g.mm && !g.sides) || !(px >= 0.0f) || !(py >= 0.0f) || !(pz >= 0.0f) || !coord_ok(px) || !coord_ok(py) ||
+        !coord_ok(pz))
         return SIDE_NONE;
     int cx = (int)px - g.cx0, cy = (int)py - g.cy0, cz = (int)pz - g.cz0;
     int lim = 4 * g.G;
     if ((unsigned)cx >= (unsigned)lim || (unsigned)cy >= (unsigned)lim || (unsigned)cz >= (unsigned)lim)
         return SIDE_NONE;
     int sb = ((cx >> 2) * g.G + (cy >> 2)) * g.G + (cz >> 2);
-    float2 mm = g.mm[sb];
-    int s = side_from_minmax(mm.x, mm.y, iso);
+    int s;
+    if (g.sides) {
+        s = (int)((g.sides[sb >> 4] >> (2 * (sb & 15))) & 3u);
+    } else {
+        float2 mm = g.mm[sb];
+        s = side_from_minmax(mm.x, mm.y, iso);
+    }
     sc.x0 = (float)(g.cx0 + (cx & ~3));
     sc.y0 = (float)(g.cy0 + (cy & ~3));
     sc.z0 = (float)(g.cz0 + (cz & ~3));

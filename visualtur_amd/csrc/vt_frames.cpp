@@ -220,6 +220,14 @@ int frames_loop(vt_frames* f, std::deque<Pending>& waiting, std::vector<Slot>& s
         a.brick_trace = nullptr;
         a.submm = ctx->skip_empty ? c.d_submm : nullptr;
         a.G = c.G;
+        // the proven sides follow the iso of the newest frame (frames in
+        // flight with another iso read the (min, max))
+        if (ctx->skip_empty && nfresh > 0) {
+            if (int rc = ensure_sides(ctx, f->h_jobs.p[a.fresh_slot[nfresh - 1]].sh.iso)) return rc;
+        }
+        a.sides = ctx->skip_empty ? c.d_sides : nullptr;
+        a.side_words = c.side_words;
+        a.side_iso = c.side_iso;
         a.ctr = ctx->d_ctr;
         a.collect_stats = 0;
         a.jobs = f->d_jobs.p;

@@ -26,6 +26,11 @@ struct BrickView {
     uint64_t brick_base;         // 8^levelC
     const float2* submm;         // per slot G^3 sub-block (min, max), or null
     int G;                       // sub-blocks per axis (cubeDim / 4)
+    // per slot, the side of iso each sub-block's (min, max) proves, 2 bits a
+    // sub-block (16 to a word, sub-block sb in word sb >> 4), computed for
+    // this pass's iso; null: decide from submm
+    const uint32_t* sides = nullptr;
+    int side_words = 0;          // words per slot
 };
 
 struct Shading {
@@ -91,6 +96,9 @@ struct RenderArgs {
     uint8_t* brick_trace;        // optional, 1 byte per brick
     const float2* submm;         // sub-block min/max per slot, or null (no skipping)
     int G;
+    const uint32_t* sides;       // per slot proven sides for iso side_iso (see BrickView), or null
+    int side_words;
+    float side_iso;              // rays whose iso differs use submm
     FrameCounters* ctr;
     float4* rgba;
     HitRecord* hits;
@@ -207,6 +215,10 @@ hipError_t launch_bricks_from_volume(const float* vol, int X, int Y, int Z,
 // (min, max) of every 4x4x4-cell sub-block (5^3 samples) of the given slots
 hipError_t launch_submm(const float* pool, const int32_t* slots, int count, int S, int inc, int G, size_t stride,
                         float2* submm, hipStream_t s);
+// proven-side words of `count` slots (slots[i], or slot i when slots is null)
+// from their sub-block (min, max) for `iso`
+hipError_t launch_sides(const float2* submm, const int32_t* slots, int count, int G, float iso, uint32_t* sides,
+                        int side_words, hipStream_t s);
 // Copy bricks of slot_elems floats from (device-visible) host memory into
 // the pool, slot b at pool + b * stride, with `split` workgroups per brick
 // (0: the default of 8).

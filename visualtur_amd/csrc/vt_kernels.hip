@@ -64,7 +64,7 @@ __device__ __forceinline__ bool visit_node(const Ray& r, const OctreeView& oct, 
     // without sub-block min/max (mm == null) no side is ever proven and
     // march_node_skip evaluates every sample: march_node's sequence exactly
     SubGrid g{bv.submm ? bv.submm + (size_t)slot * bv.G * bv.G * bv.G : nullptr, bv.G, bx + bv.inc,
-              by + bv.inc, bz + bv.inc};
+              by + bv.inc, bz + bv.inc, bv.sides ? bv.sides + (size_t)slot * bv.side_words : nullptr};
     if (!march_node_skip(r, nx, ny, nz, side, sp, bv.S, bx, by, bz, g, sh.iso, Xx, Xy, Xz, err, nsamples))
         return false;
     shade(sp, bv.S, bx, by, bz, Xx, Xy, Xz, sh.lx, sh.ly, sh.lz, sh.color_scale, out, err);
@@ -268,7 +268,7 @@ __device__ __forceinline__ bool visit_node_g(const Ray& r, const OctreeView& oct
     float Xx, Xy, Xz;
     nnodes += 1;
     SubGrid sg{bv.submm ? bv.submm + (size_t)slot * bv.G * bv.G * bv.G : nullptr, bv.G, bx + bv.inc,
-               by + bv.inc, bz + bv.inc};
+               by + bv.inc, bz + bv.inc, bv.sides ? bv.sides + (size_t)slot * bv.side_words : nullptr};
     if (!march_node_wide(r, nx, ny, nz, side, sp, bv.S, bx, by, bz, sg, sh.iso, Xx, Xy, Xz, err, nsamples, g))
         return false;
     if (g.L >= 8)
@@ -311,8 +311,10 @@ __device__ __forceinline__ void pass_ray(const RenderArgs& a, uint32_t id, bool 
         int i = (int)(pix / (uint32_t)W), j = (int)(pix % (uint32_t)W);
         Ray r;
         make_ray(cam, i, j, r);
-        BrickView bv{a.brick_slot, a.pool, a.S, a.inc, a.slot_elems, a.level_c, a.shift, a.brick_base, a.submm, a.G};
         const Shading sh = kMulti ? a.jobs[fslot].sh : Shading{a.iso, a.lx, a.ly, a.lz, a.color_scale};
+        // the proven-side words hold the sides for a.side_iso (never NaN)
+        BrickView bv{a.brick_slot, a.pool, a.S, a.inc, a.slot_elems, a.level_c, a.shift, a.brick_base, a.submm, a.G,
+                     a.sides && sh.iso == a.side_iso ? a.sides : nullptr, a.side_words};
         float4 c;
         HitRecord h;
         uint64_t bidx = 0, node = 0;
@@ -631,7 +633,8 @@ __global__ __launch_bounds__(kBlock) void k_render_persistent(RenderArgs a)
     const uint32_t unit = first ? 256u : 1u;
     const int lane = (int)__lane_id();
     const unsigned long long below = (1ull << lane) - 1ull;
-    BrickView bv{a.brick_slot, a.pool, a.S, a.inc, a.slot_elems, a.level_c, a.shift, a.brick_base, a.submm, a.G};
+    BrickView bv{a.brick_slot, a.pool, a.S, a.inc, a.slot_elems, a.level_c, a.shift, a.brick_base, a.submm, a.G,
+                 a.sides && a.iso == a.side_iso ? a.sides : nullptr, a.side_words};
     Shading sh{a.iso, a.lx, a.ly, a.lz, a.color_scale};
 
     int part = (int)(blockIdx.x & 7u);   // wave-uniform
@@ -1286,6 +1289,38 @@ hipError_t launch_submm(const float* pool, const int32_t* slots, int count, int 
 {
     if (count <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_submm, dim3(count), dim3(64), 0, s, pool, slots, S, inc, G, stride, submm);
+    return hipGetLastError();
+}
+
+// one thread per proven-side word: the sides of its 16 sub-blocks (fewer in
+// a slot's last word), from their (min, max) as proven_side decides them
+__global__ __launch_bounds__(256) void k_sides(const float2* __restrict__ submm, const int32_t* __restrict__ slots,
+                                               int count, int G, float iso, uint32_t* __restrict__ sides,
+                                               int side_words)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)count * side_words) return;
+    const int b = (int)(i / (size_t)side_words), w = (int)(i % (size_t)side_words);
+    const size_t slot = slots ? (size_t)slots[b] : (size_t)b;
+    const int nsb = G * G * G;
+    const float2* mm = submm + slot * nsb;
+    uint32_t word = 0;
+    for (int q = 0; q < 16; ++q) {
+        const int sb = w * 16 + q;
+        if (sb >= nsb) break;
+        const float2 v = mm[sb];
+        word |= (uint32_t)side_from_minmax(v.x, v.y, iso) << (2 * q);
+    }
+    sides[slot * side_words + w] = word;
+}
+
+hipError_t launch_sides(const float2* submm, const int32_t* slots, int count, int G, float iso, uint32_t* sides,
+                        int side_words, hipStream_t s)
+{
+    if (count <= 0) return hipSuccess;
+    const size_t n = (size_t)count * side_words;
+    hipLaunchKernelGGL(k_sides, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, submm, slots, count, G, iso, sides,
+                       side_words);
     return hipGetLastError();
 }
 

@@ -207,6 +207,8 @@ int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t
     }
     if (c.d_submm)
         HIP_TRY(launch_submm(c.d_pool, d_slots, (int)n_load, c.S, c.inc, c.G, c.slot_elems, c.d_submm, ctx->stream));
+    if (c.d_sides && c.side_iso == c.side_iso)
+        HIP_TRY(launch_sides(c.d_submm, d_slots, (int)n_load, c.G, c.side_iso, c.d_sides, c.side_words, ctx->stream));
     if (reset) {
         HIP_TRY(launch_finish_service(c.d_table, d_tidx, d_tval, (int)nt, *reset, ctx->stream));
     } else {
@@ -307,6 +309,8 @@ int publish_loads(vt_context* ctx, hipStream_t s)
     }
     if (c.d_submm)
         HIP_TRY(launch_submm(c.d_pool, A.slots.dev + A.pub, (int)k, c.S, c.inc, c.G, c.slot_elems, c.d_submm, s));
+    if (c.d_sides && c.side_iso == c.side_iso)
+        HIP_TRY(launch_sides(c.d_submm, A.slots.dev + A.pub, (int)k, c.G, c.side_iso, c.d_sides, c.side_words, s));
     HIP_TRY(launch_scatter_i32(c.d_table, A.tidx.dev + A.pub, A.tval.dev + A.pub, (int)k, s));
     A.pub = A.used;
     return VT_OK;
@@ -322,6 +326,22 @@ int copy_stream_idle(vt_context* ctx)
     }
     HIP_TRY(hipStreamSynchronize(ctx->cstream));
     ctx->astage.used = ctx->astage.pub = 0;
+    return VT_OK;
+}
+
+// Makes the proven-side words hold the sides for `iso` (a no-op when they
+// already do, or for a NaN iso): recomputed for every slot on the context
+// stream, where the sub-block (min, max) of every published load is already
+// written; the copy stream's later work waits for them.  A ray whose iso is
+// not side_iso reads the (min, max) instead.
+int ensure_sides(vt_context* ctx, float iso)
+{
+    BrickCache& c = *ctx->cache;
+    if (!c.d_sides || iso != iso || iso == c.side_iso) return VT_OK;
+    c.side_iso = iso;
+    HIP_TRY(launch_sides(c.d_submm, nullptr, (int)c.gpu_slots, c.G, iso, c.d_sides, c.side_words, ctx->stream));
+    HIP_TRY(hipEventRecord(ctx->ev_copy, ctx->stream));
+    HIP_TRY(hipStreamWaitEvent(ctx->cstream, ctx->ev_copy, 0));
     return VT_OK;
 }
 
@@ -697,6 +717,9 @@ int vt_cache_create(vt_context* ctx, int32_t levelC, int32_t cubeDim, int32_t cu
         c->G = cubeDim / 4;
         size_t mm = (size_t)c->gpu_slots * c->G * c->G * c->G * sizeof(float2);
         if (hipMalloc((void**)&c->d_submm, mm) != hipSuccess) return fail(VT_E_NOMEM, "sub-block min/max");
+        c->side_words = (c->G * c->G * c->G + 16 * 32 - 1) / (16 * 32) * 32;
+        if (hipMalloc((void**)&c->d_sides, (size_t)c->gpu_slots * c->side_words * 4) != hipSuccess)
+            return fail(VT_E_NOMEM, "sub-block sides");
     }
     c->table.assign((size_t)c->nbricks, -1);
     c->slot_brick.assign((size_t)c->gpu_slots, -1);
@@ -1075,6 +1098,8 @@ static int render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_p
     a.brick_trace = ctx->record_trace ? ctx->trace.p : nullptr;
     a.submm = ctx->skip_empty ? c.d_submm : nullptr;
     a.G = c.G;
+    a.sides = ctx->skip_empty ? c.d_sides : nullptr;
+    a.side_words = c.side_words;
     a.ctr = ctx->d_ctr;
     a.rgba = reinterpret_cast<float4*>(d_rgba);
     a.hits = hits;
@@ -1091,6 +1116,10 @@ static int render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_p
     a.out_list = state->list[cur];
     // bricks still arriving from a prefetch land before the first pass
     if (int rc = copy_stream_idle(ctx)) return rc;
+    if (ctx->skip_empty) {
+        if (int rc = ensure_sides(ctx, p->iso)) return rc;
+    }
+    a.side_iso = c.side_iso;
     // free slots for the loads issued while the first pass runs: at most as
     // many as the previous frame's first pass requested, so that a frame
     // whose bricks are all resident evicts nothing
@@ -1294,6 +1323,28 @@ int vt_file_create(const char* file, const char* dataset, const float* data, int
     return VT_OK;
 }
 
+int vt_file_cut(const char* file, const char* dataset, const char* out_file, int32_t x, int32_t y, int32_t z,
+                int32_t dim)
+{
+    if (!file || !dataset || !out_file) return fail(VT_E_ARG, "NULL argument");
+    if (dim <= 0 || dim > 4096) return fail(VT_E_ARG, "dim must be in [1, 4096]");
+    // the box must stay in int32 coordinates, as the box read takes them
+    const int64_t lim = (int64_t)INT32_MAX - dim;
+    if (x > lim || y > lim || z > lim) return fail(VT_E_ARG, "box end overflows");
+    vt_file* f = nullptr;
+    if (int rc = vt_file_open(file, dataset, &f, nullptr)) return rc;
+    std::unique_ptr<vt_file> own(f);
+    std::vector<float> box;
+    try {
+        box.resize((size_t)dim * dim * dim);
+    } catch (const std::bad_alloc&) {
+        return fail(VT_E_NOMEM, "cut box");
+    }
+    const int32_t s3[3] = {x, y, z}, e3[3] = {x + dim, y + dim, z + dim};
+    if (int rc = vt_file_read_box(f, s3, e3, box.data())) return rc;
+    return vt_file_create(out_file, "value", box.data(), dim, dim, dim, 0);
+}
+
 }  // extern "C"
 
 // ============================================================ builder ABI
@@ -1390,7 +1441,9 @@ int vt_octree_build(vt_context* ctx, const float* d_vol, int32_t X, int32_t Y, i
         sizes[(size_t)l] = (int32_t)runs[(size_t)l].size();
         flat.insert(flat.end(), runs[(size_t)l].begin(), runs[(size_t)l].end());
     }
-    int rc = vt_octree_upload(ctx, flat.data(), sizes.data(), n, maxLevel);
+    // an isovalue outside the volume's range gives empty run lists
+    static const uint64_t no_runs = 0;
+    int rc = vt_octree_upload(ctx, flat.empty() ? &no_runs : flat.data(), sizes.data(), n, maxLevel);
     if (rc != VT_OK) return rc;
     ctx->built_runs = std::move(runs);
     ctx->built_iso = iso;
