@@ -443,18 +443,19 @@ __device__ __forceinline__ int side_from_minmax(float lo, float hi, float iso)
     return 0;
 }
 
-// The sub-block last looked up: its voxel box [x0,x1) x [y0,y1) x [z0,z1)
-// (integer-valued floats) and its proven side.  For p >= 0, (int)p is
-// floor(p), so p lies in the box exactly when its cell lies in the
-// sub-block; NaN fails every comparison.
+// The sub-block last looked up: its voxel box [x0,x0+4) x [y0,y0+4) x
+// [z0,z0+4) (integer-valued floats, so x0 + 4 is exact) and its proven side.
+// For p >= 0, (int)p is floor(p), so p lies in the box exactly when its
+// cell lies in the sub-block; NaN fails every comparison (the empty box).
 struct SideCache {
-    float x0 = 1.0f, y0 = 1.0f, z0 = 1.0f, x1 = 0.0f, y1 = 0.0f, z1 = 0.0f;   // empty
+    float x0 = __builtin_nanf(""), y0 = __builtin_nanf(""), z0 = __builtin_nanf("");
     int side = SIDE_NONE;
 };
 
 __device__ __forceinline__ int proven_side(const SubGrid& g, float iso, float px, float py, float pz, SideCache& sc)
 {
-    if (px >= sc.x0 && px < sc.x1 && py >= sc.y0 && py < sc.y1 && pz >= sc.z0 && pz < sc.z1) return sc.side;
+    if (px >= sc.x0 && px < sc.x0 + 4.0f && py >= sc.y0 && py < sc.y0 + 4.0f && pz >= sc.z0 && pz < sc.z0 + 4.0f)
+        return sc.side;
     if ((!g.mm && !g.sides) || !(px >= 0.0f) || !(py >= 0.0f) || !(pz >= 0.0f) || !coord_ok(px) || !coord_ok(py) ||
         !coord_ok(pz))
         return SIDE_NONE;
@@ -473,11 +474,28 @@ __device__ __forceinline__ int proven_side(const SubGrid& g, float iso, float px
     sc.x0 = (float)(g.cx0 + (cx & ~3));
     sc.y0 = (float)(g.cy0 + (cy & ~3));
     sc.z0 = (float)(g.cz0 + (cz & ~3));
-    sc.x1 = sc.x0 + 4.0f;
-    sc.y1 = sc.y0 + 4.0f;
-    sc.z1 = sc.z0 + 4.0f;
     sc.side = s;
     return s;
+}
+
+// Steps, at most `lim`, that certainly keep the position inside the cached
+// sub-block box when taken from X by the accumulation X += s (each add
+// rounded).  After m adds X_a differs from X_a + m s_a by at most m half-ulps
+// of values below |X_a| + 8; a ray crosses a 4-voxel box in at most 14 steps
+// of half a voxel, so a margin of 16 * 2^-22 * (|X| + 8) + 1e-5 voxel covers
+// the drift and the rounding of this estimate.  s = d / 2, so 1 / |s_a| is
+// 2 |1 / d_a|, from the ray's inverse directions.
+__device__ __forceinline__ int steps_inside(const SideCache& sc, const Ray& r, float X0, float X1, float X2, float s0,
+                                            float s1, float s2, int lim)
+{
+    const float inv0 = 2.0f * fabsf(r.ix), inv1 = 2.0f * fabsf(r.iy), inv2 = 2.0f * fabsf(r.iz);
+    const float m = fmaxf(fmaxf(fabsf(X0), fabsf(X1)), fabsf(X2)) + 8.0f;
+    const float margin = m * 3.814697265625e-06f + 1e-5f;
+    float kf = (float)lim;
+    if (s0 != 0.0f) kf = fminf(kf, ((s0 > 0.0f ? (sc.x0 + 4.0f) - X0 : X0 - sc.x0) - margin) * inv0);
+    if (s1 != 0.0f) kf = fminf(kf, ((s1 > 0.0f ? (sc.y0 + 4.0f) - X1 : X1 - sc.y0) - margin) * inv1);
+    if (s2 != 0.0f) kf = fminf(kf, ((s2 > 0.0f ? (sc.z0 + 4.0f) - X2 : X2 - sc.z0) - margin) * inv2);
+    return kf >= 1.0f ? (int)kf : 0;   // NaN gives 0
 }
 
 __device__ __forceinline__ bool march_node_skip(const Ray& r, int nx, int ny, int nz, int side,
@@ -511,6 +529,13 @@ __device__ __forceinline__ bool march_node_skip(const Ray& r, int nx, int ny, in
         X0 = X0 + s0; X1 = X1 + s1; X2 = X2 + s2;
         int sd = proven_side(g, iso, X0, X1, X2, sc);
         if (sd != SIDE_NONE && sd == ant_side) {   // same strict side: the reference's "ant = sig"
+            // the next steps inside the same sub-block take this branch too:
+            // accumulate their positions without the box test
+            const int k = steps_inside(sc, r, X0, X1, X2, s0, s1, s2, maxStep - steps);
+            for (int q = 0; q < k; ++q) {
+                X0 = X0 + s0; X1 = X1 + s1; X2 = X2 + s2;
+            }
+            steps += k;
             F0 = X0; F1 = X1; F2 = X2;
             ant_known = false;
             continue;
