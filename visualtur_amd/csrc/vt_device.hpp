@@ -274,8 +274,49 @@ __device__ __forceinline__ uint32_t sorted_children_cached(const Ray& r, const O
     return ord;
 }
 
+// The render passes keep every level's list: the sorted children of the
+// path's ancestor at level l sit in LDS entry l of the lane's column (parent
+// code, order; code 0 = empty), so each internal node's list is computed
+// once per ray and pass however deep the walk goes below it.  Parent levels
+// are below 9 (levelO <= 9), so their codes fit 32 bits.  The last list used
+// is also kept in registers (sibling steps reuse it without an LDS read).
+constexpr int kWalkLevels = 9;
+
+struct LevelCache {
+    uint2* col;          // entry l at col[l * stride]
+    int stride;
+    uint64_t p = 0;      // last parent (0 = none)
+    uint32_t ord = 0;
+};
+
+__device__ __forceinline__ void level_cache_clear(LevelCache& c)
+{
+#pragma unroll
+    for (int l = 0; l < kWalkLevels; ++l) c.col[l * c.stride] = make_uint2(0u, 0u);
+    c.p = 0;
+}
+
+__device__ __forceinline__ uint32_t sorted_children_cached(const Ray& r, const OctreeView& o, uint64_t parent,
+                                                           int plevel, LevelCache& c)
+{
+    if (parent == c.p) return c.ord;
+    uint2* e = c.col + plevel * c.stride;
+    const uint2 v = *e;
+    uint32_t ord;
+    if (v.x == (uint32_t)parent) {
+        ord = v.y;
+    } else {
+        ord = sorted_children(r, o, parent, plevel);
+        *e = make_uint2((uint32_t)parent, ord);
+    }
+    c.p = parent;
+    c.ord = ord;
+    return ord;
+}
+
+template <class Cache>
 __device__ __forceinline__ uint64_t next_node(const Ray& r, const OctreeView& o,
-                                              uint64_t node, bool descend, OrderCache& cache)
+                                              uint64_t node, bool descend, Cache& cache)
 {
     int level = code_level(node);
     for (;;) {

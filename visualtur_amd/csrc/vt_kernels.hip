@@ -292,9 +292,15 @@ __device__ __forceinline__ bool visit_node_g(const Ray& r, const OctreeView& oct
 // pass, and the per-ray state is indexed by id with deferral lists of
 // a.defer_stride rays; otherwise the frame is a.cam and the first pass is
 // the one without an input list.
+// Render-pass workgroup: kPassBlock/64 waves, each rendering one 8x8 pixel
+// tile (the tiles of a workgroup are consecutive in row-major tile order).
+// Each lane has a column of kWalkLevels entries in the workgroup's LDS walk
+// cache (entry l at l * kPassBlock).
+constexpr int kPassBlock = 64;
+
 template <bool kWide, bool kMulti>
 __device__ __forceinline__ void pass_ray(const RenderArgs& a, uint32_t id, bool active, bool first_region,
-                                         const Lanes& g)
+                                         const Lanes& g, uint2* walk_col)
 {
     const uint32_t fslot = kMulti ? (id >> a.rshift) : 0u;
     const uint32_t pix = kMulti ? (id & ((1u << a.rshift) - 1u)) : id;
@@ -319,7 +325,8 @@ __device__ __forceinline__ void pass_ray(const RenderArgs& a, uint32_t id, bool 
         HitRecord h;
         uint64_t bidx = 0, node = 0;
         int nd = 0;
-        OrderCache oc;
+        LevelCache oc{walk_col, kPassBlock};
+        level_cache_clear(oc);
         // what the pass leaves: WALK_HIT/WALK_MISS = c/h to write, WALK_ABSENT =
         // park on `node`, kKeep = the output already holds the ray's result
         // (final when nd == 0), kBlocked = a deferred node is still absent
@@ -461,9 +468,6 @@ __device__ __forceinline__ void pass_ray(const RenderArgs& a, uint32_t id, bool 
     }
 }
 
-// Render-pass workgroup: kPassBlock/64 waves, each rendering one 8x8 pixel
-// tile (the tiles of a workgroup are consecutive in row-major tile order).
-constexpr int kPassBlock = 64;
 
 // at most 96 VGPRs: 5 waves per SIMD instead of 4, without spilling
 __global__ __launch_bounds__(kPassBlock, 5) void k_render_pass(RenderArgs a)
@@ -506,7 +510,8 @@ __global__ __launch_bounds__(kPassBlock, 5) void k_render_pass(RenderArgs a)
         id = active ? (uint32_t)(i * W + j) : 0;
     }
     const Lanes g{1, 0, (int)(threadIdx.x & 63), true};
-    pass_ray<false, false>(a, id, active, false, g);
+    __shared__ uint2 walk[kWalkLevels * kPassBlock];
+    pass_ray<false, false>(a, id, active, false, g, walk + threadIdx.x);
 }
 
 // A pass with 2^lg lanes per ray over groups of 64 rays: the first pass's
@@ -540,7 +545,8 @@ __global__ __launch_bounds__(kPassBlock) void k_render_wide(RenderArgs a, int lg
     }
     const int L = 1 << lg;
     const Lanes g{L, lane & (L - 1), lane & ~(L - 1), (lane & (L - 1)) == 0};
-    pass_ray<true, false>(a, id, active, false, g);
+    __shared__ uint2 walk[kWalkLevels * kPassBlock];
+    pass_ray<true, false>(a, id, active, false, g, walk + threadIdx.x);
 }
 
 // Multi-frame pass, one lane per ray: workgroups [0, list_blocks) take 64
@@ -575,7 +581,8 @@ __global__ __launch_bounds__(kPassBlock, 5) void k_render_multi(RenderArgs a)
         first = true;
     }
     const Lanes g{1, 0, (int)lane, true};
-    pass_ray<false, true>(a, id, active, first, g);
+    __shared__ uint2 walk[kWalkLevels * kPassBlock];
+    pass_ray<false, true>(a, id, active, first, g, walk + threadIdx.x);
 }
 
 // Multi-frame pass over in_list only, 2^lg lanes per ray (the later passes
@@ -592,7 +599,8 @@ __global__ __launch_bounds__(kPassBlock) void k_render_multi_wide(RenderArgs a, 
     const uint32_t id = active ? a.in_list[idx] : 0u;
     const int L = 1 << lg;
     const Lanes g{L, lane & (L - 1), lane & ~(L - 1), (lane & (L - 1)) == 0};
-    pass_ray<true, true>(a, id, active, false, g);
+    __shared__ uint2 walk[kWalkLevels * kPassBlock];
+    pass_ray<true, true>(a, id, active, false, g, walk + threadIdx.x);
 }
 
 // ------------------------------------------------- persistent render pass
