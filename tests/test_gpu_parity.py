@@ -41,6 +41,39 @@ def test_trace_nodes_matches_reference_dfs(vt, vto, ctx, n, level_o, seed):
             assert nodes[r, : min(cnt, K)].tolist() == exp, r
 
 
+@pytest.mark.parametrize("level_o", [3, 5])
+def test_rays_through_node_corners_and_edges(vt, vto, ctx, level_o):
+    """Rays whose direction components are equal in pairs or all three
+    cross node corners and edges, where children share tnear: the child
+    order falls back from the sorting network to the full (tnear, tfar,
+    index) order there.  Every emitted node sequence equals the oracle's
+    DFS, and frames equal the oracle's."""
+    vol = blobs_volume(32, 10, seed=19)
+    sc = Scene(vto, vol, 0.3, level_o, 2)
+    sc.setup_gpu(vt, ctx, slots=64)
+    K = 48
+    lines = [((-5.0, -5.0, -5.0), (1.0, 1.0, 1.0)), ((40.0, 40.0, 40.0), (-1.0, -1.0, -1.0)),
+             ((-3.0, -3.0, 7.25), (1.0, 1.0, 0.0)), ((16.0, -6.0, -6.0), (0.0, 1.0, 1.0)),
+             ((-8.0, 8.0, -8.0), (1.0, 0.0, 1.0)), ((36.0, -4.0, 36.0), (-1.0, 1.0, -1.0)),
+             ((8.0, 8.0, 8.0), (1.0, 1.0, 1.0)), ((-2.0, 30.0, -2.0), (1.0, -1.0, 1.0))]
+    for pos, look in lines:
+        look = np.asarray(look) / np.linalg.norm(look)
+        up = np.cross(look, [0.3, 0.5, 0.8])
+        up /= np.linalg.norm(up)
+        right = np.cross(look, up)
+        for W, H in [(1, 1), (5, 5)]:
+            cam = vt.make_camera(W, H, pos, look, up, right, fov_v_deg=10.0)
+            nodes, counts = ctx.trace_nodes(cam, level_o, K, 1 << 20)
+            dirs = vto.ray_dirs(cam)
+            for r in range(cam.W * cam.H):
+                exp, cnt = sc.O.traverse(cam.pos, dirs[r], level_o, max_out=K)
+                assert counts[r] == cnt, (pos, r)
+                assert nodes[r, : min(cnt, K)].tolist() == exp, (pos, r)
+            rg, hg, _ = sc.render_gpu(vt, ctx, cam)
+            rc, hc = sc.render_cpu(vto, cam)
+            assert_same(hg, rg, hc, rc)
+
+
 def test_degenerate_axis_aligned_rays(vt, vto, ctx):
     """Rays with zero direction components from grid-aligned origins give
     NaN slab distances; the kernel must follow the reference's literal

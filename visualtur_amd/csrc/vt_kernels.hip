@@ -587,7 +587,7 @@ __global__ __launch_bounds__(kPassBlock, 5) void k_render_multi(RenderArgs a)
 
 // Multi-frame pass over in_list only, 2^lg lanes per ray (the later passes
 // of frames in flight when no frame starts), grouped as in k_render_wide.
-__global__ __launch_bounds__(kPassBlock) void k_render_multi_wide(RenderArgs a, int lg)
+__global__ __launch_bounds__(kPassBlock, 4) void k_render_multi_wide(RenderArgs a, int lg)
 {
     const int lane = (int)(threadIdx.x & 63);
     const uint32_t b = blockIdx.x;
