@@ -1122,7 +1122,7 @@ def main() -> int:
     ap.add_argument("--max-defer", type=int, default=1,
                     help="absent-brick nodes a ray may walk past per pass (0 = park at the first)")
     ap.add_argument("--prefetch-min-prob", type=float, default=0.8)
-    ap.add_argument("--frames-in-flight", type=int, default=8, help="c3 serving leg: frames sharing launches")
+    ap.add_argument("--frames-in-flight", type=int, default=16, help="c3 serving leg: frames sharing launches")
     ap.add_argument("--train-paths", type=int, default=24)
     ap.add_argument("--train-frames", type=int, default=6)
     ap.add_argument("--train-steps", type=int, default=600)

@@ -74,6 +74,34 @@ def test_rays_through_node_corners_and_edges(vt, vto, ctx, level_o):
             assert_same(hg, rg, hc, rc)
 
 
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_tie_prone_rays_walk_like_the_reference(vt, vto, ctx, seed):
+    """Seeded cameras at lattice points (origins on node planes, inside and
+    outside the cube) looking along directions with equal components: the
+    walk meets equal plane times at many nodes, and every node sequence must
+    be the oracle's DFS."""
+    rng = np.random.default_rng(seed)
+    vol = blobs_volume(32, 12, seed=20 + seed)
+    sc = Scene(vto, vol, 0.25, 4, 2)
+    flat, sizes = flat_runs(sc.runs, 4)
+    ctx.octree_upload(flat, sizes, sc.n, 4)
+    K = 64
+    dirs_set = [(1, 1, 1), (1, 1, 0), (0, 1, 1), (1, 0, 1), (1, 2, 2), (2, 1, 0), (1, 1, 2)]
+    for t in range(12):
+        pos = rng.integers(-8, 41, 3).astype(np.float64)
+        look = np.asarray(dirs_set[int(rng.integers(len(dirs_set)))], np.float64) * rng.choice([-1.0, 1.0], 3)
+        look /= np.linalg.norm(look)
+        up = np.cross(look, [0.31, 0.57, 0.76])
+        up /= np.linalg.norm(up)
+        cam = vt.make_camera(9, 9, pos, look, up, np.cross(look, up), fov_v_deg=float(rng.uniform(5, 60)))
+        nodes, counts = ctx.trace_nodes(cam, 4, K, 1 << 20)
+        dirs = vto.ray_dirs(cam)
+        for r in range(cam.W * cam.H):
+            exp, cnt = sc.O.traverse(cam.pos, dirs[r], 4, max_out=K)
+            assert counts[r] == cnt, (t, r)
+            assert nodes[r, : min(cnt, K)].tolist() == exp, (t, r)
+
+
 def test_degenerate_axis_aligned_rays(vt, vto, ctx):
     """Rays with zero direction components from grid-aligned origins give
     NaN slab distances; the kernel must follow the reference's literal

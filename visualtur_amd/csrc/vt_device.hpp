@@ -274,46 +274,6 @@ __device__ __forceinline__ uint32_t sorted_children_cached(const Ray& r, const O
     return ord;
 }
 
-// The render passes keep every level's list: the sorted children of the
-// path's ancestor at level l sit in LDS entry l of the lane's column (parent
-// code, order; code 0 = empty), so each internal node's list is computed
-// once per ray and pass however deep the walk goes below it.  Parent levels
-// are below 9 (levelO <= 9), so their codes fit 32 bits.  The last list used
-// is also kept in registers (sibling steps reuse it without an LDS read).
-constexpr int kWalkLevels = 9;
-
-struct LevelCache {
-    uint2* col;          // entry l at col[l * stride]
-    int stride;
-    uint64_t p = 0;      // last parent (0 = none)
-    uint32_t ord = 0;
-};
-
-__device__ __forceinline__ void level_cache_clear(LevelCache& c)
-{
-#pragma unroll
-    for (int l = 0; l < kWalkLevels; ++l) c.col[l * c.stride] = make_uint2(0u, 0u);
-    c.p = 0;
-}
-
-__device__ __forceinline__ uint32_t sorted_children_cached(const Ray& r, const OctreeView& o, uint64_t parent,
-                                                           int plevel, LevelCache& c)
-{
-    if (parent == c.p) return c.ord;
-    uint2* e = c.col + plevel * c.stride;
-    const uint2 v = *e;
-    uint32_t ord;
-    if (v.x == (uint32_t)parent) {
-        ord = v.y;
-    } else {
-        ord = sorted_children(r, o, parent, plevel);
-        *e = make_uint2((uint32_t)parent, ord);
-    }
-    c.p = parent;
-    c.ord = ord;
-    return ord;
-}
-
 template <class Cache>
 __device__ __forceinline__ uint64_t next_node(const Ray& r, const OctreeView& o,
                                               uint64_t node, bool descend, Cache& cache)
@@ -353,6 +313,150 @@ __device__ __forceinline__ uint64_t next_node(const Ray& r, const OctreeView& o,
 {
     OrderCache cache;
     return next_node(r, o, node, descend, cache);
+}
+
+// The sorted child list without sorting.  Along one axis a parent's
+// children lie between three planes, whose times are the nine products
+// sorted_children computes (near, mid, far in ray order).  The children the
+// ray passes through, in time order, are the one holding the ray at
+// t0 = max(entry, 0) and then the neighbour across each mid-plane the ray
+// crosses before it leaves the parent.  When no two of the times involved
+// are equal (and none is NaN), every such child has an interval of positive
+// length, no other child has one, their tnear values are distinct, and
+// tnear order is time order: exactly the list sorted_children returns.  An
+// equality (a ray through a node edge or corner, or an origin on a plane)
+// reports !ok and the caller takes the sorted list; so does a step from a
+// child with no positive-length interval (only reached through that list).
+// Returns the first member child after `after` in that order (after < 0:
+// the first of all), or -1.
+__device__ __forceinline__ int seq_child(const Ray& r, const OctreeView& o, uint64_t parent, int plevel, int after,
+                                         bool& ok)
+{
+    ok = true;
+    const uint32_t members = child_mask(o, parent, plevel);
+    if (members == 0) return -1;
+    int px, py, pz;
+    node_min(parent, plevel, o.n, px, py, pz);
+    const int side = 1 << (o.n - plevel - 1);
+    const float x0 = ((float)px - r.ox) * r.ix, x1 = ((float)(px + side) - r.ox) * r.ix,
+                x2 = ((float)(px + 2 * side) - r.ox) * r.ix;
+    const float y0 = ((float)py - r.oy) * r.iy, y1 = ((float)(py + side) - r.oy) * r.iy,
+                y2 = ((float)(py + 2 * side) - r.oy) * r.iy;
+    const float z0 = ((float)pz - r.oz) * r.iz, z1 = ((float)(pz + side) - r.oz) * r.iz,
+                z2 = ((float)(pz + 2 * side) - r.oz) * r.iz;
+    if (x0 != x0 || x1 != x1 || x2 != x2 || y0 != y0 || y1 != y1 || y2 != y2 || z0 != z0 || z1 != z1 ||
+        z2 != z2) {
+        ok = false;
+        return -1;
+    }
+    // ray order per axis, as sorted_children picks tmin/tmax (r.i* >= 0)
+    const bool nx_ = !(r.ix >= 0), ny_ = !(r.iy >= 0), nz_ = !(r.iz >= 0);
+    const float fx = nx_ ? x0 : x2, fy = ny_ ? y0 : y2, fz = nz_ ? z0 : z2;   // far planes
+    int c;
+    if (after < 0) {
+        const float ex = nx_ ? x2 : x0, ey = ny_ ? y2 : y0, ez = nz_ ? z2 : z0;   // near planes
+        float t0 = fmaxf(fmaxf(ex, ey), ez);
+        t0 = t0 < 0.0f ? 0.0f : t0;
+        const float t1 = fminf(fminf(fx, fy), fz);
+        if (x1 == t0 || y1 == t0 || z1 == t0 || t1 == t0) {
+            ok = false;
+            return -1;
+        }
+        if (t1 < t0) return -1;   // the parent lies behind the origin
+        // a crossed mid-plane puts the ray in the far half (in ray order)
+        c = ((int)((x1 < t0) != nx_) << 2) | ((int)((y1 < t0) != ny_) << 1) | (int)((z1 < t0) != nz_);
+        if ((members >> c) & 1) return c;
+    } else {
+        // the child the walk stands on must lie on the ray for a positive
+        // time (after t = 0): one reached through the sorted list at an
+        // equality may only touch it, and the order after it is the list's
+        c = after;
+        const float ex = nx_ ? x2 : x0, ey = ny_ ? y2 : y0, ez = nz_ ? z2 : z0;
+        const bool hx = ((c >> 2) & 1) == (int)nx_, hy = ((c >> 1) & 1) == (int)ny_, hz = (c & 1) == (int)nz_;
+        const float tn = fmaxf(fmaxf(fmaxf(hx ? ex : x1, hy ? ey : y1), hz ? ez : z1), 0.0f);
+        const float tf = fminf(fminf(hx ? x1 : fx, hy ? y1 : fy), hz ? z1 : fz);
+        if (!(tn < tf)) {
+            ok = false;
+            return -1;
+        }
+    }
+    for (int q = 0; q < 3; ++q) {
+        // the child's exit: its far plane per axis (the mid-plane when the
+        // child is the near half in ray order)
+        const bool hx = ((c >> 2) & 1) == (int)nx_, hy = ((c >> 1) & 1) == (int)ny_, hz = (c & 1) == (int)nz_;
+        const float ox = hx ? x1 : fx, oy = hy ? y1 : fy, oz = hz ? z1 : fz;
+        const float m = fminf(fminf(ox, oy), oz);
+        if ((int)(ox == m) + (int)(oy == m) + (int)(oz == m) > 1) {
+            ok = false;
+            return -1;
+        }
+        if (ox == m) {
+            if (!hx) return -1;   // leaves the parent
+            c ^= 4;
+        } else if (oy == m) {
+            if (!hy) return -1;
+            c ^= 2;
+        } else {
+            if (!hz) return -1;
+            c ^= 1;
+        }
+        if ((members >> c) & 1) return c;
+    }
+    return -1;
+}
+
+// The child after k (k < 0: the first) in the sorted list: where seq_child
+// finds equal times.  Out of line, with the ray by value, so that the walk
+// keeps its registers for the common case.
+__device__ __noinline__ int sorted_child_after(float ox, float oy, float oz, float dx, float dy, float dz, float ix,
+                                               float iy, float iz, const OctreeView* o, uint64_t parent, int plevel,
+                                               int k)
+{
+    const Ray r{ox, oy, oz, dx, dy, dz, ix, iy, iz};
+    const uint32_t ord = sorted_children(r, *o, parent, plevel);
+    const int cnt = order_count(ord);
+    int p = 0;
+    if (k >= 0) {
+        while (p < cnt && order_at(ord, p) != k) ++p;
+        p += 1;
+    }
+    return p < cnt ? order_at(ord, p) : -1;
+}
+
+// next_node with the children taken in ray order by seq_child, and the
+// sorted list where seq_child finds equal times.
+__device__ __forceinline__ uint64_t next_node_seq(const Ray& r, const OctreeView& o, uint64_t node, bool descend)
+{
+    int level = code_level(node);
+    for (;;) {
+        if (descend && level == o.final_level) return node;
+        if (!descend && level == 0) return 0;
+        const uint64_t parent = descend ? node : node >> 3;
+        const int plevel = descend ? level : level - 1;
+        const int k = descend ? -1 : (int)(node & 7u);
+        bool ok;
+        int c = seq_child(r, o, parent, plevel, k, ok);
+        if (!ok) {
+            const uint32_t ord = sorted_children(r, o, parent, plevel);
+            const int cnt = order_count(ord);
+            int p = 0;
+            if (k >= 0) {
+                while (p < cnt && order_at(ord, p) != k) ++p;
+                p += 1;
+            }
+            c = p < cnt ? order_at(ord, p) : -1;
+        }
+        if (c >= 0) {
+            node = (parent << 3) | (uint64_t)c;
+            level = plevel + 1;
+            descend = true;
+        } else if (descend) {
+            descend = false;   // no child to walk: the node is done
+        } else {
+            node = parent;
+            level = plevel;
+        }
+    }
 }
 
 // Root acceptance of Octree.cu:729: level 0 holds node 1 and the ray meets

@@ -138,6 +138,8 @@ int frames_loop(vt_frames* f, std::deque<Pending>& waiting, std::vector<Slot>& s
     BrickCache& c = *ctx->cache;
     int cur = 0;
     uint32_t in_count = 0;   // parked rays of frames in flight, in list[cur]
+    // per frame slot, its parked rays (its list at slot << rshift in list[cur])
+    uint32_t in_counts[kMaxFrames] = {};
     bool first_launch = true;
     for (;;) {
         {
@@ -237,6 +239,11 @@ int frames_loop(vt_frames* f, std::deque<Pending>& waiting, std::vector<Slot>& s
         a.nfresh = nfresh;
         a.in_list = in_count ? f->list[cur].p : nullptr;
         a.in_count = in_count;
+        a.list_block0[0] = 0;
+        for (int k = 0; k < kMaxFrames; ++k) {
+            a.list_count[k] = k < f->max_frames ? in_counts[k] : 0u;
+            a.list_block0[k + 1] = a.list_block0[k] + (a.list_count[k] + 63) / 64;
+        }
         a.out_list = f->list[cur ^ 1].p;
         if (first_launch) {   // later launches follow a service that resets these
             HIP_TRY(hipMemsetAsync(ctx->d_ctr, 0, sizeof(FrameCounters), ctx->stream));
@@ -284,7 +291,11 @@ int frames_loop(vt_frames* f, std::deque<Pending>& waiting, std::vector<Slot>& s
         HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         f->stats.launches++;
         f->stats.ms_kernel += ms;
-        const uint32_t parked = ctx->h_ctr->parked_count;
+        uint32_t parked = 0;
+        for (int k = 0; k < f->max_frames; ++k) {
+            in_counts[k] = f->h_parked.p[k];
+            parked += in_counts[k];
+        }
         const uint32_t nreq = ctx->h_ctr->req_count;
         const uint32_t err = ctx->h_ctr->err;
         if (nreq > (uint32_t)c.nbricks) return fail(VT_E_STATE, "request list overflow");
@@ -379,7 +390,7 @@ extern "C" {
 int vt_frames_create(vt_context* ctx, int32_t max_frames, int64_t max_rays, vt_frames** out)
 {
     if (!ctx || !out) return fail(VT_E_ARG, "NULL argument");
-    if (max_frames < 1 || max_frames > kMaxFrames) return fail(VT_E_ARG, "max_frames must be in [1, 8]");
+    if (max_frames < 1 || max_frames > kMaxFrames) return fail(VT_E_ARG, "max_frames must be in [1, 16]");
     if (max_rays < 1 || max_rays > (1ll << 26)) return fail(VT_E_ARG, "max_rays must be in [1, 2^26]");
     if (!ctx->octree_ready) return fail(VT_E_STATE, "no octree uploaded");
     if (!ctx->cache) return fail(VT_E_STATE, "no brick cache");

@@ -65,7 +65,7 @@ struct RenderJob {
     float4* rgba;
     HitRecord* hits;
 };
-constexpr int kMaxFrames = 8;
+constexpr int kMaxFrames = 16;
 
 struct RenderArgs {
     Camera cam;
@@ -112,7 +112,12 @@ struct RenderArgs {
     int rshift;                  // ray id = frame slot << rshift | pixel
     uint32_t defer_stride;       // rays per deferral list (frame slots << rshift)
     int max_defer_late;          // deferral of the resumed rays (max_defer: of the first passes)
-    uint32_t* job_parked;        // per frame slot: rays parked in this launch
+    uint32_t* job_parked;        // per frame slot: rays parked in this launch (their count in its list)
+    // in_list/out_list hold one list per frame slot, at slot << rshift; the
+    // in_list groups of 64 entries of frame s are groups
+    // [list_block0[s], list_block0[s + 1]) of the launch
+    uint32_t list_count[kMaxFrames];
+    uint32_t list_block0[kMaxFrames + 1];
     int nfresh;
     uint32_t fresh_slot[kMaxFrames];
     uint32_t fresh_tile0[kMaxFrames + 1];   // first 8x8 tile of each fresh frame in the launch (prefix sums)

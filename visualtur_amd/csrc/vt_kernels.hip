@@ -286,23 +286,21 @@ __device__ __forceinline__ bool visit_node_g(const Ray& r, const OctreeView& oct
     return true;
 }
 
-// One ray of a render pass (ray `id`, when `active`; group-uniform).  With
-// kMulti the ray belongs to frame slot id >> a.rshift (its camera, shading
-// and outputs in a.jobs), `first` says whether this is its frame's first
-// pass, and the per-ray state is indexed by id with deferral lists of
-// a.defer_stride rays; otherwise the frame is a.cam and the first pass is
-// the one without an input list.
 // Render-pass workgroup: kPassBlock/64 waves, each rendering one 8x8 pixel
 // tile (the tiles of a workgroup are consecutive in row-major tile order).
-// Each lane has a column of kWalkLevels entries in the workgroup's LDS walk
-// cache (entry l at l * kPassBlock).
 constexpr int kPassBlock = 64;
+
+// One ray of a render pass (ray `id`, when `active`; group-uniform).  With
+// kMulti the ray belongs to frame slot `fslot` (wave-uniform; id = fslot <<
+// a.rshift | pixel; its camera, shading and outputs in a.jobs), `first` says
+// whether this is its frame's first pass, and the per-ray state is indexed
+// by id with deferral lists of a.defer_stride rays; otherwise the frame is
+// a.cam and the first pass is the one without an input list.
 
 template <bool kWide, bool kMulti>
 __device__ __forceinline__ void pass_ray(const RenderArgs& a, uint32_t id, bool active, bool first_region,
-                                         const Lanes& g, uint2* walk_col)
+                                         const Lanes& g, uint32_t fslot)
 {
-    const uint32_t fslot = kMulti ? (id >> a.rshift) : 0u;
     const uint32_t pix = kMulti ? (id & ((1u << a.rshift) - 1u)) : id;
     const Camera& cam = kMulti ? a.jobs[fslot].cam : a.cam;
     const int W = cam.W, H = cam.H;
@@ -325,8 +323,6 @@ __device__ __forceinline__ void pass_ray(const RenderArgs& a, uint32_t id, bool 
         HitRecord h;
         uint64_t bidx = 0, node = 0;
         int nd = 0;
-        LevelCache oc{walk_col, kPassBlock};
-        level_cache_clear(oc);
         // what the pass leaves: WALK_HIT/WALK_MISS = c/h to write, WALK_ABSENT =
         // park on `node`, kKeep = the output already holds the ray's result
         // (final when nd == 0), kBlocked = a deferred node is still absent
@@ -369,7 +365,7 @@ __device__ __forceinline__ void pass_ray(const RenderArgs& a, uint32_t id, bool 
                     break;
                 }
                 if (adv) {
-                    node = next_node(r, a.oct, node, descend, oc);
+                    node = next_node_seq(r, a.oct, node, descend);
                     adv = descend = false;
                 }
                 if (node == 0) {
@@ -439,20 +435,15 @@ __device__ __forceinline__ void pass_ray(const RenderArgs& a, uint32_t id, bool 
         }
     }
 
-    // parked rays: one atomic per wave
+    // parked rays: one atomic per wave (a multi-frame wave holds rays of one
+    // frame, which park in that frame's list)
     park = park && g.leader;
-    uint32_t pslot = wave_atomic_add(&a.ctr->parked_count, park);
-    if (park) a.out_list[pslot] = id;
     if (kMulti) {
-        // parked rays per frame: one atomic per frame present in the wave
-        unsigned long long left = __ballot(park);
-        while (left) {
-            const int l = __ffsll((long long)left) - 1;
-            const uint32_t s = (uint32_t)__shfl((int)fslot, l);
-            const unsigned long long m = __ballot(park && fslot == s);
-            if ((int)__lane_id() == l) atomicAdd(&a.job_parked[s], (uint32_t)__popcll(m));
-            left &= ~m;
-        }
+        const uint32_t pslot = wave_atomic_add(&a.job_parked[fslot], park);
+        if (park) a.out_list[(fslot << a.rshift) + pslot] = id;
+    } else {
+        const uint32_t pslot = wave_atomic_add(&a.ctr->parked_count, park);
+        if (park) a.out_list[pslot] = id;
     }
     if (err) atomicOr(&a.ctr->err, err);
     if (a.collect_stats) {
@@ -510,8 +501,7 @@ __global__ __launch_bounds__(kPassBlock, 5) void k_render_pass(RenderArgs a)
         id = active ? (uint32_t)(i * W + j) : 0;
     }
     const Lanes g{1, 0, (int)(threadIdx.x & 63), true};
-    __shared__ uint2 walk[kWalkLevels * kPassBlock];
-    pass_ray<false, false>(a, id, active, false, g, walk + threadIdx.x);
+    pass_ray<false, false>(a, id, active, false, g, 0u);
 }
 
 // A pass with 2^lg lanes per ray over groups of 64 rays: the first pass's
@@ -545,31 +535,42 @@ __global__ __launch_bounds__(kPassBlock) void k_render_wide(RenderArgs a, int lg
     }
     const int L = 1 << lg;
     const Lanes g{L, lane & (L - 1), lane & ~(L - 1), (lane & (L - 1)) == 0};
-    __shared__ uint2 walk[kWalkLevels * kPassBlock];
-    pass_ray<true, false>(a, id, active, false, g, walk + threadIdx.x);
+    pass_ray<true, false>(a, id, active, false, g, 0u);
 }
 
-// Multi-frame pass, one lane per ray: workgroups [0, list_blocks) take 64
-// consecutive in_list entries (the parked rays of frames in flight, started
-// first since they are the long walks), the rest take the 8x8 tiles of the
-// fresh frames in order (fresh_tile0 prefix sums), dealt round-robin to the
-// XCDs as in k_render_pass.
+// The frame of list group `grp` (64 consecutive entries of one frame's
+// parked list; list_block0 holds the prefix sums of the frames' groups) and
+// the entry's offset in that list.
+__device__ __forceinline__ uint32_t list_frame(const RenderArgs& a, uint32_t grp)
+{
+    uint32_t s = 0;
+    while (s + 1 < (uint32_t)kMaxFrames && grp >= a.list_block0[s + 1]) ++s;
+    return s;
+}
+
+// Multi-frame pass, one lane per ray: workgroups [0, list groups) take 64
+// consecutive entries of one frame's parked list (the later passes of the
+// frames in flight, started first since they are the long walks), the rest
+// take the 8x8 tiles of the fresh frames in order (fresh_tile0 prefix sums),
+// dealt round-robin to the XCDs as in k_render_pass.  Every wave holds rays
+// of one frame, so the frame's camera, shading and outputs are wave-uniform.
 __global__ __launch_bounds__(kPassBlock, 5) void k_render_multi(RenderArgs a)
 {
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t list_blocks = (a.in_count + 63) / 64;
-    uint32_t id = 0;
+    const uint32_t list_blocks = a.list_block0[kMaxFrames];
+    uint32_t id = 0, fs;
     bool active, first;
     if (blockIdx.x < list_blocks) {
-        const uint32_t idx = blockIdx.x * 64 + lane;
-        active = idx < a.in_count;
-        id = active ? a.in_list[idx] : 0u;
+        fs = list_frame(a, blockIdx.x);
+        const uint32_t off = (blockIdx.x - a.list_block0[fs]) * 64 + lane;
+        active = off < a.list_count[fs];
+        id = active ? a.in_list[(fs << a.rshift) + off] : 0u;
         first = false;
     } else {
         const uint32_t t = blockIdx.x - list_blocks;
         int k = 0;
         while (k + 1 < a.nfresh && t >= a.fresh_tile0[k + 1]) ++k;
-        const uint32_t fs = a.fresh_slot[k];
+        fs = a.fresh_slot[k];
         const Camera& cam = a.jobs[fs].cam;
         const int tx8 = (cam.W + 7) >> 3;
         const uint32_t tt = t - a.fresh_tile0[k];
@@ -581,8 +582,7 @@ __global__ __launch_bounds__(kPassBlock, 5) void k_render_multi(RenderArgs a)
         first = true;
     }
     const Lanes g{1, 0, (int)lane, true};
-    __shared__ uint2 walk[kWalkLevels * kPassBlock];
-    pass_ray<false, true>(a, id, active, first, g, walk + threadIdx.x);
+    pass_ray<false, true>(a, id, active, first, g, fs);
 }
 
 // Multi-frame pass over in_list only, 2^lg lanes per ray (the later passes
@@ -594,13 +594,13 @@ __global__ __launch_bounds__(kPassBlock, 4) void k_render_multi_wide(RenderArgs 
     const uint32_t grp = ((b >> 3) >> lg) * 8u + (b & 7u);
     const uint32_t w = (b >> 3) & ((1u << lg) - 1u);
     const uint32_t p = (w << (6 - lg)) + ((uint32_t)lane >> lg);
-    const uint32_t idx = grp * 64u + p;
-    const bool active = idx < a.in_count;
-    const uint32_t id = active ? a.in_list[idx] : 0u;
+    const uint32_t fs = list_frame(a, grp);
+    const uint32_t off = (grp - a.list_block0[fs]) * 64u + p;
+    const bool active = off < a.list_count[fs];
+    const uint32_t id = active ? a.in_list[(fs << a.rshift) + off] : 0u;
     const int L = 1 << lg;
     const Lanes g{L, lane & (L - 1), lane & ~(L - 1), (lane & (L - 1)) == 0};
-    __shared__ uint2 walk[kWalkLevels * kPassBlock];
-    pass_ray<true, true>(a, id, active, false, g, walk + threadIdx.x);
+    pass_ray<true, true>(a, id, active, false, g, fs);
 }
 
 // ------------------------------------------------- persistent render pass
@@ -655,7 +655,6 @@ __global__ __launch_bounds__(kBlock) void k_render_persistent(RenderArgs a)
     uint64_t node = 0;
     int nd = 0, k = 0, nprev = 0;
     uint8_t tent = 0;
-    OrderCache oc;
 
     for (;;) {
         // ---- refill idle lanes
@@ -701,11 +700,10 @@ __global__ __launch_bounds__(kBlock) void k_render_persistent(RenderArgs a)
                 }
                 if (valid) {
                     make_ray(a.cam, (int)(id / (uint32_t)W), (int)(id % (uint32_t)W), r);
-                    oc = OrderCache();
                     nd = 0;
                     k = 0;
                     if (first) {
-                        node = root_ok(r, a.oct) ? next_node(r, a.oct, 1ull, true, oc) : 0ull;
+                        node = root_ok(r, a.oct) ? next_node_seq(r, a.oct, 1ull, true) : 0ull;
                         nprev = 0;
                         tent = 0;
                         mode = LANE_WALK;
@@ -795,7 +793,7 @@ __global__ __launch_bounds__(kBlock) void k_render_persistent(RenderArgs a)
                     }
                 }
             }
-            if (advance) node = next_node(r, a.oct, target, false, oc);
+            if (advance) node = next_node_seq(r, a.oct, target, false);
         }
         const uint32_t pslot = wave_atomic_add(&a.ctr->parked_count, park);
         if (park) a.out_list[pslot] = id;
@@ -878,14 +876,15 @@ hipError_t launch_render_multi(const RenderArgs& a, int lg, hipStream_t s)
 {
     if (!a.jobs || a.nfresh < 0 || a.nfresh > kMaxFrames || lg < 0 || lg > 6) return hipErrorInvalidValue;
     const uint32_t tiles = a.nfresh > 0 ? a.fresh_tile0[a.nfresh] : 0u;
+    const uint32_t list_groups = a.list_block0[kMaxFrames];
     if (tiles == 0 && lg > 0) {
-        if (a.in_count == 0) return hipSuccess;
-        const uint64_t groups = ((uint64_t)a.in_count + 63) / 64;
+        if (list_groups == 0) return hipSuccess;
+        const uint64_t groups = list_groups;
         const unsigned grid = (unsigned)(((groups + 7) / 8 * 8) << lg);
         hipLaunchKernelGGL(k_render_multi_wide, dim3(grid), dim3(kPassBlock), 0, s, a, lg);
         return hipGetLastError();
     }
-    const uint64_t grid = ((uint64_t)a.in_count + 63) / 64 + tiles;
+    const uint64_t grid = (uint64_t)list_groups + tiles;
     if (grid == 0) return hipSuccess;
     hipLaunchKernelGGL(k_render_multi, dim3((unsigned)grid), dim3(kPassBlock), 0, s, a);
     return hipGetLastError();
@@ -904,8 +903,7 @@ __device__ __forceinline__ void serve_ray(const ServeJob& J, int i, int j, const
 {
     Ray r;
     make_ray(J.cam, i, j, r);
-    OrderCache oc;
-    uint64_t node = root_ok(r, J.oct) ? next_node(r, J.oct, 1ull, true, oc) : 0ull;
+    uint64_t node = root_ok(r, J.oct) ? next_node_seq(r, J.oct, 1ull, true) : 0ull;
     float4 c;
     HitRecord h;
     uint32_t ns = 0, nn = 0;
@@ -923,7 +921,7 @@ __device__ __forceinline__ void serve_ray(const ServeJob& J, int i, int j, const
         }
         if (visit_node_g<kWide>(r, J.oct, J.bv, J.sh, node, brick, slot, nullptr, nullptr, c, h, err, ns, nn, g))
             break;
-        node = next_node(r, J.oct, node, false, oc);
+        node = next_node_seq(r, J.oct, node, false);
     }
     if (g.leader) {
         const uint32_t id = (uint32_t)(i * J.cam.W + j);
@@ -984,12 +982,12 @@ __global__ __launch_bounds__(kBlock) void k_trace_nodes(TraceArgs a)
     make_ray(a.cam, i, j, r);
     int count = 0;
     if (root_ok(r, a.oct)) {
-        OrderCache oc;
-        uint64_t node = next_node(r, a.oct, 1ull, true, oc);
+        // the render passes' walker (ray-order children, the sorted list on ties)
+        uint64_t node = next_node_seq(r, a.oct, 1ull, true);
         while (node != 0 && count < a.cap_total) {
             if (count < a.max_nodes) a.nodes[(size_t)id * a.max_nodes + count] = node;
             count++;
-            node = next_node(r, a.oct, node, false, oc);
+            node = next_node_seq(r, a.oct, node, false);
         }
     }
     a.counts[id] = count;
@@ -1024,7 +1022,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_traverse(StageTraverseArgs a)
     if (c->state != ST_NOCUBE) return;
     uint8_t ts = a.trav[id];
     if (ts == TRAV_EXHAUSTED) return;
-    uint64_t node = (ts == TRAV_FRESH) ? next_node(r, a.oct, 1ull, true) : next_node(r, a.oct, c->id, false);
+    uint64_t node = (ts == TRAV_FRESH) ? next_node_seq(r, a.oct, 1ull, true) : next_node_seq(r, a.oct, c->id, false);
     if (node != 0) {
         c->id = node;
         c->state = ST_CUBE;
