@@ -883,11 +883,11 @@ def run_c5(ctx, args):
             kept["batched_alone"][k][1].tobytes() == kept["per_request_alone"][k][1].tobytes() and
             kept["batched_alone"][k][0].tobytes() == kept["per_request_alone"][k][0].tobytes()
             for k in kept["batched_alone"])
-    if args.frames_in_flight > 0:
+    if args.c5_frames_in_flight > 0:
         import threading
 
         cache_setup(gpu_slots)
-        J = args.frames_in_flight
+        J = args.c5_frames_in_flight
         win = J + 2
         ring = [(ctx.alloc(R * 16), ctx.alloc(R * 32)) for _ in range(win)]
         beside = {}
@@ -1126,6 +1126,7 @@ def main() -> int:
                     help="absent-brick nodes a ray may walk past per pass (0 = park at the first)")
     ap.add_argument("--prefetch-min-prob", type=float, default=0.8)
     ap.add_argument("--frames-in-flight", type=int, default=16, help="c3 serving leg: frames sharing launches")
+    ap.add_argument("--c5-frames-in-flight", type=int, default=4, help="c5 serving leg: frames sharing launches")
     ap.add_argument("--train-paths", type=int, default=24)
     ap.add_argument("--train-frames", type=int, default=6)
     ap.add_argument("--train-steps", type=int, default=600)
