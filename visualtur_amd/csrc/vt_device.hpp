@@ -29,6 +29,18 @@ __host__ __device__ __forceinline__ uint32_t compact3(uint64_t x)
     return (uint32_t)x;
 }
 
+// compact3 for codes below 2^30 (nodes of levels <= 9 without their
+// leading 1 bit), in 32-bit arithmetic: half the instructions
+__host__ __device__ __forceinline__ uint32_t compact3_30(uint32_t x)
+{
+    x &= 0x09249249u;
+    x = (x ^ (x >> 2)) & 0x030c30c3u;
+    x = (x ^ (x >> 4)) & 0x0300f00fu;
+    x = (x ^ (x >> 8)) & 0x030000ffu;
+    x = (x ^ (x >> 16)) & 0x000003ffu;
+    return x;
+}
+
 __host__ __device__ __forceinline__ uint64_t dilate3(uint32_t v)
 {
     uint64_t x = v & 0x1fffffu;
@@ -55,6 +67,16 @@ __host__ __device__ __forceinline__ void node_min(uint64_t c, int l, int n, int&
     x = (int)(compact3(m >> 2) << s);
     y = (int)(compact3(m >> 1) << s);
     z = (int)(compact3(m) << s);
+}
+
+// node_min for levels l <= 9 (the octree's levelO and everything above it)
+__host__ __device__ __forceinline__ void node_min9(uint64_t c, int l, int n, int& x, int& y, int& z)
+{
+    const uint32_t m = (uint32_t)c ^ (1u << (3 * l));
+    const int s = n - l;
+    x = (int)(compact3_30(m >> 2) << s);
+    y = (int)(compact3_30(m >> 1) << s);
+    z = (int)(compact3_30(m) << s);
 }
 
 // ------------------------------------------------------------------- rays
@@ -181,7 +203,7 @@ __device__ __forceinline__ uint32_t sorted_children(const Ray& r, const OctreeVi
     uint32_t members = child_mask(o, parent, plevel);
     if (members == 0) return 0;
     int px, py, pz;
-    node_min(parent, plevel, o.n, px, py, pz);
+    node_min9(parent, plevel, o.n, px, py, pz);
     int side = 1 << (o.n - plevel - 1);
     // The children's faces lie on three planes per axis (parent min, middle,
     // max).  slab_box computes (plane - origin) * inverse per face, so these
@@ -336,7 +358,7 @@ __device__ __forceinline__ int seq_child(const Ray& r, const OctreeView& o, uint
     const uint32_t members = child_mask(o, parent, plevel);
     if (members == 0) return -1;
     int px, py, pz;
-    node_min(parent, plevel, o.n, px, py, pz);
+    node_min9(parent, plevel, o.n, px, py, pz);
     const int side = 1 << (o.n - plevel - 1);
     const float x0 = ((float)px - r.ox) * r.ix, x1 = ((float)(px + side) - r.ox) * r.ix,
                 x2 = ((float)(px + 2 * side) - r.ox) * r.ix;
@@ -344,8 +366,12 @@ __device__ __forceinline__ int seq_child(const Ray& r, const OctreeView& o, uint
                 y2 = ((float)(py + 2 * side) - r.oy) * r.iy;
     const float z0 = ((float)pz - r.oz) * r.iz, z1 = ((float)(pz + side) - r.oz) * r.iz,
                 z2 = ((float)(pz + 2 * side) - r.oz) * r.iz;
-    if (x0 != x0 || x1 != x1 || x2 != x2 || y0 != y0 || y1 != y1 || y2 != y2 || z0 != z0 || z1 != z1 ||
-        z2 != z2) {
+    // a product is NaN only as 0 * inf: a zero direction component (its
+    // inverse infinite) with the origin on one of the planes
+    const bool inf_dir = !(fabsf(r.ix) < __builtin_inff()) || !(fabsf(r.iy) < __builtin_inff()) ||
+                         !(fabsf(r.iz) < __builtin_inff());
+    if (inf_dir && (x0 != x0 || x1 != x1 || x2 != x2 || y0 != y0 || y1 != y1 || y2 != y2 || z0 != z0 ||
+                    z1 != z1 || z2 != z2)) {
         ok = false;
         return -1;
     }

@@ -55,8 +55,8 @@ __device__ __forceinline__ bool visit_node(const Ray& r, const OctreeView& oct, 
     if (slot_ref) slot_ref[slot] = 1;
     if (brick_trace) brick_trace[brick - bv.brick_base] = 1;
     int nx, ny, nz, bx, by, bz;
-    node_min(node, levelO, n, nx, ny, nz);
-    node_min(brick, bv.level_c, n, bx, by, bz);
+    node_min9(node, levelO, n, nx, ny, nz);
+    node_min9(brick, bv.level_c, n, bx, by, bz);
     bx -= bv.inc; by -= bv.inc; bz -= bv.inc;
     const float* sp = bv.pool + (size_t)slot * bv.slot_elems;
     float Xx, Xy, Xz;
@@ -261,8 +261,8 @@ __device__ __forceinline__ bool visit_node_g(const Ray& r, const OctreeView& oct
         if (brick_trace) brick_trace[brick - bv.brick_base] = 1;
     }
     int nx, ny, nz, bx, by, bz;
-    node_min(node, levelO, n, nx, ny, nz);
-    node_min(brick, bv.level_c, n, bx, by, bz);
+    node_min9(node, levelO, n, nx, ny, nz);
+    node_min9(brick, bv.level_c, n, bx, by, bz);
     bx -= bv.inc; by -= bv.inc; bz -= bv.inc;
     const float* sp = bv.pool + (size_t)slot * bv.slot_elems;
     float Xx, Xy, Xz;
