@@ -156,9 +156,9 @@ def orbit(vt_mod, W, H, center, radius, deg, elev_deg=0.0, fov=45.0):
 def setup_c2(ctx, args) -> Workload:
     w = Workload()
     w.name = "c2"
-    w.desc = ("1024^3 value-noise fBm (6 oct, lac 2, gain 0.5), iso 0, HDF5 32^3 chunks, 32^3 bricks + 3-voxel "
-              "halo, device cache 25% (8192 of 32768 bricks), pinned host tier, 1920x1080, 45deg, orbit r=2.2 "
-              "extents, 5.6 deg/frame")
+    w.desc = ("1024^3 value-noise fBm (6 oct, lac 2, gain 0.5), iso 0, HDF5 of 32^3 bricks with a 3-voxel halo "
+              "(and the 32^3-chunked volume for the CPU baseline), device cache 25% (8192 of 32768 bricks), pinned "
+              "host tier, 1920x1080, 45deg, orbit r=2.2 extents, 5.6 deg/frame")
     dims = (1024, 1024, 1024)
     w.iso, w.level_o, w.level_c, w.inc = 0.0, 5, 5, 3
     t0 = time.perf_counter()
@@ -176,15 +176,28 @@ def setup_c2(ctx, args) -> Workload:
     t0 = time.perf_counter()
     ctx.volume_write_hdf5(d_vol, dims, h5, "value", 32)
     log(f"hdf5 written in {time.perf_counter() - t0:.2f}s")
+    # the same bricks as a brick-major file (32^3 bricks with their halo, one
+    # record each): the host tier's source unless --c2-source volume
+    from visualtur_amd import prefetch as pf
+
+    codes = pf.candidate_codes(w.runs[w.level_c])
+    h5b = os.path.join(w.tmpdir, "c2_bricks.h5")
+    t0 = time.perf_counter()
+    ctx.volume_write_bricks_hdf5(d_vol, dims, h5b, w.level_c, 1 << (w.n - w.level_c), w.inc, codes)
+    log(f"brick-major hdf5 written in {time.perf_counter() - t0:.2f}s ({codes.size} bricks)")
     d_vol.free()
     w.cube = 1 << (w.n - w.level_c)
     nbricks = 1 << (3 * w.level_c)
     w.slots = nbricks // 4
     ctx.cache_create(w.level_c, w.cube, w.inc, w.slots, nbricks)
-    ctx.cache_attach_hdf5(h5, "value")
+    if args.c2_source == "bricks":
+        ctx.cache_attach_hdf5_bricks(h5b)
+    else:
+        ctx.cache_attach_hdf5(h5, "value")
     t0 = time.perf_counter()
     ctx.cache_prefetch_host(wait=True)
-    log(f"host tier filled from hdf5 in {time.perf_counter() - t0:.2f}s")
+    fill_s = time.perf_counter() - t0
+    log(f"host tier filled from the {args.c2_source} file in {fill_s:.2f}s")
     w.W, w.H = 1920, 1080
     w.cameras = lambda k: orbit(vt, w.W, w.H, (512.0, 512.0, 512.0), 2.2 * 1024.0, 5.6 * k)
     w.prm = vt.render_params(w.level_o, w.level_c, w.iso)
@@ -192,6 +205,7 @@ def setup_c2(ctx, args) -> Workload:
     w.oracle_runs = lambda: (w.n, w.runs)
     w.cpu_cache = (h5, w.slots)   # the CPU baseline reads the same file through an LRU of the same capacity
     w.extra = {"volume": "1024^3", "bricks": nbricks, "cache_slots": w.slots, "host_tier_slots": nbricks,
+               "host_tier_source": f"{args.c2_source} file ({'one record per brick with halo' if args.c2_source == 'bricks' else '32^3 chunks'}), filled in {fill_s:.2f}s before timing",
                "frame": "1920x1080", "frames": "orbit"}
     return w
 
@@ -1126,6 +1140,8 @@ def main() -> int:
                     help="absent-brick nodes a ray may walk past per pass (0 = park at the first)")
     ap.add_argument("--prefetch-min-prob", type=float, default=0.8)
     ap.add_argument("--frames-in-flight", type=int, default=16, help="c3 serving leg: frames sharing launches")
+    ap.add_argument("--c2-source", default="bricks", choices=["bricks", "volume"],
+                    help="c2 host tier source: brick-major file or the 32^3-chunked volume file")
     ap.add_argument("--c5-frames-in-flight", type=int, default=4, help="c5 serving leg: frames sharing launches")
     ap.add_argument("--train-paths", type=int, default=24)
     ap.add_argument("--train-frames", type=int, default=6)

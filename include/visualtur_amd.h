@@ -148,6 +148,13 @@ int vt_cache_create(vt_context *ctx, int32_t levelC, int32_t cubeDim, int32_t cu
                     int64_t gpuSlots, int64_t cpuSlots);
 /* Brick source = HDF5 dataset (lruCache.cu:118 opens a FileManager). */
 int vt_cache_attach_hdf5(vt_context *ctx, const char *file, const char *dataset);
+/* Brick source = a brick-major file written by vt_volume_write_bricks_hdf5:
+ * each brick one record (no reference counterpart: the reference reads
+ * every brick as a box of the volume dataset, lruCache.cu:226-266 through
+ * FileManager.cu:86-167; the records hold exactly those boxes).  Its
+ * levelC/cubeDim/cubeInc must be the cache's; a brick the octree needs that
+ * the file lacks is VT_E_ARG. */
+int vt_cache_attach_hdf5_bricks(vt_context *ctx, const char *file, const char *dataset);
 /* Brick source = a volume already in HBM, C order [X][Y][Z] (misses are
  * assembled on the device; no host tier involved). */
 int vt_cache_attach_device_volume(vt_context *ctx, const float *d_vol, int32_t X, int32_t Y,
@@ -329,6 +336,17 @@ int vt_volume_generate(vt_context *ctx, const vt_gen_params *p, int64_t X, int64
 /* Write a device volume to an HDF5 dataset, x-slab by x-slab
  * (FileManager::CreateFile semantics, FileManager.cu:169-243); chunk 0 =
  * contiguous. */
+/* Write the bricks `codes` (ascending Morton codes at levelC) of a device
+ * volume as a brick-major HDF5 file: dataset `dataset` of shape (K, S, S, S),
+ * S = cubeDim + 2 cubeInc, record k = the zero-padded box [min - cubeInc,
+ * min - cubeInc + S) of brick codes[k] (what FileManager's box read returns
+ * for it, FileManager.cu:86-167), one record per chunk; `dataset`_codes
+ * holds the codes and attributes hold levelC, cubeDim, cubeInc and the
+ * volume dims.  (The configuration's "32^3 chunks with a halo": a brick is
+ * then one contiguous read instead of a box over up to eight chunks.) */
+int vt_volume_write_bricks_hdf5(vt_context *ctx, const float *d_vol, int64_t X, int64_t Y, int64_t Z, int32_t levelC,
+                                int32_t cubeDim, int32_t cubeInc, const uint64_t *codes, int64_t ncodes,
+                                const char *file, const char *dataset);
 int vt_volume_write_hdf5(vt_context *ctx, const float *d_vol, int64_t X, int64_t Y, int64_t Z,
                          const char *file, const char *dataset, int32_t chunk);
 

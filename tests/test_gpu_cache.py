@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from conftest import blobs_volume
+from gpu_scenes import assert_bits, orbit_camera
 
 pytestmark = pytest.mark.gpu
 
@@ -191,3 +192,96 @@ def test_brick_source_failure_is_reported_and_recoverable(vt, vto, ctx, tmp_path
         ctx.cache_destroy()
         d_rgba.free()
         d_hits.free()
+
+
+@pytest.mark.parametrize("tier", ["full", "on_demand"])
+def test_brick_major_file_equals_box_reads(vt, vto, ctx, tmp_path, tier):
+    """A brick-major file (one zero-padded brick with halo per record) holds
+    exactly FileManager's box read of each brick (FileManager.cu:86-167):
+    every slot loaded from it, through a host tier filled ahead and through
+    an on-demand one of 9 slots, equals the box read of the volume file, and
+    a frame rendered from it equals the oracle's."""
+    from visualtur_amd import prefetch as pf
+
+    dims = (64, 60, 50)
+    vol = blobs_volume(64, 10, seed=32, dims=dims)
+    h5 = tmp_path / "vol.h5"
+    vt.file_create(h5, "value", vol, chunk=16)
+    n, runs = vto.build_octree(vol, 0.5, literal=False)
+    level_c, cube, inc = 2, 16, 3
+    S = cube + 2 * inc
+    flat = np.concatenate(runs[: level_c + 1]).astype(np.uint64)
+    ctx.octree_upload(flat, [r.size for r in runs[: level_c + 1]], n, level_c)
+    codes = pf.candidate_codes(runs[level_c])
+    assert codes.size >= 16
+    d_vol = ctx.to_device(vol)
+    bricks = tmp_path / "bricks.h5"
+    ctx.volume_write_bricks_hdf5(d_vol, dims, bricks, level_c, cube, inc, codes)
+    d_vol.free()
+    nb = 8 ** level_c
+    ctx.cache_create(level_c, cube, inc, nb, nb if tier == "full" else 9)
+    try:
+        ctx.cache_attach_hdf5_bricks(bricks)
+        if tier == "full":
+            ctx.cache_prefetch_host(wait=True)
+        with vt.File(h5, "value") as f:
+            for code in codes:
+                ctx.cache_load_bricks([code])
+                mn = np.array(vto.decode(int(code), level_c, n), np.int64)
+                box = f.read_box(tuple(mn - inc), tuple(mn + cube + inc))
+                got = ctx.cache_read_brick(int(code), S)
+                assert got.view(np.uint32).tobytes() == box.view(np.uint32).tobytes(), (tier, int(code))
+        # a brick the file holds no record for
+        missing = next(c for c in range(nb, 2 * nb) if c not in set(int(x) for x in codes))
+        if tier == "on_demand":
+            with pytest.raises(vt.VTError):
+                ctx.cache_load_bricks([missing])
+        ctx.cache_clear()
+        # a frame through the brick-major source equals the oracle's
+        prm = vt.render_params(level_c, level_c, 0.5)
+        cam = orbit_camera(vt, 80, 64, (32, 30, 25), 110.0, 40.0)
+        R = cam.W * cam.H
+        rg, hg = ctx.alloc(R * 16), ctx.alloc(R * 32)
+        ctx.render_frame(cam, prm, rg, hg)
+        V, O = vto.Volume(vol), vto.Octree(runs, n, level_c)
+        rc, hc, _ = vto.render(V, O, cam, vto.params(level_c, level_c, 0.5))
+        assert_bits(hg.download(vt.HIT_DTYPE, (R,)), rg.download(np.float32, (R, 4)), hc, rc)
+        rg.free()
+        hg.free()
+    finally:
+        ctx.cache_destroy()
+
+
+def test_brick_major_file_refusals(vt, vto, ctx, tmp_path):
+    """A brick file whose layout differs from the cache's, or that lacks a
+    brick the octree needs, is refused; a missing file is VT_E_IO."""
+    from visualtur_amd import prefetch as pf
+
+    dims = (32, 32, 32)
+    vol = blobs_volume(32, 6, seed=33, dims=dims)
+    n, runs = vto.build_octree(vol, 0.5, literal=False)
+    flat = np.concatenate(runs[:2]).astype(np.uint64)
+    ctx.octree_upload(flat, [r.size for r in runs[:2]], n, 1)
+    codes = pf.candidate_codes(runs[1])
+    assert codes.size >= 2
+    d_vol = ctx.to_device(vol)
+    full, part = tmp_path / "full.h5", tmp_path / "part.h5"
+    ctx.volume_write_bricks_hdf5(d_vol, dims, full, 1, 16, 3, codes)
+    ctx.volume_write_bricks_hdf5(d_vol, dims, part, 1, 16, 3, codes[1:])
+    with pytest.raises(vt.VTError):   # codes not ascending
+        ctx.volume_write_bricks_hdf5(d_vol, dims, tmp_path / "bad.h5", 1, 16, 3, codes[::-1].copy())
+    d_vol.free()
+    try:
+        ctx.cache_create(1, 16, 2, 8, 8)
+        with pytest.raises(vt.VTError):   # cubeInc 3 in the file, 2 in the cache
+            ctx.cache_attach_hdf5_bricks(full)
+        ctx.cache_destroy()
+        ctx.cache_create(1, 16, 3, 8, 8)
+        with pytest.raises(vt.VTError) as e:
+            ctx.cache_attach_hdf5_bricks(tmp_path / "nope.h5")
+        assert e.value.code == -3
+        ctx.cache_attach_hdf5_bricks(part)
+        with pytest.raises(vt.VTError):   # the first brick with nodes has no record
+            ctx.cache_prefetch_host(wait=True)
+    finally:
+        ctx.cache_destroy()

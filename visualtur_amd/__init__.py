@@ -196,6 +196,10 @@ _SIGS = {
     "vt_file_read_box": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), _P]),
     "vt_file_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _P, ctypes.c_int64, ctypes.c_int64,
                                       ctypes.c_int64, ctypes.c_int32]),
+    "vt_volume_write_bricks_hdf5": (ctypes.c_int, [_P, _P, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                                   ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P, ctypes.c_int64,
+                                                   ctypes.c_char_p, ctypes.c_char_p]),
+    "vt_cache_attach_hdf5_bricks": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_char_p]),
     "vt_file_cut": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32,
                                    ctypes.c_int32, ctypes.c_int32]),
 }
@@ -407,6 +411,18 @@ class Context:
     def cache_attach_device_volume(self, vol: DeviceBuffer, dims) -> None:
         self._keep.append(vol)
         self._c(lib().vt_cache_attach_device_volume(self.handle, vol.ptr, *[int(d) for d in dims]))
+
+    def cache_attach_hdf5_bricks(self, file: str, dataset: str = "bricks") -> None:
+        """Brick source = a brick-major file (volume_write_bricks_hdf5)."""
+        self._c(lib().vt_cache_attach_hdf5_bricks(self.handle, str(file).encode(), dataset.encode()))
+
+    def volume_write_bricks_hdf5(self, d_vol: DeviceBuffer, dims, path, level_c: int, cube: int, inc: int, codes,
+                                 dataset: str = "bricks") -> None:
+        """The bricks `codes` (ascending level-level_c codes) of a device volume
+        as a brick-major HDF5 file, one zero-padded brick with halo per record."""
+        c = np.ascontiguousarray(codes, np.uint64)
+        self._c(lib().vt_volume_write_bricks_hdf5(self.handle, d_vol.ptr, *[int(d) for d in dims], level_c, cube,
+                                                  inc, c.ctypes.data, c.size, str(path).encode(), dataset.encode()))
 
     def cache_attach_host_from_device(self, vol: DeviceBuffer, dims) -> None:
         """A pinned host tier holding the bricks with nodes, filled from a device volume."""

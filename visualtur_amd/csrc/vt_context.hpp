@@ -100,6 +100,12 @@ struct HostTier {
     float* pool_dev = nullptr;    // the same memory as seen by kernels
     std::string file, dataset;
     H5Volume sync_reader;         // on-demand reads (service thread)
+    // brick-major source (H5BrickReader layout): record of each brick index,
+    // -1 when the file holds none
+    bool bricked = false;
+    std::vector<int64_t> rec_of;
+    H5BrickReader brick_sync;
+    int64_t recs_done = 0;        // records the reader has passed (full mode)
     std::vector<int32_t> slot_of; // brick index -> host slot
     std::vector<int64_t> brick_of;
     std::vector<uint8_t> ref;
@@ -133,8 +139,47 @@ struct HostTier {
                dilate3((uint32_t)bz);
     }
 
+    void reader_failure(const std::string& e)
+    {
+        std::lock_guard<std::mutex> g(m);
+        reader_failed = true;
+        reader_err = e;
+        cv.notify_all();
+    }
+
+    // full mode from a brick-major file: records in file order, each read
+    // straight into its host slot
+    void run_prefetch_bricked()
+    {
+        H5BrickReader rd;
+        std::string err;
+        if (!rd.open(file, dataset, err)) return reader_failure(err);
+        const uint64_t base = 1ull << (3 * level_c);
+        const int64_t K = (int64_t)rd.codes.size();
+        int64_t placed = 0;
+        for (int64_t k = 0; k < K; ++k) {
+            const int64_t bidx = (int64_t)(rd.codes[(size_t)k] - base);
+            if (slot_of[(size_t)bidx] >= 0) {
+                if (!rd.read(k, pool + (size_t)slot_of[(size_t)bidx] * slot_elems, err)) return reader_failure(err);
+                placed++;
+            }
+            if ((k & 63) == 63 || k + 1 == K) {
+                {
+                    std::lock_guard<std::mutex> g(m);
+                    recs_done = k + 1;
+                    file_reads += placed;
+                    filled += placed;
+                    placed = 0;
+                    if (k + 1 == K) complete.store(true, std::memory_order_release);
+                }
+                cv.notify_all();
+            }
+        }
+    }
+
     void run_prefetch()
     {
+        if (bricked) return run_prefetch_bricked();
         H5Volume vol;
         std::string err;
         if (!vol.open(file, dataset, 0, err)) {
@@ -190,7 +235,21 @@ struct HostTier {
     {
         started = true;
         n = n_levels;
-        if (capacity >= nbricks) {
+        if (bricked && runs_c) {
+            // a brick-major file is read ahead only for the bricks with
+            // nodes, and only when it holds a record for each of them
+            const uint64_t base = 1ull << (3 * level_c);
+            int64_t count = 0;
+            for (size_t j = 0; j + 1 < runs_c->size(); j += 2)
+                for (uint64_t code = (*runs_c)[j]; code <= (*runs_c)[j + 1]; ++code, ++count)
+                    if (rec_of[(size_t)(code - base)] < 0)
+                        return fail(VT_E_ARG, "the brick file holds no record for brick code " + std::to_string(code) +
+                                                  ", which the octree needs");
+            if (count > capacity) return VT_OK;   // on demand
+        } else if (bricked) {
+            return VT_OK;   // on demand
+        }
+        if (capacity >= nbricks && !bricked) {
             full_mode = true;
             for (int64_t b = 0; b < nbricks; ++b) { slot_of[b] = (int32_t)b; brick_of[b] = b; }
             used = nbricks;
@@ -220,6 +279,15 @@ struct HostTier {
             *out = pool_dev + (size_t)slot_of[bidx] * slot_elems;
             return VT_OK;
         }
+        if (full_mode && bricked) {
+            if (slot_of[bidx] < 0) return fail(VT_E_ARG, "requested brick holds no octree node");
+            const int64_t rec = rec_of[(size_t)bidx];
+            std::unique_lock<std::mutex> lk(m);
+            cv.wait(lk, [&] { return reader_failed || recs_done > rec; });
+            if (reader_failed) return fail(VT_E_IO, "host tier reader: " + reader_err);
+            *out = pool_dev + (size_t)slot_of[bidx] * slot_elems;
+            return VT_OK;
+        }
         if (full_mode) {
             int32_t code_bx;
             {
@@ -242,7 +310,14 @@ struct HostTier {
             return VT_OK;
         }
         // the file first: a source that cannot be opened takes no slot
-        if (!sync_reader.is_open()) {
+        if (bricked) {
+            if (rec_of[(size_t)bidx] < 0)
+                return fail(VT_E_ARG, "the brick file holds no record for brick index " + std::to_string(bidx));
+            if (!brick_sync.is_open()) {
+                std::string err;
+                if (!brick_sync.open(file, dataset, err)) return fail(VT_E_IO, err);
+            }
+        } else if (!sync_reader.is_open()) {
             std::string err;
             if (!sync_reader.open(file, dataset, 64ull << 20, err)) return fail(VT_E_IO, err);
         }
@@ -266,12 +341,16 @@ struct HostTier {
             slot_of[(size_t)brick_of[slot]] = -1;
             brick_of[slot] = -1;
         }
-        int x, y, z;
-        node_min(bidx + (1ull << (3 * level_c)), level_c, n, x, y, z);
-        int s3[3] = {x - inc, y - inc, z - inc};
-        int e3[3] = {x - inc + S, y - inc + S, z - inc + S};
         std::string err;
-        if (!sync_reader.read_box(s3, e3, pool + (size_t)slot * slot_elems, err)) return fail(VT_E_IO, err);
+        if (bricked) {
+            if (!brick_sync.read(rec_of[(size_t)bidx], pool + (size_t)slot * slot_elems, err)) return fail(VT_E_IO, err);
+        } else {
+            int x, y, z;
+            node_min(bidx + (1ull << (3 * level_c)), level_c, n, x, y, z);
+            int s3[3] = {x - inc, y - inc, z - inc};
+            int e3[3] = {x - inc + S, y - inc + S, z - inc + S};
+            if (!sync_reader.read_box(s3, e3, pool + (size_t)slot * slot_elems, err)) return fail(VT_E_IO, err);
+        }
         brick_of[slot] = bidx;
         slot_of[bidx] = (int32_t)slot;
         ref[slot] = 1;
@@ -455,7 +534,7 @@ int ensure_sides(vt_context* ctx, float iso);
 int publish_loads(vt_context* ctx, hipStream_t s);
 int make_reserve(vt_context* ctx, int64_t reserve);
 int load_during_pass(vt_context* ctx, hipEvent_t done, int64_t* loaded, int64_t* misses);
-void host_tier_start(vt_context* ctx);
+int host_tier_start(vt_context* ctx);
 struct vt_mlp;
 // vt_prefetch's scoring, selection and loads; without `publish` the copies'
 // sub-block min/max and page-table entries wait for publish_loads, so a

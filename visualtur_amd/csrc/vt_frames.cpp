@@ -413,7 +413,7 @@ int vt_frames_create(vt_context* ctx, int32_t max_frames, int64_t max_rays, vt_f
     HIP_TRY(hipMemset(f->d_jobs.p, 0, (size_t)max_frames * sizeof(RenderJob)));
     BrickCache& c = *ctx->cache;
     c.n = ctx->n;
-    host_tier_start(ctx);
+    if (int rc = host_tier_start(ctx)) return rc;
     f->st_t0 = now_ms();
     vt_frames* raw = f.release();
     raw->worker = std::thread(frames_worker, raw);

@@ -157,4 +157,176 @@ bool H5Writer::close(std::string& err)
     return ok;
 }
 
+// ------------------------------------------------------ brick-major files
+
+namespace {
+
+bool put_attr_i64(hid_t obj, const char* name, const int64_t* v, hsize_t n)
+{
+    hid_t sp = H5Screate_simple(1, &n, nullptr);
+    hid_t a = H5Acreate2(obj, name, H5T_STD_I64LE, sp, H5P_DEFAULT, H5P_DEFAULT);
+    const bool ok = a >= 0 && H5Awrite(a, H5T_NATIVE_INT64, v) >= 0;
+    if (a >= 0) H5Aclose(a);
+    H5Sclose(sp);
+    return ok;
+}
+
+bool get_attr_i64(hid_t obj, const char* name, int64_t* v, hsize_t n)
+{
+    if (H5Aexists(obj, name) <= 0) return false;
+    hid_t a = H5Aopen(obj, name, H5P_DEFAULT);
+    if (a < 0) return false;
+    hid_t sp = H5Aget_space(a);
+    const bool ok = H5Sget_simple_extent_npoints(sp) == (hssize_t)n && H5Aread(a, H5T_NATIVE_INT64, v) >= 0;
+    H5Sclose(sp);
+    H5Aclose(a);
+    return ok;
+}
+
+}  // namespace
+
+H5BrickWriter::~H5BrickWriter()
+{
+    std::string e;
+    close(e);
+}
+
+bool H5BrickWriter::create(const std::string& file, const std::string& name, const uint64_t* codes, int64_t K,
+                           int level_c, int cube, int inc, const int64_t dims[3], std::string& err)
+{
+    ErrSilencer quiet;
+    if (K <= 0) { err = "no bricks to write"; return false; }
+    hid_t f = H5Fcreate(file.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+    if (f < 0) { err = "unable to create the HDF5 file: " + file; return false; }
+    file_ = f;
+    S_ = cube + 2 * inc;
+    K_ = K;
+    hsize_t d[4] = {(hsize_t)K, (hsize_t)S_, (hsize_t)S_, (hsize_t)S_};
+    hid_t sp = H5Screate_simple(4, d, nullptr);
+    space_ = sp;
+    hid_t dcpl = H5Pcreate(H5P_DATASET_CREATE);
+    hsize_t c[4] = {1, (hsize_t)S_, (hsize_t)S_, (hsize_t)S_};
+    H5Pset_chunk(dcpl, 4, c);
+    hid_t ds = H5Dcreate2(f, name.c_str(), H5T_IEEE_F32LE, sp, H5P_DEFAULT, dcpl, H5P_DEFAULT);
+    H5Pclose(dcpl);
+    if (ds < 0) { err = "unable to create the brick data set"; return false; }
+    dset_ = ds;
+    const int64_t meta[3] = {level_c, cube, inc};
+    if (!put_attr_i64(ds, "levelC_cubeDim_cubeInc", meta, 3) || !put_attr_i64(ds, "dims", dims, 3)) {
+        err = "unable to write the brick attributes";
+        return false;
+    }
+    hsize_t kk = (hsize_t)K;
+    hid_t csp = H5Screate_simple(1, &kk, nullptr);
+    hid_t cds = H5Dcreate2(f, (name + "_codes").c_str(), H5T_STD_U64LE, csp, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+    const bool ok = cds >= 0 && H5Dwrite(cds, H5T_NATIVE_UINT64, H5S_ALL, H5S_ALL, H5P_DEFAULT, codes) >= 0;
+    if (cds >= 0) H5Dclose(cds);
+    H5Sclose(csp);
+    if (!ok) { err = "unable to write the brick codes"; return false; }
+    return true;
+}
+
+bool H5BrickWriter::write(int64_t k0, int64_t n, const float* data, std::string& err)
+{
+    if (dset_ < 0) { err = "writer not open"; return false; }
+    if (k0 < 0 || n <= 0 || k0 + n > K_) { err = "records outside the data set"; return false; }
+    ErrSilencer quiet;
+    hsize_t off[4] = {(hsize_t)k0, 0, 0, 0};
+    hsize_t cnt[4] = {(hsize_t)n, (hsize_t)S_, (hsize_t)S_, (hsize_t)S_};
+    hid_t fsp = H5Scopy((hid_t)space_);
+    H5Sselect_hyperslab(fsp, H5S_SELECT_SET, off, nullptr, cnt, nullptr);
+    hid_t msp = H5Screate_simple(4, cnt, nullptr);
+    herr_t st = H5Dwrite((hid_t)dset_, H5T_IEEE_F32LE, msp, fsp, H5P_DEFAULT, data);
+    H5Sclose(msp);
+    H5Sclose(fsp);
+    if (st < 0) { err = "unable to write bricks"; return false; }
+    return true;
+}
+
+bool H5BrickWriter::close(std::string& err)
+{
+    bool ok = true;
+    if (dset_ >= 0 && H5Dclose((hid_t)dset_) < 0) { err = "unable to close the data set"; ok = false; }
+    if (space_ >= 0) H5Sclose((hid_t)space_);
+    if (file_ >= 0 && H5Fclose((hid_t)file_) < 0) { err = "unable to close the file"; ok = false; }
+    dset_ = space_ = file_ = -1;
+    return ok;
+}
+
+H5BrickReader::~H5BrickReader() { close(); }
+
+void H5BrickReader::close()
+{
+    if (space_ >= 0) H5Sclose((hid_t)space_);
+    if (dset_ >= 0) H5Dclose((hid_t)dset_);
+    if (file_ >= 0) H5Fclose((hid_t)file_);
+    space_ = dset_ = file_ = -1;
+}
+
+bool H5BrickReader::open(const std::string& file, const std::string& name, std::string& err)
+{
+    ErrSilencer quiet;
+    close();
+    hid_t f = H5Fopen(file.c_str(), H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (f < 0) { err = "unable to open the HDF5 file: " + file; return false; }
+    file_ = f;
+    hid_t d = H5Dopen2(f, name.c_str(), H5P_DEFAULT);
+    if (d < 0) { err = "unable to open the brick data set " + name; return false; }
+    dset_ = d;
+    hid_t sp = H5Dget_space(d);
+    space_ = sp;
+    hsize_t dd[4] = {0, 0, 0, 0};
+    if (H5Sget_simple_extent_ndims(sp) != 4 || H5Sget_simple_extent_dims(sp, dd, nullptr) != 4 || dd[1] != dd[2] ||
+        dd[1] != dd[3]) {
+        err = "the brick data set is not (K, S, S, S)";
+        return false;
+    }
+    int64_t meta[3];
+    if (!get_attr_i64(d, "levelC_cubeDim_cubeInc", meta, 3) || !get_attr_i64(d, "dims", dims, 3)) {
+        err = "the brick data set lacks its levelC/cubeDim/cubeInc and dims attributes";
+        return false;
+    }
+    level_c = (int)meta[0];
+    cube = (int)meta[1];
+    inc = (int)meta[2];
+    S = cube + 2 * inc;
+    if ((hsize_t)S != dd[1]) { err = "brick side does not match cubeDim + 2 cubeInc"; return false; }
+    hid_t c = H5Dopen2(f, (name + "_codes").c_str(), H5P_DEFAULT);
+    if (c < 0) { err = "unable to open the brick codes " + name + "_codes"; return false; }
+    hid_t csp = H5Dget_space(c);
+    const hssize_t K = H5Sget_simple_extent_npoints(csp);
+    H5Sclose(csp);
+    bool ok = K == (hssize_t)dd[0];
+    if (ok) {
+        codes.assign((size_t)K, 0);
+        ok = H5Dread(c, H5T_NATIVE_UINT64, H5S_ALL, H5S_ALL, H5P_DEFAULT, codes.data()) >= 0;
+    }
+    H5Dclose(c);
+    if (!ok) { err = "unable to read the brick codes"; return false; }
+    const uint64_t lo = 1ull << (3 * level_c), hi = 1ull << (3 * level_c + 3);
+    for (size_t k = 0; k < codes.size(); ++k)
+        if (codes[k] < lo || codes[k] >= hi || (k > 0 && codes[k] <= codes[k - 1])) {
+            err = "brick codes are not ascending codes of level " + std::to_string(level_c);
+            return false;
+        }
+    return true;
+}
+
+bool H5BrickReader::read(int64_t k, float* out, std::string& err)
+{
+    if (dset_ < 0) { err = "brick file not open"; return false; }
+    if (k < 0 || k >= (int64_t)codes.size()) { err = "record outside the brick data set"; return false; }
+    ErrSilencer quiet;
+    hsize_t off[4] = {(hsize_t)k, 0, 0, 0};
+    hsize_t cnt[4] = {1, (hsize_t)S, (hsize_t)S, (hsize_t)S};
+    hid_t fsp = H5Scopy((hid_t)space_);
+    H5Sselect_hyperslab(fsp, H5S_SELECT_SET, off, nullptr, cnt, nullptr);
+    hid_t msp = H5Screate_simple(4, cnt, nullptr);
+    herr_t st = H5Dread((hid_t)dset_, H5T_IEEE_F32LE, msp, fsp, H5P_DEFAULT, out);
+    H5Sclose(msp);
+    H5Sclose(fsp);
+    if (st < 0) { err = "unable to read brick record " + std::to_string(k); return false; }
+    return true;
+}
+
 }  // namespace vt

@@ -2,6 +2,7 @@
 #pragma once
 #include <stdint.h>
 #include <string>
+#include <vector>
 
 namespace vt {
 
@@ -50,6 +51,41 @@ public:
 private:
     int64_t file_ = -1, dset_ = -1, space_ = -1;
     int64_t dims_[3] = {0, 0, 0};
+};
+
+// A brick-major volume file: dataset `name` of shape (K, S, S, S) float32,
+// record k the brick of Morton code codes[k] at levelC (its cube plus an
+// inc-voxel halo, zero outside the volume: the FileManager box read of
+// [min - inc, min - inc + S)), one record per chunk, so that a brick is one
+// contiguous read; dataset `name`_codes holds the K codes (ascending), and
+// attributes on `name` hold levelC, cubeDim, cubeInc and the volume dims.
+class H5BrickWriter {
+public:
+    ~H5BrickWriter();
+    bool create(const std::string& file, const std::string& name, const uint64_t* codes, int64_t K, int level_c,
+                int cube, int inc, const int64_t dims[3], std::string& err);
+    bool write(int64_t k0, int64_t n, const float* data, std::string& err);   // records [k0, k0 + n)
+    bool close(std::string& err);
+
+private:
+    int64_t file_ = -1, dset_ = -1, space_ = -1;
+    int64_t K_ = 0;
+    int S_ = 0;
+};
+
+class H5BrickReader {
+public:
+    ~H5BrickReader();
+    bool open(const std::string& file, const std::string& name, std::string& err);
+    bool read(int64_t k, float* out, std::string& err);   // record k, S^3 floats
+    void close();
+    bool is_open() const { return dset_ >= 0; }
+    int level_c = 0, cube = 0, inc = 0, S = 0;
+    int64_t dims[3] = {0, 0, 0};
+    std::vector<uint64_t> codes;
+
+private:
+    int64_t file_ = -1, dset_ = -1, space_ = -1;
 };
 
 }  // namespace vt

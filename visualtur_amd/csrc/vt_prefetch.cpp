@@ -342,7 +342,7 @@ int prefetch_bricks(vt_context* ctx, vt_mlp* m, const vt_camera* cam, int64_t bu
     if (budget < 0 || min_prob < 0.0f || min_prob >= 1.0f) return fail(VT_E_ARG, "bad budget or threshold");
     HIP_TRY(hipSetDevice(ctx->device));
     c.n = ctx->n;
-    host_tier_start(ctx);
+    if (int rc = host_tier_start(ctx)) return rc;
     hipStream_t s = ctx->stream;
     // candidates: the bricks that hold octree nodes (the level-levelC run list)
     if (ctx->cand_level != c.level_c) {

@@ -92,12 +92,12 @@ int load_bricks(vt_context* ctx, const uint32_t* bricks, int64_t n_load, int64_t
 // Starts the host tier once the brick geometry is known; with the octree
 // uploaded down to levelC, a tier that holds every brick with octree nodes
 // is filled ahead by the reader thread.
-void host_tier_start(vt_context* ctx)
+int host_tier_start(vt_context* ctx)
 {
     BrickCache& c = *ctx->cache;
-    if (!c.host || c.host->started) return;
+    if (!c.host || c.host->started) return VT_OK;
     const bool have = ctx->octree_ready && ctx->max_level >= c.level_c && (int)ctx->up_runs.size() > c.level_c;
-    c.host->start(ctx->n, have ? &ctx->up_runs[(size_t)c.level_c] : nullptr);
+    return c.host->start(ctx->n, have ? &ctx->up_runs[(size_t)c.level_c] : nullptr);
 }
 
 // Load `count` requested bricks (indices relative to 8^levelC) into the
@@ -762,17 +762,13 @@ int vt_cache_attach_device_volume(vt_context* ctx, const float* d_vol, int32_t X
     return VT_OK;
 }
 
-int vt_cache_attach_hdf5(vt_context* ctx, const char* file, const char* dataset)
+namespace {
+
+// the pinned host tier of the cache, reading `dataset` of `file` (opened by
+// the caller's check)
+int attach_host_tier(vt_context* ctx, const char* file, const char* dataset)
 {
-    if (!ctx || !file || !dataset) return fail(VT_E_ARG, "NULL argument");
-    if (!ctx->cache) return fail(VT_E_STATE, "no cache");
     BrickCache& c = *ctx->cache;
-    if (c.cpu_slots <= 0) return fail(VT_E_ARG, "an HDF5 source needs a host tier (cpuSlots > 0)");
-    {
-        H5Volume probe;
-        std::string err;
-        if (!probe.open(file, dataset, 0, err)) return fail(VT_E_IO, err);
-    }
     std::unique_ptr<HostTier> h(new HostTier);
     h->level_c = c.level_c;
     h->cube = c.cube;
@@ -794,6 +790,21 @@ int vt_cache_attach_hdf5(vt_context* ctx, const char* file, const char* dataset)
     c.d_vol = nullptr;
     c.host = std::move(h);
     return VT_OK;
+}
+
+}  // namespace
+
+int vt_cache_attach_hdf5(vt_context* ctx, const char* file, const char* dataset)
+{
+    if (!ctx || !file || !dataset) return fail(VT_E_ARG, "NULL argument");
+    if (!ctx->cache) return fail(VT_E_STATE, "no cache");
+    if (ctx->cache->cpu_slots <= 0) return fail(VT_E_ARG, "an HDF5 source needs a host tier (cpuSlots > 0)");
+    {
+        H5Volume probe;
+        std::string err;
+        if (!probe.open(file, dataset, 0, err)) return fail(VT_E_IO, err);
+    }
+    return attach_host_tier(ctx, file, dataset);
 }
 
 int vt_cache_attach_host_from_device(vt_context* ctx, const float* d_vol, int32_t X, int32_t Y, int32_t Z)
@@ -874,10 +885,10 @@ int vt_cache_prefetch_host(vt_context* ctx, int wait)
     if (!ctx->octree_ready) return fail(VT_E_STATE, "upload the octree first (brick geometry needs nLevels)");
     HostTier& h = *ctx->cache->host;
     ctx->cache->n = ctx->n;
-    host_tier_start(ctx);
+    if (int rc = host_tier_start(ctx)) return rc;
     if (wait && h.full_mode) {
         std::unique_lock<std::mutex> lk(h.m);
-        h.cv.wait(lk, [&] { return h.reader_failed || h.slabs_done >= (int)h.side_bricks(); });
+        h.cv.wait(lk, [&] { return h.reader_failed || h.complete.load(std::memory_order_acquire); });
         if (h.reader_failed) return fail(VT_E_IO, "host tier reader: " + h.reader_err);
     }
     return VT_OK;
@@ -892,7 +903,7 @@ int vt_cache_load_bricks(vt_context* ctx, const uint64_t* codes, int64_t count)
     if (!c.d_vol && !c.host) return fail(VT_E_STATE, "no brick source attached");
     if (int rc = set_device(ctx)) return rc;
     c.n = ctx->n;
-    host_tier_start(ctx);
+    if (int rc = host_tier_start(ctx)) return rc;
     std::vector<uint32_t> req;
     uint64_t base = 1ull << (3 * c.level_c);
     for (int64_t i = 0; i < count; ++i) {
@@ -1041,7 +1052,7 @@ static int render_frame(vt_context* ctx, const vt_camera* cam, const vt_render_p
     if ((1 << (ctx->n - c.level_c)) != c.cube) return fail(VT_E_ARG, "cubeDim must equal 2^(nLevels-levelC)");
     if (int rc = set_device(ctx)) return rc;
     c.n = ctx->n;
-    host_tier_start(ctx);
+    if (int rc = host_tier_start(ctx)) return rc;
 
     const uint32_t R = (uint32_t)cam->W * (uint32_t)cam->H;
     RayState own;
@@ -1519,6 +1530,75 @@ int vt_volume_generate(vt_context* ctx, const vt_gen_params* p, int64_t X, int64
     return VT_OK;
 }
 
+int vt_volume_write_bricks_hdf5(vt_context* ctx, const float* d_vol, int64_t X, int64_t Y, int64_t Z, int32_t levelC,
+                                int32_t cubeDim, int32_t cubeInc, const uint64_t* codes, int64_t ncodes,
+                                const char* file, const char* dataset)
+{
+    if (!ctx || !d_vol || !codes || !file || !dataset) return fail(VT_E_ARG, "NULL argument");
+    if (X <= 0 || Y <= 0 || Z <= 0 || X > INT32_MAX || Y > INT32_MAX || Z > INT32_MAX)
+        return fail(VT_E_ARG, "bad dimensions");
+    if (levelC < 0 || levelC > 9 || cubeDim <= 0 || (cubeDim & (cubeDim - 1)) || cubeInc < 0 || ncodes <= 0)
+        return fail(VT_E_ARG, "bad brick layout");
+    const uint64_t lo = 1ull << (3 * levelC), hi = 1ull << (3 * levelC + 3);
+    for (int64_t k = 0; k < ncodes; ++k)
+        if (codes[k] < lo || codes[k] >= hi || (k > 0 && codes[k] <= codes[k - 1]))
+            return fail(VT_E_ARG, "codes must be ascending codes of level levelC");
+    int n = levelC;
+    while ((1 << (n - levelC)) < cubeDim) ++n;
+    if (int rc = set_device(ctx)) return rc;
+    const int S = cubeDim + 2 * cubeInc;
+    const size_t elems = (size_t)S * S * S;
+    const int64_t dims[3] = {X, Y, Z};
+    std::string err;
+    H5BrickWriter w;
+    if (!w.create(file, dataset, codes, ncodes, levelC, cubeDim, cubeInc, dims, err)) return fail(VT_E_IO, err);
+    // batches of bricks gathered on the device (k_bricks_from_volume) and
+    // written as consecutive records
+    const int64_t per = std::max<int64_t>(1, std::min<int64_t>(ncodes, (int64_t)((256ull << 20) / (elems * 4))));
+    DevBuf<float> d_bricks;
+    DevBuf<int64_t> d_codes;
+    DevBuf<int32_t> d_slots;
+    PinBuf<float> h_bricks;
+    if (d_bricks.ensure((size_t)per * elems) != hipSuccess || d_codes.ensure((size_t)per) != hipSuccess ||
+        d_slots.ensure((size_t)per) != hipSuccess || h_bricks.ensure((size_t)per * elems) != hipSuccess)
+        return fail(VT_E_NOMEM, "brick staging");
+    std::vector<int32_t> slots((size_t)per);
+    for (int64_t i = 0; i < per; ++i) slots[(size_t)i] = (int32_t)i;
+    HIP_TRY(hipMemcpyAsync(d_slots.p, slots.data(), (size_t)per * 4, hipMemcpyHostToDevice, ctx->stream));
+    for (int64_t k0 = 0; k0 < ncodes; k0 += per) {
+        const int64_t nb = std::min(per, ncodes - k0);
+        HIP_TRY(hipMemcpyAsync(d_codes.p, codes + k0, (size_t)nb * 8, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(launch_bricks_from_volume(d_vol, (int)X, (int)Y, (int)Z, d_codes.p, d_slots.p, (int)nb, levelC, n,
+                                          cubeDim, cubeInc, elems, d_bricks.p, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(h_bricks.p, d_bricks.p, (size_t)nb * elems * 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (!w.write(k0, nb, h_bricks.p, err)) return fail(VT_E_IO, err);
+    }
+    if (!w.close(err)) return fail(VT_E_IO, err);
+    return VT_OK;
+}
+
+int vt_cache_attach_hdf5_bricks(vt_context* ctx, const char* file, const char* dataset)
+{
+    if (!ctx || !file || !dataset) return fail(VT_E_ARG, "NULL argument");
+    if (!ctx->cache) return fail(VT_E_STATE, "no cache");
+    BrickCache& c = *ctx->cache;
+    std::string err;
+    H5BrickReader rd;
+    if (!rd.open(file, dataset, err)) return fail(VT_E_IO, err);
+    if (rd.level_c != c.level_c || rd.cube != c.cube || rd.inc != c.inc)
+        return fail(VT_E_ARG, "the brick file's levelC/cubeDim/cubeInc (" + std::to_string(rd.level_c) + "/" +
+                                  std::to_string(rd.cube) + "/" + std::to_string(rd.inc) + ") differ from the cache's");
+    if (c.cpu_slots <= 0) return fail(VT_E_ARG, "an HDF5 source needs a host tier (cpuSlots > 0)");
+    if (int rc = attach_host_tier(ctx, file, dataset)) return rc;
+    HostTier& h = *c.host;
+    h.bricked = true;
+    h.rec_of.assign((size_t)c.nbricks, -1);
+    const uint64_t base = 1ull << (3 * c.level_c);
+    for (size_t k = 0; k < rd.codes.size(); ++k) h.rec_of[(size_t)(rd.codes[k] - base)] = (int64_t)k;
+    return VT_OK;
+}
+
 int vt_volume_write_hdf5(vt_context* ctx, const float* d_vol, int64_t X, int64_t Y, int64_t Z, const char* file,
                          const char* dataset, int32_t chunk)
 {
@@ -1603,7 +1683,7 @@ int vt_stage_cache(vt_context* ctx, vt_visible_cube* h_cubes, int64_t num, int32
     if ((1 << (nLevels - c.level_c)) != c.cube) return fail(VT_E_ARG, "cubeDim must equal 2^(nLevels-levelC)");
     if (int rc = set_device(ctx)) return rc;
     c.n = nLevels;
-    host_tier_start(ctx);
+    if (int rc = host_tier_start(ctx)) return rc;
     const int shift = 3 * (levelO - c.level_c);
     const uint64_t base = 1ull << (3 * c.level_c);
     std::vector<uint32_t> need;
