@@ -147,17 +147,21 @@ __device__ __forceinline__ bool slab(const Ray& r, int mnx, int mny, int mnz, in
 // list.  Built on the host from the reference run lists.
 
 struct OctreeView {
-    const uint8_t* child_bytes;   // levels 1..maxLevel concatenated
-    uint64_t level_off[24];       // byte offset of level l's table (index by l)
+    const uint8_t* child_bytes;   // levels 1..maxLevel concatenated (level l: 8^(l-1) bytes)
+    uint64_t level_off[24];       // byte offset of level l's table, (8^(l-1) - 1) / 7
     int n;                        // cube side 2^n
     int final_level;              // levelO: the level the traversal emits
     int root_valid;               // level-0 run list contains node 1
 };
 
+// The tables follow each other from level 1, so the offset of level
+// plevel + 1 is (8^plevel - 1) / 7, computed here rather than looked up in
+// level_off by a lane-varying index (which would be one more dependent load
+// per walk step).  Parent levels are below 9, so 32 bits hold every index.
 __device__ __forceinline__ uint32_t child_mask(const OctreeView& o, uint64_t parent, int plevel)
 {
-    uint64_t idx = parent - (1ull << (3 * plevel));
-    return o.child_bytes[o.level_off[plevel + 1] + idx];
+    const uint32_t p8 = 1u << (3 * plevel);
+    return o.child_bytes[(uint32_t)parent - p8 + (p8 - 1u) / 7u];
 }
 
 // Literal restatement of the bubble sort of Octree.cu:665-706 on <= 8 keys.
