@@ -247,10 +247,11 @@ int vt_set_early_loads(vt_context *ctx, int64_t reserve);
  * finishes in the later passes (with the wide march).  Default 16; 0: no
  * limit. */
 int vt_set_first_pass_budget(vt_context *ctx, int32_t nodes);
-/* Tile order of the one-lane first pass (no reference counterpart; results
- * are identical for every setting): supertile S > 1 deals S x S blocks of
- * 8x8-pixel tiles round-robin to the XCDs, so tiles that share bricks share
- * one L2; 1 (default) deals single tiles.  S in [1, 16]. */
+/* Tile order of the one-lane first pass, of single frames and of frames in
+ * flight (no reference counterpart; results are identical for every
+ * setting): supertile S > 1 deals S x S blocks of 8x8-pixel tiles
+ * round-robin to the XCDs, so tiles that share bricks share one L2
+ * (default 4); 1 deals single tiles.  S in [1, 16]. */
 int vt_set_first_pass_tiles(vt_context *ctx, int32_t supertile);
 /* Visited-brick trace of the last frame: out[i] = 1 if brick
  * (i + 8^levelC) was marched by at least one ray.  n = 8^levelC. */

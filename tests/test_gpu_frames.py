@@ -68,6 +68,36 @@ def test_frames_in_flight_equal_single_frames(vt, vto, ctx):
     assert_bits(single[0][1], single[0][0], hc, rc)
 
 
+def test_frames_with_tile_blocks_equal_single_frames(vt, vto, ctx):
+    """The multi-frame first passes dealt to the XCDs as S x S blocks of 8x8
+    tiles (frame regions padded to whole rounds of blocks, image sizes that
+    leave partial blocks) give the bytes of single frames with single tiles."""
+    from visualtur_amd.prefetch import random_walk
+
+    vol = blobs_volume(64, 14, seed=73)
+    sc = Scene(vto, vol, 0.3, 3, 2)
+    sc.setup_gpu(vt, ctx, slots=9)
+    sizes = [(96, 80), (33, 47), (120, 50), (17, 129), (64, 64)]
+    cams = [random_walk(W, H, 64, 1, seed=950 + k, radius=(1.4, 2.2))[0] for k, (W, H) in enumerate(sizes)]
+    single = []
+    try:
+        ctx.set_first_pass_tiles(1)
+        for c in cams:
+            rg, hg, st = sc.render_gpu(vt, ctx, c)
+            single.append((rg, hg))
+        for S, max_frames in ((1, 3), (2, 3), (3, 4), (4, 2), (8, 5)):
+            print(f"supertile {S} frames in flight {max_frames}", flush=True)
+            ctx.set_first_pass_tiles(S)
+            ctx.cache_clear()
+            out, stats, agg = _frames(vt, ctx, sc, cams, max_frames)
+            for k in range(len(cams)):
+                assert out[k][1].tobytes() == single[k][1].tobytes(), (S, k)
+                assert out[k][0].tobytes() == single[k][0].tobytes(), (S, k)
+    finally:
+        ctx.set_first_pass_tiles()
+        ctx.cache_destroy()
+
+
 def test_frames_with_mixed_isos_equal_single_frames(vt, vto, ctx):
     """Frames of two isovalues in flight together over one octree: the
     proven sub-block sides are kept for one iso at a time (the newest

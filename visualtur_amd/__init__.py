@@ -485,9 +485,10 @@ class Context:
         in the later passes (0: no limit)."""
         self._c(lib().vt_set_first_pass_budget(self.handle, int(nodes)))
 
-    def set_first_pass_tiles(self, supertile: int = 1) -> None:
+    def set_first_pass_tiles(self, supertile: int = 4) -> None:
         """Deal S x S blocks of 8x8 tiles to the XCDs in the one-lane first
-        pass (1: single tiles)."""
+        passes, of single frames and of frames in flight (1: single tiles;
+        4: the default)."""
         self._c(lib().vt_set_first_pass_tiles(self.handle, int(supertile)))
 
     def brick_trace(self, level_c: int) -> np.ndarray:

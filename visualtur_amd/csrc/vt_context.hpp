@@ -585,7 +585,7 @@ struct vt_context {
     int64_t wide_first = 1 << 21;   // ... and in the first pass
     int64_t early_reserve = 256;    // free slots kept for loads issued while the first pass runs (0: off)
     int first_nodes = 16;           // walk nodes per ray in the first pass before it parks (0: no limit)
-    int supertile = 1;             // first pass: S x S tile blocks per XCD (1: single tiles)
+    int supertile = 4;             // first passes: S x S tile blocks per XCD (1: single tiles)
     bool pass_state_clean = false;  // the last frame left counters and request bits at zero
     int64_t early_last = INT64_MAX;  // bricks the previous frame's first pass requested
     int persistent = 0;   // persistent render pass with lane refill (measured slower; option)

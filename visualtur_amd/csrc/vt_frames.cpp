@@ -191,7 +191,13 @@ int frames_loop(vt_frames* f, std::deque<Pending>& waiting, std::vector<Slot>& s
             if (!s.used || !s.fresh) continue;
             const Camera& cam = f->h_jobs.p[k].cam;
             a.fresh_slot[nfresh] = (uint32_t)k;
-            tiles += (uint32_t)(((cam.W + 7) / 8) * ((cam.H + 7) / 8));
+            const uint32_t tx8 = (uint32_t)((cam.W + 7) / 8), ty8 = (uint32_t)((cam.H + 7) / 8);
+            if (ctx->supertile > 1) {   // whole rounds of S x S blocks for the 8 XCDs (k_render_multi)
+                const uint32_t S = (uint32_t)ctx->supertile, blocks = ((tx8 + S - 1) / S) * ((ty8 + S - 1) / S);
+                tiles += (blocks + 7) / 8 * 8 * S * S;
+            } else {
+                tiles += tx8 * ty8;
+            }
             a.fresh_tile0[++nfresh] = tiles;
         }
         if (nfresh == 0 && in_count == 0) {
@@ -237,6 +243,7 @@ int frames_loop(vt_frames* f, std::deque<Pending>& waiting, std::vector<Slot>& s
         a.defer_stride = (uint32_t)f->max_frames << f->rshift;
         a.job_parked = f->d_parked.p;
         a.nfresh = nfresh;
+        a.supertile = ctx->supertile;
         a.in_list = in_count ? f->list[cur].p : nullptr;
         a.in_count = in_count;
         a.list_block0[0] = 0;

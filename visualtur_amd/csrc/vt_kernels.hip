@@ -552,12 +552,17 @@ __device__ __forceinline__ uint32_t list_frame(const RenderArgs& a, uint32_t grp
 // consecutive entries of one frame's parked list (the later passes of the
 // frames in flight, started first since they are the long walks), the rest
 // take the 8x8 tiles of the fresh frames in order (fresh_tile0 prefix sums),
-// dealt round-robin to the XCDs as in k_render_pass.  Every wave holds rays
-// of one frame, so the frame's camera, shading and outputs are wave-uniform.
+// dealt round-robin to the XCDs as in k_render_pass.  With supertile S > 1
+// the list region is padded to a multiple of 8 workgroups and each fresh
+// frame's region holds ceil(blocks / 8) * 8 * S^2 workgroups, so that
+// workgroup b still runs on XCD b % 8 and every XCD is dealt whole S x S
+// blocks of a frame's tiles, as in k_render_pass.  Every wave holds rays of
+// one frame, so the frame's camera, shading and outputs are wave-uniform.
 __global__ __launch_bounds__(kPassBlock, 5) void k_render_multi(RenderArgs a)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t list_blocks = a.list_block0[kMaxFrames];
+    const uint32_t list_region = a.supertile > 1 ? (list_blocks + 7u) & ~7u : list_blocks;
     uint32_t id = 0, fs;
     bool active, first;
     if (blockIdx.x < list_blocks) {
@@ -566,17 +571,33 @@ __global__ __launch_bounds__(kPassBlock, 5) void k_render_multi(RenderArgs a)
         active = off < a.list_count[fs];
         id = active ? a.in_list[(fs << a.rshift) + off] : 0u;
         first = false;
+    } else if (blockIdx.x < list_region) {   // padding
+        fs = 0;
+        active = false;
+        first = false;
     } else {
-        const uint32_t t = blockIdx.x - list_blocks;
+        const uint32_t t = blockIdx.x - list_region;
         int k = 0;
         while (k + 1 < a.nfresh && t >= a.fresh_tile0[k + 1]) ++k;
         fs = a.fresh_slot[k];
         const Camera& cam = a.jobs[fs].cam;
-        const int tx8 = (cam.W + 7) >> 3;
+        const int tx8 = (cam.W + 7) >> 3, ty8 = (cam.H + 7) >> 3;
         const uint32_t tt = t - a.fresh_tile0[k];
         active = t < a.fresh_tile0[a.nfresh];
-        const int i = (int)(tt / (uint32_t)tx8) * 8 + (int)(lane >> 3);
-        const int j = (int)(tt % (uint32_t)tx8) * 8 + (int)(lane & 7);
+        int ti, tj;
+        if (a.supertile > 1) {
+            const int S = a.supertile, sx = (tx8 + S - 1) / S, sy = (ty8 + S - 1) / S;
+            const int kk = (int)(tt >> 3), r = kk % (S * S);
+            const int q = (kk / (S * S)) * 8 + (int)(tt & 7u);
+            tj = (q % sx) * S + r % S;
+            ti = (q / sx) * S + r / S;
+            active = active && q < sx * sy && tj < tx8 && ti < ty8;
+        } else {
+            ti = (int)(tt / (uint32_t)tx8);
+            tj = (int)(tt % (uint32_t)tx8);
+        }
+        const int i = ti * 8 + (int)(lane >> 3);
+        const int j = tj * 8 + (int)(lane & 7);
         active = active && i < cam.H && j < cam.W;
         id = active ? (fs << a.rshift) | (uint32_t)(i * cam.W + j) : 0u;
         first = true;
@@ -884,7 +905,9 @@ hipError_t launch_render_multi(const RenderArgs& a, int lg, hipStream_t s)
         hipLaunchKernelGGL(k_render_multi_wide, dim3(grid), dim3(kPassBlock), 0, s, a, lg);
         return hipGetLastError();
     }
-    const uint64_t grid = (uint64_t)list_groups + tiles;
+    const uint32_t list_region = a.supertile > 1 ? (list_groups + 7u) & ~7u : list_groups;
+    if (a.supertile > 1 && tiles % 8u != 0) return hipErrorInvalidValue;   // frame regions in whole rounds of 8
+    const uint64_t grid = (uint64_t)list_region + tiles;
     if (grid == 0) return hipSuccess;
     hipLaunchKernelGGL(k_render_multi, dim3((unsigned)grid), dim3(kPassBlock), 0, s, a);
     return hipGetLastError();
