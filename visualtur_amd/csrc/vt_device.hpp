@@ -273,74 +273,6 @@ __device__ __forceinline__ uint32_t sorted_children(const Ray& r, const OctreeVi
 __device__ __forceinline__ int order_count(uint32_t order) { return (int)(order >> 24); }
 __device__ __forceinline__ int order_at(uint32_t order, int p) { return (int)((order >> (3 * p)) & 7u); }
 
-// Next node of the reference's DFS emission sequence at level final_level.
-//   descend = true : `node` itself has not been explored yet
-//   descend = false: `node` (and its subtree) is finished
-// Returns 0 when the traversal is exhausted.  This is the stack DFS of
-// Octree.cu:620-720 without the stack: the parent of a node is code>>3 and
-// its position among its siblings is found by recomputing the parent's
-// sorted child list, which is a pure function of (parent, ray).
-// Sorted child lists of the last two parents expanded by this ray: the
-// next node is usually a sibling, whose parent's list is then reused.
-struct OrderCache {
-    uint64_t p[2] = {0, 0};   // parent codes (0 = empty; codes are >= 1)
-    uint32_t ord[2] = {0, 0};
-};
-
-__device__ __forceinline__ uint32_t sorted_children_cached(const Ray& r, const OctreeView& o, uint64_t parent,
-                                                           int plevel, OrderCache& c)
-{
-    if (parent == c.p[0]) return c.ord[0];
-    if (parent == c.p[1]) return c.ord[1];
-    uint32_t ord = sorted_children(r, o, parent, plevel);
-    c.p[1] = c.p[0];
-    c.ord[1] = c.ord[0];
-    c.p[0] = parent;
-    c.ord[0] = ord;
-    return ord;
-}
-
-template <class Cache>
-__device__ __forceinline__ uint64_t next_node(const Ray& r, const OctreeView& o,
-                                              uint64_t node, bool descend, Cache& cache)
-{
-    int level = code_level(node);
-    for (;;) {
-        // one sorted-children site per iteration: the node's own list when
-        // descending, its parent's list when moving to the next sibling
-        if (descend && level == o.final_level) return node;
-        if (!descend && level == 0) return 0;
-        const uint64_t parent = descend ? node : node >> 3;
-        const uint32_t ord = sorted_children_cached(r, o, parent, descend ? level : level - 1, cache);
-        const int cnt = order_count(ord);
-        if (descend) {
-            if (cnt > 0) {
-                node = (node << 3) | (uint64_t)order_at(ord, 0);
-                level += 1;
-            } else {
-                descend = false;
-            }
-            continue;
-        }
-        int k = (int)(node & 7u);
-        int p = 0;
-        while (p < cnt && order_at(ord, p) != k) ++p;
-        if (p + 1 < cnt) {
-            node = (parent << 3) | (uint64_t)order_at(ord, p + 1);
-            descend = true;
-        } else {
-            node = parent;
-            level -= 1;
-        }
-    }
-}
-
-__device__ __forceinline__ uint64_t next_node(const Ray& r, const OctreeView& o, uint64_t node, bool descend)
-{
-    OrderCache cache;
-    return next_node(r, o, node, descend, cache);
-}
-
 // The sorted child list without sorting.  Along one axis a parent's
 // children lie between three planes, whose times are the nine products
 // sorted_children computes (near, mid, far in ray order).  The children the
@@ -435,26 +367,14 @@ __device__ __forceinline__ int seq_child(const Ray& r, const OctreeView& o, uint
     return -1;
 }
 
-// The child after k (k < 0: the first) in the sorted list: where seq_child
-// finds equal times.  Out of line, with the ray by value, so that the walk
-// keeps its registers for the common case.
-__device__ __noinline__ int sorted_child_after(float ox, float oy, float oz, float dx, float dy, float dz, float ix,
-                                               float iy, float iz, const OctreeView* o, uint64_t parent, int plevel,
-                                               int k)
-{
-    const Ray r{ox, oy, oz, dx, dy, dz, ix, iy, iz};
-    const uint32_t ord = sorted_children(r, *o, parent, plevel);
-    const int cnt = order_count(ord);
-    int p = 0;
-    if (k >= 0) {
-        while (p < cnt && order_at(ord, p) != k) ++p;
-        p += 1;
-    }
-    return p < cnt ? order_at(ord, p) : -1;
-}
-
-// next_node with the children taken in ray order by seq_child, and the
-// sorted list where seq_child finds equal times.
+// Next node of the reference's DFS emission sequence at level final_level.
+//   descend = true : `node` itself has not been explored yet
+//   descend = false: `node` (and its subtree) is finished
+// Returns 0 when the traversal is exhausted.  This is the stack DFS of
+// Octree.cu:620-720 without the stack: the parent of a node is code>>3 and
+// the next child after it is found from the parent's planes (seq_child, in
+// ray order), or from the parent's sorted child list where seq_child finds
+// equal times; both are pure functions of (parent, ray).
 __device__ __forceinline__ uint64_t next_node_seq(const Ray& r, const OctreeView& o, uint64_t node, bool descend)
 {
     int level = code_level(node);
