@@ -1139,7 +1139,7 @@ def main() -> int:
     ap.add_argument("--max-defer", type=int, default=1,
                     help="absent-brick nodes a ray may walk past per pass (0 = park at the first)")
     ap.add_argument("--prefetch-min-prob", type=float, default=0.8)
-    ap.add_argument("--frames-in-flight", type=int, default=16, help="c3 serving leg: frames sharing launches")
+    ap.add_argument("--frames-in-flight", type=int, default=32, help="c3 serving leg: frames sharing launches")
     ap.add_argument("--c2-source", default="bricks", choices=["bricks", "volume"],
                     help="c2 host tier source: brick-major file or the 32^3-chunked volume file")
     ap.add_argument("--c5-frames-in-flight", type=int, default=4, help="c5 serving leg: frames sharing launches")

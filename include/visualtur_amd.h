@@ -422,7 +422,7 @@ typedef struct {
     double ms_kernel;           /* device time of the launches */
     double wall_s, busy_s;      /* since creation or the last vt_frames_get_stats; busy: frames in flight */
 } vt_frames_stats;
-/* max_frames: 1..16; max_rays: the largest W*H a frame may have. */
+/* max_frames: 1..32; max_rays: the largest W*H a frame may have. */
 int vt_frames_create(vt_context *ctx, int32_t max_frames, int64_t max_rays, vt_frames **out);
 /* m = NULL turns the prefetcher off; budget and min_prob as in vt_prefetch. */
 int vt_frames_set_prefetch(vt_frames *f, vt_mlp *m, int64_t budget, float min_prob);

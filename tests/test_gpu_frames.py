@@ -45,7 +45,7 @@ def test_frames_in_flight_equal_single_frames(vt, vto, ctx):
         passes += st.passes
     net = vt.MLP(ctx, seed=3, max_batch=4096)
     try:
-        for max_frames, pre in ((1, False), (2, False), (4, False), (8, False), (16, False), (3, True)):
+        for max_frames, pre in ((1, False), (2, False), (4, False), (8, False), (16, False), (32, False), (3, True)):
             ctx.cache_clear()
             # min_prob 0 with the untrained network: every absent candidate qualifies, so
             # the prefetcher loads at every boundary where a frame waits
@@ -153,7 +153,7 @@ def test_frames_refuse_bad_requests(vt, vto, ctx):
     big = orbit_camera(vt, 64, 64, (16, 16, 16), 60.0, 10.0)
     r, h = ctx.alloc(64 * 64 * 16), ctx.alloc(64 * 64 * 32)
     try:
-        for bad in (0, 17):
+        for bad in (0, 33):
             with pytest.raises(vt.VTError):
                 vt.Frames(ctx, max_frames=bad, max_rays=32 * 32)
         with vt.Frames(ctx, max_frames=2, max_rays=32 * 32) as fr:

@@ -397,7 +397,7 @@ extern "C" {
 int vt_frames_create(vt_context* ctx, int32_t max_frames, int64_t max_rays, vt_frames** out)
 {
     if (!ctx || !out) return fail(VT_E_ARG, "NULL argument");
-    if (max_frames < 1 || max_frames > kMaxFrames) return fail(VT_E_ARG, "max_frames must be in [1, 16]");
+    if (max_frames < 1 || max_frames > kMaxFrames) return fail(VT_E_ARG, "max_frames must be in [1, 32]");
     if (max_rays < 1 || max_rays > (1ll << 26)) return fail(VT_E_ARG, "max_rays must be in [1, 2^26]");
     if (!ctx->octree_ready) return fail(VT_E_STATE, "no octree uploaded");
     if (!ctx->cache) return fail(VT_E_STATE, "no brick cache");

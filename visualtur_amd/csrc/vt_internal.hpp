@@ -65,7 +65,7 @@ struct RenderJob {
     float4* rgba;
     HitRecord* hits;
 };
-constexpr int kMaxFrames = 16;
+constexpr int kMaxFrames = 32;
 
 struct RenderArgs {
     Camera cam;
