@@ -109,3 +109,21 @@ def test_ppm_writer_layout(vt, tmp_path):
     assert px[3 - 1 - 1, 1, 1] == 127      # ray 4 = (x=1, y=1) -> row 1
     assert px[3 - 1 - 2, 1, 2] == 255      # ray 5 = (x=1, y=2) -> row 0
     assert px[3 - 1 - 1, 0, 1] == 0        # ray 1 = (x=0, y=1) -> row 1, NaN
+
+
+def test_headless_viewer_refuses_bad_input(tmp_path):
+    """The headless viewer (drawPixel.cu without a window) checks its
+    arguments and the display size it reads before touching a device, and a
+    missing HDF5 file is an error, not a crash."""
+    viewer = ROOT / "visualtur_amd" / "_build" / "vt_view"
+    assert viewer.exists(), "build() compiles the viewer"
+    r = subprocess.run([str(viewer)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
+    for stdin in ("", "0\n5\n1\n2\n3\n", "8\n8\n1\n"):
+        r = subprocess.run([str(viewer), "a.h5", "value", "a.oct"], input=stdin, capture_output=True, text=True,
+                           timeout=60)
+        assert r.returncode == 2 and "positive width" in r.stderr, stdin
+    r = subprocess.run([str(viewer), str(tmp_path / "missing.h5"), "value", "a.oct", "0", str(tmp_path)],
+                       input="8\n8\n1\n2\n3\n", capture_output=True, text=True, timeout=60)
+    assert r.returncode == 4 and "unable to open" in r.stderr
+

@@ -81,6 +81,9 @@ def build(verbose: bool = False) -> Path:
     _run([HIPCC, "-shared", "-fPIC", "-o", str(cls), *[str(obj / (s + ".o")) for s in classes],
           f"-L{BUILD}", "-lvt_hip", "-Wl,-rpath,$ORIGIN", f"-L{deps}", "-lhdf5",
           "-Wl,-rpath,$ORIGIN/deps"], verbose)
+    # the headless viewer (drawPixel.cu without a window) on the class surface
+    _run([HIPCC, *COMMON, *INCS, str(CSRC / "vt_view.cpp"), "-o", str(BUILD / "vt_view"), f"-L{BUILD}", "-lvisualtur",
+          "-lvt_hip", "-Wl,-rpath,$ORIGIN", f"-L{deps}", "-lhdf5", "-Wl,-rpath,$ORIGIN/deps"], verbose)
     # C++ drivers of the class surface used by the tests
     for drv in sorted((ROOT / "tests" / "cpp").glob("*.cpp")):
         _run([HIPCC, *COMMON, *INCS, str(drv), "-o", str(BUILD / drv.stem), f"-L{BUILD}", "-lvisualtur",

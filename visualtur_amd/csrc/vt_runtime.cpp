@@ -525,6 +525,17 @@ int vt_synchronize(vt_context* ctx)
     return VT_OK;
 }
 
+int vt_device_memory(int32_t device, size_t* free_bytes, size_t* total_bytes)
+{
+    if (!free_bytes || !total_bytes) return fail(VT_E_ARG, "NULL argument");
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail(VT_E_ARG, "no such device");
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipMemGetInfo(free_bytes, total_bytes));
+    return VT_OK;
+}
+
 int vt_malloc(vt_context* ctx, size_t bytes, void** out)
 {
     if (!ctx || !out) return fail(VT_E_ARG, "NULL argument");
