@@ -116,7 +116,9 @@ int vt_memcpy_h2d(vt_context *ctx, void *d_dst, const void *src, size_t bytes);
 int vt_memcpy_d2h(vt_context *ctx, void *dst, const void *d_src, size_t bytes);
 int vt_memset(vt_context *ctx, void *d_dst, int value, size_t bytes);
 /* Free and total device memory of `device` (the cudaMemGetInfo that sizes
- * the viewer's brick cache, drawPixel.cu:214-217). */
+ * the viewer's brick cache, drawPixel.cu:214-217).  Like the reference's
+ * cudaSetDevice before it (drawPixel.cu:205), it makes `device` the calling
+ * thread's current device, which vt_context_create(-1, ...) then uses. */
 int vt_device_memory(int32_t device, size_t *free_bytes, size_t *total_bytes);
 
 /* -------------------------------------------------------------- octree */
